@@ -1,0 +1,192 @@
+/*
+ * tapsim_gpu.h -- C ABI of the MI355X (gfx950) per-event electrostatics of
+ * TAPSim (ashtonmv/tapsim_modified).
+ *
+ * Plain pointers and sizes only; every call returns an int status
+ * (TSG_OK or a negative TSG_E_*), with tsg_last_error() describing the last
+ * failure of the calling thread.  No exception crosses this boundary.  One
+ * HIP stream per grid handle; calls on one handle are serialised by the
+ * caller (the reference's Grid_3d::Table is not re-entrant either,
+ * tapsim/grid_3d.cpp:1238-1282).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef TAPSIM_GPU_H
+#define TAPSIM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSG_OK 0
+#define TSG_E_ARG (-1)
+#define TSG_E_HIP (-2)
+#define TSG_E_NAN (-3)          /* std::runtime_error of computePotential & co. */
+#define TSG_E_GEOM (-4)
+#define TSG_E_RELAX_FAULT (-5)  /* Grid_3d::Table::RelaxationFault */
+#define TSG_E_SURFACE (-6)
+#define TSG_E_STATE (-7)
+
+/* precision / method of a grid (selected at creation) */
+#define TSG_PLAIN_F64 0         /* reference algorithm promoted to FP64 (drop-in, bit-exact to the FP64 oracle) */
+#define TSG_FAITHFUL_F32 1      /* the reference's own float storage, double accumulation */
+#define TSG_MIXED_F32_REFINE 2  /* FP32 sweeps on the correction, FP64 residual and refinement */
+
+/* node flags, tapsim/grid_3d.h:84 */
+#define TSG_FLAG_BOUNDARY 0x01
+#define TSG_FLAG_DIRICHLET 0x02
+#define TSG_FLAG_NEUMANN 0x04
+
+const char* tsg_last_error(void);
+int tsg_version(void);
+void tsg_free(void* p);
+
+/* ------------------------------------------------------------------ mesh */
+
+typedef struct tsg_specimen_params
+{
+	double a;                 /* BCC lattice constant, m */
+	double radius;            /* tip radius, m */
+	double shank;             /* cylinder height below the cap centre, m */
+	double box[3];            /* box extent, m (x, y centred on the axis; z from the base) */
+	double shell;             /* vacuum lattice shell, in lattice constants */
+	long long vacuum_target;  /* graded random vacuum nodes (approximate) */
+	double decay;             /* vacuum density ~ distance^-decay */
+	double face_spacing;      /* box surface node spacing, m */
+	double cluster_density;   /* solute clusters per m^3 (0: one phase) */
+	double cluster_radius;    /* m */
+	unsigned long long seed;
+	int id_vacuum, id_top, id_bottom, id_sides, id_matrix, id_solute;
+} tsg_specimen_params;
+
+typedef struct tsg_mesh tsg_mesh;
+
+/* Synthetic BCC [110] tip node set (replaces the node file that the
+   reference reads with File_Io::readGeomGrid, tapsim/file_io.cpp:822).
+   *xyz (3n floats) and *ids (n shorts) are released with tsg_free. */
+int tsg_specimen_generate(const tsg_specimen_params* p, int* n, float** xyz, short** ids, long long counts[5]);
+
+/* Delaunay tetrahedralisation + topology (replaces
+   Geometry_3d::Table::tetrahedralize, tapsim/geometry_3d.cpp:759-810). */
+int tsg_mesh_create(int n, const float* xyz, tsg_mesh** out);
+/* Adopt a given tetrahedralisation (neighbours[i] opposite vertices[i], -1
+   on the hull; Geometry_3d::Table::set_metaInformation, geometry_3d.cpp:695) */
+int tsg_mesh_create_from_tets(int n, const float* xyz, int m, const int* tet_v, const int* tet_n, tsg_mesh** out);
+int tsg_mesh_info(const tsg_mesh*, int* n, int* m, long long* nnz);
+int tsg_mesh_get_tets(const tsg_mesh*, int* tet_v, int* tet_n);
+int tsg_mesh_get_csr(const tsg_mesh*, int* ptr, int* idx);          /* adjacentNodes, ascending */
+int tsg_mesh_get_nodeinfo(const tsg_mesh*, int* assoc, unsigned char* boundary);
+void tsg_mesh_destroy(tsg_mesh*);
+
+
+/* ------------------------------------------------------------------ grid */
+
+/* One species of the configuration table (Configuration::NodeData,
+   tapsim/configuration.h:100-150); the table is indexed by NodeId (0..127). */
+typedef struct tsg_species
+{
+	int defined;
+	float charge_density;     /* C/m^3 */
+	float epsilon;
+	int removable, neumann, dirichlet;
+	float phi;                /* preset potential, V */
+	float mass;               /* amu */
+	int evap_charge;
+	float evap_field;         /* V/m */
+	float evap_energy;        /* eV */
+} tsg_species;
+
+typedef struct tsg_grid tsg_grid;
+
+/* Builds the device grid on a mesh: Voronoi face areas on the GPU, then
+   Grid_3d::Table::fastSync (tapsim/grid_3d.cpp:491-553); potentials start at
+   the species presets (Process::initialization, process.cpp:65-80).
+   mode: TSG_PLAIN_F64 or TSG_FAITHFUL_F32. */
+int tsg_grid_create(const tsg_mesh* mesh, const short* ids, const tsg_species* species /* [128] */,
+                    float temperature, int mode, tsg_grid** out);
+void tsg_grid_destroy(tsg_grid*);
+int tsg_grid_info(const tsg_grid*, int* n, long long* nnz, int* nrows);
+
+/* Grid_3d::Table::relax(cycleSize) (grid_3d.cpp:591-609); returns the deviation */
+int tsg_relax_cycles(tsg_grid*, int cycles, double* deviation);
+/* Grid_3d::Table::relax(threshold, cycleSize, queueSize) (grid_3d.cpp:611-671);
+   *iterations = -1 - iteration when a RelaxationFault (zero slope) ends it */
+int tsg_relax_threshold(tsg_grid*, double threshold, int cycles, int queue, long long* iterations,
+                        double* fault_dev, double* fault_slope, double* last_dev);
+/* Grid_3d::Table::localRelax(index, order, threshold, cycleSize, queueSize)
+   (grid_3d.cpp:707-795) */
+int tsg_local_relax(tsg_grid*, int node, int order, double threshold, int cycles, int queue, long long* iterations);
+
+/* potential buffers (Node::phi, grid_3d.h:57-58) and the buffer switches */
+int tsg_get_phi(tsg_grid*, int buffer, double* out);
+int tsg_set_phi(tsg_grid*, const double* phi0, const double* phi1, int phi_switch, int local_switch);
+int tsg_phi_switches(const tsg_grid*, int* phi_switch, int* local_switch);
+/* Grid_3d::Table::potential(i) (grid_3d.h:184) */
+int tsg_potential(tsg_grid*, int node, double* out);
+/* Grid_3d::Table::field_o1(i) (grid_3d.cpp:909-991), 3 values per node */
+int tsg_field_o1(tsg_grid*, int count, const int* nodes, double* out);
+/* the host mirror of Node::coupling for every directed entry of the CSR */
+int tsg_get_couplings(const tsg_grid*, double* out);
+int tsg_get_flags(const tsg_grid*, unsigned char* out);
+int tsg_get_ids(const tsg_grid*, short* out);
+/* per directed entry: Voronoi face area closed at |pi-pj|/2 (float and
+   double) and closed at 1 m (float) -- Geometry_3d::Table::voronoiArea */
+int tsg_get_areas(const tsg_grid*, float* half32, double* half64, float* one32);
+/* Node::setId + Grid_3d::Table::fastResync (grid_3d.cpp:555-589) */
+int tsg_set_id(tsg_grid*, int node, int id);
+
+/* Surface_3d::Table::init (surface_3d.cpp:50-81) */
+int tsg_surface_init(tsg_grid*, int vacuum_id);
+int tsg_surface_size(const tsg_grid*, int* count);
+int tsg_surface_nodes(const tsg_grid*, int* out);
+/* evap_compute_specificFields + evap_compute_probabilities (linear field),
+   surface_3d.cpp:194-219,266-297; exact_sum also forms the normaliser */
+int tsg_surface_evaluate(tsg_grid*, int exact_sum, double* scaling_ref);
+int tsg_surface_values(tsg_grid*, double* out);
+int tsg_surface_probabilities(tsg_grid*, float* out);
+/* evapMaximum (surface_3d.cpp:423-431) */
+int tsg_surface_candidate(tsg_grid*, int* node, int* near_tie);
+/* evap_takeAway (surface_3d.cpp:507-564) */
+int tsg_take_away(tsg_grid*, int node);
+
+typedef struct tsg_evap_options
+{
+	int prob_mode, evap_mode, vacuum_id;
+	double relax_threshold;
+	int relax_shell, relax_cycle, relax_queue;
+	int relax_global_cycles;
+	int refresh_interval;
+	double refresh_threshold;
+	int refresh_cycle, refresh_queue;
+	int exact_probability;
+} tsg_evap_options;
+
+typedef struct tsg_event
+{
+	int node, id;
+	float probability;
+	double pot_before, field_before[3];
+	double pot_after, field_after[3];
+	long long local_iter, global_iter;
+	int surface_size;
+	double scaling_ref;
+	int near_tie;
+} tsg_event;
+
+/* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k
+   events; *done is the number recorded in out */
+int tsg_evaporate(tsg_grid*, const tsg_evap_options*, int k, tsg_event* out, int* done);
+
+/* Measurement: algorithmic bytes of one sweep (column index + coupling per
+   entry; node, degree, coupling sum, charge and written potential per row;
+   one read of every referenced potential) and the accumulated device time of
+   the threshold relaxations (HIP events on the grid's stream).  No reference
+   counterpart: the reference has no instrumentation of its sweep. */
+int tsg_sweep_stats(const tsg_grid*, double* bytes, long long* rows, long long* entries, long long* referenced,
+                    double* ms, long long* sweeps, long long* launched);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,474 @@
+/*
+ * Fixture driver for the reference TAPSim build (test infrastructure only).
+ *
+ * Links the reference's own objects (built by oracle/Makefile.ref from the
+ * reference sources) and runs their Grid_3d / Surface_3d / Process code on a
+ * mesh given to it, writing raw little-endian arrays that tests/golden/
+ * turns into committed fixtures.  Nothing in the product links this.
+ *
+ * usage: ref_driver IN.bin CONFIG.aly OUTDIR STEP [STEP ...]
+ *
+ * IN.bin (all little endian):
+ *   char[4] "TSGM", int32 version (1), int32 N, int32 M,
+ *   float32 coords[3N] (metres), int16 ids[N], int32 tets[4M]
+ *   M == 0 -> the reference meshes the nodes itself with TetGen
+ *             (Geometry_3d::Table::tetrahedralize, tapsim/geometry_3d.cpp:759)
+ *   M  > 0 -> the given tets are used (positively oriented) and the
+ *             reference derives neighbours/boundary/associated tets
+ *             (Geometry_3d::Table::set_metaInformation, geometry_3d.cpp:695)
+ *
+ * STEPs are applied in order to one System_3d:
+ *   mesh                 build the tetrahedralisation
+ *   sync                 Grid_3d::Table::fastSync      (grid_3d.cpp:491)
+ *   dumpmesh             tets, neighbours, associated tets, boundary flags
+ *   dumpgrid             CSR neighbours, couplings, charges, flags
+ *   dumpphi:TAG          both potential buffers + potential() buffer index
+ *   relax:K:C            K x Grid_3d::Table::relax(C)   (grid_3d.cpp:591)
+ *   relaxthr:T:C:Q       Grid_3d::Table::relax(T,C,Q)   (grid_3d.cpp:611)
+ *   local:NODE:ORD:T:C:Q Grid_3d::Table::localRelax     (grid_3d.cpp:707)
+ *   evapnode:NODE        setId(vacuum) + fastResync     (surface_3d.cpp:513)
+ *   field:TAG            field_o1 / field_o2 of every node (grid_3d.cpp:909,993)
+ *   vorarea:TAG          voronoiArea(i,j,1.0f) both directions for every edge
+ *   evap:...             the steps I-VI of Process::evaporation (process.cpp:680-1037)
+ *   traj:TAG:NODE:INT:STEP  one Trajectory_3d from a node (trajectory_3d.cpp:99)
+ */
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <set>
+#include <fstream>
+#include <stdexcept>
+#include <queue>
+#include <limits>
+#include <cmath>
+
+#include "system_3d.h"
+#include "file_io.h"
+#include "process.h"
+#include "surface_3d.h"
+#include "trajectory_3d.h"
+
+namespace
+{
+	std::string g_out;
+
+	template<class T>
+	void writeArray(const std::string& name, const std::vector<T>& v)
+	{
+		const std::string path = g_out + "/" + name;
+		FILE* f = std::fopen(path.c_str(), "wb");
+		if (f == 0) throw std::runtime_error("cannot write " + path);
+		if (!v.empty()) std::fwrite(&v[0], sizeof(T), v.size(), f);
+		std::fclose(f);
+	}
+
+	std::vector<std::string> split(const std::string& s, char c)
+	{
+		std::vector<std::string> out;
+		std::string cur;
+		for (size_t i = 0; i < s.size(); i++)
+		{
+			if (s[i] == c) { out.push_back(cur); cur.clear(); }
+			else cur += s[i];
+		}
+		out.push_back(cur);
+		return out;
+	}
+
+	struct Input
+	{
+		int n, m;
+		std::vector<float> coords;
+		std::vector<short> ids;
+		std::vector<int> tets;
+	};
+
+	Input readInput(const char* path)
+	{
+		Input in;
+		FILE* f = std::fopen(path, "rb");
+		if (f == 0) throw std::runtime_error("cannot open input");
+		char magic[4];
+		int version;
+		if (std::fread(magic, 1, 4, f) != 4 || std::memcmp(magic, "TSGM", 4) != 0) throw std::runtime_error("bad magic");
+		if (std::fread(&version, 4, 1, f) != 1 || version != 1) throw std::runtime_error("bad version");
+		if (std::fread(&in.n, 4, 1, f) != 1 || std::fread(&in.m, 4, 1, f) != 1) throw std::runtime_error("bad header");
+		in.coords.resize(3 * (size_t)in.n);
+		in.ids.resize(in.n);
+		in.tets.resize(4 * (size_t)in.m);
+		if (std::fread(&in.coords[0], 4, in.coords.size(), f) != in.coords.size()) throw std::runtime_error("short coords");
+		if (std::fread(&in.ids[0], 2, in.ids.size(), f) != in.ids.size()) throw std::runtime_error("short ids");
+		if (in.m > 0 && std::fread(&in.tets[0], 4, in.tets.size(), f) != in.tets.size()) throw std::runtime_error("short tets");
+		std::fclose(f);
+		return in;
+	}
+
+	void dumpPhi(const System_3d& s, const std::string& tag)
+	{
+		const int n = s.gridTable.numNodes();
+		std::vector<float> p0(n), p1(n), pot(n);
+		for (int i = 0; i < n; i++)
+		{
+			p0[i] = s.gridTable.node(i).phi(0);
+			p1[i] = s.gridTable.node(i).phi(1);
+			pot[i] = s.gridTable.potential(i);
+		}
+		writeArray("phi0_" + tag + ".f32", p0);
+		writeArray("phi1_" + tag + ".f32", p1);
+		writeArray("potential_" + tag + ".f32", pot);
+	}
+}
+
+int main(int argc, char** argv)
+{
+	if (argc < 5)
+	{
+		std::fprintf(stderr, "usage: ref_driver IN.bin CONFIG.aly OUTDIR STEP...\n");
+		return 2;
+	}
+
+	std::srand(99); // as tapsim/tapsim.cpp:235
+
+	try
+	{
+		const Input in = readInput(argv[1]);
+		g_out = argv[3];
+
+		System_3d system;
+		system.gridTable.setThreadNum(1);
+		File_Io::readConfig(argv[2], &system.configTable);
+
+		system.geomTable.allocate(in.n);
+		system.gridTable.allocate(in.n);
+		for (int i = 0; i < in.n; i++)
+		{
+			system.geomTable.node(i).coords = Geometry_3d::Point(in.coords[3*i], in.coords[3*i+1], in.coords[3*i+2]);
+			system.geomTable.node(i).boundary = false;
+			system.gridTable.node(i).setId(Configuration::NodeId::fromValue(in.ids[i]));
+			system.gridTable.node(i).setNumber(Configuration::NodeNumber::fromValue(i + 1));
+			const float phi = system.configTable[system.gridTable.id(i)].phi();
+			system.gridTable.node(i).setPhi(0, phi);
+			system.gridTable.node(i).setPhi(1, phi);
+		}
+
+		std::vector<double> relaxDeviations;
+		std::vector<long long> relaxIterations;
+
+		for (int a = 4; a < argc; a++)
+		{
+			const std::vector<std::string> tok = split(argv[a], ':');
+			const std::string& cmd = tok[0];
+			std::fprintf(stderr, "[ref_driver] step %s\n", argv[a]);
+
+			if (cmd == "mesh")
+			{
+				if (in.m == 0)
+					system.geomTable.tetrahedralize();
+				else
+				{
+					system.geomTable.allocateTets(in.m);
+					for (int t = 0; t < in.m; t++)
+						for (int k = 0; k < 4; k++)
+							system.geomTable.tetrahedron(t).vertices[k] = in.tets[4*t+k];
+					system.geomTable.set_metaInformation();
+				}
+			}
+			else if (cmd == "sync")
+			{
+				system.gridTable.fastSync(system.geomTable, system.configTable);
+			}
+			else if (cmd == "dumpmesh")
+			{
+				const int m = system.geomTable.numTetrahedra();
+				std::vector<int> tets(4 * (size_t)m), neigh(4 * (size_t)m), assoc(in.n);
+				std::vector<unsigned char> boundary(in.n);
+				for (int t = 0; t < m; t++)
+					for (int k = 0; k < 4; k++)
+					{
+						tets[4*t+k] = system.geomTable.tetrahedron(t).vertices[k];
+						neigh[4*t+k] = system.geomTable.tetrahedron(t).neighbours[k];
+					}
+				for (int i = 0; i < in.n; i++)
+				{
+					assoc[i] = system.geomTable.node(i).associatedTetrahedron;
+					boundary[i] = system.geomTable.node(i).boundary ? 1 : 0;
+				}
+				writeArray("tets.i32", tets);
+				writeArray("neigh.i32", neigh);
+				writeArray("assoc.i32", assoc);
+				writeArray("boundary.u8", boundary);
+			}
+			else if (cmd == "dumpgrid")
+			{
+				const int n = system.gridTable.numNodes();
+				std::vector<int> ptr(n + 1, 0), idx;
+				std::vector<float> coup, charge(n);
+				std::vector<unsigned char> flags(n);
+				std::vector<short> ids(n);
+				for (int i = 0; i < n; i++)
+				{
+					const Grid_3d::Node& node = system.gridTable.node(i);
+					for (int j = 0; j < node.numNeighbours(); j++)
+					{
+						idx.push_back(node.neighbour(j));
+						coup.push_back(node.coupling(j));
+					}
+					ptr[i+1] = (int) idx.size();
+					charge[i] = node.charge();
+					flags[i] = (node.boundary() ? 1 : 0) | (node.dirichletBoundary() ? 2 : 0) | (node.neumannBoundary() ? 4 : 0);
+					ids[i] = node.id().toValue();
+				}
+				writeArray("nbr_ptr.i32", ptr);
+				writeArray("nbr_idx.i32", idx);
+				writeArray("coupling.f32", coup);
+				writeArray("charge.f32", charge);
+				writeArray("flags.u8", flags);
+				writeArray("ids.i16", ids);
+			}
+			else if (cmd == "dumpphi")
+			{
+				dumpPhi(system, tok.at(1));
+			}
+			else if (cmd == "relax")
+			{
+				const int k = std::atoi(tok.at(1).c_str());
+				const unsigned int c = std::atoi(tok.at(2).c_str());
+				for (int i = 0; i < k; i++)
+					relaxDeviations.push_back(system.gridTable.relax(c));
+				writeArray("relax_dev.f64", relaxDeviations);
+			}
+			else if (cmd == "relaxthr")
+			{
+				const double thr = std::atof(tok.at(1).c_str());
+				const unsigned int c = std::atoi(tok.at(2).c_str());
+				const unsigned int q = std::atoi(tok.at(3).c_str());
+				long long it;
+				double faultDev = -1.0, faultSlope = 0.0;
+				try { it = system.gridTable.relax(thr, c, q); }
+				catch (Grid_3d::Table::RelaxationFault& e) { it = -1 - e.iteration(); faultDev = e.deviation(); faultSlope = e.slope(); }
+				relaxIterations.push_back(it);
+				std::vector<double> fault(2);
+				fault[0] = faultDev; fault[1] = faultSlope;
+				writeArray("relaxthr_iter.i64", relaxIterations);
+				writeArray("relaxthr_fault_" + tok.at(1) + ".f64", fault);
+			}
+			else if (cmd == "local")
+			{
+				const int node = std::atoi(tok.at(1).c_str());
+				const unsigned int order = std::atoi(tok.at(2).c_str());
+				const double thr = std::atof(tok.at(3).c_str());
+				const unsigned int c = std::atoi(tok.at(4).c_str());
+				const unsigned int q = std::atoi(tok.at(5).c_str());
+				long long it;
+				try { it = system.gridTable.localRelax(node, order, thr, c, q); }
+				catch (Grid_3d::Table::RelaxationFault& e) { it = -1 - e.iteration(); }
+				relaxIterations.push_back(it);
+				writeArray("local_iter.i64", relaxIterations);
+			}
+			else if (cmd == "evapnode")
+			{
+				const int node = std::atoi(tok.at(1).c_str());
+				system.gridTable.node(node).setId(system.configTable["Vacuum"].id());
+				system.gridTable.fastResync(node, system.geomTable, system.configTable);
+			}
+			else if (cmd == "field")
+			{
+				const int n = system.gridTable.numNodes();
+				std::vector<float> e1(3 * (size_t)n), e2(3 * (size_t)n), q2(n);
+				for (int i = 0; i < n; i++)
+				{
+					const MathVector3d<float> f1 = system.gridTable.field_o1(i, system.geomTable);
+					float q = 0.0f;
+					const MathVector3d<float> f2 = system.gridTable.field_o2(i, system.geomTable, &q);
+					for (int k = 0; k < 3; k++) { e1[3*i+k] = f1[k]; e2[3*i+k] = f2[k]; }
+					q2[i] = q;
+				}
+				writeArray("field_o1_" + tok.at(1) + ".f32", e1);
+				writeArray("field_o2_" + tok.at(1) + ".f32", e2);
+				writeArray("charge_o2_" + tok.at(1) + ".f32", q2);
+			}
+			else if (cmd == "vorarea")
+			{
+				const int n = system.gridTable.numNodes();
+				std::vector<float> a;
+				for (int i = 0; i < n; i++)
+				{
+					const Grid_3d::Node& node = system.gridTable.node(i);
+					for (int j = 0; j < node.numNeighbours(); j++)
+						a.push_back(system.geomTable.voronoiArea(i, node.neighbour(j), 1.0f));
+				}
+				writeArray("vorarea_" + tok.at(1) + ".f32", a);
+			}
+			else if (cmd == "evap")
+			{
+				// evap:EVENTS:PROB:CHOICE:LTHR:SHELL:LCYC:LQUE:GLOBAL:RINT:RTHR:RCYC:RQUE:INTEG:STEPPER
+				// The steps I-VI of Process::evaporation (process.cpp:680-1037)
+				// composed of the reference's own functions.  The fork's
+				// update_coordination (process.cpp:647,770) is left out: it adds
+				// neighbour counts to species ids every event and the
+				// configuration lookup then throws for the undefined ids
+				// (configuration.cpp:214-232), so upstream TAPSim semantics are
+				// what can be run and compared.
+				const int events = std::atoi(tok.at(1).c_str());
+				const int probMode = std::atoi(tok.at(2).c_str());
+				const int evapMode = std::atoi(tok.at(3).c_str());
+				const float lthr = (float) std::atof(tok.at(4).c_str());
+				const unsigned int shell = std::atoi(tok.at(5).c_str());
+				const unsigned int lcyc = std::atoi(tok.at(6).c_str());
+				const unsigned int lque = std::atoi(tok.at(7).c_str());
+				const unsigned int gcyc = std::atoi(tok.at(8).c_str());
+				const int rint = std::atoi(tok.at(9).c_str());
+				const float rthr = (float) std::atof(tok.at(10).c_str());
+				const unsigned int rcyc = std::atoi(tok.at(11).c_str());
+				const unsigned int rque = std::atoi(tok.at(12).c_str());
+				const int integ = std::atoi(tok.at(13).c_str());
+				const int stepper = std::atoi(tok.at(14).c_str());
+
+				Surface_3d::Table surf;
+				surf.setVacuumId(system.configTable["Vacuum"].id());
+				surf.init(system);
+				Surface_3d::evap_compute_specificFields(&surf, system);
+				Surface_3d::evap_compute_probabilities(probMode, &surf, system);
+
+				std::vector<float> rec;
+				std::vector<int> recNode;
+				std::vector<long long> recIter;
+				std::queue<float> voltageQueue;
+				float voltageQueueSum = 0.0f;
+				for (int ev = 1; ev <= events && surf.nodes().size() > 0; ev++)
+				{
+					const float scalingReference = surf.scalingReference();
+					const Surface_3d::Nodeset::const_iterator it = Surface_3d::evap_findCandidate(evapMode, surf);
+					const int node = it->index();
+					const float probability = it->probability();
+					const float potBefore = system.gridTable.potential(node);
+					const MathVector3d<float> fieldBefore = system.gridTable.field_o1(node, system.geomTable);
+					const Geometry_3d::Point apex = system.geomTable.nodeCoords(surf.apex(system)->index());
+					const Configuration::NodeId nodeId = system.gridTable.id(node);
+
+					evap_takeAway(&system, &surf, node);
+
+					long long localIter = 0, refreshIter = 0;
+					try { localIter = system.gridTable.localRelax(node, shell, lthr, lcyc, lque); }
+					catch (Grid_3d::Table::RelaxationFault& e) { localIter = -1 - e.iteration(); }
+					try { system.gridTable.relax(gcyc); }
+					catch (Grid_3d::Table::RelaxationFault&) {}
+					if (rint > 0 && ev % rint == 0)
+					{
+						try { refreshIter = system.gridTable.relax(rthr, rcyc, rque); }
+						catch (Grid_3d::Table::RelaxationFault& e) { refreshIter = -1 - e.iteration(); }
+					}
+
+					Surface_3d::evap_compute_specificFields(&surf, system);
+					Surface_3d::evap_compute_probabilities(probMode, &surf, system);
+
+					float charge = system.configTable[nodeId].evapCharge();
+					charge *= Trajectory_3d::eCharge;
+					float mass = system.configTable[nodeId].mass();
+					mass *= Trajectory_3d::amu2kg;
+					Trajectory_3d traj;
+					traj.setGeometry(&system.geomTable);
+					traj.setGrid(&system.gridTable);
+					traj.setIntegratorType(integ);
+					traj.setStepperType(stepper);
+					traj.setTimeStepLimit(1e-16f);
+					traj.setStepperLimit(std::numeric_limits<unsigned long>::max());
+					traj.setIterationLimit(std::numeric_limits<unsigned long>::max());
+					traj.init(system.geomTable.nodeCoords(node), MathVector3d<float>(0.0f), charge, mass,
+					          system.geomTable.node(node).associatedTetrahedron);
+					traj.integrate(1e-14f, 1e-6f);
+					traj.extrapolate(Geometry_3d::Point::Z, system.geomTable.max().z());
+
+					const float potAfter = system.gridTable.potential(node);
+					const MathVector3d<float> fieldAfter = system.gridTable.field_o1(node, system.geomTable);
+					float voltage, timeScale;
+					std::vector<Trajectory_3d::phaseVector>::const_reverse_iterator back = traj.data().rbegin();
+					while (traj.data().rend() != back && back->tetIndex() < 0) back++;
+					if (traj.data().rend() != back && fieldBefore.length() != 0.0f)
+					{
+						float v = potBefore;
+						v -= Grid_3d::potential(system.geomTable, system.gridTable, back->position(), back->tetIndex());
+						v *= system.configTable[nodeId].evapField();
+						v /= fieldBefore.length();
+						v /= scalingReference;
+						voltageQueue.push(v);
+						voltageQueueSum += v;
+						while (voltageQueue.size() > 250) { voltageQueueSum -= voltageQueue.front(); voltageQueue.pop(); }
+						voltage = voltageQueueSum;
+						voltage /= voltageQueue.size();
+						timeScale = potAfter;
+						timeScale -= Grid_3d::potential(system.geomTable, system.gridTable, back->position(), back->tetIndex());
+						timeScale /= voltage;
+						timeScale = std::sqrt(timeScale);
+					}
+					else if (fieldBefore.length() == 0.0f) { voltage = 1.0f; timeScale = 1.0f; }
+					else { voltage = 0.0f; timeScale = 1.0f; }
+
+					recNode.push_back(node);
+					recNode.push_back(nodeId.toValue());
+					recNode.push_back((int) traj.data().size());
+					recNode.push_back((int) surf.nodes().size());
+					recIter.push_back(localIter);
+					recIter.push_back(refreshIter);
+					rec.push_back(probability);
+					rec.push_back(potBefore);
+					for (int c = 0; c < 3; c++) rec.push_back(fieldBefore[c]);
+					rec.push_back(potAfter);
+					for (int c = 0; c < 3; c++) rec.push_back(fieldAfter[c]);
+					rec.push_back(voltage);
+					rec.push_back(timeScale);
+					for (int c = 0; c < 3; c++) rec.push_back(traj.data().front().position()[c]);
+					for (int c = 0; c < 3; c++) rec.push_back(traj.data().back().position()[c]);
+					rec.push_back(traj.data().back().time() * timeScale);
+					for (int c = 0; c < 3; c++) rec.push_back(apex[c]);
+					rec.push_back(surf.scalingReference());
+				}
+				writeArray("evap_int.i32", recNode);
+				writeArray("evap_iter.i64", recIter);
+				writeArray("evap_f.f32", rec);
+			}
+			else if (cmd == "traj")
+			{
+				// traj:TAG:NODE:INTEGRATOR:STEPPER  (charge 2e, mass 55.845 u)
+				const int node = std::atoi(tok.at(2).c_str());
+				Trajectory_3d t;
+				t.setGeometry(&system.geomTable);
+				t.setGrid(&system.gridTable);
+				t.setIntegratorType(std::atoi(tok.at(3).c_str()));
+				t.setStepperType(std::atoi(tok.at(4).c_str()));
+				t.setTimeStepLimit(1e-16f);
+				t.setStepperLimit(100000);
+				t.setIterationLimit(1000000);
+				const float charge = 2.0f * Trajectory_3d::eCharge;
+				const float mass = 55.845f * Trajectory_3d::amu2kg;
+				t.init(system.geomTable.nodeCoords(node), MathVector3d<float>(0.0f), charge, mass,
+				       system.geomTable.node(node).associatedTetrahedron);
+				const int status = t.integrate(1e-14f, 1e-6f);
+				t.extrapolate(Geometry_3d::Point::Z, system.geomTable.max().z());
+				std::vector<float> data;
+				for (size_t k = 0; k < t.data().size(); k++)
+				{
+					const Trajectory_3d::phaseVector& p = t.data()[k];
+					data.push_back(p.time());
+					for (int c = 0; c < 3; c++) data.push_back(p.position(c));
+					for (int c = 0; c < 3; c++) data.push_back(p.velocity(c));
+					data.push_back((float) p.tetIndex());
+				}
+				std::vector<int> st(1, status);
+				writeArray("traj_" + tok.at(1) + ".f32", data);
+				writeArray("traj_status_" + tok.at(1) + ".i32", st);
+			}
+			else
+				throw std::runtime_error("unknown step " + cmd);
+		}
+	}
+	catch (std::exception& e)
+	{
+		std::fprintf(stderr, "ref_driver: %s\n", e.what());
+		return 1;
+	}
+	return 0;
+}

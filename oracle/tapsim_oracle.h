@@ -1,0 +1,146 @@
+/*
+ * CPU oracle: a restatement of TAPSim's per-event electrostatics
+ * (ashtonmv/tapsim_modified, TAPSim 1.0b r3225) used ONLY as the checker in
+ * tests/, in __graft_entry__.smoke() and as bench.py's cpu_baseline leg.
+ * The product library never links or calls it.
+ *
+ * Two precisions are restated:
+ *   ORA_F32  "faithful": float storage of potentials, couplings and
+ *            geometry exactly as the reference (tapsim/grid_3d.h:89-97),
+ *            double accumulation in computePotential (grid_3d.cpp:1317).
+ *   ORA_F64  the same algorithm with every float promoted to double; this
+ *            is the plain-FP64 oracle the GPU path must match bit for bit.
+ *
+ * Parity pinning: every function is checked in tests/ against fixtures
+ * produced by the reference itself (oracle/_ref, built from the reference
+ * sources by oracle/Makefile.ref) and against closed forms (linear fixed
+ * points, BCC couplings, exact linear fields, uniform-field flights).
+ */
+#ifndef TAPSIM_ORACLE_H
+#define TAPSIM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_F32 = 0, ORA_F64 = 1 };
+
+enum { ORA_OK = 0, ORA_E_ARG = -1, ORA_E_NAN = -2, ORA_E_GEOM = -3, ORA_E_SURF = -4 };
+
+typedef struct
+{
+	int defined;
+	float charge_density;  /* C/m^3 */
+	float epsilon;
+	int removable;
+	int neumann;
+	int dirichlet;
+	float phi;
+	float mass;            /* amu */
+	int evap_charge;
+	float evap_field;      /* V/m */
+	float evap_energy;     /* eV */
+} ora_species;
+
+typedef struct
+{
+	int prob_mode;         /* 0 linear field, 1 boltzmann, 2 linear force, 3 voronoi flux */
+	int evap_mode;         /* 0 maximum, 1 monte carlo */
+	int vacuum_id;
+	float relax_threshold; int relax_shell; int relax_cycle; int relax_queue;
+	int relax_global_cycles;
+	int refresh_interval; float refresh_threshold; int refresh_cycle; int refresh_queue;
+	int trajectory;        /* 0: skip step V, 1: integrate */
+	int integrator; int stepper;
+	float dt0; float dt_min; float err_limit;
+	unsigned int voltage_queue;
+	float shrink_limit;
+	int inplace_order;     /* 0 ascending sequential (one thread), 1 colour classes */
+} ora_evap_opts;
+
+typedef struct
+{
+	int node;
+	int id;
+	float probability;
+	float pot_before; float field_before[3];
+	float pot_after;  float field_after[3];
+	float voltage; float time_scale;
+	float start[3]; float stop[3]; float tof;
+	int traj_len; int traj_status;
+	long long local_iter; long long refresh_iter;
+	int surface_size;
+	float scaling_ref;
+	float apex[3];
+} ora_event;
+
+typedef struct ora_sys ora_sys;
+
+ora_sys* ora_create(int mode, int n, const float* xyz, const short* ids,
+                    int m, const int* tet_v, const int* tet_n,
+                    const ora_species* species /* 128 */, float temperature);
+void ora_destroy(ora_sys*);
+const char* ora_last_error(void);
+
+int ora_sync(ora_sys*);                                   /* fastSync */
+int ora_set_couplings(ora_sys*, const double* coupling);
+int ora_resync(ora_sys*, int node);                       /* fastResync */
+int ora_set_id(ora_sys*, int node, int id);
+
+int ora_nnz(const ora_sys*);
+void ora_get_csr(const ora_sys*, int* ptr, int* idx);
+void ora_get_coupling(const ora_sys*, double* out);
+void ora_get_charge(const ora_sys*, double* out);
+void ora_get_flags(const ora_sys*, unsigned char* out);
+void ora_get_ids(const ora_sys*, short* out);
+void ora_get_assoc(const ora_sys*, int* out);
+void ora_get_boundary(const ora_sys*, unsigned char* out);
+
+void ora_get_phi(const ora_sys*, int buffer, double* out);
+void ora_set_phi(ora_sys*, const double* phi0, const double* phi1, int phi_switch, int local_switch);
+int ora_phi_switch(const ora_sys*);
+int ora_local_switch(const ora_sys*);
+double ora_potential(const ora_sys*, int node);
+void ora_set_inplace_order(ora_sys*, int kind, const int* colour, int ncolours);
+
+double ora_relax(ora_sys*, int cycles, int* err);
+long long ora_relax_threshold(ora_sys*, double threshold, int cycles, int queue,
+                              double* fault_dev, double* fault_slope, int* err);
+int ora_find_local_nodes(const ora_sys*, int node, int order, int* out, int cap);
+double ora_local_relax(ora_sys*, int node, int order, int cycles, int* err);
+long long ora_local_relax_threshold(ora_sys*, int node, int order, double threshold,
+                                    int cycles, int queue, int* err);
+
+int ora_field_o1(const ora_sys*, int node, double out[3]);
+int ora_field_o2(const ora_sys*, int node, double out[3], double* charge);
+double ora_voronoi_area(const ora_sys*, int n1, int n2, double inf);
+double ora_voronoi_volume(const ora_sys*, int node, double inf);
+double ora_distance(const ora_sys*, int n1, int n2);
+
+/* surface / selection */
+int ora_surface_init(ora_sys*, int vacuum_id);
+int ora_surface_size(const ora_sys*);
+void ora_surface_nodes(const ora_sys*, int* out);
+void ora_surface_probabilities(const ora_sys*, float* out);
+int ora_compute_probabilities(ora_sys*, int mode);
+float ora_specific_fields(ora_sys*);
+int ora_find_candidate(const ora_sys*, int mode);
+int ora_take_away(ora_sys*, int node);
+
+/* trajectory (float, as trajectory_3d.cpp) */
+void ora_srand(unsigned int seed);
+int ora_find_tetrahedron(const ora_sys*, const float pos[3], int guess);
+void ora_tet_weights(const ora_sys*, int tet, const float pos[3], float w[4]);
+float ora_point_potential(const ora_sys*, const float pos[3], int guess);
+int ora_point_field(const ora_sys*, const float pos[3], int guess, float out[3]);
+int ora_trajectory(const ora_sys*, int node, float charge, float mass, int integrator,
+                   int stepper, float dt0, float dt_min, float err_limit,
+                   float* out, int cap, int* len);
+
+int ora_evaporate(ora_sys*, const ora_evap_opts*, int nevents, ora_event* out, int* done);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,374 @@
+"""MI355X (gfx950) per-event electrostatics of TAPSim.
+
+Python mirror of the reference's C++ call surface for this path
+(Grid_3d::Table, Surface_3d::Table, Process::evaporation in
+ashtonmv/tapsim_modified tapsim/grid_3d.h, surface_3d.h, process.h) on top of
+the C ABI in include/tapsim_gpu.h.  The work runs in the in-tree HIP library
+libtapsim_gpu.so; there is no CPU fallback: a missing library or GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "load", "TsgError", "RelaxationFault", "SpecimenParams", "Species", "Mesh", "Grid3D",
+    "generate_specimen", "EvapOptions", "PLAIN_F64", "FAITHFUL_F32",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtapsim_gpu.so")
+
+PLAIN_F64 = 0
+FAITHFUL_F32 = 1
+
+FLAG_BOUNDARY, FLAG_DIRICHLET, FLAG_NEUMANN = 1, 2, 4
+
+_lib = None
+
+
+class TsgError(RuntimeError):
+    """A failed call of the C ABI (status code and message)."""
+
+    def __init__(self, code, msg):
+        super().__init__("tsg error %d: %s" % (code, msg))
+        self.code = code
+
+
+class RelaxationFault(TsgError):
+    """Grid_3d::Table::RelaxationFault (tapsim/grid_3d.h:103-129)."""
+
+    def __init__(self, iteration, deviation, slope):
+        TsgError.__init__(self, -5, "relaxation stagnated at iteration %d (deviation %g, slope %g)"
+                          % (iteration, deviation, slope))
+        self.iteration, self.deviation, self.slope = iteration, deviation, slope
+
+
+class SpecimenParams(ctypes.Structure):
+    _fields_ = [
+        ("a", ctypes.c_double), ("radius", ctypes.c_double), ("shank", ctypes.c_double),
+        ("box", ctypes.c_double * 3), ("shell", ctypes.c_double), ("vacuum_target", ctypes.c_longlong),
+        ("decay", ctypes.c_double), ("face_spacing", ctypes.c_double), ("cluster_density", ctypes.c_double),
+        ("cluster_radius", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+        ("id_vacuum", ctypes.c_int), ("id_top", ctypes.c_int), ("id_bottom", ctypes.c_int),
+        ("id_sides", ctypes.c_int), ("id_matrix", ctypes.c_int), ("id_solute", ctypes.c_int),
+    ]
+
+
+class Species(ctypes.Structure):
+    """Configuration::NodeData (tapsim/configuration.h:100-150)."""
+    _fields_ = [
+        ("defined", ctypes.c_int), ("charge_density", ctypes.c_float), ("epsilon", ctypes.c_float),
+        ("removable", ctypes.c_int), ("neumann", ctypes.c_int), ("dirichlet", ctypes.c_int),
+        ("phi", ctypes.c_float), ("mass", ctypes.c_float), ("evap_charge", ctypes.c_int),
+        ("evap_field", ctypes.c_float), ("evap_energy", ctypes.c_float),
+    ]
+
+
+class EvapOptions(ctypes.Structure):
+    """The hot-path knobs of Process::EvaporationOptions (tapsim/process.h:39-122)."""
+    _fields_ = [
+        ("prob_mode", ctypes.c_int), ("evap_mode", ctypes.c_int), ("vacuum_id", ctypes.c_int),
+        ("relax_threshold", ctypes.c_double), ("relax_shell", ctypes.c_int), ("relax_cycle", ctypes.c_int),
+        ("relax_queue", ctypes.c_int), ("relax_global_cycles", ctypes.c_int), ("refresh_interval", ctypes.c_int),
+        ("refresh_threshold", ctypes.c_double), ("refresh_cycle", ctypes.c_int), ("refresh_queue", ctypes.c_int),
+        ("exact_probability", ctypes.c_int),
+    ]
+
+
+class Event(ctypes.Structure):
+    _fields_ = [
+        ("node", ctypes.c_int), ("id", ctypes.c_int), ("probability", ctypes.c_float),
+        ("pot_before", ctypes.c_double), ("field_before", ctypes.c_double * 3),
+        ("pot_after", ctypes.c_double), ("field_after", ctypes.c_double * 3),
+        ("local_iter", ctypes.c_longlong), ("global_iter", ctypes.c_longlong),
+        ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_double), ("near_tie", ctypes.c_int),
+    ]
+
+
+# exported symbols and their argument types
+_P = ctypes.c_void_p
+_SIGS = {
+    "tsg_last_error": ([], ctypes.c_char_p),
+    "tsg_version": ([], ctypes.c_int),
+    "tsg_free": ([_P], None),
+    "tsg_specimen_generate": ([_P, _P, _P, _P, _P], ctypes.c_int),
+    "tsg_mesh_create": ([ctypes.c_int, _P, _P], ctypes.c_int),
+    "tsg_mesh_create_from_tets": ([ctypes.c_int, _P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "tsg_mesh_info": ([_P, _P, _P, _P], ctypes.c_int),
+    "tsg_mesh_get_tets": ([_P, _P, _P], ctypes.c_int),
+    "tsg_mesh_get_csr": ([_P, _P, _P], ctypes.c_int),
+    "tsg_mesh_get_nodeinfo": ([_P, _P, _P], ctypes.c_int),
+    "tsg_mesh_destroy": ([_P], None),
+    "tsg_grid_create": ([_P, _P, _P, ctypes.c_float, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_grid_destroy": ([_P], None),
+    "tsg_grid_info": ([_P, _P, _P, _P], ctypes.c_int),
+    "tsg_relax_cycles": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_relax_threshold": ([_P, ctypes.c_double, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P], ctypes.c_int),
+    "tsg_local_relax": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_get_phi": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_set_phi": ([_P, _P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "tsg_phi_switches": ([_P, _P, _P], ctypes.c_int),
+    "tsg_potential": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "tsg_get_couplings": ([_P, _P], ctypes.c_int),
+    "tsg_get_flags": ([_P, _P], ctypes.c_int),
+    "tsg_get_ids": ([_P, _P], ctypes.c_int),
+    "tsg_get_areas": ([_P, _P, _P, _P], ctypes.c_int),
+    "tsg_set_id": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "tsg_surface_init": ([_P, ctypes.c_int], ctypes.c_int),
+    "tsg_surface_size": ([_P, _P], ctypes.c_int),
+    "tsg_surface_nodes": ([_P, _P], ctypes.c_int),
+    "tsg_surface_evaluate": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_surface_values": ([_P, _P], ctypes.c_int),
+    "tsg_surface_probabilities": ([_P, _P], ctypes.c_int),
+    "tsg_surface_candidate": ([_P, _P, _P], ctypes.c_int),
+    "tsg_take_away": ([_P, ctypes.c_int], ctypes.c_int),
+    "tsg_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "tsg_sweep_stats": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+}
+
+
+def load():
+    """Loads libtapsim_gpu.so (built by __graft_entry__.build()); raises when absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TsgError(-2, "libtapsim_gpu.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _chk(rc):
+    if rc != 0:
+        raise TsgError(rc, load().tsg_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def generate_specimen(params):
+    """Seeded BCC [110] tip node set; returns (xyz float32 (n,3), ids int16 (n,), counts)."""
+    lib = load()
+    n = ctypes.c_int()
+    xyz = ctypes.POINTER(ctypes.c_float)()
+    ids = ctypes.POINTER(ctypes.c_short)()
+    counts = (ctypes.c_longlong * 5)()
+    _chk(lib.tsg_specimen_generate(ctypes.byref(params), ctypes.byref(n), ctypes.byref(xyz), ctypes.byref(ids), counts))
+    try:
+        X = np.ctypeslib.as_array(xyz, (n.value * 3,)).copy().reshape(-1, 3)
+        I = np.ctypeslib.as_array(ids, (n.value,)).copy()
+    finally:
+        lib.tsg_free(ctypes.cast(xyz, ctypes.c_void_p))
+        lib.tsg_free(ctypes.cast(ids, ctypes.c_void_p))
+    keys = ("atoms", "solute", "vacuum_lattice", "vacuum_random", "box")
+    return X, I, dict(zip(keys, list(counts)))
+
+
+class Mesh:
+    """Delaunay tetrahedralisation of a node set plus its topology
+    (Geometry_3d::Table after tetrahedralize, tapsim/geometry_3d.cpp:759-810)."""
+
+    def __init__(self, xyz, tets=None, neighbours=None):
+        lib = load()
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        if tets is None:
+            _chk(lib.tsg_mesh_create(len(self.xyz), _ptr(self.xyz), ctypes.byref(h)))
+        else:
+            tv = np.ascontiguousarray(tets, dtype=np.int32)
+            tn = np.ascontiguousarray(neighbours, dtype=np.int32)
+            _chk(lib.tsg_mesh_create_from_tets(len(self.xyz), _ptr(self.xyz), len(tv), _ptr(tv), _ptr(tn), ctypes.byref(h)))
+        self._h = h
+        n, m, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        _chk(lib.tsg_mesh_info(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nnz)))
+        self.n, self.m, self.nnz = n.value, m.value, nnz.value
+
+    def tets(self):
+        tv = np.zeros((self.m, 4), np.int32)
+        tn = np.zeros((self.m, 4), np.int32)
+        _chk(load().tsg_mesh_get_tets(self._h, _ptr(tv), _ptr(tn)))
+        return tv, tn
+
+    def csr(self):
+        ptr = np.zeros(self.n + 1, np.int32)
+        idx = np.zeros(self.nnz, np.int32)
+        _chk(load().tsg_mesh_get_csr(self._h, _ptr(ptr), _ptr(idx)))
+        return ptr, idx
+
+    def nodeinfo(self):
+        assoc = np.zeros(self.n, np.int32)
+        bnd = np.zeros(self.n, np.uint8)
+        _chk(load().tsg_mesh_get_nodeinfo(self._h, _ptr(assoc), _ptr(bnd)))
+        return assoc, bnd
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tsg_mesh_destroy(self._h)
+            self._h = None
+
+
+def species_table(entries):
+    """entries: {id: dict(field=value)} -> ctypes array of 128 Species."""
+    table = (Species * 128)()
+    for sid, kw in entries.items():
+        s = table[sid]
+        s.defined = 1
+        s.epsilon = 1.0
+        for k, v in kw.items():
+            setattr(s, k, v)
+    return table
+
+
+class Grid3D:
+    """Device grid with the reference's Grid_3d::Table semantics
+    (tapsim/grid_3d.h:100-295): two potential buffers, the _phiSwitch /
+    _phiLocalSwitch rules and one-cycle-stale potential()."""
+
+    def __init__(self, mesh, ids, species, temperature=0.0, mode=PLAIN_F64):
+        lib = load()
+        self.mesh = mesh
+        self.ids0 = np.ascontiguousarray(ids, dtype=np.int16)
+        self.species = species
+        self.mode = mode
+        h = ctypes.c_void_p()
+        _chk(lib.tsg_grid_create(mesh._h, _ptr(self.ids0), ctypes.byref(species), float(temperature), mode, ctypes.byref(h)))
+        self._h = h
+        n, nnz, nrows = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+        _chk(lib.tsg_grid_info(h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nrows)))
+        self.n, self.nnz = n.value, nnz.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tsg_grid_destroy(self._h)
+            self._h = None
+
+    # -- Grid_3d::Table::relax / localRelax
+    def relax(self, cycles=1):
+        dev = ctypes.c_double()
+        _chk(load().tsg_relax_cycles(self._h, cycles, ctypes.byref(dev)))
+        return dev.value
+
+    def relax_threshold(self, threshold, cycles=1, queue=0, raise_fault=True):
+        it, fd, fs, ld = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _chk(load().tsg_relax_threshold(self._h, threshold, cycles, queue, ctypes.byref(it), ctypes.byref(fd),
+                                        ctypes.byref(fs), ctypes.byref(ld)))
+        if it.value < 0 and raise_fault:
+            raise RelaxationFault(-1 - it.value, fd.value, fs.value)
+        return it.value
+
+    def local_relax(self, node, order, threshold, cycles, queue=0):
+        it = ctypes.c_longlong()
+        _chk(load().tsg_local_relax(self._h, node, order, threshold, cycles, queue, ctypes.byref(it)))
+        return it.value
+
+    # -- state
+    def phi(self, buffer):
+        out = np.zeros(self.n, np.float64)
+        _chk(load().tsg_get_phi(self._h, buffer, _ptr(out)))
+        return out
+
+    def set_phi(self, phi0, phi1, phi_switch=0, local_switch=0):
+        p0 = np.ascontiguousarray(phi0, dtype=np.float64)
+        p1 = np.ascontiguousarray(phi1, dtype=np.float64)
+        _chk(load().tsg_set_phi(self._h, _ptr(p0), _ptr(p1), phi_switch, local_switch))
+
+    def switches(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _chk(load().tsg_phi_switches(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def potential(self, node):
+        out = ctypes.c_double()
+        _chk(load().tsg_potential(self._h, node, ctypes.byref(out)))
+        return out.value
+
+    def field_o1(self, nodes):
+        nodes = np.ascontiguousarray(np.atleast_1d(nodes), dtype=np.int32)
+        out = np.zeros((len(nodes), 3), np.float64)
+        _chk(load().tsg_field_o1(self._h, len(nodes), _ptr(nodes), _ptr(out)))
+        return out
+
+    def couplings(self):
+        out = np.zeros(self.nnz, np.float64)
+        _chk(load().tsg_get_couplings(self._h, _ptr(out)))
+        return out
+
+    def flags(self):
+        out = np.zeros(self.n, np.uint8)
+        _chk(load().tsg_get_flags(self._h, _ptr(out)))
+        return out
+
+    def ids(self):
+        out = np.zeros(self.n, np.int16)
+        _chk(load().tsg_get_ids(self._h, _ptr(out)))
+        return out
+
+    def areas(self):
+        h32 = np.zeros(self.nnz, np.float32)
+        h64 = np.zeros(self.nnz, np.float64)
+        o32 = np.zeros(self.nnz, np.float32)
+        _chk(load().tsg_get_areas(self._h, _ptr(h32), _ptr(h64), _ptr(o32)))
+        return h32, h64, o32
+
+    def set_id(self, node, sid):
+        _chk(load().tsg_set_id(self._h, node, sid))
+
+    # -- Surface_3d
+    def surface_init(self, vacuum_id):
+        _chk(load().tsg_surface_init(self._h, vacuum_id))
+
+    def surface_nodes(self):
+        c = ctypes.c_int()
+        _chk(load().tsg_surface_size(self._h, ctypes.byref(c)))
+        out = np.zeros(c.value, np.int32)
+        _chk(load().tsg_surface_nodes(self._h, _ptr(out)))
+        return out
+
+    def surface_evaluate(self, exact_sum=True):
+        ref = ctypes.c_double()
+        _chk(load().tsg_surface_evaluate(self._h, 1 if exact_sum else 0, ctypes.byref(ref)))
+        return ref.value
+
+    def surface_values(self):
+        out = np.zeros(len(self.surface_nodes()), np.float64)
+        _chk(load().tsg_surface_values(self._h, _ptr(out)))
+        return out
+
+    def surface_probabilities(self):
+        out = np.zeros(len(self.surface_nodes()), np.float32)
+        _chk(load().tsg_surface_probabilities(self._h, _ptr(out)))
+        return out
+
+    def candidate(self):
+        node, tie = ctypes.c_int(), ctypes.c_int()
+        _chk(load().tsg_surface_candidate(self._h, ctypes.byref(node), ctypes.byref(tie)))
+        return node.value, tie.value
+
+    def take_away(self, node):
+        _chk(load().tsg_take_away(self._h, node))
+
+    def sweep_stats(self):
+        b, ms = ctypes.c_double(), ctypes.c_double()
+        r, e, ref, sw, la = (ctypes.c_longlong() for _ in range(5))
+        _chk(load().tsg_sweep_stats(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(e), ctypes.byref(ref),
+                                    ctypes.byref(ms), ctypes.byref(sw), ctypes.byref(la)))
+        return dict(bytes=b.value, rows=r.value, entries=e.value, referenced=ref.value, ms=ms.value,
+                    sweeps=sw.value, launched=la.value)
+
+    def evaporate(self, options, k):
+        out = (Event * max(1, k))()
+        done = ctypes.c_int()
+        _chk(load().tsg_evaporate(self._h, ctypes.byref(options), k, out, ctypes.byref(done)))
+        return [out[i] for i in range(done.value)]

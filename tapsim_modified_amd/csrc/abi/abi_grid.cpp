@@ -1,0 +1,213 @@
+// C ABI over the device grid (include/tapsim_gpu.h).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../../include/tapsim_gpu.h"
+#include "../host/grid.hpp"
+#include "errors.hpp"
+
+struct tsg_grid {
+	std::unique_ptr<tsg::Grid> g;
+};
+
+namespace {
+static_assert(sizeof(tsg_species) == sizeof(tsg::Species), "species layout");
+static_assert(sizeof(tsg_event) == sizeof(tsg::EventRecord), "event layout");
+
+int check(tsg_grid* h, int rc)
+{
+	if (rc) tsg::set_error(h->g->error());
+	return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int tsg_grid_create(const tsg_mesh* mesh, const short* ids, const tsg_species* species, float temperature, int mode,
+                    tsg_grid** out)
+{
+	if (!mesh || !ids || !species || !out) return tsg::fail(TSG_E_ARG, "tsg_grid_create: null argument");
+	if (mode != TSG_PLAIN_F64 && mode != TSG_FAITHFUL_F32) return tsg::fail(TSG_E_ARG, "tsg_grid_create: unknown mode");
+	int dev = 0;
+	if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0) return tsg::fail(TSG_E_HIP, "tsg_grid_create: no HIP device");
+	try {
+		tsg::Species sp[128];
+		std::memcpy(sp, species, sizeof(sp));
+		auto* h = new tsg_grid();
+		h->g.reset(new tsg::Grid(mode, mesh->m, ids, sp, temperature));
+		if (h->g->status()) {
+			const int rc = h->g->status();
+			tsg::set_error(h->g->error());
+			delete h;
+			return rc;
+		}
+		*out = h;
+		return TSG_OK;
+	} catch (const std::exception& e) {
+		return tsg::fail(TSG_E_HIP, std::string("tsg_grid_create: ") + e.what());
+	}
+}
+
+void tsg_grid_destroy(tsg_grid* h) { delete h; }
+
+int tsg_grid_info(const tsg_grid* h, int* n, long long* nnz, int* nrows)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	if (n) *n = h->g->n();
+	if (nnz) *nnz = h->g->nnz();
+	if (nrows) *nrows = h->g->nrows();
+	return TSG_OK;
+}
+
+int tsg_relax_cycles(tsg_grid* h, int cycles, double* deviation)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->relax_cycles(cycles, deviation));
+}
+
+int tsg_relax_threshold(tsg_grid* h, double threshold, int cycles, int queue, long long* iterations, double* fault_dev,
+                        double* fault_slope, double* last_dev)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->relax_threshold(threshold, cycles, queue, iterations, fault_dev, fault_slope, last_dev));
+}
+
+int tsg_local_relax(tsg_grid* h, int node, int order, double threshold, int cycles, int queue, long long* iterations)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->local_relax(node, order, threshold, cycles, queue, iterations));
+}
+
+int tsg_get_phi(tsg_grid* h, int buffer, double* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->get_phi(buffer, out));
+}
+
+int tsg_set_phi(tsg_grid* h, const double* p0, const double* p1, int sw, int lsw)
+{
+	if (!h || !p0 || !p1) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->set_phi(p0, p1, sw, lsw));
+}
+
+int tsg_phi_switches(const tsg_grid* h, int* sw, int* lsw)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	if (sw) *sw = h->g->phi_switch();
+	if (lsw) *lsw = h->g->local_switch();
+	return TSG_OK;
+}
+
+int tsg_potential(tsg_grid* h, int node, double* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->potential(node, out));
+}
+
+int tsg_field_o1(tsg_grid* h, int count, const int* nodes, double* out)
+{
+	if (!h || (count > 0 && (!nodes || !out))) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->field_o1(count, nodes, out));
+}
+
+int tsg_get_couplings(const tsg_grid* h, double* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->get_couplings(out);
+}
+
+int tsg_get_flags(const tsg_grid* h, unsigned char* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->get_flags(out);
+}
+
+int tsg_get_ids(const tsg_grid* h, short* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->get_ids((int16_t*) out);
+}
+
+int tsg_get_areas(const tsg_grid* h, float* half32, double* half64, float* one32)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->get_areas(half32, half64, one32);
+}
+
+int tsg_set_id(tsg_grid* h, int node, int id)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->set_id_and_resync(node, id));
+}
+
+int tsg_surface_init(tsg_grid* h, int vacuum_id)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->surface_init(vacuum_id));
+}
+
+int tsg_surface_size(const tsg_grid* h, int* count)
+{
+	if (!h || !count) return tsg::fail(TSG_E_ARG, "null argument");
+	*count = h->g->surface_size();
+	return TSG_OK;
+}
+
+int tsg_surface_nodes(const tsg_grid* h, int* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->surface_nodes(out);
+}
+
+int tsg_surface_evaluate(tsg_grid* h, int exact_sum, double* scaling_ref)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->surface_evaluate(exact_sum != 0, scaling_ref));
+}
+
+int tsg_surface_values(tsg_grid* h, double* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->surface_values(out));
+}
+
+int tsg_surface_probabilities(tsg_grid* h, float* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->surface_probabilities(out));
+}
+
+int tsg_surface_candidate(tsg_grid* h, int* node, int* near_tie)
+{
+	if (!h || !node) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->surface_candidate(node, near_tie));
+}
+
+int tsg_take_away(tsg_grid* h, int node)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->take_away(node));
+}
+
+int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out, int* done)
+{
+	if (!h || !o || !out || !done) return tsg::fail(TSG_E_ARG, "null argument");
+	tsg::EvapOptions e;
+	e.prob_mode = o->prob_mode; e.evap_mode = o->evap_mode; e.vacuum_id = o->vacuum_id;
+	e.relax_threshold = o->relax_threshold; e.relax_shell = o->relax_shell; e.relax_cycle = o->relax_cycle;
+	e.relax_queue = o->relax_queue; e.relax_global_cycles = o->relax_global_cycles;
+	e.refresh_interval = o->refresh_interval; e.refresh_threshold = o->refresh_threshold;
+	e.refresh_cycle = o->refresh_cycle; e.refresh_queue = o->refresh_queue;
+	e.exact_probability = o->exact_probability;
+	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
+}
+
+int tsg_sweep_stats(const tsg_grid* h, double* bytes, long long* rows, long long* entries, long long* referenced,
+                    double* ms, long long* sweeps, long long* launched)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return h->g->sweep_stats(bytes, rows, entries, referenced, ms, sweeps, launched);
+}
+
+}  // extern "C"

@@ -1,0 +1,1064 @@
+// Host side of the device grid (see grid.hpp).
+#include "grid.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <limits>
+
+namespace tsg {
+
+namespace {
+constexpr int kSlice = 64;
+constexpr int kSpareSlices = 64;  // 4096 rows for atoms that turn into vacuum
+constexpr double kEpsilon0 = 8.85418781762e-12;  // grid_3d.h:285
+}  // namespace
+
+DevBuf::~DevBuf() { free(); }
+void DevBuf::alloc(size_t bytes)
+{
+	free();
+	if (bytes == 0) bytes = 8;
+	if (hipMalloc(&p_, bytes) != hipSuccess) { p_ = nullptr; n_ = 0; return; }
+	n_ = bytes;
+}
+void DevBuf::free()
+{
+	if (p_) (void) hipFree(p_);
+	p_ = nullptr;
+	n_ = 0;
+}
+
+int Grid::fail(int code, const std::string& msg)
+{
+	err_ = msg;
+	status_ = code;
+	return code;
+}
+
+int Grid::hip_check(hipError_t e, const char* where)
+{
+	if (e == hipSuccess) return 0;
+	return fail(-2, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(x)                                      \
+	do {                                               \
+		if (int rc__ = hip_check((x), #x)) return rc__; \
+	} while (0)
+
+Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* species, float temperature)
+    : mode_(mode), f64_(mode == 0), mesh_(mesh), temperature_(temperature)
+{
+	const int n = mesh_.n;
+	ids_.assign(ids, ids + n);
+	for (int s = 0; s < 128; s++) species_[s] = species[s];
+	for (int i = 0; i < n; i++)
+		if (ids_[i] < 0 || ids_[i] > 127 || !species_[ids_[i]].defined) { fail(-1, "node with an undefined species id"); return; }
+	flags_.resize(n);
+	for (int i = 0; i < n; i++) {
+		const Species& sp = species_[ids_[i]];
+		flags_[i] = (mesh_.boundary[i] ? 1 : 0) | (sp.dirichlet ? 2 : 0) | (sp.neumann ? 4 : 0);
+	}
+	for (int i = 0; i < n; i++)
+		if (mesh_.assoc[i] < 0 || mesh_.assoc[i] >= mesh_.m() || mesh_.ptr[i + 1] == mesh_.ptr[i]) {
+			fail(-4, "node " + std::to_string(i) + " belongs to no tetrahedron");
+			return;
+		}
+	const long long nnz = (long long) mesh_.idx.size();
+	row_of_entry_.resize(nnz);
+	mirror_.resize(nnz);
+	for (int i = 0; i < n; i++)
+		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++) row_of_entry_[k] = i;
+	for (long long k = 0; k < nnz; k++) {
+		const int i = (int) row_of_entry_[k], j = mesh_.idx[k];
+		const int* b = mesh_.idx.data() + mesh_.ptr[j];
+		const int* e = mesh_.idx.data() + mesh_.ptr[j + 1];
+		const int* f = std::lower_bound(b, e, i);
+		if (f == e || *f != i) { fail(-4, "asymmetric node adjacency"); return; }
+		mirror_[k] = f - mesh_.idx.data();
+	}
+	if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
+
+	// ---- geometry precompute on the device
+	const int m = mesh_.m();
+	DevBuf d_tv, d_tn, d_assoc, d_ntp, d_nti, d_rowof, d_cc64, d_cc32;
+	DevBuf d_ah32, d_ah64, d_a132, d_d32, d_d64;
+	d_xyz_.alloc(sizeof(float) * 3 * (size_t) n);
+	d_tv.alloc(sizeof(int) * 4 * (size_t) m);
+	d_tn.alloc(sizeof(int) * 4 * (size_t) m);
+	d_assoc.alloc(sizeof(int) * (size_t) n);
+	d_ntp.alloc(sizeof(int) * ((size_t) n + 1));
+	d_nti.alloc(sizeof(int) * mesh_.nt_idx.size());
+	d_ptr_.alloc(sizeof(int) * ((size_t) n + 1));
+	d_idx_.alloc(sizeof(int) * (size_t) nnz);
+	d_rowof.alloc(sizeof(int) * (size_t) nnz);
+	d_cc64.alloc(sizeof(double) * 3 * (size_t) m);
+	d_cc32.alloc(sizeof(float) * 3 * (size_t) m);
+	d_ah32.alloc(sizeof(float) * (size_t) nnz);
+	d_ah64.alloc(sizeof(double) * (size_t) nnz);
+	d_a132.alloc(sizeof(float) * (size_t) nnz);
+	d_d32.alloc(sizeof(float) * (size_t) nnz);
+	d_d64.alloc(sizeof(double) * (size_t) nnz);
+	d_status_.alloc(sizeof(int) * 4);
+	if (!d_xyz_.ptr() || !d_tv.ptr() || !d_tn.ptr() || !d_nti.ptr() || !d_idx_.ptr() || !d_cc64.ptr() || !d_ah64.ptr() ||
+	    !d_d64.ptr() || !d_status_.ptr()) {
+		fail(-2, "device allocation failed (geometry)");
+		return;
+	}
+	std::vector<int> rowof32(nnz);
+	for (long long k = 0; k < nnz; k++) rowof32[k] = (int) row_of_entry_[k];
+	if (hip_check(hipMemcpy(d_xyz_.ptr(), mesh_.xyz.data(), d_xyz_.bytes(), hipMemcpyHostToDevice), "upload xyz") ||
+	    hip_check(hipMemcpy(d_tv.ptr(), mesh_.tet_v.data(), d_tv.bytes(), hipMemcpyHostToDevice), "upload tets") ||
+	    hip_check(hipMemcpy(d_tn.ptr(), mesh_.tet_n.data(), d_tn.bytes(), hipMemcpyHostToDevice), "upload neighbours") ||
+	    hip_check(hipMemcpy(d_assoc.ptr(), mesh_.assoc.data(), d_assoc.bytes(), hipMemcpyHostToDevice), "upload assoc") ||
+	    hip_check(hipMemcpy(d_ntp.ptr(), mesh_.nt_ptr.data(), d_ntp.bytes(), hipMemcpyHostToDevice), "upload nt_ptr") ||
+	    hip_check(hipMemcpy(d_nti.ptr(), mesh_.nt_idx.data(), d_nti.bytes(), hipMemcpyHostToDevice), "upload nt_idx") ||
+	    hip_check(hipMemcpy(d_ptr_.ptr(), mesh_.ptr.data(), d_ptr_.bytes(), hipMemcpyHostToDevice), "upload ptr") ||
+	    hip_check(hipMemcpy(d_idx_.ptr(), mesh_.idx.data(), d_idx_.bytes(), hipMemcpyHostToDevice), "upload idx") ||
+	    hip_check(hipMemcpy(d_rowof.ptr(), rowof32.data(), d_rowof.bytes(), hipMemcpyHostToDevice), "upload row_of") ||
+	    hip_check(hipMemsetAsync(d_status_.ptr(), 0, d_status_.bytes(), stream_), "clear status"))
+		return;
+	if (launch_circumcentres(stream_, m, d_xyz_.as<float>(), d_tv.as<int>(), d_cc64.as<double>(), d_cc32.as<float>(),
+	                         d_status_.as<int>()) ||
+	    launch_edge_areas(stream_, n, nnz, d_xyz_.as<float>(), d_tv.as<int>(), d_tn.as<int>(), d_assoc.as<int>(),
+	                      d_ntp.as<int>(), d_nti.as<int>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_rowof.as<int>(),
+	                      d_cc64.as<double>(), d_cc32.as<float>(), d_ah32.as<float>(), d_ah64.as<double>(),
+	                      d_a132.as<float>(), d_d32.as<float>(), d_d64.as<double>(), d_status_.as<int>())) {
+		fail(-2, "geometry kernel launch failed");
+		return;
+	}
+	int gst = 0;
+	a_half32_.resize(nnz); a_half64_.resize(nnz); a_one32_.resize(nnz); dist32_.resize(nnz); dist64_.resize(nnz);
+	if (hip_check(hipStreamSynchronize(stream_), "geometry kernels") ||
+	    hip_check(hipMemcpy(&gst, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost), "status") ||
+	    hip_check(hipMemcpy(a_half32_.data(), d_ah32.ptr(), d_ah32.bytes(), hipMemcpyDeviceToHost), "areas") ||
+	    hip_check(hipMemcpy(a_half64_.data(), d_ah64.ptr(), d_ah64.bytes(), hipMemcpyDeviceToHost), "areas") ||
+	    hip_check(hipMemcpy(a_one32_.data(), d_a132.ptr(), d_a132.bytes(), hipMemcpyDeviceToHost), "areas") ||
+	    hip_check(hipMemcpy(dist32_.data(), d_d32.ptr(), d_d32.bytes(), hipMemcpyDeviceToHost), "distances") ||
+	    hip_check(hipMemcpy(dist64_.data(), d_d64.ptr(), d_d64.bytes(), hipMemcpyDeviceToHost), "distances"))
+		return;
+	if (gst & 1) { fail(-4, "degenerate tetrahedron (circumcentre)"); return; }
+	if (gst & 2) { fail(-4, "Voronoi face walk failed"); return; }
+
+	// ---- device state of the relaxation
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	d_coupfull_.alloc(rs * (size_t) nnz);
+	d_flags_.alloc((size_t) n);
+	d_evapF_.alloc(sizeof(float) * (size_t) n);
+	d_phi0_.alloc(rs * (size_t) n);
+	d_phi1_.alloc(rs * (size_t) n);
+	d_ctl_.alloc(sizeof(RelaxCtl));
+	d_argmax_.alloc(sizeof(long long) * 2);
+	d_maxval_.alloc(sizeof(double));
+	d_sum_.alloc(sizeof(double));
+	d_local_out_.alloc(sizeof(long long) * 4);
+	if (!d_coupfull_.ptr() || !d_phi0_.ptr() || !d_phi1_.ptr()) { fail(-2, "device allocation failed (grid)"); return; }
+	if (sync()) return;
+	// initial potentials: the species presets (Process::initialization, process.cpp:65-80)
+	std::vector<double> phi(n);
+	for (int i = 0; i < n; i++) phi[i] = species_[ids_[i]].phi;
+	if (set_phi(phi.data(), phi.data(), 0, 0)) return;
+	std::vector<float> evf(n);
+	for (int i = 0; i < n; i++) evf[i] = species_[ids_[i]].evap_field;
+	if (hip_check(hipMemcpy(d_evapF_.ptr(), evf.data(), d_evapF_.bytes(), hipMemcpyHostToDevice), "upload evapField")) return;
+}
+
+Grid::~Grid()
+{
+	if (stream_) (void) hipStreamSynchronize(stream_);
+	if (ev_start_) (void) hipEventDestroy(ev_start_);
+	if (ev_stop_) (void) hipEventDestroy(ev_stop_);
+	if (h_pinned_) (void) hipHostFree(h_pinned_);
+	if (stream_) (void) hipStreamDestroy(stream_);
+}
+
+// Grid_3d::Table::computeCoupling (grid_3d.cpp:1392-1409) from the
+// precomputed directed-face area of entry (n1 -> n2)
+double Grid::compute_coupling(int n1, int n2, long long entry) const
+{
+	if (!(flags_[n1] & 4) && (flags_[n2] & 4)) return 0.0;
+	const Species& s1 = species_[ids_[n1]];
+	const Species& s2 = species_[ids_[n2]];
+	double value = 2.0;
+	if (f64_) {
+		value *= (double) s1.epsilon * (double) s2.epsilon;
+		value /= (double) s1.epsilon + (double) s2.epsilon;
+		value *= a_half64_[entry];
+		value /= dist64_[entry];
+	} else {
+		value *= s1.epsilon * s2.epsilon;   // float product, as the reference
+		value /= s1.epsilon + s2.epsilon;
+		value *= a_half32_[entry];
+		value /= dist32_[entry];
+		value = (double) (float) value;     // stored as float
+	}
+	return value;
+}
+
+int Grid::sync()
+{
+	const int n = mesh_.n;
+	const long long nnz = (long long) mesh_.idx.size();
+	coup_.assign(nnz, std::numeric_limits<double>::quiet_NaN());
+	charge_.assign(n, 0.0);
+	for (int i = 0; i < n; i++)
+		if (species_[ids_[i]].charge_density != 0.0f)
+			return fail(-7, "non-zero charge density (Voronoi cell volumes) is not supported on the device path");
+	// couplings once per pair as computeCoupling(lower, higher), mirrored
+	for (int i = 0; i < n; i++)
+		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++) {
+			if (coup_[k] == coup_[k]) continue;
+			const long long kk = mirror_[k];
+			if (coup_[kk] != coup_[kk]) {
+				const double c = compute_coupling(i, mesh_.idx[k], k);
+				if (c != c) return fail(-3, "Grid_3d::Table::computeCoupling(): NaN");
+				coup_[k] = c;
+				coup_[kk] = c;
+			} else
+				coup_[k] = coup_[kk];
+		}
+	// upload full-CSR couplings and flags
+	if (f64_) {
+		HIPCHK(hipMemcpy(d_coupfull_.ptr(), coup_.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+	} else {
+		std::vector<float> c32(nnz);
+		for (long long k = 0; k < nnz; k++) c32[k] = (float) coup_[k];
+		HIPCHK(hipMemcpy(d_coupfull_.ptr(), c32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+	}
+	HIPCHK(hipMemcpy(d_flags_.ptr(), flags_.data(), (size_t) n, hipMemcpyHostToDevice));
+	return build_rows();
+}
+
+bool Grid::participates(int row, int col) const
+{
+	if (flags_[row] & 4) return true;       // Neumann rows use every neighbour
+	return !(flags_[col] & 4);              // others skip Neumann neighbours
+}
+
+// the entries of a row in ascending neighbour order and their coupling sum,
+// accumulated in double in that order (computePotential's weightSum)
+void Grid::row_entries(int node, std::vector<int>* cols, std::vector<double>* coups, double* ws) const
+{
+	cols->clear();
+	coups->clear();
+	double sum = 0.0;
+	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) {
+		const int j = mesh_.idx[k];
+		if (!participates(node, j)) continue;
+		cols->push_back(j);
+		coups->push_back(coup_[k]);
+		sum += coup_[k];
+	}
+	*ws = sum;
+}
+
+int Grid::build_rows()
+{
+	const int n = mesh_.n;
+	slot_of_.assign(n, -1);
+	std::vector<int> rows;
+	for (int i = 0; i < n; i++)
+		if (!(flags_[i] & 2)) rows.push_back(i);
+	nrows_ = (int) rows.size();
+	const int nslices_main = (nrows_ + kSlice - 1) / kSlice;
+	int maxdeg = 0;
+	for (int i = 0; i < n; i++) maxdeg = std::max(maxdeg, mesh_.ptr[i + 1] - mesh_.ptr[i]);
+	spare_width_ = maxdeg;
+	const int nslices = nslices_main + kSpareSlices;
+	cap_slots_ = nslices * kSlice;
+	nslots_ = nslices_main * kSlice;
+	next_spare_ = nslots_;
+	slot_node_.assign(cap_slots_, -1);
+	slot_deg_.assign(cap_slots_, 0);
+	slice_width_.assign(nslices, spare_width_);
+	slice_off_.assign((size_t) nslices + 1, 0);
+	std::vector<std::vector<int>> ecols(nrows_);
+	std::vector<std::vector<double>> ecoup(nrows_);
+	std::vector<double> ws(cap_slots_, 1.0), q(cap_slots_, 0.0);
+	for (int r = 0; r < nrows_; r++) {
+		double w;
+		row_entries(rows[r], &ecols[r], &ecoup[r], &w);
+		slot_node_[r] = rows[r];
+		slot_deg_[r] = (int) ecols[r].size();
+		slot_of_[rows[r]] = r;
+		ws[r] = w;
+		q[r] = charge_[rows[r]];
+	}
+	for (int s = 0; s < nslices_main; s++) {
+		int w = 0;
+		for (int l = 0; l < kSlice; l++) {
+			const int r = s * kSlice + l;
+			if (r < nrows_) w = std::max(w, slot_deg_[r]);
+		}
+		slice_width_[s] = w;
+	}
+	for (int s = 0; s < nslices; s++) slice_off_[s + 1] = slice_off_[s] + (long long) slice_width_[s] * kSlice;
+	const long long nent = slice_off_[nslices];
+	std::vector<int> col(nent, 0);
+	std::vector<double> cp(nent, 0.0);
+	for (int r = 0; r < nrows_; r++) {
+		const long long base = slice_off_[r / kSlice] + (r % kSlice);
+		for (int k = 0; k < slot_deg_[r]; k++) {
+			col[base + (long long) kSlice * k] = ecols[r][k];
+			cp[base + (long long) kSlice * k] = ecoup[r][k];
+		}
+	}
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	d_slot_node_.alloc(sizeof(int) * cap_slots_);
+	d_slot_deg_.alloc(sizeof(int) * cap_slots_);
+	d_slice_off_.alloc(sizeof(long long) * slice_off_.size());
+	d_col_.alloc(sizeof(int) * (size_t) nent);
+	d_coup_.alloc(rs * (size_t) nent);
+	d_ws_.alloc(sizeof(double) * cap_slots_);
+	d_q_.alloc(sizeof(double) * cap_slots_);
+	if (!d_col_.ptr() || !d_coup_.ptr()) return fail(-2, "device allocation failed (rows)");
+	HIPCHK(hipMemcpy(d_slot_node_.ptr(), slot_node_.data(), d_slot_node_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_slot_deg_.ptr(), slot_deg_.data(), d_slot_deg_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_slice_off_.ptr(), slice_off_.data(), d_slice_off_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_col_.ptr(), col.data(), d_col_.bytes(), hipMemcpyHostToDevice));
+	if (f64_) {
+		HIPCHK(hipMemcpy(d_coup_.ptr(), cp.data(), d_coup_.bytes(), hipMemcpyHostToDevice));
+	} else {
+		std::vector<float> c32(cp.begin(), cp.end());
+		HIPCHK(hipMemcpy(d_coup_.ptr(), c32.data(), d_coup_.bytes(), hipMemcpyHostToDevice));
+	}
+	HIPCHK(hipMemcpy(d_ws_.ptr(), ws.data(), d_ws_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_q_.ptr(), q.data(), d_q_.bytes(), hipMemcpyHostToDevice));
+	return 0;
+}
+
+SweepArgs Grid::sweep_args()
+{
+	SweepArgs a;
+	a.nslots = next_spare_;
+	a.slot_node = d_slot_node_.as<int>();
+	a.slot_deg = d_slot_deg_.as<int>();
+	a.slice_off = d_slice_off_.as<long long>();
+	a.col = d_col_.as<int>();
+	a.coup = d_coup_.ptr();
+	a.ws = d_ws_.as<double>();
+	a.q = d_q_.as<double>();
+	a.phi0 = d_phi0_.ptr();
+	a.phi1 = d_phi1_.ptr();
+	a.ctl = d_ctl_.as<RelaxCtl>();
+	a.threshold_mode = 1;
+	return a;
+}
+
+int Grid::arm_ctl(double threshold, int cycles, int queue)
+{
+	if (queue < 0 || queue > 1024) return fail(-1, "queue size must be in [0, 1024]");
+	RelaxCtl c;
+	std::memset(&c, 0, sizeof(c));
+	c.threshold = threshold;
+	c.cycle = cycles;
+	c.queue_size = queue;
+	HIPCHK(hipMemcpyAsync(d_ctl_.ptr(), &c, offsetof(RelaxCtl, queue), hipMemcpyHostToDevice, stream_));
+	return 0;
+}
+
+int Grid::relax_cycles(int cycles, double* deviation)
+{
+	if (cycles != 1) return fail(-1, "relax(cycleSize) on the device supports cycleSize 1 (Jacobi); in-place cycles run in localRelax");
+	if (int rc = arm_ctl(-1.0, 1, 0)) return rc;
+	SweepArgs a = sweep_args();
+	if (launch_jacobi_sweep(stream_, a, sw_, f64_)) return fail(-2, "sweep launch failed");
+	RelaxCtl c;
+	HIPCHK(hipMemcpyAsync(&c, d_ctl_.ptr(), offsetof(RelaxCtl, queue), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (c.done == 3) return fail(-3, "Grid_3d::Table::computePotential(): NaN");
+	sw_ ^= 1;
+	if (deviation) *deviation = c.last_dev;
+	return 0;
+}
+
+int Grid::relax_threshold(double threshold, int cycles, int queue, long long* iterations, double* fault_dev,
+                          double* fault_slope, double* last_dev)
+{
+	if (cycles != 1) return fail(-1, "relax(threshold, cycleSize, queue) on the device supports cycleSize 1");
+	if (int rc = arm_ctl(threshold, 1, queue)) return rc;
+	SweepArgs a = sweep_args();
+	const int sw0 = sw_;
+	long long launched = 0;
+	// the first batch is sized from the previous call's sweep count, later
+	// batches are short so that few sweeps are launched after convergence
+	// (those return at once on the device's done flag)
+	int batch = (int) std::min<long long>(4096, std::max<long long>(8, last_sweeps_ * 7 / 8));
+	RelaxCtl c;
+	if (!ev_start_) {
+		HIPCHK(hipEventCreate(&ev_start_));
+		HIPCHK(hipEventCreate(&ev_stop_));
+	}
+	HIPCHK(hipEventRecord(ev_start_, stream_));
+	for (;;) {
+		for (int b = 0; b < batch; b++) {
+			if (launch_jacobi_sweep(stream_, a, (int) ((sw0 + launched) & 1), f64_)) return fail(-2, "sweep launch failed");
+			launched++;
+		}
+		HIPCHK(hipMemcpyAsync(&c, d_ctl_.ptr(), offsetof(RelaxCtl, queue), hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipStreamSynchronize(stream_));
+		if (c.done) break;
+		batch = (int) std::min<long long>(64, std::max<long long>(4, (last_sweeps_ - c.sweeps) / 2));
+	}
+	HIPCHK(hipEventRecord(ev_stop_, stream_));
+	HIPCHK(hipEventSynchronize(ev_stop_));
+	float ms = 0.0f;
+	HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+	stat_sweep_ms_ += ms;
+	stat_sweeps_ += c.sweeps;
+	stat_launched_ += launched;
+	last_sweeps_ = c.sweeps;
+	sw_ = (int) ((sw0 + c.sweeps) & 1);
+	if (last_dev) *last_dev = c.last_dev;
+	if (c.done == 3) return fail(-3, "Grid_3d::Table::computePotential(): NaN");
+	if (c.done == 2) {
+		if (iterations) *iterations = -1 - c.iteration;
+		if (fault_dev) *fault_dev = c.fault_dev;
+		if (fault_slope) *fault_slope = c.fault_slope;
+		return 0;
+	}
+	if (iterations) *iterations = c.iteration;
+	return 0;
+}
+
+int Grid::local_relax(int node, int order, double threshold, int cycles, int queue, long long* iterations)
+{
+	const int n = mesh_.n;
+	if (node < 0 || node >= n || order < 0 || cycles < 1) return fail(-1, "localRelax: bad arguments");
+	if (queue < 0 || queue > 1024) return fail(-1, "queue size must be in [0, 1024]");
+	// find_localNodes (grid_3d.cpp:1284-1315): BFS levels into an ascending
+	// set (the primary node enters only as a neighbour's neighbour), then the
+	// Dirichlet nodes are removed (grid_3d.cpp:713-723)
+	std::vector<int> in_set;
+	std::vector<int> cur(1, node), next;
+	static thread_local std::vector<uint8_t> mark;
+	if ((int) mark.size() < n) mark.assign(n, 0);
+	for (int lvl = 0; lvl < order; lvl++) {
+		next.clear();
+		for (int u : cur)
+			for (int k = mesh_.ptr[u]; k < mesh_.ptr[u + 1]; k++) {
+				const int v = mesh_.idx[k];
+				if (!mark[v]) { mark[v] = 1; next.push_back(v); in_set.push_back(v); }
+			}
+		cur.swap(next);
+	}
+	for (int v : in_set) mark[v] = 0;
+	std::sort(in_set.begin(), in_set.end());
+	std::vector<int> nodes, slots;
+	for (int v : in_set)
+		if (!(flags_[v] & 2)) { nodes.push_back(v); slots.push_back(slot_of_[v]); }
+	const int count = (int) nodes.size();
+	if (count == 0) {
+		// an empty shell: one call of doLocalRelax toggles the switch, the
+		// deviation over no nodes is 0
+		lsw_ = (lsw_ + cycles) % 2;
+		if (iterations) *iterations = cycles;
+		return 0;
+	}
+	// level schedule of the ascending in-place sweep
+	std::vector<int> level(count, 0), pos_of;
+	int nlev = 1;
+	{
+		// map node -> shell position through a sorted search
+		for (int p = 0; p < count; p++) {
+			const int v = nodes[p];
+			int lv = 0;
+			for (int k = mesh_.ptr[v]; k < mesh_.ptr[v + 1]; k++) {
+				const int j = mesh_.idx[k];
+				if (j >= v) break;  // ascending: only lower-index neighbours precede
+				if (!participates(v, j)) continue;
+				const auto it = std::lower_bound(nodes.begin(), nodes.end(), j);
+				if (it != nodes.end() && *it == j) lv = std::max(lv, level[it - nodes.begin()] + 1);
+			}
+			level[p] = lv;
+			nlev = std::max(nlev, lv + 1);
+		}
+	}
+	std::vector<int> level_ptr(nlev + 1, 0), order_v(count);
+	for (int p = 0; p < count; p++) level_ptr[level[p] + 1]++;
+	for (int l = 0; l < nlev; l++) level_ptr[l + 1] += level_ptr[l];
+	{
+		std::vector<int> fill(level_ptr.begin(), level_ptr.end() - 1);
+		for (int p = 0; p < count; p++) order_v[fill[level[p]]++] = p;
+	}
+	if (d_shell_nodes_.bytes() < sizeof(int) * (size_t) count) {
+		d_shell_nodes_.alloc(sizeof(int) * 2 * (size_t) count);
+		d_shell_slots_.alloc(sizeof(int) * 2 * (size_t) count);
+		d_order_.alloc(sizeof(int) * 2 * (size_t) count);
+	}
+	if (d_level_ptr_.bytes() < sizeof(int) * level_ptr.size()) d_level_ptr_.alloc(sizeof(int) * 2 * level_ptr.size());
+	HIPCHK(hipMemcpyAsync(d_shell_nodes_.ptr(), nodes.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_shell_slots_.ptr(), slots.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_order_.ptr(), order_v.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_level_ptr_.ptr(), level_ptr.data(), sizeof(int) * level_ptr.size(), hipMemcpyHostToDevice, stream_));
+	LocalArgs L;
+	L.count = count;
+	L.nodes = d_shell_nodes_.as<int>();
+	L.slots = d_shell_slots_.as<int>();
+	L.level_ptr = d_level_ptr_.as<int>();
+	L.order = d_order_.as<int>();
+	L.nlevels = nlev;
+	L.read_buf = sw_;
+	L.local_switch = lsw_;
+	L.cycle = cycles;
+	L.queue = queue;
+	L.threshold = threshold;
+	L.max_iterations = 100000000ll;
+	L.out = d_local_out_.as<long long>();
+	SweepArgs a = sweep_args();
+	if (launch_local_relax(stream_, a, L, f64_)) return fail(-2, "local relax launch failed");
+	long long out[3];
+	HIPCHK(hipMemcpyAsync(out, d_local_out_.ptr(), sizeof(out), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	lsw_ = (int) out[1];
+	if (out[2] == 3) return fail(-3, "Grid_3d::Table::computePotential(): NaN");
+	if (out[2] == 4) return fail(-5, "localRelax: iteration limit");
+	if (iterations) *iterations = out[0];
+	return 0;
+}
+
+int Grid::get_phi(int buffer, double* out)
+{
+	const int n = mesh_.n;
+	const void* src = buffer ? d_phi1_.ptr() : d_phi0_.ptr();
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (f64_) {
+		HIPCHK(hipMemcpy(out, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> t(n);
+		HIPCHK(hipMemcpy(t.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+		for (int i = 0; i < n; i++) out[i] = t[i];
+	}
+	return 0;
+}
+
+int Grid::set_phi(const double* p0, const double* p1, int phi_switch, int local_switch)
+{
+	const int n = mesh_.n;
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (f64_) {
+		HIPCHK(hipMemcpy(d_phi0_.ptr(), p0, sizeof(double) * n, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d_phi1_.ptr(), p1, sizeof(double) * n, hipMemcpyHostToDevice));
+	} else {
+		std::vector<float> t(n);
+		for (int i = 0; i < n; i++) t[i] = (float) p0[i];
+		HIPCHK(hipMemcpy(d_phi0_.ptr(), t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+		for (int i = 0; i < n; i++) t[i] = (float) p1[i];
+		HIPCHK(hipMemcpy(d_phi1_.ptr(), t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	}
+	sw_ = phi_switch & 1;
+	lsw_ = local_switch & 1;
+	return 0;
+}
+
+// Grid_3d::Table::potential(i) = phi[(_phiSwitch + 1) % 2] (grid_3d.h:184)
+int Grid::potential(int node, double* out)
+{
+	if (node < 0 || node >= mesh_.n) return fail(-1, "potential: bad node");
+	const int b = (sw_ + 1) % 2;
+	const char* base = (const char*) (b ? d_phi1_.ptr() : d_phi0_.ptr());
+	if (f64_) {
+		HIPCHK(hipMemcpyAsync(out, base + sizeof(double) * node, sizeof(double), hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipStreamSynchronize(stream_));
+	} else {
+		float v;
+		HIPCHK(hipMemcpyAsync(&v, base + sizeof(float) * node, sizeof(float), hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipStreamSynchronize(stream_));
+		*out = v;
+	}
+	return 0;
+}
+
+int Grid::field_o1(int count, const int* nodes, double* out)
+{
+	if (count <= 0) return 0;
+	for (int i = 0; i < count; i++)
+		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "field_o1: bad node");
+	if (d_nodes_.bytes() < sizeof(int) * (size_t) count) {
+		d_nodes_.alloc(sizeof(int) * (size_t) count);
+		d_fields_.alloc(sizeof(double) * 3 * (size_t) count);
+	}
+	HIPCHK(hipMemcpyAsync(d_nodes_.ptr(), nodes, sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	if (launch_node_fields(stream_, count, d_nodes_.as<int>(), d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(),
+	                       d_coupfull_.ptr(), d_flags_.as<unsigned char>(), phi, f64_, d_fields_.as<double>(),
+	                       d_status_.as<int>()))
+		return fail(-2, "field launch failed");
+	int st = 0;
+	HIPCHK(hipMemcpyAsync(out, d_fields_.ptr(), sizeof(double) * 3 * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	return 0;
+}
+
+int Grid::get_couplings(double* out) const
+{
+	std::copy(coup_.begin(), coup_.end(), out);
+	return 0;
+}
+int Grid::get_flags(unsigned char* out) const
+{
+	std::copy(flags_.begin(), flags_.end(), out);
+	return 0;
+}
+int Grid::get_ids(int16_t* out) const
+{
+	std::copy(ids_.begin(), ids_.end(), out);
+	return 0;
+}
+int Grid::get_areas(float* a_half32, double* a_half64, float* a_one32) const
+{
+	if (a_half32) std::copy(a_half32_.begin(), a_half32_.end(), a_half32);
+	if (a_half64) std::copy(a_half64_.begin(), a_half64_.end(), a_half64);
+	if (a_one32) std::copy(a_one32_.begin(), a_one32_.end(), a_one32);
+	return 0;
+}
+
+long long Grid::entry_index(int row, int col) const
+{
+	const int* b = mesh_.idx.data() + mesh_.ptr[row];
+	const int* e = mesh_.idx.data() + mesh_.ptr[row + 1];
+	const int* f = std::lower_bound(b, e, col);
+	if (f == e || *f != col) return -1;
+	return f - mesh_.idx.data();
+}
+
+// (re)writes a row into a slot: slot arrays (int patches to target 0/1),
+// entries (col int patches target 2, coupling real patches target 0), and
+// ws/q (real patches targets 1/2)
+int Grid::write_slot(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* ar,
+                     std::vector<double>* vr, std::vector<long long>* aw, std::vector<double>* vw)
+{
+	std::vector<int> cols;
+	std::vector<double> coups;
+	double ws;
+	row_entries(node, &cols, &coups, &ws);
+	const int deg = (int) cols.size();
+	if (deg > slice_width_[slot / kSlice]) return -1;
+	slot_node_[slot] = node;
+	slot_deg_[slot] = deg;
+	slot_of_[node] = slot;
+	// int patches are encoded as (array id << 60) | index
+	ai->push_back((0ll << 60) | slot); vi->push_back(node);
+	ai->push_back((1ll << 60) | slot); vi->push_back(deg);
+	const long long base = slice_off_[slot / kSlice] + (slot % kSlice);
+	for (int k = 0; k < deg; k++) {
+		ai->push_back((2ll << 60) | (base + (long long) kSlice * k)); vi->push_back(cols[k]);
+		ar->push_back(base + (long long) kSlice * k); vr->push_back(coups[k]);
+	}
+	aw->push_back((1ll << 60) | slot); vw->push_back(ws);
+	aw->push_back((2ll << 60) | slot); vw->push_back(charge_[node]);
+	return 0;
+}
+
+int Grid::set_id_and_resync(int node, int id)
+{
+	if (node < 0 || node >= mesh_.n || id < 0 || id > 127 || !species_[id].defined)
+		return fail(-1, "setId: bad node or species");
+	if (species_[id].charge_density != 0.0f) return fail(-7, "non-zero charge density is not supported on the device path");
+	ids_[node] = (int16_t) id;
+	const Species& sp = species_[id];
+	flags_[node] = (mesh_.boundary[node] ? 1 : 0) | (sp.dirichlet ? 2 : 0) | (sp.neumann ? 4 : 0);
+
+	// couplings of the node's faces as computeCoupling(node, neighbour)
+	std::vector<long long> full_addr;
+	std::vector<double> full_val;
+	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) {
+		const int j = mesh_.idx[k];
+		const double c = compute_coupling(node, j, k);
+		if (c != c) return fail(-3, "Grid_3d::Table::computeCoupling(): NaN");
+		coup_[k] = c;
+		coup_[mirror_[k]] = c;
+		full_addr.push_back(k); full_val.push_back(c);
+		full_addr.push_back(mirror_[k]); full_val.push_back(c);
+	}
+	charge_[node] = 0.0;
+
+	// rows touched: the node itself and every neighbour that owns a row
+	std::vector<long long> ai, ar, aw;
+	std::vector<int> vi;
+	std::vector<double> vr, vw;
+	std::vector<int> touched;
+	touched.push_back(node);
+	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) touched.push_back(mesh_.idx[k]);
+	for (int r : touched) {
+		const bool unknown = !(flags_[r] & 2);
+		int slot = slot_of_[r];
+		if (!unknown) {
+			if (slot >= 0) {  // a row that became Dirichlet is switched off
+				slot_node_[slot] = -1;
+				ai.push_back((0ll << 60) | slot); vi.push_back(-1);
+				slot_of_[r] = -1;
+			}
+			continue;
+		}
+		if (slot >= 0 && write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw) == 0) continue;
+		if (slot >= 0) {  // does not fit its slice any more
+			slot_node_[slot] = -1;
+			ai.push_back((0ll << 60) | slot); vi.push_back(-1);
+		}
+		if (next_spare_ >= cap_slots_) return fail(-7, "row slots exhausted (more than 4096 new unknown nodes)");
+		slot = next_spare_++;
+		if (write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw)) return fail(-7, "row wider than the spare slices");
+		nrows_++;
+	}
+
+	// ship all patches through the pinned staging area
+	const size_t ni = ai.size(), nr = ar.size(), nw = aw.size(), nf = full_addr.size();
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	const size_t need = ni * 12 + nr * (8 + rs) + nw * 16 + nf * (8 + rs) + 256;
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (need > h_pinned_bytes_) {
+		if (h_pinned_) (void) hipHostFree(h_pinned_);
+		h_pinned_bytes_ = std::max(need * 2, (size_t) 1 << 20);
+		HIPCHK(hipHostMalloc(&h_pinned_, h_pinned_bytes_, hipHostMallocDefault));
+		d_patch_addr_.alloc(h_pinned_bytes_);
+	}
+	char* h = (char*) h_pinned_;
+	size_t off = 0;
+	auto put = [&](const void* src, size_t bytes) {
+		std::memcpy(h + off, src, bytes);
+		const size_t at = off;
+		off += (bytes + 15) & ~(size_t) 15;
+		return at;
+	};
+	// split int patches by target array
+	std::vector<long long> ia[3];
+	std::vector<int> iv[3];
+	for (size_t t = 0; t < ni; t++) {
+		const int arr = (int) (ai[t] >> 60);
+		ia[arr].push_back(ai[t] & ((1ll << 60) - 1));
+		iv[arr].push_back(vi[t]);
+	}
+	std::vector<long long> wa[3];
+	std::vector<double> wv[3];
+	for (size_t t = 0; t < nw; t++) {
+		const int arr = (int) (aw[t] >> 60);
+		wa[arr].push_back(aw[t] & ((1ll << 60) - 1));
+		wv[arr].push_back(vw[t]);
+	}
+	size_t o_ia[3], o_iv[3], o_wa[3], o_wv[3];
+	for (int t = 0; t < 3; t++) {
+		o_ia[t] = put(ia[t].data(), sizeof(long long) * ia[t].size());
+		o_iv[t] = put(iv[t].data(), sizeof(int) * iv[t].size());
+		o_wa[t] = put(wa[t].data(), sizeof(long long) * wa[t].size());
+		o_wv[t] = put(wv[t].data(), sizeof(double) * wv[t].size());
+	}
+	const size_t o_ra = put(ar.data(), sizeof(long long) * nr);
+	const size_t o_fa = put(full_addr.data(), sizeof(long long) * nf);
+	size_t o_rv, o_fv;
+	if (f64_) {
+		o_rv = put(vr.data(), sizeof(double) * nr);
+		o_fv = put(full_val.data(), sizeof(double) * nf);
+	} else {
+		std::vector<float> a32(vr.begin(), vr.end()), b32(full_val.begin(), full_val.end());
+		o_rv = put(a32.data(), sizeof(float) * nr);
+		o_fv = put(b32.data(), sizeof(float) * nf);
+	}
+	char* d = (char*) d_patch_addr_.ptr();
+	HIPCHK(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, stream_));
+	int* iarr[3] = {d_slot_node_.as<int>(), d_slot_deg_.as<int>(), d_col_.as<int>()};
+	for (int t = 0; t < 3; t++)
+		if (launch_scatter_i32(stream_, (int) ia[t].size(), (const long long*) (d + o_ia[t]), (const int*) (d + o_iv[t]), iarr[t]))
+			return fail(-2, "patch launch failed");
+	double* warr[3] = {nullptr, d_ws_.as<double>(), d_q_.as<double>()};
+	for (int t = 1; t < 3; t++)
+		if (launch_scatter_f64(stream_, (int) wa[t].size(), (const long long*) (d + o_wa[t]), (const double*) (d + o_wv[t]), warr[t]))
+			return fail(-2, "patch launch failed");
+	int rc;
+	if (f64_) {
+		rc = launch_scatter_f64(stream_, (int) nr, (const long long*) (d + o_ra), (const double*) (d + o_rv), d_coup_.as<double>()) |
+		     launch_scatter_f64(stream_, (int) nf, (const long long*) (d + o_fa), (const double*) (d + o_fv), d_coupfull_.as<double>());
+	} else {
+		rc = launch_scatter_f32(stream_, (int) nr, (const long long*) (d + o_ra), (const float*) (d + o_rv), d_coup_.as<float>()) |
+		     launch_scatter_f32(stream_, (int) nf, (const long long*) (d + o_fa), (const float*) (d + o_fv), d_coupfull_.as<float>());
+	}
+	if (rc) return fail(-2, "patch launch failed");
+	HIPCHK(hipMemcpyAsync(d_flags_.as<unsigned char>() + node, &flags_[node], 1, hipMemcpyHostToDevice, stream_));
+	float evf = species_[id].evap_field;
+	HIPCHK(hipMemcpyAsync(d_evapF_.as<float>() + node, &evf, sizeof(float), hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	return 0;
+}
+
+// ---------------------------------------------------------------- surface
+
+int Grid::surface_init(int vacuum_id)
+{
+	vacuum_id_ = vacuum_id;
+	surface_.clear();
+	const int n = mesh_.n;
+	for (int i = 0; i < n; i++) {
+		if (ids_[i] == vacuum_id) continue;
+		if (!species_[ids_[i]].removable) continue;
+		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++)
+			if (ids_[mesh_.idx[k]] == vacuum_id && a_one32_[k] > 0.0f) { surface_.insert(i); break; }
+	}
+	surface_dirty_ = true;
+	return 0;
+}
+
+int Grid::surface_nodes(int* out) const
+{
+	int p = 0;
+	for (int v : surface_) out[p++] = v;
+	return 0;
+}
+
+int Grid::upload_surface()
+{
+	if (!surface_dirty_) return 0;
+	surface_vec_.assign(surface_.begin(), surface_.end());
+	const size_t cnt = surface_vec_.size();
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	if (d_surf_.bytes() < sizeof(int) * cnt) {
+		d_surf_.alloc(sizeof(int) * (cnt + cnt / 2 + 1024));
+		d_surf_val_.alloc(rs * (cnt + cnt / 2 + 1024));
+	}
+	HIPCHK(hipMemcpyAsync(d_surf_.ptr(), surface_vec_.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	surface_dirty_ = false;
+	return 0;
+}
+
+int Grid::surface_evaluate(bool exact_sum, double* scaling_ref)
+{
+	if (int rc = upload_surface()) return rc;
+	const int cnt = (int) surface_vec_.size();
+	if (cnt == 0) return 0;
+	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
+	FieldArgs f;
+	f.count = cnt;
+	f.nodes = d_surf_.as<int>();
+	f.xyz = d_xyz_.as<float>();
+	f.ptr = d_ptr_.as<int>();
+	f.idx = d_idx_.as<int>();
+	f.coup = d_coupfull_.ptr();
+	f.flags = d_flags_.as<unsigned char>();
+	f.phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	f.evap_field = d_evapF_.as<float>();
+	f.value = d_surf_val_.ptr();
+	f.field = nullptr;
+	f.status = d_status_.as<int>();
+	if (launch_surface_fields(stream_, f, f64_)) return fail(-2, "field launch failed");
+	if (launch_surface_argmax(stream_, cnt, d_surf_val_.ptr(), f64_, d_argmax_.as<long long>(), d_maxval_.as<double>()))
+		return fail(-2, "argmax launch failed");
+	if (exact_sum && launch_surface_sum(stream_, cnt, d_surf_val_.ptr(), f64_, d_sum_.as<double>()))
+		return fail(-2, "sum launch failed");
+	int st = 0;
+	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (st & 1) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	if (st & 2) return fail(-3, "Grid_3d::Table::field_o1(): NaN");
+	if (scaling_ref) *scaling_ref = scaling_ref_;
+	return 0;
+}
+
+int Grid::surface_values(double* out)
+{
+	const int cnt = (int) surface_vec_.size();
+	if (cnt == 0) return 0;
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (f64_) {
+		HIPCHK(hipMemcpy(out, d_surf_val_.ptr(), sizeof(double) * cnt, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> t(cnt);
+		HIPCHK(hipMemcpy(t.data(), d_surf_val_.ptr(), sizeof(float) * cnt, hipMemcpyDeviceToHost));
+		for (int i = 0; i < cnt; i++) out[i] = t[i];
+	}
+	return 0;
+}
+
+// probLinear_field normalisation (surface_3d.cpp:276-296): sequential sum in
+// node order, then value / sum in the element type
+int Grid::surface_probabilities(float* out)
+{
+	const int cnt = (int) surface_vec_.size();
+	std::vector<double> v(cnt);
+	if (int rc = surface_values(v.data())) return rc;
+	if (f64_) {
+		double sum = 0.0;
+		for (int i = 0; i < cnt; i++) sum += v[i];
+		for (int i = 0; i < cnt; i++) out[i] = (float) (v[i] / sum);
+	} else {
+		float sum = 0.0f;
+		for (int i = 0; i < cnt; i++) sum += (float) v[i];
+		for (int i = 0; i < cnt; i++) out[i] = (float) v[i] / sum;
+	}
+	return 0;
+}
+
+int Grid::resolve_candidate(int* node, int* near_tie)
+{
+	long long am[2];
+	HIPCHK(hipMemcpyAsync(am, d_argmax_.ptr(), sizeof(am), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	int w = (int) am[0];
+	*near_tie = (int) am[1];
+	if (am[1]) {
+		// exact rule: first maximum of the normalised values
+		const int cnt = (int) surface_vec_.size();
+		std::vector<double> v(cnt);
+		if (int rc = surface_values(v.data())) return rc;
+		if (f64_) {
+			double sum = 0.0;
+			for (int i = 0; i < cnt; i++) sum += v[i];
+			int pick = 0;
+			for (int i = 0; i < cnt; i++)
+				if (v[pick] / sum < v[i] / sum) pick = i;
+			w = pick;
+		} else {
+			float sum = 0.0f;
+			for (int i = 0; i < cnt; i++) sum += (float) v[i];
+			int pick = 0;
+			for (int i = 0; i < cnt; i++)
+				if ((float) v[pick] / sum < (float) v[i] / sum) pick = i;
+			w = pick;
+		}
+	}
+	*node = surface_vec_[w];
+	return 0;
+}
+
+int Grid::surface_candidate(int* node, int* near_tie)
+{
+	if (surface_vec_.empty()) return fail(-6, "empty surface");
+	int nt;
+	const int rc = resolve_candidate(node, &nt);
+	if (near_tie) *near_tie = nt;
+	return rc;
+}
+
+// Surface_3d::Table::update (surface_3d.cpp:83-119)
+static bool has_vacuum_face(const Mesh& m, const std::vector<int16_t>& ids, const std::vector<float>& a1, int vac, int i)
+{
+	for (int k = m.ptr[i]; k < m.ptr[i + 1]; k++)
+		if (ids[m.idx[k]] == vac && a1[k] > 0.0f) return true;
+	return false;
+}
+
+int Grid::take_away(int node)
+{
+	if (vacuum_id_ < 0) return fail(-6, "surface not initialised");
+	auto update = [&](int v) -> int {
+		if (surface_.erase(v) != 1) return fail(-6, "Surface_3d::Table::update(): node not on the surface");
+		for (int k = mesh_.ptr[v]; k < mesh_.ptr[v + 1]; k++) {
+			const int j = mesh_.idx[k];
+			if (ids_[j] == vacuum_id_) continue;
+			if (!species_[ids_[j]].removable) continue;
+			if (has_vacuum_face(mesh_, ids_, a_one32_, vacuum_id_, j)) surface_.insert(j);
+		}
+		surface_dirty_ = true;
+		return 0;
+	};
+	if (int rc = set_id_and_resync(node, vacuum_id_)) return rc;
+	if (int rc = update(node)) return rc;
+	// detached surface atoms (surface_3d.cpp:521-562)
+	std::vector<int> lonely;
+	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) {
+		const int j = mesh_.idx[k];
+		if (ids_[j] == vacuum_id_) continue;
+		bool flag = true;
+		for (int kk = mesh_.ptr[j]; kk < mesh_.ptr[j + 1]; kk++)
+			if (ids_[mesh_.idx[kk]] != vacuum_id_) { flag = false; break; }
+		if (flag) lonely.push_back(j);
+	}
+	for (int j : lonely) {
+		if (int rc = set_id_and_resync(j, vacuum_id_)) return rc;
+		if (int rc = update(j)) return rc;
+	}
+	return 0;
+}
+
+int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
+{
+	*done = 0;
+	if (o.prob_mode != 0 || o.evap_mode != 0)
+		return fail(-1, "the device event loop implements the linear-field probability with the maximum choice");
+	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
+	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr)) return rc;
+	for (int ev = 0; ev < k; ev++) {
+		if (surface_.empty()) break;
+		EventRecord& r = out[ev];
+		std::memset(&r, 0, sizeof(r));
+		int node, nt;
+		if (int rc = surface_candidate(&node, &nt)) return rc;
+		r.node = node;
+		r.id = ids_[node];
+		r.near_tie = nt;
+		if (o.exact_probability) {
+			double sum;
+			HIPCHK(hipMemcpyAsync(&sum, d_sum_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+			double vw;
+			const int slot = (int) (std::lower_bound(surface_vec_.begin(), surface_vec_.end(), node) - surface_vec_.begin());
+			const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+			if (f64_) {
+				HIPCHK(hipMemcpyAsync(&vw, d_surf_val_.as<char>() + rs * slot, sizeof(double), hipMemcpyDeviceToHost, stream_));
+				HIPCHK(hipStreamSynchronize(stream_));
+				r.probability = (float) (vw / sum);
+			} else {
+				float v32;
+				HIPCHK(hipMemcpyAsync(&v32, d_surf_val_.as<char>() + rs * slot, sizeof(float), hipMemcpyDeviceToHost, stream_));
+				HIPCHK(hipStreamSynchronize(stream_));
+				r.probability = v32 / (float) sum;
+			}
+		}
+		if (int rc = potential(node, &r.pot_before)) return rc;
+		if (int rc = field_o1(1, &node, r.field_before)) return rc;
+		if (int rc = take_away(node)) return rc;
+		if (int rc = local_relax(node, o.relax_shell, o.relax_threshold, o.relax_cycle, o.relax_queue, &r.local_iter)) return rc;
+		for (int c = 0; c < o.relax_global_cycles; c++)
+			if (int rc = relax_cycles(1, nullptr)) return rc;
+		if (o.refresh_interval > 0 && (ev + 1) % o.refresh_interval == 0) {
+			double fd, fs, ld;
+			if (int rc = relax_threshold(o.refresh_threshold, o.refresh_cycle, o.refresh_queue, &r.global_iter, &fd, &fs, &ld)) return rc;
+		}
+		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref)) return rc;
+		r.surface_size = (int) surface_.size();
+		if (int rc = potential(node, &r.pot_after)) return rc;
+		if (int rc = field_o1(1, &node, r.field_after)) return rc;
+		*done = ev + 1;
+	}
+	return 0;
+}
+
+}  // namespace tsg
+
+namespace tsg {
+
+// algorithmic bytes of one FP64 (or float) Jacobi sweep over the active rows:
+// per entry the column index and the coupling, per row its node index,
+// degree, coupling sum, charge and the written potential, and every
+// referenced potential read once
+int Grid::sweep_stats(double* bytes, long long* rows, long long* entries, long long* referenced, double* ms,
+                      long long* sweeps, long long* launched) const
+{
+	const int rsz = f64_ ? 8 : 4;
+	long long nr = 0, ne = 0;
+	std::vector<uint8_t> ref(mesh_.n, 0);
+	for (int s = 0; s < next_spare_; s++) {
+		const int node = slot_node_[s];
+		if (node < 0) continue;
+		nr++;
+		ref[node] = 1;
+		for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++)
+			if (participates(node, mesh_.idx[k])) { ne++; ref[mesh_.idx[k]] = 1; }
+	}
+	long long nref = 0;
+	for (uint8_t v : ref) nref += v;
+	if (bytes) *bytes = (double) ne * (4 + rsz) + (double) nr * (4 + 4 + 8 + 8 + rsz) + (double) nref * rsz;
+	if (rows) *rows = nr;
+	if (entries) *entries = ne;
+	if (referenced) *referenced = nref;
+	if (ms) *ms = stat_sweep_ms_;
+	if (sweeps) *sweeps = stat_sweeps_;
+	if (launched) *launched = stat_launched_;
+	return 0;
+}
+
+}  // namespace tsg

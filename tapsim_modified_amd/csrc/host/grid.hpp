@@ -1,0 +1,185 @@
+// Host side of the device grid: the counterpart of Grid_3d::Table
+// (tapsim/grid_3d.h:100-295) together with the surface table and the
+// per-event loop of Process::evaporation that drive it.  Hot data lives on
+// the device; the host keeps the small mirrors needed to patch it when an
+// atom is removed (flags, ids, couplings, row layout, surface set).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../kernels/kernels.hpp"
+#include "mesh.hpp"
+
+namespace tsg {
+
+struct Species {
+	int defined = 0;
+	float charge_density = 0, epsilon = 1;
+	int removable = 0, neumann = 0, dirichlet = 0;
+	float phi = 0, mass = 0;
+	int evap_charge = 0;
+	float evap_field = 0, evap_energy = 0;
+};
+
+// one evaporation event as recorded by File_Io::LogResults::Dataset
+// (tapsim/logResults.h:41-66) plus the relaxation counters
+struct EventRecord {
+	int node, id;
+	float probability;
+	double pot_before, field_before[3];
+	double pot_after, field_after[3];
+	long long local_iter, global_iter;
+	int surface_size;
+	double scaling_ref;
+	int near_tie;
+};
+
+struct EvapOptions {
+	int prob_mode = 0, evap_mode = 0, vacuum_id = 0;
+	double relax_threshold = 1e-2;
+	int relax_shell = 3, relax_cycle = 5, relax_queue = 5;
+	int relax_global_cycles = 1;
+	int refresh_interval = 0;
+	double refresh_threshold = 0.0;
+	int refresh_cycle = 1, refresh_queue = 0;
+	int exact_probability = 1;  // compute the sequential normaliser every event
+};
+
+class DevBuf {
+public:
+	DevBuf() = default;
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	~DevBuf();
+	void alloc(size_t bytes);
+	void free();
+	template <typename T> T* as() const { return (T*) p_; }
+	void* ptr() const { return p_; }
+	size_t bytes() const { return n_; }
+
+private:
+	void* p_ = nullptr;
+	size_t n_ = 0;
+};
+
+class Grid {
+public:
+	Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* species, float temperature);
+	~Grid();
+
+	int status() const { return status_; }
+	const std::string& error() const { return err_; }
+	bool f64() const { return f64_; }
+	int n() const { return mesh_.n; }
+	long long nnz() const { return (long long) mesh_.idx.size(); }
+	int nrows() const { return nrows_; }
+
+	// Grid_3d::Table::fastSync (grid_3d.cpp:491-553); runs at construction
+	int sync();
+	// Grid_3d::Table::fastResync after setId (grid_3d.cpp:555-589)
+	int set_id_and_resync(int node, int id);
+
+	// Grid_3d::Table::relax(cycleSize) (grid_3d.cpp:591-609)
+	int relax_cycles(int cycles, double* deviation);
+	// Grid_3d::Table::relax(threshold, cycleSize, queueSize) (grid_3d.cpp:611-671)
+	int relax_threshold(double threshold, int cycles, int queue, long long* iterations, double* fault_dev,
+	                    double* fault_slope, double* last_dev);
+	// Grid_3d::Table::localRelax(index, order, threshold, cycleSize, queueSize) (grid_3d.cpp:707-795)
+	int local_relax(int node, int order, double threshold, int cycles, int queue, long long* iterations);
+
+	int get_phi(int buffer, double* out);
+	int set_phi(const double* phi0, const double* phi1, int phi_switch, int local_switch);
+	int phi_switch() const { return sw_; }
+	int local_switch() const { return lsw_; }
+	int potential(int node, double* out);
+
+	int field_o1(int count, const int* nodes, double* out);
+	int get_couplings(double* out) const;
+	int get_flags(unsigned char* out) const;
+	int get_ids(int16_t* out) const;
+	int get_areas(float* a_half32, double* a_half64, float* a_one32) const;
+
+	// surface (Surface_3d::Table, surface_3d.cpp:50-119) and selection
+	int surface_init(int vacuum_id);
+	int surface_size() const { return (int) surface_.size(); }
+	int surface_nodes(int* out) const;
+	// evap_compute_specificFields + evap_compute_probabilities(linear field)
+	int surface_evaluate(bool exact_sum, double* scaling_ref);
+	int surface_values(double* out);                  // |E|/F per site, site order
+	int surface_probabilities(float* out);            // normalised as the reference
+	int surface_candidate(int* node, int* near_tie);  // evapMaximum
+	// evap_takeAway (surface_3d.cpp:507-564)
+	int take_away(int node);
+
+	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
+	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done);
+
+	hipStream_t stream() const { return stream_; }
+	// algorithmic bytes per sweep and the accumulated sweep timing (HIP events
+	// around the threshold relaxations on the grid's stream)
+	int sweep_stats(double* bytes, long long* rows, long long* entries, long long* referenced, double* ms,
+	                long long* sweeps, long long* launched) const;
+	// device pointers for the benchmark's own timing of the sweep kernel
+	SweepArgs sweep_args();
+
+private:
+	int fail(int code, const std::string& msg);
+	int hip_check(hipError_t e, const char* where);
+	double compute_coupling(int n1, int n2, long long entry) const;
+	bool participates(int row, int col) const;
+	void row_entries(int node, std::vector<int>* cols, std::vector<double>* coups, double* ws) const;
+	int build_rows();
+	int upload_surface();
+	int write_slot(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* ar,
+	               std::vector<double>* vr, std::vector<long long>* aw, std::vector<double>* vw);
+	long long entry_index(int row, int col) const;
+	int arm_ctl(double threshold, int cycles, int queue);
+	int resolve_candidate(int* node, int* near_tie);
+
+	int mode_;
+	bool f64_;
+	int status_ = 0;
+	std::string err_;
+	Mesh mesh_;
+	std::vector<int16_t> ids_;
+	std::vector<uint8_t> flags_;
+	Species species_[128];
+	float temperature_;
+	std::vector<long long> row_of_entry_;  // row index of every directed entry (int64 for >2^31)
+	std::vector<long long> mirror_;        // entry of (j,i) for entry (i,j)
+	std::vector<float> a_half32_, a_one32_, dist32_;
+	std::vector<double> a_half64_, dist64_;
+	std::vector<double> coup_;             // host mirror (values exact in R)
+	std::vector<double> charge_;
+	int sw_ = 0, lsw_ = 0;
+
+	// rows (SELL-64)
+	int nrows_ = 0, nslots_ = 0, cap_slots_ = 0, next_spare_ = 0, spare_width_ = 0;
+	std::vector<int> slot_of_, slot_node_, slot_deg_, slice_width_;
+	std::vector<long long> slice_off_;
+
+	std::set<int> surface_;
+	int vacuum_id_ = -1;
+	bool surface_dirty_ = true;
+	std::vector<int> surface_vec_;
+	double scaling_ref_ = 1.0;
+
+	hipStream_t stream_ = nullptr;
+	DevBuf d_xyz_, d_ptr_, d_idx_, d_coupfull_, d_flags_, d_evapF_, d_phi0_, d_phi1_;
+	DevBuf d_slot_node_, d_slot_deg_, d_slice_off_, d_col_, d_coup_, d_ws_, d_q_, d_ctl_;
+	DevBuf d_surf_, d_surf_val_, d_argmax_, d_maxval_, d_sum_, d_status_;
+	DevBuf d_shell_nodes_, d_shell_slots_, d_level_ptr_, d_order_, d_local_out_;
+	DevBuf d_patch_addr_, d_patch_i_, d_patch_r_, d_nodes_, d_fields_;
+	void* h_pinned_ = nullptr;
+	size_t h_pinned_bytes_ = 0;
+	hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+	double stat_sweep_ms_ = 0.0;
+	long long stat_sweeps_ = 0, stat_launched_ = 0, last_sweeps_ = 0;
+};
+
+}  // namespace tsg

@@ -1,0 +1,109 @@
+// Host-visible launchers of the device kernels (one translation unit per
+// kernel family; the host Grid calls only these).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tsg {
+
+// ---------------------------------------------------------------- geometry
+int launch_circumcentres(hipStream_t, int m, const float* xyz, const int* tv, double* cc64, float* cc32, int* status);
+int launch_edge_areas(hipStream_t, int n, long long nnz, const float* xyz, const int* tv, const int* tn,
+                      const int* assoc, const int* nt_ptr, const int* nt_idx, const int* ptr, const int* idx,
+                      const int* row_of, const double* cc64, const float* cc32, float* a_half32, double* a_half64,
+                      float* a_one32, float* dist32, double* dist64, int* status);
+
+// ---------------------------------------------------------------- relaxation
+
+// Device-resident control block of a threshold relaxation
+// (Grid_3d::Table::relax(threshold, cycleSize, queueSize), grid_3d.cpp:611-671).
+struct RelaxCtl {
+	int done;                 // 0 running, 1 converged, 2 RelaxationFault, 3 NaN
+	int nblocks;              // blocks per sweep launch
+	unsigned int counter;     // blocks finished in the current sweep
+	int queue_size;
+	unsigned long long devbits;  // running max |dphi| of the current sweep (double bits)
+	long long iteration;      // iterations counted as the reference does
+	long long sweeps;         // sweeps executed since the control block was armed
+	double threshold;
+	double last_dev;
+	double fault_dev, fault_slope;
+	int cycle;                // cycle size (1: every sweep is one relax(1) call)
+	int qhead, qsize;
+	int pad;
+	double queue[1024];
+};
+
+struct SweepArgs {
+	int nslots;
+	const int* slot_node;
+	const int* slot_deg;
+	const long long* slice_off;
+	const int* col;
+	const void* coup;         // R per entry
+	const double* ws;         // per slot
+	const double* q;          // per slot
+	void* phi0;               // R per node
+	void* phi1;
+	RelaxCtl* ctl;
+	int threshold_mode;       // 1: use ctl convergence/early exit; 0: plain fixed sweeps
+};
+
+// One Jacobi cycle (read buffer `read`, write the other); the deviation
+// max|phi_new - phi_old| over the rows is reduced into ctl.
+int launch_jacobi_sweep(hipStream_t, const SweepArgs&, int read, bool f64);
+
+// Local relaxation (Grid_3d::Table::localRelax(index, order, threshold,
+// cycleSize, queueSize), grid_3d.cpp:707-795) of a node shell in one
+// workgroup: the shell (ascending node index, Dirichlet removed) with its
+// level schedule for the in-place cycles.
+struct LocalArgs {
+	int count;
+	const int* nodes;         // shell nodes (ascending)
+	const int* slots;         // their row slots
+	const int* level_ptr;     // in-place levels: positions level_ptr[l]..level_ptr[l+1] of order[]
+	const int* order;         // shell positions grouped by level
+	int nlevels;
+	int read_buf;             // _phiSwitch
+	int local_switch;         // _phiLocalSwitch on entry
+	int cycle, queue;
+	double threshold;
+	long long max_iterations;
+	long long* out;           // [0] iterations (or -1-iteration on fault), [1] local switch after, [2] status
+};
+int launch_local_relax(hipStream_t, const SweepArgs&, const LocalArgs&, bool f64);
+
+// generic scatter of patches: dst[addr[k]] = val[k]
+int launch_scatter_f64(hipStream_t, int n, const long long* addr, const double* val, double* dst);
+int launch_scatter_f32(hipStream_t, int n, const long long* addr, const float* val, float* dst);
+int launch_scatter_i32(hipStream_t, int n, const long long* addr, const int* val, int* dst);
+
+// ---------------------------------------------------------------- surface
+struct FieldArgs {
+	int count;
+	const int* nodes;         // surface nodes, ascending
+	const float* xyz;
+	const int* ptr;           // full node CSR (ascending neighbours)
+	const int* idx;
+	const void* coup;         // R per full-CSR entry
+	const unsigned char* flags;
+	const void* phi;          // potential() buffer
+	const float* evap_field;  // per node
+	void* value;              // R per surface slot: |E| / F
+	float* field;             // optional 3 floats per slot (E vector) or null
+	int* status;
+};
+int launch_surface_fields(hipStream_t, const FieldArgs&, bool f64);
+
+// argmax with first-index tie-break plus the near-tie flag
+int launch_surface_argmax(hipStream_t, int count, const void* value, bool f64, long long* out /* [slot, tie_flag] */,
+                          double* maxval);
+// sequential sum in slot order (the reference's float/double normaliser)
+int launch_surface_sum(hipStream_t, int count, const void* value, bool f64, double* out);
+
+// field_o1 of arbitrary nodes (one thread each)
+int launch_node_fields(hipStream_t, int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
+                       const void* coup, const unsigned char* flags, const void* phi, bool f64, double* out, int* status);
+
+}  // namespace tsg

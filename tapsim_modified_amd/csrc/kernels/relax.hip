@@ -1,0 +1,284 @@
+// Relaxation kernels: one Jacobi cycle over the unknown rows (global
+// relaxation) and the whole local relaxation of a node shell.
+//
+// Storage: the unknown rows (non-Dirichlet nodes) live in row slots laid
+// out as SELL-64: slice s holds slots 64s..64s+63, entry k of the slot in
+// lane l sits at slice_off[s] + 64k + l, so a wavefront reads 64 consecutive
+// column indices and couplings per step.  A row lists the neighbours that
+// take part in Grid_3d::Table::computePotential (grid_3d.cpp:1317-1371) in
+// ascending node index -- all neighbours for a Neumann row, the non-Neumann
+// ones otherwise -- and its coupling sum is precomputed on the host in that
+// same order, so the value is bit-identical to the reference's
+//     lv = sum_k (double) phi[j_k] * c_k ;  lv -= charge ;  lv /= sum_k c_k
+// (double accumulation for both float and double storage; the build uses
+// -ffp-contract=off so no product is fused into an add).
+#include "device_math.hpp"
+#include "kernels.hpp"
+
+namespace tsg {
+namespace dev {
+
+__device__ __forceinline__ double wave_max(double v)
+{
+	for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+	return v;
+}
+
+// the deviation queue and least-squares slope test of
+// Grid_3d::Table::relax(threshold,cycleSize,queueSize), grid_3d.cpp:621-654;
+// returns true when the slope is exactly zero (RelaxationFault)
+__device__ bool push_and_test(double* queue, int* qhead, int* qsize, int qcap, int queueSize, int cycle, double dev,
+                              double* slope_out)
+{
+	if (queueSize <= 0) return false;
+	if (*qsize < qcap) {
+		queue[(*qhead + *qsize) % qcap] = dev;
+		(*qsize)++;
+	} else {
+		queue[*qhead] = dev;
+		*qhead = (*qhead + 1) % qcap;
+	}
+	while (*qsize > queueSize) { *qhead = (*qhead + 1) % qcap; (*qsize)--; }
+	if (*qsize != queueSize) return false;
+	const double xMean = (double) ((float) (queueSize - 1) / 2.0f);
+	double yMean = (double) 0.0f;
+	for (int i = 0; i < *qsize; i++) yMean += queue[(*qhead + i) % qcap];
+	yMean /= (double) (unsigned) queueSize;
+	double nom = (double) 0.0f;
+	for (int x = 0; x < *qsize; x++) nom += (x - xMean) * (queue[(*qhead + x) % qcap] - yMean);
+	double den = (double) (unsigned) queueSize;
+	den *= (double) (unsigned) queueSize;
+	den -= 1;
+	den *= (double) (unsigned) queueSize;
+	den *= (double) (unsigned) cycle;
+	den /= 12.0f;
+	const double slope = nom / den;
+	*slope_out = slope;
+	return slope == 0.0f;
+}
+
+// value of one row (computePotential) from the given read buffer
+template <typename R, bool kAtomicRead>
+__device__ __forceinline__ double row_value(const SweepArgs& a, int slot, const R* __restrict__ pin)
+{
+	const int deg = a.slot_deg[slot];
+	const long long base = a.slice_off[slot >> 6] + (slot & 63);
+	const R* coup = (const R*) a.coup;
+	double lv = 0.0;
+	for (int k = 0; k < deg; k++) {
+		const long long e = base + 64ll * k;
+		const int j = a.col[e];
+		const double c = (double) coup[e];
+		double adj;
+		if constexpr (kAtomicRead)
+			adj = (double) __hip_atomic_load(pin + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		else
+			adj = (double) pin[j];
+		adj *= c;
+		lv += adj;
+	}
+	lv -= a.q[slot];
+	lv /= a.ws[slot];
+	return lv;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
+{
+	RelaxCtl* ctl = a.ctl;
+	if (ctl->done) return;  // uniform per launch: converged earlier in this batch
+	const R* pin = (const R*) (read ? a.phi1 : a.phi0);
+	R* pout = (R*) (read ? a.phi0 : a.phi1);
+	const int slot = blockIdx.x * 256 + threadIdx.x;
+	double dev = 0.0;
+	int nan = 0;
+	if (slot < a.nslots) {
+		const int node = a.slot_node[slot];
+		if (node >= 0) {
+			const double lv = row_value<R, false>(a, slot, pin);
+			if (lv != lv) nan = 1;
+			const R nv = (R) lv;
+			pout[node] = nv;
+			// relax(cycleSize): |phi(0) - phi(1)| in double (grid_3d.cpp:597-605)
+			double ld = (double) nv;
+			ld -= (double) pin[node];
+			dev = fabs(ld);
+		}
+	}
+	__shared__ double smax[4];
+	__shared__ int snan[4];
+	__shared__ int last;
+	dev = wave_max(dev);
+	const int anynan = __any(nan);
+	const int w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0) { smax[w] = dev; snan[w] = anynan; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const double m = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+		if (snan[0] | snan[1] | snan[2] | snan[3]) atomicOr(&ctl->pad, 1);
+		atomicMax(&ctl->devbits, (unsigned long long) __double_as_longlong(m));
+		__threadfence();
+		const unsigned int ticket = atomicAdd(&ctl->counter, 1u);
+		last = (ticket == gridDim.x - 1);
+	}
+	__syncthreads();
+	if (!last || threadIdx.x != 0) return;
+	__threadfence();
+	const double d = __longlong_as_double((long long) atomicAdd(&ctl->devbits, 0ull));
+	ctl->devbits = 0ull;
+	ctl->counter = 0u;
+	ctl->sweeps += 1;
+	ctl->last_dev = d;
+	if (ctl->pad & 1) { ctl->done = 3; return; }
+	double slope = 0.0;
+	if (push_and_test(ctl->queue, &ctl->qhead, &ctl->qsize, 1024, ctl->queue_size, ctl->cycle, d, &slope)) {
+		ctl->fault_dev = d;
+		ctl->fault_slope = slope;
+		ctl->done = 2;
+		return;
+	}
+	ctl->iteration += ctl->cycle;
+	if (!(d > ctl->threshold)) ctl->done = 1;
+}
+
+// Whole localRelax(index, order, threshold, cycleSize, queueSize) in one
+// workgroup.  doLocalRelax (grid_3d.cpp:1260-1282, 1516-1575) reads buffer
+// _phiSwitch and writes the buffer after _phiLocalSwitch, which toggles per
+// cycle: a cycle whose write buffer differs from the read buffer is Jacobi,
+// otherwise it updates in place in ascending node order (one reference
+// thread).  The in-place order is reproduced exactly with a level schedule:
+// a node's level exceeds the levels of all its lower-index shell neighbours,
+// so every node reads exactly the values the sequential sweep would read.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_local(SweepArgs a, LocalArgs L)
+{
+	R* phi[2] = {(R*) a.phi0, (R*) a.phi1};
+	const R* pr = phi[L.read_buf];
+	__shared__ double sred[16];
+	__shared__ int sflag;
+	__shared__ double squeue[1024];
+	__shared__ int qhead, qsize;
+	__shared__ long long iteration;
+	__shared__ int ps, status;
+	if (threadIdx.x == 0) {
+		qhead = 0; qsize = 0; iteration = 0; ps = L.local_switch; status = 0;
+	}
+	__syncthreads();
+	for (;;) {
+		for (int c = 0; c < L.cycle; c++) {
+			const int wbuf = (ps + 1) % 2;
+			R* pw = phi[wbuf];
+			if (wbuf != L.read_buf) {
+				for (int p = threadIdx.x; p < L.count; p += blockDim.x) {
+					const double lv = row_value<R, true>(a, L.slots[p], pr);
+					if (lv != lv) atomicOr(&status, 3);
+					__hip_atomic_store(pw + L.nodes[p], (R) lv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				}
+				__syncthreads();
+			} else {
+				for (int l = 0; l < L.nlevels; l++) {
+					for (int q = L.level_ptr[l] + threadIdx.x; q < L.level_ptr[l + 1]; q += blockDim.x) {
+						const int p = L.order[q];
+						const double lv = row_value<R, true>(a, L.slots[p], pr);
+						if (lv != lv) atomicOr(&status, 3);
+						__hip_atomic_store(pw + L.nodes[p], (R) lv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+					}
+					__threadfence_block();
+					__syncthreads();
+				}
+			}
+			if (threadIdx.x == 0) ps = (ps + 1) % 2;
+			__syncthreads();
+		}
+		// deviation over the shell (grid_3d.cpp:748-757)
+		double dev = 0.0;
+		for (int p = threadIdx.x; p < L.count; p += blockDim.x) {
+			const int node = L.nodes[p];
+			double ld = (double) __hip_atomic_load(phi[0] + node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			ld -= (double) __hip_atomic_load(phi[1] + node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			ld = fabs(ld);
+			if (ld > dev) dev = ld;
+		}
+		dev = wave_max(dev);
+		if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = dev;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			double m = 0.0;
+			for (int w = 0; w < (int) (blockDim.x >> 6); w++) m = fmax(m, sred[w]);
+			double slope;
+			int stop = 0;
+			if (status) stop = 1;
+			else if (push_and_test(squeue, &qhead, &qsize, 1024, L.queue, L.cycle, m, &slope)) {
+				iteration = -1 - iteration;
+				stop = 1;
+			} else {
+				iteration += L.cycle;
+				if (!(m > L.threshold)) stop = 1;
+				else if (iteration >= L.max_iterations) { status = 4; stop = 1; }
+			}
+			sflag = stop;
+		}
+		__syncthreads();
+		if (sflag) break;
+	}
+	if (threadIdx.x == 0) {
+		L.out[0] = iteration;
+		L.out[1] = ps;
+		L.out[2] = status;
+	}
+}
+
+__global__ void k_scatter_f64(int n, const long long* addr, const double* val, double* dst)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dst[addr[i]] = val[i];
+}
+__global__ void k_scatter_f32(int n, const long long* addr, const float* val, float* dst)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dst[addr[i]] = val[i];
+}
+__global__ void k_scatter_i32(int n, const long long* addr, const int* val, int* dst)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dst[addr[i]] = val[i];
+}
+
+}  // namespace dev
+
+int launch_jacobi_sweep(hipStream_t s, const SweepArgs& a, int read, bool f64)
+{
+	const int blocks = (a.nslots + 255) / 256;
+	if (blocks == 0) return 0;
+	if (f64) hipLaunchKernelGGL(dev::k_jacobi<double>, dim3(blocks), dim3(256), 0, s, a, read);
+	else hipLaunchKernelGGL(dev::k_jacobi<float>, dim3(blocks), dim3(256), 0, s, a, read);
+	return (int) hipGetLastError();
+}
+
+int launch_local_relax(hipStream_t s, const SweepArgs& a, const LocalArgs& L, bool f64)
+{
+	if (f64) hipLaunchKernelGGL(dev::k_local<double>, dim3(1), dim3(1024), 0, s, a, L);
+	else hipLaunchKernelGGL(dev::k_local<float>, dim3(1), dim3(1024), 0, s, a, L);
+	return (int) hipGetLastError();
+}
+
+int launch_scatter_f64(hipStream_t s, int n, const long long* addr, const double* val, double* dst)
+{
+	if (n <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_scatter_f64, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
+	return (int) hipGetLastError();
+}
+int launch_scatter_f32(hipStream_t s, int n, const long long* addr, const float* val, float* dst)
+{
+	if (n <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_scatter_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
+	return (int) hipGetLastError();
+}
+int launch_scatter_i32(hipStream_t s, int n, const long long* addr, const int* val, int* dst)
+{
+	if (n <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_scatter_i32, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
+	return (int) hipGetLastError();
+}
+
+}  // namespace tsg

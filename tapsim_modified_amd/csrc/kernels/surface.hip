@@ -1,0 +1,179 @@
+// Surface evaluation: node fields by least squares (Grid_3d::Table::field_o1,
+// grid_3d.cpp:909-991), the evaporation score |E| / F_evap of every surface
+// site (Surface_3d::evap_compute_specificFields / probLinear_field,
+// surface_3d.cpp:194-219, 266-297), the first-maximum choice
+// (evapMaximum, surface_3d.cpp:423-431) and the sequential normaliser.
+#include "device_math.hpp"
+#include "kernels.hpp"
+
+namespace tsg {
+namespace dev {
+
+// field_o1 in element type R; returns false on a singular system
+template <typename R>
+__device__ bool field_o1(int node, const float* __restrict__ xyz, const int* __restrict__ ptr,
+                         const int* __restrict__ idx, const R* __restrict__ coup, const unsigned char* __restrict__ flags,
+                         const R* __restrict__ phi, R E[3])
+{
+	E[0] = E[1] = E[2] = (R) 0;
+	if (flags[node] & 0x01) return true;  // geometric boundary: zero field
+	const V3<R> pi = load_coord<R>(xyz, node);
+	const R p0 = phi[node];
+	R a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0;
+	R b[3] = {0, 0, 0};
+	// the reference forms each of the nine sums in its own loop over the same
+	// ascending neighbour list; the sums are independent, so one pass with
+	// nine accumulators performs exactly the same roundings
+	for (int k = ptr[node]; k < ptr[node + 1]; k++) {
+		if (coup[k] == (R) 0) continue;
+		const int j = idx[k];
+		const V3<R> c = vsub(load_coord<R>(xyz, j), pi);
+		a00 += c.x[0] * c.x[0];
+		a01 += c.x[0] * c.x[1];
+		a02 += c.x[0] * c.x[2];
+		a11 += c.x[1] * c.x[1];
+		a12 += c.x[1] * c.x[2];
+		a22 += c.x[2] * c.x[2];
+		const R dp = phi[j] - p0;
+		for (int i = 0; i < 3; i++) b[i] += dp * c.x[i];
+	}
+	R a[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
+	R d;
+	int perm[3];
+	if (!lu_decmp<R, 3>(a, perm, &d)) return false;
+	lu_solve<R, 3>(a, perm, b);
+	for (int i = 0; i < 3; i++) E[i] = (R) (-1.0f * b[i]);
+	return true;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_surface_fields(FieldArgs f)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= f.count) return;
+	const int node = f.nodes[s];
+	R E[3];
+	if (!field_o1<R>(node, f.xyz, f.ptr, f.idx, (const R*) f.coup, f.flags, (const R*) f.phi, E)) {
+		atomicOr(f.status, 1);
+		E[0] = E[1] = E[2] = (R) 0;
+	}
+	if (E[0] != E[0] || E[1] != E[1] || E[2] != E[2]) atomicOr(f.status, 2);
+	V3<R> v = v3<R>(E[0], E[1], E[2]);
+	R value = vlen(v);
+	value /= (R) f.evap_field[node];
+	((R*) f.value)[s] = value;
+	if (f.field)
+		for (int i = 0; i < 3; i++) f.field[3 * (size_t) s + i] = (float) E[i];
+}
+
+template <typename R>
+__global__ void k_node_fields(int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
+                              const R* coup, const unsigned char* flags, const R* phi, double* out, int* status)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= count) return;
+	R E[3];
+	if (!field_o1<R>(nodes[s], xyz, ptr, idx, coup, flags, phi, E)) atomicOr(status, 1);
+	for (int i = 0; i < 3; i++) out[3 * (size_t) s + i] = (double) E[i];
+}
+
+// first maximum in slot order (slots are in ascending node order); a slot
+// below the winner whose value lies within a few ulps of the maximum could
+// tie with it after the reference's float normalisation (v / sum), which
+// sets the near-tie flag for the host to resolve with the exact normaliser
+template <typename R>
+__global__ __launch_bounds__(1024) void k_argmax(int count, const R* __restrict__ v, long long* out, double* maxval)
+{
+	__shared__ R sv[1024];
+	__shared__ int ss[1024];
+	R best = (R) -1;
+	int bs = 0x7fffffff;
+	for (int s = threadIdx.x; s < count; s += blockDim.x) {
+		const R x = v[s];
+		if (x > best) { best = x; bs = s; }  // increasing s: ties keep the first
+	}
+	sv[threadIdx.x] = best;
+	ss[threadIdx.x] = bs;
+	__syncthreads();
+	for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+		if (threadIdx.x < h) {
+			const R o = sv[threadIdx.x + h];
+			const int os = ss[threadIdx.x + h];
+			if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && os < ss[threadIdx.x])) {
+				sv[threadIdx.x] = o;
+				ss[threadIdx.x] = os;
+			}
+		}
+		__syncthreads();
+	}
+	const R vmax = sv[0];
+	const int w = ss[0];
+	__shared__ int near;
+	if (threadIdx.x == 0) near = 0;
+	__syncthreads();
+	const R eps = (sizeof(R) == 4) ? (R) (8.0 / 16777216.0) : (R) (8.0 / 9007199254740992.0);
+	const R lim = vmax - vmax * eps;
+	for (int s = threadIdx.x; s < w; s += blockDim.x)
+		if (v[s] >= lim) near = 1;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		out[0] = w;
+		out[1] = near;
+		*maxval = (double) vmax;
+	}
+}
+
+// sequential sum in slot order: one wavefront loads 64 values at a time and
+// every lane accumulates them in lane order, reproducing the reference's
+// float (or double) running sum exactly
+template <typename R>
+__global__ __launch_bounds__(64) void k_seq_sum(int count, const R* __restrict__ v, double* out)
+{
+	R sum = (R) 0;
+	for (int base = 0; base < count; base += 64) {
+		const int s = base + threadIdx.x;
+		const R x = s < count ? v[s] : (R) 0;
+		const int lim = min(64, count - base);
+		for (int l = 0; l < lim; l++) sum += __shfl(x, l, 64);
+	}
+	if (threadIdx.x == 0) *out = (double) sum;
+}
+
+}  // namespace dev
+
+int launch_surface_fields(hipStream_t st, const FieldArgs& f, bool f64)
+{
+	if (f.count <= 0) return 0;
+	if (f64) hipLaunchKernelGGL(dev::k_surface_fields<double>, dim3((f.count + 255) / 256), dim3(256), 0, st, f);
+	else hipLaunchKernelGGL(dev::k_surface_fields<float>, dim3((f.count + 255) / 256), dim3(256), 0, st, f);
+	return (int) hipGetLastError();
+}
+
+int launch_surface_argmax(hipStream_t st, int count, const void* value, bool f64, long long* out, double* maxval)
+{
+	if (f64) hipLaunchKernelGGL(dev::k_argmax<double>, dim3(1), dim3(1024), 0, st, count, (const double*) value, out, maxval);
+	else hipLaunchKernelGGL(dev::k_argmax<float>, dim3(1), dim3(1024), 0, st, count, (const float*) value, out, maxval);
+	return (int) hipGetLastError();
+}
+
+int launch_surface_sum(hipStream_t st, int count, const void* value, bool f64, double* out)
+{
+	if (f64) hipLaunchKernelGGL(dev::k_seq_sum<double>, dim3(1), dim3(64), 0, st, count, (const double*) value, out);
+	else hipLaunchKernelGGL(dev::k_seq_sum<float>, dim3(1), dim3(64), 0, st, count, (const float*) value, out);
+	return (int) hipGetLastError();
+}
+
+int launch_node_fields(hipStream_t st, int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
+                       const void* coup, const unsigned char* flags, const void* phi, bool f64, double* out, int* status)
+{
+	if (count <= 0) return 0;
+	if (f64)
+		hipLaunchKernelGGL(dev::k_node_fields<double>, dim3((count + 63) / 64), dim3(64), 0, st, count, nodes, xyz, ptr,
+		                   idx, (const double*) coup, flags, (const double*) phi, out, status);
+	else
+		hipLaunchKernelGGL(dev::k_node_fields<float>, dim3((count + 63) / 64), dim3(64), 0, st, count, nodes, xyz, ptr,
+		                   idx, (const float*) coup, flags, (const float*) phi, out, status);
+	return (int) hipGetLastError();
+}
+
+}  // namespace tsg

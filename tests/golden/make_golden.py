@@ -1,0 +1,186 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+The reference TAPSim sources are compiled (never copied) by
+oracle/Makefile.ref into oracle/_ref/ref_driver, which runs the reference's own
+Grid_3d / Surface_3d / Process code on a small specimen generated and meshed
+by this repository.  Only inputs and the reference's outputs are stored.
+
+    python tests/golden/make_golden.py      (needs /root/reference and the built library)
+"""
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import tapsim_modified_amd as tsg  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+OUT = os.path.join(ROOT, "tests", "golden")
+
+# species of the fixture (ids as in the reference's examples/example.aly)
+SPECIES = {
+    0: dict(name="Vacuum", eps=1.0, rem=0, neu=0, dir=0, phi=0.0),
+    1: dict(name="Top", eps=1.0, rem=0, neu=0, dir=1, phi=0.0),
+    2: dict(name="Bottom", eps=1.0, rem=0, neu=1, dir=0, phi=0.0),
+    3: dict(name="Sides", eps=1.0, rem=0, neu=0, dir=1, phi=0.0),
+    10: dict(name="Matrix", eps=1.0e6, rem=1, neu=0, dir=1, phi=5000.0, mass=55.845, charge=2, field=3.0e10, energy=1.0),
+    11: dict(name="Solute", eps=1.0e6, rem=1, neu=0, dir=1, phi=5000.0, mass=58.69, charge=2, field=2.5e10, energy=1.0),
+}
+
+
+def aly_text(species, temperature=0.0):
+    out = ["ASCII", "", "TEMPERATURE = %.5e" % temperature]
+    for sid in sorted(species):
+        s = species[sid]
+        out += ["", "ID = %d" % sid, "NAME = %s" % s["name"], "CHARGE_DENSITY = 0.00000e+00",
+                "DIELECTRICITY = %.8e" % s["eps"], "REMOVABLE = %d" % s["rem"], "NEUMANN_BOUNDARY = %d" % s["neu"],
+                "DIRICHLET_BOUNDARY = %d" % s["dir"], "POTENTIAL = %.8e" % s["phi"],
+                "MASS = %.8e" % s.get("mass", 0.0), "EVAPORATION_CHARGE_STATE = %d" % s.get("charge", 0),
+                "EVAPORATION_FIELD_STRENGTH = %.8e" % s.get("field", 0.0),
+                "EVAPORATION_ACTIVATION_ENERGY = %.8e" % s.get("energy", 0.0)]
+    return "\n".join(out) + "\n"
+
+
+def write_input(path, xyz, ids, tets=None):
+    with open(path, "wb") as f:
+        f.write(b"TSGM")
+        f.write(struct.pack("<iii", 1, len(xyz), 0 if tets is None else len(tets)))
+        f.write(np.ascontiguousarray(xyz, "<f4").tobytes())
+        f.write(np.ascontiguousarray(ids, "<i2").tobytes())
+        if tets is not None:
+            f.write(np.ascontiguousarray(tets, "<i4").tobytes())
+
+
+def run(workdir, steps, tets=True):
+    r = subprocess.run([DRIVER, os.path.join(workdir, "in.bin"), os.path.join(workdir, "cfg.aly"), workdir] + steps,
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("ref_driver failed: %s\n%s" % (steps, r.stderr[-2000:]))
+    return r
+
+
+def rd(workdir, name, dtype):
+    return np.fromfile(os.path.join(workdir, name), dtype=dtype)
+
+
+def specimen(radius, shank, box, vac, spacing, seed):
+    p = tsg.SpecimenParams()
+    p.a = 0.287e-9
+    p.radius, p.shank = radius, shank
+    for k in range(3):
+        p.box[k] = box[k]
+    p.shell, p.vacuum_target, p.decay, p.face_spacing = 2.0, vac, 3.0, spacing
+    p.cluster_density, p.cluster_radius, p.seed = 0.0, 2e-9, seed
+    p.id_vacuum, p.id_top, p.id_bottom, p.id_sides, p.id_matrix, p.id_solute = 0, 1, 2, 3, 10, 11
+    return tsg.generate_specimen(p)
+
+
+def make_small():
+    xyz, ids, counts = specimen(1.2e-9, 1.5e-9, (8e-9, 8e-9, 11e-9), 1500, 2.5e-9, 11)
+    # a few solute atoms so that two evaporation fields take part
+    rng = np.random.default_rng(5)
+    atoms = np.where(ids == 10)[0]
+    ids[rng.choice(atoms, size=len(atoms) // 10, replace=False)] = 11
+    mesh = tsg.Mesh(xyz)
+    tv, tn = mesh.tets()
+    fx = dict(xyz=xyz, ids=ids, tets=tv, neigh=tn)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        # A) topology, couplings, a few Jacobi cycles, fields
+        run(wd, ["mesh", "sync", "dumpmesh", "dumpgrid", "vorarea:one", "dumpphi:init", "relax:6:1", "dumpphi:r6",
+                 "field:r6"])
+        fx["ref_neigh"] = rd(wd, "neigh.i32", np.int32).reshape(-1, 4)
+        fx["ref_assoc"] = rd(wd, "assoc.i32", np.int32)
+        fx["ref_boundary"] = rd(wd, "boundary.u8", np.uint8)
+        fx["ref_ptr"] = rd(wd, "nbr_ptr.i32", np.int32)
+        fx["ref_idx"] = rd(wd, "nbr_idx.i32", np.int32)
+        fx["ref_coupling"] = rd(wd, "coupling.f32", np.float32)
+        fx["ref_flags"] = rd(wd, "flags.u8", np.uint8)
+        fx["ref_area_one"] = rd(wd, "vorarea_one.f32", np.float32)
+        fx["ref_dev_r6"] = rd(wd, "relax_dev.f64", np.float64)
+        fx["ref_phi0_r6"] = rd(wd, "phi0_r6.f32", np.float32)
+        fx["ref_phi1_r6"] = rd(wd, "phi1_r6.f32", np.float32)
+        fx["ref_field_r6"] = rd(wd, "field_o1_r6.f32", np.float32).reshape(-1, 3)
+        fx["ref_field2_r6"] = rd(wd, "field_o2_r6.f32", np.float32).reshape(-1, 3)
+        fx["ref_charge2_r6"] = rd(wd, "charge_o2_r6.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        # B) threshold relaxation with a deviation queue, and in-place cycles
+        run(wd, ["mesh", "sync", "relaxthr:0.5:1:4", "dumpphi:thr", "relax:3:4", "dumpphi:c4"])
+        fx["ref_thr_iter"] = rd(wd, "relaxthr_iter.i64", np.int64)
+        fx["ref_phi0_thr"] = rd(wd, "phi0_thr.f32", np.float32)
+        fx["ref_phi1_thr"] = rd(wd, "phi1_thr.f32", np.float32)
+        fx["ref_dev_c4"] = rd(wd, "relax_dev.f64", np.float64)
+        fx["ref_phi0_c4"] = rd(wd, "phi0_c4.f32", np.float32)
+        fx["ref_phi1_c4"] = rd(wd, "phi1_c4.f32", np.float32)
+    # the atom with the largest z among the removable atoms (the apex)
+    rem = np.where(np.isin(ids, [10, 11]))[0]
+    node = int(rem[np.argmax(xyz[rem, 2])])
+    fx["local_node"] = np.int32(node)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        # C) an atom becomes vacuum (fastResync), then localRelax on its shell
+        run(wd, ["mesh", "sync", "relax:40:1", "evapnode:%d" % node, "dumpgrid", "local:%d:3:1e-2:5:5" % node,
+                 "dumpphi:loc"])
+        fx["ref_coupling_resync"] = rd(wd, "coupling.f32", np.float32)
+        fx["ref_local_iter"] = rd(wd, "local_iter.i64", np.int64)
+        fx["ref_phi0_loc"] = rd(wd, "phi0_loc.f32", np.float32)
+        fx["ref_phi1_loc"] = rd(wd, "phi1_loc.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        # D) Process::evaporation: 25 events with local relaxation, one global
+        #    cycle and a refresh relaxation every 5 events; Cash-Karp flights
+        run(wd, ["mesh", "sync", "relax:300:1",
+                 "evap:25:0:0:1e-2:3:5:5:1:5:1e-3:1:0:1:1", "dumpphi:evap"])
+        ints = rd(wd, "evap_int.i32", np.int32).reshape(-1, 4)
+        fx["ref_evap_node"], fx["ref_evap_id"] = ints[:, 0], ints[:, 1]
+        fx["ref_evap_trajlen"], fx["ref_evap_surface"] = ints[:, 2], ints[:, 3]
+        fx["ref_evap_iter"] = rd(wd, "evap_iter.i64", np.int64).reshape(-1, 2)
+        fx["ref_evap_f"] = rd(wd, "evap_f.f32", np.float32).reshape(-1, 22)
+        fx["ref_phi0_evap"] = rd(wd, "phi0_evap.f32", np.float32)
+        fx["ref_phi1_evap"] = rd(wd, "phi1_evap.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        # E) one trajectory from the apex atom after relaxation (O5 + error-restricted)
+        run(wd, ["mesh", "sync", "relax:300:1", "evapnode:%d" % node, "relax:1:1", "traj:t:%d:1:1" % node])
+        fx["ref_traj"] = rd(wd, "traj_t.f32", np.float32).reshape(-1, 8)
+        fx["ref_traj_status"] = rd(wd, "traj_status_t.i32", np.int32)
+    np.savez_compressed(os.path.join(OUT, "tip_small.npz"), **fx)
+    print("tip_small: nodes %d tets %d counts %s" % (len(xyz), len(tv), counts))
+
+
+def make_tetgen_mesh():
+    """Delaunay fixture: TetGen's tetrahedralisation of random points in a
+    unit-ish box (general position) -- my mesher must reproduce it exactly."""
+    rng = np.random.default_rng(3)
+    pts = (rng.random((600, 3)) * 4e-9).astype(np.float32)
+    # hull corners jittered off the cube so that no five points are cospherical
+    corners = np.array([[x, y, z] for x in (-1e-9, 5e-9) for y in (-1e-9, 5e-9) for z in (-1e-9, 5e-9)])
+    corners = (corners + (rng.random(corners.shape) - 0.5) * 0.4e-9).astype(np.float32)
+    xyz = np.vstack([corners, pts])
+    ids = np.zeros(len(xyz), np.int16)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids)
+        run(wd, ["mesh", "dumpmesh"])
+        tets = rd(wd, "tets.i32", np.int32).reshape(-1, 4)
+    np.savez_compressed(os.path.join(OUT, "tetgen_random.npz"), xyz=xyz, tets=tets)
+    print("tetgen_random: %d tets" % len(tets))
+
+
+if __name__ == "__main__":
+    if not os.path.exists(DRIVER):
+        subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref"], cwd=ROOT, check=True)
+    make_tetgen_mesh()
+    make_small()

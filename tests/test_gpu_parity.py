@@ -1,0 +1,178 @@
+"""GPU parity: the HIP path through the C ABI against the reference fixtures
+(float-faithful mode) and against the oracle (plain FP64 mode).  Bit-exact
+unless stated."""
+import sys
+
+import numpy as np
+import pytest
+
+import tapsim_modified_amd as tsg
+from oracle import oracle as O
+from tests.conftest import FIXTURE_SPECIES
+
+pytestmark = pytest.mark.gpu
+
+F1E2 = float(np.float32(1e-2))
+F1E3 = float(np.float32(1e-3))
+
+
+def log(msg):
+    print("[gpu-test] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def grid(fx, mode):
+    mesh = tsg.Mesh(fx["xyz"], fx["tets"], fx["ref_neigh"])
+    return tsg.Grid3D(mesh, fx["ids"], tsg.species_table(FIXTURE_SPECIES), 0.0, mode)
+
+
+def oracle(fx, mode):
+    return O.Oracle(mode, fx["xyz"], fx["ids"], fx["tets"], fx["ref_neigh"], O.species_table(FIXTURE_SPECIES))
+
+
+def same_bits(a, b, dtype):
+    a = np.asarray(a, dtype)
+    b = np.asarray(b, dtype)
+    it = np.uint32 if dtype == np.float32 else np.uint64
+    return np.array_equal(a.view(it), b.view(it))
+
+
+def test_geometry_precompute_f32_matches_reference(tip_small):
+    log("geometry f32 tip_small")
+    g = grid(tip_small, tsg.FAITHFUL_F32)
+    assert same_bits(g.couplings(), tip_small["ref_coupling"], np.float32)
+    h32, h64, o32 = g.areas()
+    assert same_bits(o32, tip_small["ref_area_one"], np.float32)
+    assert np.array_equal(g.flags(), tip_small["ref_flags"])
+
+
+def test_geometry_precompute_f64_matches_oracle(tip_small):
+    log("geometry f64 tip_small")
+    g = grid(tip_small, tsg.PLAIN_F64)
+    o = oracle(tip_small, O.F64)
+    o.sync()
+    assert same_bits(g.couplings(), o.couplings(), np.float64)
+
+
+@pytest.mark.parametrize("mode", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_jacobi_cycles(tip_small, mode):
+    log("jacobi mode %d" % mode)
+    g = grid(tip_small, mode)
+    o = oracle(tip_small, O.F32 if mode == tsg.FAITHFUL_F32 else O.F64)
+    o.sync()
+    dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
+    for k in range(6):
+        dg = g.relax(1)
+        do = o.relax(1)
+        assert dg == do, (k, dg, do)
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+    assert g.switches()[0] == o.switches()[0]
+    if mode == tsg.FAITHFUL_F32:
+        assert same_bits(g.phi(0), tip_small["ref_phi0_r6"], np.float32)
+        assert np.array_equal([g.relax(1) for _ in range(0)], [])
+    nodes = np.arange(g.n, dtype=np.int32)
+    fg = g.field_o1(nodes)
+    fo = np.array([o.field_o1(i) for i in range(g.n)])
+    assert same_bits(fg, fo, dt)
+    if mode == tsg.FAITHFUL_F32:
+        assert same_bits(fg, tip_small["ref_field_r6"], np.float32)
+
+
+@pytest.mark.parametrize("mode", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_threshold_relaxation_with_queue(tip_small, mode):
+    log("threshold mode %d" % mode)
+    g = grid(tip_small, mode)
+    o = oracle(tip_small, O.F32 if mode == tsg.FAITHFUL_F32 else O.F64)
+    o.sync()
+    dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
+    it_g = g.relax_threshold(0.5, 1, 4)
+    it_o, _, _ = o.relax_threshold(0.5, 1, 4)
+    assert it_g == it_o
+    if mode == tsg.FAITHFUL_F32:
+        assert it_g == tip_small["ref_thr_iter"][0]
+        assert same_bits(g.phi(0), tip_small["ref_phi0_thr"], np.float32)
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+    # a tighter threshold from there on
+    it_g = g.relax_threshold(1e-4, 1, 0)
+    it_o, _, _ = o.relax_threshold(1e-4, 1, 0)
+    assert it_g == it_o
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+
+
+@pytest.mark.parametrize("mode", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_resync_and_local_relax(tip_small, mode):
+    log("local relax mode %d" % mode)
+    node = int(tip_small["local_node"])
+    g = grid(tip_small, mode)
+    o = oracle(tip_small, O.F32 if mode == tsg.FAITHFUL_F32 else O.F64)
+    o.sync()
+    dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
+    for _ in range(40):
+        g.relax(1)
+        o.relax(1)
+    g.set_id(node, 0)
+    o.set_id(node, 0)
+    o.resync(node)
+    assert same_bits(g.couplings(), o.couplings(), dt)
+    it_g = g.local_relax(node, 3, F1E2, 5, 5)
+    it_o = o.local_relax_threshold(node, 3, F1E2, 5, 5)
+    assert it_g == it_o
+    assert g.switches() == o.switches()
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+    if mode == tsg.FAITHFUL_F32:
+        assert same_bits(g.couplings(), tip_small["ref_coupling_resync"], np.float32)
+        assert it_g == tip_small["ref_local_iter"][0]
+        assert same_bits(g.phi(0), tip_small["ref_phi0_loc"], np.float32)
+        assert same_bits(g.phi(1), tip_small["ref_phi1_loc"], np.float32)
+    # the global cycles after the local relaxation read the new row
+    for _ in range(3):
+        assert g.relax(1) == o.relax(1)
+    assert same_bits(g.phi(0), o.phi(0), dt)
+
+
+def evap_opts(vac=0):
+    e = tsg.EvapOptions()
+    e.prob_mode, e.evap_mode, e.vacuum_id = 0, 0, vac
+    e.relax_threshold, e.relax_shell, e.relax_cycle, e.relax_queue = F1E2, 3, 5, 5
+    e.relax_global_cycles = 1
+    e.refresh_interval, e.refresh_threshold, e.refresh_cycle, e.refresh_queue = 5, F1E3, 1, 0
+    e.exact_probability = 1
+    return e
+
+
+def ora_opts():
+    return O.OraEvapOpts(prob_mode=0, evap_mode=0, vacuum_id=0, relax_threshold=1e-2, relax_shell=3, relax_cycle=5,
+                         relax_queue=5, relax_global_cycles=1, refresh_interval=5, refresh_threshold=1e-3,
+                         refresh_cycle=1, refresh_queue=0, trajectory=0, integrator=1, stepper=1, dt0=1e-14,
+                         dt_min=1e-16, err_limit=1e-6, voltage_queue=250, shrink_limit=1.0, inplace_order=0)
+
+
+@pytest.mark.parametrize("mode", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_evaporation_sequence(tip_small, mode):
+    log("evaporation mode %d" % mode)
+    g = grid(tip_small, mode)
+    o = oracle(tip_small, O.F32 if mode == tsg.FAITHFUL_F32 else O.F64)
+    o.sync()
+    for _ in range(300):
+        g.relax(1)
+        o.relax(1)
+    evg = g.evaporate(evap_opts(), 25)
+    evo = o.evaporate(ora_opts(), 25)
+    assert [e.node for e in evg] == [e.node for e in evo]
+    assert [e.probability for e in evg] == [e.probability for e in evo]
+    fdt = np.float32
+    assert same_bits([e.pot_before for e in evg], [e.pot_before for e in evo], fdt)
+    assert same_bits([list(e.field_before) for e in evg], [list(e.field_before) for e in evo], fdt)
+    assert same_bits([e.pot_after for e in evg], [e.pot_after for e in evo], fdt)
+    assert [e.local_iter for e in evg] == [e.local_iter for e in evo]
+    assert [e.global_iter for e in evg] == [e.refresh_iter for e in evo]
+    dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+    if mode == tsg.FAITHFUL_F32:
+        F = tip_small["ref_evap_f"]
+        assert np.array_equal([e.node for e in evg], tip_small["ref_evap_node"])
+        assert same_bits([e.probability for e in evg], F[:, 0], np.float32)
+        assert same_bits([e.pot_before for e in evg], F[:, 1], np.float32)
+        assert same_bits([list(e.field_before) for e in evg], F[:, 2:5], np.float32)
+        assert same_bits([e.pot_after for e in evg], F[:, 5], np.float32)
+        assert same_bits(g.phi(0), tip_small["ref_phi0_evap"], np.float32)
