@@ -159,6 +159,8 @@ typedef struct tsg_evap_options
 	double refresh_threshold;
 	int refresh_cycle, refresh_queue;
 	int exact_probability;
+	int method;               /* 0: the reference's relaxation (steps III as process.cpp:824-900), 1: accelerated */
+	double accel_tol;         /* accelerated: max |J(phi) - phi| target, V */
 } tsg_evap_options;
 
 typedef struct tsg_event
@@ -176,6 +178,21 @@ typedef struct tsg_event
 /* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k
    events; *done is the number recorded in out */
 int tsg_evaporate(tsg_grid*, const tsg_evap_options*, int k, tsg_event* out, int* done);
+
+/* Accelerated relaxation (no reference counterpart; replaces step III's
+   relaxations when selected): FP32 Chebyshev-accelerated Jacobi correction
+   solves inside FP64 residual refinement until max|J(phi)-phi| <= tol, where
+   J is one Jacobi sweep (the reference's deviation for that sweep).  Both
+   potential buffers then hold the result.  setup estimates the spectral
+   radius (power iteration) with the given safety factor (0 < safety <= 1). */
+int tsg_accel_setup(tsg_grid*, double safety, double* rho);
+int tsg_accel_relax(tsg_grid*, double tol, int* outer, long long* inner, double* final_residual);
+
+/* Measurement: average device time of one launch (HIP events on the grid's
+   stream around reps launches) of kind 0 = the Jacobi cycle, kind 1 = the
+   FP32 Chebyshev correction step, and that kernel's algorithmic bytes per
+   launch.  Potentials are restored afterwards. */
+int tsg_time_kernel(tsg_grid*, int kind, int reps, double* avg_ms, double* bytes);
 
 /* Measurement: algorithmic bytes of one sweep (column index + coupling per
    entry; node, degree, coupling sum, charge and written potential per row;

@@ -388,8 +388,9 @@ typedef struct { double v[4096]; int head, size; } ora_queue;
 static int queue_slope_zero(ora_queue* q, double dev, unsigned int qsize, unsigned int cycles, double* slope_out)
 {
 	if (qsize == 0) return 0;
-	if (q->size < (int) qsize) { q->v[(q->head + q->size) % 4096] = dev; q->size++; }
-	else { q->v[q->head] = dev; q->head = (q->head + 1) % 4096; }
+	/* std::deque push_back then pop_front while longer than queueSize */
+	q->v[(q->head + q->size) % 4096] = dev;
+	q->size++;
 	while (q->size > (int) qsize) { q->head = (q->head + 1) % 4096; q->size--; }
 	if (q->size != (int) qsize) return 0;
 	const double xMean = (double) ((float) (qsize - 1) / 2.0f);

@@ -72,7 +72,7 @@ class EvapOptions(ctypes.Structure):
         ("relax_threshold", ctypes.c_double), ("relax_shell", ctypes.c_int), ("relax_cycle", ctypes.c_int),
         ("relax_queue", ctypes.c_int), ("relax_global_cycles", ctypes.c_int), ("refresh_interval", ctypes.c_int),
         ("refresh_threshold", ctypes.c_double), ("refresh_cycle", ctypes.c_int), ("refresh_queue", ctypes.c_int),
-        ("exact_probability", ctypes.c_int),
+        ("exact_probability", ctypes.c_int), ("method", ctypes.c_int), ("accel_tol", ctypes.c_double),
     ]
 
 
@@ -126,6 +126,9 @@ _SIGS = {
     "tsg_take_away": ([_P, ctypes.c_int], ctypes.c_int),
     "tsg_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_sweep_stats": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "tsg_accel_setup": ([_P, ctypes.c_double, _P], ctypes.c_int),
+    "tsg_time_kernel": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
+    "tsg_accel_relax": ([_P, ctypes.c_double, _P, _P, _P], ctypes.c_int),
 }
 
 
@@ -366,6 +369,21 @@ class Grid3D:
                                     ctypes.byref(ms), ctypes.byref(sw), ctypes.byref(la)))
         return dict(bytes=b.value, rows=r.value, entries=e.value, referenced=ref.value, ms=ms.value,
                     sweeps=sw.value, launched=la.value)
+
+    def time_kernel(self, kind, reps=200):
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        _chk(load().tsg_time_kernel(self._h, kind, reps, ctypes.byref(ms), ctypes.byref(b)))
+        return ms.value, b.value
+
+    def accel_setup(self, safety=0.5):
+        rho = ctypes.c_double()
+        _chk(load().tsg_accel_setup(self._h, safety, ctypes.byref(rho)))
+        return rho.value
+
+    def accel_relax(self, tol=1e-8):
+        outer, inner, res = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_double()
+        _chk(load().tsg_accel_relax(self._h, tol, ctypes.byref(outer), ctypes.byref(inner), ctypes.byref(res)))
+        return outer.value, inner.value, res.value
 
     def evaporate(self, options, k):
         out = (Event * max(1, k))()

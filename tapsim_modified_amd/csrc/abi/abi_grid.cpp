@@ -200,7 +200,29 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.refresh_interval = o->refresh_interval; e.refresh_threshold = o->refresh_threshold;
 	e.refresh_cycle = o->refresh_cycle; e.refresh_queue = o->refresh_queue;
 	e.exact_probability = o->exact_probability;
+	e.method = o->method;
+	e.accel_tol = o->accel_tol;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
+}
+
+int tsg_accel_setup(tsg_grid* h, double safety, double* rho)
+{
+	if (!h || !(safety > 0.0 && safety <= 1.0)) return tsg::fail(TSG_E_ARG, "tsg_accel_setup: bad arguments");
+	const int rc = check(h, h->g->accel_setup(safety));
+	if (!rc && rho) *rho = h->g->accel_rho();
+	return rc;
+}
+
+int tsg_accel_relax(tsg_grid* h, double tol, int* outer, long long* inner, double* final_residual)
+{
+	if (!h || !(tol > 0.0)) return tsg::fail(TSG_E_ARG, "tsg_accel_relax: bad arguments");
+	return check(h, h->g->accel_relax(tol, outer, inner, final_residual));
+}
+
+int tsg_time_kernel(tsg_grid* h, int kind, int reps, double* avg_ms, double* bytes)
+{
+	if (!h || !avg_ms) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->time_kernel(kind, reps, avg_ms, bytes));
 }
 
 int tsg_sweep_stats(const tsg_grid* h, double* bytes, long long* rows, long long* entries, long long* referenced,

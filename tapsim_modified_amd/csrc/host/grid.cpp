@@ -313,6 +313,8 @@ int Grid::build_rows()
 	d_coup_.alloc(rs * (size_t) nent);
 	d_ws_.alloc(sizeof(double) * cap_slots_);
 	d_q_.alloc(sizeof(double) * cap_slots_);
+	d_block_max_.alloc(sizeof(double) * (cap_slots_ / 256 + 2));
+	d_block_flag_.alloc(sizeof(int) * (cap_slots_ / 256 + 2));
 	if (!d_col_.ptr() || !d_coup_.ptr()) return fail(-2, "device allocation failed (rows)");
 	HIPCHK(hipMemcpy(d_slot_node_.ptr(), slot_node_.data(), d_slot_node_.bytes(), hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_slot_deg_.ptr(), slot_deg_.data(), d_slot_deg_.bytes(), hipMemcpyHostToDevice));
@@ -343,6 +345,8 @@ SweepArgs Grid::sweep_args()
 	a.phi0 = d_phi0_.ptr();
 	a.phi1 = d_phi1_.ptr();
 	a.ctl = d_ctl_.as<RelaxCtl>();
+	a.block_max = d_block_max_.as<double>();
+	a.block_flag = d_block_flag_.as<int>();
 	a.threshold_mode = 1;
 	return a;
 }
@@ -706,11 +710,18 @@ int Grid::set_id_and_resync(int node, int id)
 		if (write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw)) return fail(-7, "row wider than the spare slices");
 		nrows_++;
 	}
+	// the FP32 correction rows of the accelerated relaxation follow the
+	// same slots
+	std::vector<long long> fa;
+	std::vector<float> fv;
+	if (accel_)
+		for (int r : touched)
+			if (slot_of_[r] >= 0 && write_inner(slot_of_[r], r, &ai, &vi, &fa, &fv)) return fail(-7, "inner row too wide");
 
 	// ship all patches through the pinned staging area
-	const size_t ni = ai.size(), nr = ar.size(), nw = aw.size(), nf = full_addr.size();
+	const size_t nr = ar.size(), nw = aw.size(), nf = full_addr.size();
 	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
-	const size_t need = ni * 12 + nr * (8 + rs) + nw * 16 + nf * (8 + rs) + 256;
+	const size_t need = ai.size() * 12 + nr * (8 + rs) + nw * 16 + nf * (8 + rs) + fa.size() * 12 + 512;
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (need > h_pinned_bytes_) {
 		if (h_pinned_) (void) hipHostFree(h_pinned_);
@@ -727,9 +738,9 @@ int Grid::set_id_and_resync(int node, int id)
 		return at;
 	};
 	// split int patches by target array
-	std::vector<long long> ia[3];
-	std::vector<int> iv[3];
-	for (size_t t = 0; t < ni; t++) {
+	std::vector<long long> ia[5];
+	std::vector<int> iv[5];
+	for (size_t t = 0; t < ai.size(); t++) {
 		const int arr = (int) (ai[t] >> 60);
 		ia[arr].push_back(ai[t] & ((1ll << 60) - 1));
 		iv[arr].push_back(vi[t]);
@@ -741,13 +752,17 @@ int Grid::set_id_and_resync(int node, int id)
 		wa[arr].push_back(aw[t] & ((1ll << 60) - 1));
 		wv[arr].push_back(vw[t]);
 	}
-	size_t o_ia[3], o_iv[3], o_wa[3], o_wv[3];
-	for (int t = 0; t < 3; t++) {
+	size_t o_ia[5], o_iv[5], o_wa[3], o_wv[3];
+	for (int t = 0; t < 5; t++) {
 		o_ia[t] = put(ia[t].data(), sizeof(long long) * ia[t].size());
 		o_iv[t] = put(iv[t].data(), sizeof(int) * iv[t].size());
+	}
+	for (int t = 0; t < 3; t++) {
 		o_wa[t] = put(wa[t].data(), sizeof(long long) * wa[t].size());
 		o_wv[t] = put(wv[t].data(), sizeof(double) * wv[t].size());
 	}
+	const size_t o_ia3 = put(fa.data(), sizeof(long long) * fa.size());
+	const size_t o_iw3 = put(fv.data(), sizeof(float) * fv.size());
 	const size_t o_ra = put(ar.data(), sizeof(long long) * nr);
 	const size_t o_fa = put(full_addr.data(), sizeof(long long) * nf);
 	size_t o_rv, o_fv;
@@ -761,8 +776,8 @@ int Grid::set_id_and_resync(int node, int id)
 	}
 	char* d = (char*) d_patch_addr_.ptr();
 	HIPCHK(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, stream_));
-	int* iarr[3] = {d_slot_node_.as<int>(), d_slot_deg_.as<int>(), d_col_.as<int>()};
-	for (int t = 0; t < 3; t++)
+	int* iarr[5] = {d_slot_node_.as<int>(), d_slot_deg_.as<int>(), d_col_.as<int>(), d_inner_deg_.as<int>(), d_icol_.as<int>()};
+	for (int t = 0; t < 5; t++)
 		if (launch_scatter_i32(stream_, (int) ia[t].size(), (const long long*) (d + o_ia[t]), (const int*) (d + o_iv[t]), iarr[t]))
 			return fail(-2, "patch launch failed");
 	double* warr[3] = {nullptr, d_ws_.as<double>(), d_q_.as<double>()};
@@ -778,6 +793,9 @@ int Grid::set_id_and_resync(int node, int id)
 		     launch_scatter_f32(stream_, (int) nf, (const long long*) (d + o_fa), (const float*) (d + o_fv), d_coupfull_.as<float>());
 	}
 	if (rc) return fail(-2, "patch launch failed");
+	if (!fa.empty() &&
+	    launch_scatter_f32(stream_, (int) fa.size(), (const long long*) (d + o_ia3), (const float*) (d + o_iw3), d_iw_.as<float>()))
+		return fail(-2, "patch launch failed");
 	HIPCHK(hipMemcpyAsync(d_flags_.as<unsigned char>() + node, &flags_[node], 1, hipMemcpyHostToDevice, stream_));
 	float evf = species_[id].evap_field;
 	HIPCHK(hipMemcpyAsync(d_evapF_.as<float>() + node, &evf, sizeof(float), hipMemcpyHostToDevice, stream_));
@@ -980,6 +998,8 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 	*done = 0;
 	if (o.prob_mode != 0 || o.evap_mode != 0)
 		return fail(-1, "the device event loop implements the linear-field probability with the maximum choice");
+	if (o.method == 1 && !accel_)
+		if (int rc = accel_setup(0.5)) return rc;
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr)) return rc;
 	for (int ev = 0; ev < k; ev++) {
@@ -1011,12 +1031,19 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		if (int rc = potential(node, &r.pot_before)) return rc;
 		if (int rc = field_o1(1, &node, r.field_before)) return rc;
 		if (int rc = take_away(node)) return rc;
-		if (int rc = local_relax(node, o.relax_shell, o.relax_threshold, o.relax_cycle, o.relax_queue, &r.local_iter)) return rc;
-		for (int c = 0; c < o.relax_global_cycles; c++)
-			if (int rc = relax_cycles(1, nullptr)) return rc;
-		if (o.refresh_interval > 0 && (ev + 1) % o.refresh_interval == 0) {
-			double fd, fs, ld;
-			if (int rc = relax_threshold(o.refresh_threshold, o.refresh_cycle, o.refresh_queue, &r.global_iter, &fd, &fs, &ld)) return rc;
+		if (o.method == 1) {
+			int outer;
+			double fr;
+			if (int rc = accel_relax(o.accel_tol, &outer, &r.global_iter, &fr)) return rc;
+			r.local_iter = outer;
+		} else {
+			if (int rc = local_relax(node, o.relax_shell, o.relax_threshold, o.relax_cycle, o.relax_queue, &r.local_iter)) return rc;
+			for (int c = 0; c < o.relax_global_cycles; c++)
+				if (int rc = relax_cycles(1, nullptr)) return rc;
+			if (o.refresh_interval > 0 && (ev + 1) % o.refresh_interval == 0) {
+				double fd, fs, ld;
+				if (int rc = relax_threshold(o.refresh_threshold, o.refresh_cycle, o.refresh_queue, &r.global_iter, &fd, &fs, &ld)) return rc;
+			}
 		}
 		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref)) return rc;
 		r.surface_size = (int) surface_.size();
@@ -1058,6 +1085,264 @@ int Grid::sweep_stats(double* bytes, long long* rows, long long* entries, long l
 	if (ms) *ms = stat_sweep_ms_;
 	if (sweeps) *sweeps = stat_sweeps_;
 	if (launched) *launched = stat_launched_;
+	return 0;
+}
+
+}  // namespace tsg
+
+namespace tsg {
+
+AccelArgs Grid::accel_args() const
+{
+	AccelArgs x;
+	x.nslots = next_spare_;
+	x.slot_node = d_slot_node_.as<int>();
+	x.inner_deg = d_inner_deg_.as<int>();
+	x.slice_off = d_slice_off_.as<long long>();
+	x.col = d_icol_.as<int>();
+	x.w = d_iw_.as<float>();
+	return x;
+}
+
+// the FP32 correction row of a slot: unknown participating neighbours with
+// weight c_ij / (row coupling sum); int patches target arrays 3 (inner_deg)
+// and 4 (inner columns), float patches the weights
+int Grid::write_inner(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* af,
+                      std::vector<float>* vf)
+{
+	std::vector<int> cols;
+	std::vector<double> coups;
+	double ws;
+	row_entries(node, &cols, &coups, &ws);
+	const long long base = slice_off_[slot / kSlice] + (slot % kSlice);
+	int deg = 0;
+	for (size_t k = 0; k < cols.size(); k++) {
+		if (flags_[cols[k]] & 2) continue;
+		const long long e = base + (long long) kSlice * deg;
+		ai->push_back((4ll << 60) | e); vi->push_back(cols[k]);
+		af->push_back(e); vf->push_back((float) (coups[k] / ws));
+		deg++;
+	}
+	if (deg > slice_width_[slot / kSlice]) return -1;
+	inner_deg_[slot] = deg;
+	ai->push_back((3ll << 60) | slot); vi->push_back(deg);
+	return 0;
+}
+
+int Grid::accel_setup(double safety)
+{
+	if (!f64_) return fail(-1, "the accelerated relaxation runs on a TSG_PLAIN_F64 grid");
+	const int n = mesh_.n;
+	const long long nent = slice_off_.back();
+	inner_deg_.assign(cap_slots_, 0);
+	std::vector<int> icol(nent, 0);
+	std::vector<float> iw(nent, 0.0f);
+	for (int slot = 0; slot < next_spare_; slot++) {
+		const int node = slot_node_[slot];
+		if (node < 0) continue;
+		std::vector<long long> ai, af;
+		std::vector<int> vi;
+		std::vector<float> vf;
+		if (write_inner(slot, node, &ai, &vi, &af, &vf)) return fail(-7, "inner row too wide");
+		for (size_t k = 0; k < af.size(); k++) iw[af[k]] = vf[k];
+		for (size_t k = 0; k < ai.size(); k++)
+			if ((ai[k] >> 60) == 4) icol[ai[k] & ((1ll << 60) - 1)] = vi[k];
+	}
+	d_inner_deg_.alloc(sizeof(int) * cap_slots_);
+	d_icol_.alloc(sizeof(int) * nent);
+	d_iw_.alloc(sizeof(float) * nent);
+	d_d0_.alloc(sizeof(float) * n);
+	d_d1_.alloc(sizeof(float) * n);
+	d_r32_.alloc(sizeof(float) * n);
+	d_redout_.alloc(sizeof(double) * 2);
+	if (!d_icol_.ptr() || !d_iw_.ptr() || !d_d0_.ptr()) return fail(-2, "device allocation failed (accelerated rows)");
+	HIPCHK(hipMemcpy(d_inner_deg_.ptr(), inner_deg_.data(), sizeof(int) * cap_slots_, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_icol_.ptr(), icol.data(), sizeof(int) * nent, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_iw_.ptr(), iw.data(), sizeof(float) * nent, hipMemcpyHostToDevice));
+	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * n));
+	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * n));
+	HIPCHK(hipMemset(d_r32_.ptr(), 0, sizeof(float) * n));
+	accel_ = true;
+
+	// spectral radius of G by power iteration with a D-weighted Rayleigh
+	// quotient (G = D^-1 C is self-adjoint in that inner product)
+	std::vector<float> x(n, 0.0f), y(n, 0.0f);
+	std::vector<double> d(n, 0.0);
+	for (int slot = 0; slot < next_spare_; slot++) {
+		const int node = slot_node_[slot];
+		if (node < 0) continue;
+		x[node] = 1.0f;
+		double ws = 0.0;
+		for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++)
+			if (participates(node, mesh_.idx[k])) ws += coup_[k];
+		d[node] = ws;
+	}
+	AccelArgs xa = accel_args();
+	double lambda = 0.0;
+	for (int chunk = 0; chunk < 8; chunk++) {
+		HIPCHK(hipMemcpy(d_d0_.ptr(), x.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+		float* in = d_d0_.as<float>();
+		float* out = d_d1_.as<float>();
+		for (int it = 0; it < 150; it++) {
+			if (launch_apply_g(stream_, xa, in, out)) return fail(-2, "power iteration launch failed");
+			std::swap(in, out);
+		}
+		if (launch_apply_g(stream_, xa, in, out)) return fail(-2, "power iteration launch failed");
+		HIPCHK(hipStreamSynchronize(stream_));
+		HIPCHK(hipMemcpy(x.data(), in, sizeof(float) * n, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(y.data(), out, sizeof(float) * n, hipMemcpyDeviceToHost));
+		double num = 0.0, den = 0.0, mx = 0.0;
+		for (int i = 0; i < n; i++) {
+			num += d[i] * (double) x[i] * (double) y[i];
+			den += d[i] * (double) x[i] * (double) x[i];
+			mx = std::max(mx, (double) std::fabs(y[i]));
+		}
+		if (den <= 0.0 || mx <= 0.0) return fail(-4, "power iteration lost its vector");
+		lambda = num / den;
+		for (int i = 0; i < n; i++) x[i] = (float) (y[i] / mx);
+	}
+	rho_ = 1.0 - (1.0 - lambda) * safety;
+	if (!(rho_ > 0.0 && rho_ < 1.0)) return fail(-4, "spectral radius estimate out of (0,1)");
+	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * n));
+	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * n));
+	return 0;
+}
+
+int Grid::accel_relax(double tol, int* outer, long long* inner, double* final_residual)
+{
+	if (!accel_)
+		if (int rc = accel_setup(0.5)) return rc;
+	const int n = mesh_.n;
+	double* phi = (sw_ ? d_phi1_ : d_phi0_).as<double>();
+	double* other = (sw_ ? d_phi0_ : d_phi1_).as<double>();
+	SweepArgs a = sweep_args();
+	AccelArgs x = accel_args();
+	const double sigma = rho_ / (1.0 + std::sqrt(1.0 - rho_ * rho_));
+	long long steps = 0;
+	int o = 0;
+	double rmax = 0.0;
+	if (!ev_start_) {
+		HIPCHK(hipEventCreate(&ev_start_));
+		HIPCHK(hipEventCreate(&ev_stop_));
+	}
+	HIPCHK(hipEventRecord(ev_start_, stream_));
+	for (;; o++) {
+		double red[2];
+		if (launch_resid64(stream_, a, phi, d_r32_.as<float>(), d_redout_.as<double>())) return fail(-2, "residual launch failed");
+		HIPCHK(hipMemcpyAsync(red, d_redout_.ptr(), sizeof(red), hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipStreamSynchronize(stream_));
+		stat_resid_++;
+		rmax = red[0];
+		if (red[1] != 0.0) return fail(-3, "accelerated relaxation: NaN residual");
+		if (rmax <= tol) break;
+		if (o >= 64) return fail(-5, "accelerated relaxation did not converge in 64 refinement steps");
+		// Chebyshev steps for a reduction by f (the FP32 correction carries ~6 digits)
+		const double f = std::max(1e-5, 0.5 * tol / rmax);
+		int m = (int) std::ceil(std::log(2.0 / f) / std::log(1.0 / sigma));
+		m = std::max(2, std::min(m, 200000));
+		float* dprev = d_d0_.as<float>();
+		float* dcur = d_d1_.as<float>();
+		if (launch_cheb_init(stream_, x, d_r32_.as<float>(), dprev, dcur)) return fail(-2, "chebyshev launch failed");
+		double omega = 1.0;
+		for (int k = 2; k <= m; k++) {
+			omega = (k == 2) ? 1.0 / (1.0 - 0.5 * rho_ * rho_) : 1.0 / (1.0 - 0.25 * rho_ * rho_ * omega);
+			if (launch_cheb_step(stream_, x, d_r32_.as<float>(), dcur, dprev, (float) omega))
+				return fail(-2, "chebyshev launch failed");
+			std::swap(dprev, dcur);
+		}
+		steps += m;
+		if (launch_apply64(stream_, x, dcur, phi, nullptr)) return fail(-2, "update launch failed");
+	}
+	// both buffers carry the converged potentials, so that potential()
+	// (the buffer not written last) reads them
+	HIPCHK(hipMemcpyAsync(other, phi, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+	HIPCHK(hipEventRecord(ev_stop_, stream_));
+	HIPCHK(hipEventSynchronize(ev_stop_));
+	float ms = 0.0f;
+	HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+	stat_accel_ms_ += ms;
+	stat_cheb_ += steps;
+	if (outer) *outer = o;
+	if (inner) *inner = steps;
+	if (final_residual) *final_residual = rmax;
+	return 0;
+}
+
+}  // namespace tsg
+
+namespace tsg {
+
+// average device time of one launch of a relaxation kernel on the grid's
+// stream (HIP events around reps back-to-back launches): kind 0 = one FP64
+// (or float) Jacobi cycle with its decision kernel, kind 1 = one FP32
+// Chebyshev correction step.  The potentials are left as they were.
+int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
+{
+	if (reps <= 0) return fail(-1, "time_kernel: reps");
+	if (!ev_start_) {
+		HIPCHK(hipEventCreate(&ev_start_));
+		HIPCHK(hipEventCreate(&ev_stop_));
+	}
+	const int n = mesh_.n;
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	DevBuf save0, save1;
+	save0.alloc(rs * n);
+	save1.alloc(rs * n);
+	HIPCHK(hipMemcpyAsync(save0.ptr(), d_phi0_.ptr(), rs * n, hipMemcpyDeviceToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(save1.ptr(), d_phi1_.ptr(), rs * n, hipMemcpyDeviceToDevice, stream_));
+	float ms = 0.0f;
+	if (kind == 0) {
+		if (int rc = arm_ctl(-1.0, 1, 0)) return rc;
+		SweepArgs a = sweep_args();
+		if (launch_jacobi_sweep(stream_, a, 0, f64_)) return fail(-2, "sweep launch failed");
+		HIPCHK(hipEventRecord(ev_start_, stream_));
+		for (int r = 0; r < reps; r++)
+			if (launch_jacobi_sweep(stream_, a, (r + 1) & 1, f64_)) return fail(-2, "sweep launch failed");
+		HIPCHK(hipEventRecord(ev_stop_, stream_));
+		HIPCHK(hipEventSynchronize(ev_stop_));
+		HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+		if (bytes) sweep_stats(bytes, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+	} else if (kind == 1) {
+		if (!accel_)
+			if (int rc = accel_setup(0.5)) return rc;
+		AccelArgs x = accel_args();
+		float* d0 = d_d0_.as<float>();
+		float* d1 = d_d1_.as<float>();
+		if (launch_cheb_step(stream_, x, d_r32_.as<float>(), d0, d1, 1.0f)) return fail(-2, "chebyshev launch failed");
+		HIPCHK(hipEventRecord(ev_start_, stream_));
+		for (int r = 0; r < reps; r++) {
+			if (launch_cheb_step(stream_, x, d_r32_.as<float>(), d0, d1, 1.5f)) return fail(-2, "chebyshev launch failed");
+			std::swap(d0, d1);
+		}
+		HIPCHK(hipEventRecord(ev_stop_, stream_));
+		HIPCHK(hipEventSynchronize(ev_stop_));
+		HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+		HIPCHK(hipMemsetAsync(d_d0_.ptr(), 0, sizeof(float) * n, stream_));
+		HIPCHK(hipMemsetAsync(d_d1_.ptr(), 0, sizeof(float) * n, stream_));
+		if (bytes) {
+			// per entry column + weight, per row node, degree, r, delta read and written,
+			// every referenced delta once
+			long long nr = 0, ne = 0;
+			std::vector<uint8_t> ref(n, 0);
+			for (int sl = 0; sl < next_spare_; sl++) {
+				const int node = slot_node_[sl];
+				if (node < 0) continue;
+				nr++;
+				ne += inner_deg_[sl];
+				ref[node] = 1;
+				for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++)
+					if (participates(node, mesh_.idx[k]) && !(flags_[mesh_.idx[k]] & 2)) ref[mesh_.idx[k]] = 1;
+			}
+			long long nref = 0;
+			for (uint8_t v : ref) nref += v;
+			*bytes = (double) ne * 8 + (double) nr * (4 + 4 + 4 + 4 + 4) + (double) nref * 4;
+		}
+	} else
+		return fail(-1, "time_kernel: kind");
+	HIPCHK(hipMemcpyAsync(d_phi0_.ptr(), save0.ptr(), rs * n, hipMemcpyDeviceToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_phi1_.ptr(), save1.ptr(), rs * n, hipMemcpyDeviceToDevice, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	*avg_ms = ms / reps;
 	return 0;
 }
 

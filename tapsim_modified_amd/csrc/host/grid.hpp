@@ -48,6 +48,8 @@ struct EvapOptions {
 	double refresh_threshold = 0.0;
 	int refresh_cycle = 1, refresh_queue = 0;
 	int exact_probability = 1;  // compute the sequential normaliser every event
+	int method = 0;             // 0: the reference's relaxation, 1: accelerated (mixed precision)
+	double accel_tol = 1e-8;    // scaled-residual target of the accelerated relaxation
 };
 
 class DevBuf {
@@ -116,6 +118,13 @@ public:
 	// evap_takeAway (surface_3d.cpp:507-564)
 	int take_away(int node);
 
+	// accelerated relaxation: FP32 Chebyshev-Jacobi correction solves inside
+	// FP64 residual refinement, to max|J(phi)-phi| <= tol
+	int accel_setup(double safety);
+	int accel_relax(double tol, int* outer, long long* inner, double* final_residual);
+	double accel_rho() const { return rho_; }
+	int time_kernel(int kind, int reps, double* avg_ms, double* bytes);
+
 	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
 	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done);
 
@@ -171,13 +180,23 @@ private:
 
 	hipStream_t stream_ = nullptr;
 	DevBuf d_xyz_, d_ptr_, d_idx_, d_coupfull_, d_flags_, d_evapF_, d_phi0_, d_phi1_;
-	DevBuf d_slot_node_, d_slot_deg_, d_slice_off_, d_col_, d_coup_, d_ws_, d_q_, d_ctl_;
+	DevBuf d_slot_node_, d_slot_deg_, d_slice_off_, d_col_, d_coup_, d_ws_, d_q_, d_ctl_, d_block_max_, d_block_flag_;
 	DevBuf d_surf_, d_surf_val_, d_argmax_, d_maxval_, d_sum_, d_status_;
 	DevBuf d_shell_nodes_, d_shell_slots_, d_level_ptr_, d_order_, d_local_out_;
 	DevBuf d_patch_addr_, d_patch_i_, d_patch_r_, d_nodes_, d_fields_;
 	void* h_pinned_ = nullptr;
 	size_t h_pinned_bytes_ = 0;
 	hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+	// accelerated relaxation state
+	bool accel_ = false;
+	double rho_ = 0.0;
+	std::vector<int> inner_deg_;
+	DevBuf d_inner_deg_, d_icol_, d_iw_, d_d0_, d_d1_, d_r32_, d_redout_;
+	int write_inner(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* af,
+	                std::vector<float>* vf);
+	AccelArgs accel_args() const;
+	double stat_accel_ms_ = 0.0;
+	long long stat_cheb_ = 0, stat_resid_ = 0;
 	double stat_sweep_ms_ = 0.0;
 	long long stat_sweeps_ = 0, stat_launched_ = 0, last_sweeps_ = 0;
 };

@@ -47,11 +47,13 @@ struct SweepArgs {
 	void* phi0;               // R per node
 	void* phi1;
 	RelaxCtl* ctl;
+	double* block_max;        // per-block partial max |dphi| of the current cycle
+	int* block_flag;          // per-block NaN flag
 	int threshold_mode;       // 1: use ctl convergence/early exit; 0: plain fixed sweeps
 };
 
-// One Jacobi cycle (read buffer `read`, write the other); the deviation
-// max|phi_new - phi_old| over the rows is reduced into ctl.
+// One Jacobi cycle (read buffer `read`, write the other) followed by the
+// one-block decision kernel that reduces max|phi_new - phi_old| into ctl.
 int launch_jacobi_sweep(hipStream_t, const SweepArgs&, int read, bool f64);
 
 // Local relaxation (Grid_3d::Table::localRelax(index, order, threshold,
@@ -78,6 +80,24 @@ int launch_local_relax(hipStream_t, const SweepArgs&, const LocalArgs&, bool f64
 int launch_scatter_f64(hipStream_t, int n, const long long* addr, const double* val, double* dst);
 int launch_scatter_f32(hipStream_t, int n, const long long* addr, const float* val, float* dst);
 int launch_scatter_i32(hipStream_t, int n, const long long* addr, const int* val, int* dst);
+
+// ---------------------------------------------------------------- accelerated relaxation
+// FP32 correction rows: the unknown neighbours of every row slot with the
+// weights c_ij / sum_k c_ik, in the slot layout of the FP64 rows
+struct AccelArgs {
+	int nslots;
+	const int* slot_node;
+	const int* inner_deg;
+	const long long* slice_off;
+	const int* col;
+	const float* w;
+};
+// FP64 scaled residual r = J(phi) - phi into r32, max |r| into out[0], NaN flag into out[1]
+int launch_resid64(hipStream_t, const SweepArgs&, const double* phi, float* r32, double* out);
+int launch_cheb_init(hipStream_t, const AccelArgs&, const float* r32, float* dprev, float* dcur);
+int launch_cheb_step(hipStream_t, const AccelArgs&, const float* r32, const float* dcur, float* dprev_next, float omega);
+int launch_apply64(hipStream_t, const AccelArgs&, const float* delta, double* phi, double* phi_other);
+int launch_apply_g(hipStream_t, const AccelArgs&, const float* in, float* out);
 
 // ---------------------------------------------------------------- surface
 struct FieldArgs {

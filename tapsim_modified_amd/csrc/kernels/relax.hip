@@ -57,36 +57,60 @@ __device__ bool push_and_test(double* queue, int* qhead, int* qsize, int qcap, i
 	return slope == 0.0f;
 }
 
-// value of one row (computePotential) from the given read buffer
+// value of one row (computePotential) from the given read buffer.  Loads
+// are issued eight entries at a time (column indices, then couplings and the
+// gathered potentials) so that a wavefront keeps many requests in flight;
+// the products are then added strictly in entry order, which keeps the sum
+// bit-identical to the reference's sequential loop.
 template <typename R, bool kAtomicRead>
 __device__ __forceinline__ double row_value(const SweepArgs& a, int slot, const R* __restrict__ pin)
 {
+	constexpr int B = 8;
 	const int deg = a.slot_deg[slot];
 	const long long base = a.slice_off[slot >> 6] + (slot & 63);
-	const R* coup = (const R*) a.coup;
+	const R* __restrict__ coup = (const R*) a.coup;
+	const int* __restrict__ col = a.col;
 	double lv = 0.0;
-	for (int k = 0; k < deg; k++) {
-		const long long e = base + 64ll * k;
-		const int j = a.col[e];
-		const double c = (double) coup[e];
-		double adj;
-		if constexpr (kAtomicRead)
-			adj = (double) __hip_atomic_load(pin + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		else
-			adj = (double) pin[j];
-		adj *= c;
-		lv += adj;
+	for (int k = 0; k < deg; k += B) {
+		int j[B];
+		R c[B];
+		R p[B];
+#pragma unroll
+		for (int u = 0; u < B; u++) j[u] = (k + u < deg) ? col[base + 64ll * (k + u)] : 0;
+#pragma unroll
+		for (int u = 0; u < B; u++) c[u] = (k + u < deg) ? coup[base + 64ll * (k + u)] : (R) 0;
+#pragma unroll
+		for (int u = 0; u < B; u++) {
+			if (k + u < deg) {
+				if constexpr (kAtomicRead)
+					p[u] = __hip_atomic_load(pin + j[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				else
+					p[u] = pin[j[u]];
+			} else
+				p[u] = (R) 0;
+		}
+#pragma unroll
+		for (int u = 0; u < B; u++) {
+			if (k + u < deg) {
+				double adj = (double) p[u];
+				adj *= (double) c[u];
+				lv += adj;
+			}
+		}
 	}
 	lv -= a.q[slot];
 	lv /= a.ws[slot];
 	return lv;
 }
 
+// One Jacobi cycle: each block writes its max |dphi| (and a NaN flag) to its
+// own slot; the cycle's decision is taken by k_finalize.  No fence or atomic
+// per block: on a multi-XCD device an agent-scope fence writes back the L2,
+// and same-address atomics from thousands of blocks serialise.
 template <typename R>
 __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 {
-	RelaxCtl* ctl = a.ctl;
-	if (ctl->done) return;  // uniform per launch: converged earlier in this batch
+	if (a.ctl->done) return;  // uniform per launch: converged earlier in this batch
 	const R* pin = (const R*) (read ? a.phi1 : a.phi0);
 	R* pout = (R*) (read ? a.phi0 : a.phi1);
 	const int slot = blockIdx.x * 256 + threadIdx.x;
@@ -107,29 +131,39 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 	}
 	__shared__ double smax[4];
 	__shared__ int snan[4];
-	__shared__ int last;
 	dev = wave_max(dev);
 	const int anynan = __any(nan);
 	const int w = threadIdx.x >> 6;
 	if ((threadIdx.x & 63) == 0) { smax[w] = dev; snan[w] = anynan; }
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		const double m = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
-		if (snan[0] | snan[1] | snan[2] | snan[3]) atomicOr(&ctl->pad, 1);
-		atomicMax(&ctl->devbits, (unsigned long long) __double_as_longlong(m));
-		__threadfence();
-		const unsigned int ticket = atomicAdd(&ctl->counter, 1u);
-		last = (ticket == gridDim.x - 1);
+		a.block_max[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+		a.block_flag[blockIdx.x] = snan[0] | snan[1] | snan[2] | snan[3];
 	}
+}
+
+// the decision of one relax(1) call: the deviation over all blocks, the
+// deviation queue / zero-slope test and the threshold (grid_3d.cpp:611-671)
+__global__ __launch_bounds__(256) void k_finalize(int nblocks, RelaxCtl* ctl, const double* block_max, const int* block_flag)
+{
+	if (ctl->done) return;
+	double m = 0.0;
+	int nan = 0;
+	for (int b = threadIdx.x; b < nblocks; b += 256) {
+		m = fmax(m, block_max[b]);
+		nan |= block_flag[b];
+	}
+	__shared__ double smax[4];
+	__shared__ int snan[4];
+	m = wave_max(m);
+	const int anynan = __any(nan);
+	if ((threadIdx.x & 63) == 0) { smax[threadIdx.x >> 6] = m; snan[threadIdx.x >> 6] = anynan; }
 	__syncthreads();
-	if (!last || threadIdx.x != 0) return;
-	__threadfence();
-	const double d = __longlong_as_double((long long) atomicAdd(&ctl->devbits, 0ull));
-	ctl->devbits = 0ull;
-	ctl->counter = 0u;
+	if (threadIdx.x != 0) return;
+	const double d = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
 	ctl->sweeps += 1;
 	ctl->last_dev = d;
-	if (ctl->pad & 1) { ctl->done = 3; return; }
+	if (snan[0] | snan[1] | snan[2] | snan[3]) { ctl->done = 3; return; }
 	double slope = 0.0;
 	if (push_and_test(ctl->queue, &ctl->qhead, &ctl->qsize, 1024, ctl->queue_size, ctl->cycle, d, &slope)) {
 		ctl->fault_dev = d;
@@ -252,6 +286,8 @@ int launch_jacobi_sweep(hipStream_t s, const SweepArgs& a, int read, bool f64)
 	if (blocks == 0) return 0;
 	if (f64) hipLaunchKernelGGL(dev::k_jacobi<double>, dim3(blocks), dim3(256), 0, s, a, read);
 	else hipLaunchKernelGGL(dev::k_jacobi<float>, dim3(blocks), dim3(256), 0, s, a, read);
+	hipLaunchKernelGGL(dev::k_finalize, dim3(1), dim3(256), 0, s, blocks, a.ctl, (const double*) a.block_max,
+	                   (const int*) a.block_flag);
 	return (int) hipGetLastError();
 }
 

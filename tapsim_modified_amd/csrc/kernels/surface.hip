@@ -123,20 +123,24 @@ __global__ __launch_bounds__(1024) void k_argmax(int count, const R* __restrict_
 	}
 }
 
-// sequential sum in slot order: one wavefront loads 64 values at a time and
-// every lane accumulates them in lane order, reproducing the reference's
-// float (or double) running sum exactly
+// sequential sum in slot order, reproducing the reference's running float
+// (or double) sum exactly: one thread, loads issued 32 at a time ahead of the
+// ordered additions
 template <typename R>
 __global__ __launch_bounds__(64) void k_seq_sum(int count, const R* __restrict__ v, double* out)
 {
+	if (threadIdx.x != 0) return;
 	R sum = (R) 0;
-	for (int base = 0; base < count; base += 64) {
-		const int s = base + threadIdx.x;
-		const R x = s < count ? v[s] : (R) 0;
-		const int lim = min(64, count - base);
-		for (int l = 0; l < lim; l++) sum += __shfl(x, l, 64);
+	int s = 0;
+	for (; s + 32 <= count; s += 32) {
+		R x[32];
+#pragma unroll
+		for (int u = 0; u < 32; u++) x[u] = v[s + u];
+#pragma unroll
+		for (int u = 0; u < 32; u++) sum += x[u];
 	}
-	if (threadIdx.x == 0) *out = (double) sum;
+	for (; s < count; s++) sum += v[s];
+	*out = (double) sum;
 }
 
 }  // namespace dev

@@ -176,3 +176,50 @@ def test_evaporation_sequence(tip_small, mode):
         assert same_bits([list(e.field_before) for e in evg], F[:, 2:5], np.float32)
         assert same_bits([e.pot_after for e in evg], F[:, 5], np.float32)
         assert same_bits(g.phi(0), tip_small["ref_phi0_evap"], np.float32)
+
+
+def test_accelerated_relaxation_residual_and_error(tip_small):
+    """The mixed FP32/FP64 relaxation reaches max|J(phi)-phi| <= 1e-8 V, checked
+    by one FP64 Jacobi sweep of the oracle on the device result, and lands
+    within 1e-6 V of the oracle's solution relaxed to 1e-13 V."""
+    log("accelerated relaxation")
+    g = grid(tip_small, tsg.PLAIN_F64)
+    rho = g.accel_setup(0.5)
+    assert 0.5 < rho < 1.0
+    outer, inner, res = g.accel_relax(1e-8)
+    assert res <= 1e-8 and outer >= 1
+    phi = g.phi(0)
+    assert np.array_equal(phi, g.phi(1))
+    o = oracle(tip_small, O.F64)
+    o.sync()
+    o.set_phi(phi, phi, 0, 0)
+    dev = o.relax(1)
+    assert dev <= 1e-8, dev
+    ex = oracle(tip_small, O.F64)
+    ex.sync()
+    ex.relax_threshold(1e-13, 1, 0)
+    err = np.max(np.abs(phi - ex.phi(ex.switches()[0])))
+    log("accelerated: rho %.6f outer %d inner %d residual %.2e error %.2e" % (rho, outer, inner, res, err))
+    assert err < 1e-6
+
+
+def test_accelerated_events_track_plain_sequence(tip_small):
+    """Events with the accelerated relaxation: every relaxation ends below the
+    residual target and the evaporation sequence follows the plain FP64
+    path's (both relaxed to 1e-8 V; near ties may legitimately differ)."""
+    log("accelerated events")
+    g = grid(tip_small, tsg.PLAIN_F64)
+    g.accel_setup(0.5)
+    g.accel_relax(1e-10)
+    p0, p1 = g.phi(0), g.phi(1)
+    ga = grid(tip_small, tsg.PLAIN_F64)
+    ga.set_phi(p0, p1, *g.switches())
+    op = evap_opts()
+    op.refresh_interval, op.refresh_threshold = 1, 1e-8
+    oa = evap_opts()
+    oa.method, oa.accel_tol = 1, 1e-8
+    ep = g.evaporate(op, 15)
+    ea = ga.evaporate(oa, 15)
+    np_, na = [e.node for e in ep], [e.node for e in ea]
+    log("sequences plain %s accel %s" % (np_, na))
+    assert np_ == na
