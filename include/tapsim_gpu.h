@@ -202,6 +202,57 @@ int tsg_time_kernel(tsg_grid*, int kind, int reps, double* avg_ms, double* bytes
 int tsg_sweep_stats(const tsg_grid*, double* bytes, long long* rows, long long* entries, long long* referenced,
                     double* ms, long long* sweeps, long long* launched);
 
+/* ------------------------------------------------------------------ learned warm start
+ * No counterpart in the reference (TAPSim starts every relaxation from the
+ * previous potentials).  A message-passing network over the 3-hop shell of
+ * the evaporated atom predicts the change of the potentials that the
+ * relaxation will produce; the prediction is added before the relaxation,
+ * which still runs to its residual target (the network never decides the
+ * result, only where the relaxation starts).
+ *
+ *   h0   = relu(X W_in + b_in)                        X: n x 32, W_in: 32 x 64
+ *   h_l  = relu([h_{l-1} | A h_{l-1}] W_l + b_l) + h_{l-1},  l = 1..3, W_l: 128 x 64
+ *   y    = h_3 w_out + b_out
+ * A holds symmetric edge weights.  The GEMMs run on bfloat16 MFMA with fp32
+ * accumulation; parameters are one fp32 array in this layout (row-major W[k][o]):
+ */
+#define TSG_MPNN_IN 32
+#define TSG_MPNN_HID 64
+#define TSG_MPNN_LAYERS 3
+#define TSG_MPNN_W_IN 0
+#define TSG_MPNN_B_IN (TSG_MPNN_IN * TSG_MPNN_HID)
+#define TSG_MPNN_LAYER0 (TSG_MPNN_B_IN + TSG_MPNN_HID)
+#define TSG_MPNN_LAYER_STRIDE (2 * TSG_MPNN_HID * TSG_MPNN_HID + TSG_MPNN_HID)
+#define TSG_MPNN_W_OUT (TSG_MPNN_LAYER0 + TSG_MPNN_LAYERS * TSG_MPNN_LAYER_STRIDE)
+#define TSG_MPNN_B_OUT (TSG_MPNN_W_OUT + TSG_MPNN_HID)
+#define TSG_MPNN_NPARAM (TSG_MPNN_B_OUT + 1)
+
+typedef struct tsg_mpnn tsg_mpnn;
+
+/* a network with its graph store in device memory: batches of up to
+ * max_batch_rows nodes; the store holds data_rows nodes and data_edges edges */
+int tsg_mpnn_create(int max_batch_rows, long long data_rows, long long data_edges, unsigned seed, tsg_mpnn** out);
+void tsg_mpnn_destroy(tsg_mpnn*);
+int tsg_mpnn_set_params(tsg_mpnn*, const float* params);
+int tsg_mpnn_get_params(tsg_mpnn*, float* params);
+/* one graph from the host (local CSR ptr[n+1]/col/a, x: n*32 features,
+ * targets t and loss weights mask): loss = sum mask (y-t)^2 / max(1, sum mask),
+ * y (n) and d loss / d params (TSG_MPNN_NPARAM); each output may be NULL */
+int tsg_mpnn_eval(tsg_mpnn*, int n, const int* ptr, const int* col, const float* a, const float* x, const float* t,
+                  const float* mask, double* loss, float* y, float* grad);
+/* the training store: graphs recorded by evaporation events (see
+ * tsg_grid_attach_mpnn) or added from the host */
+int tsg_mpnn_add_graph(tsg_mpnn*, int n, const int* ptr, const int* col, const float* a, const float* x,
+                       const float* t, const float* mask);
+int tsg_mpnn_dataset(const tsg_mpnn*, int* graphs, long long* rows, long long* edges);
+int tsg_mpnn_clear(tsg_mpnn*);
+/* Adam over graphs [g_begin, g_end), batches of up to batch_graphs
+ * consecutive graphs in shuffled order; mean batch loss of the first and
+ * last epoch */
+int tsg_mpnn_train(tsg_mpnn*, int g_begin, int g_end, int epochs, int batch_graphs, float lr, unsigned seed,
+                   double* first_loss, double* last_loss);
+int tsg_mpnn_evaluate(tsg_mpnn*, int g_begin, int g_end, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

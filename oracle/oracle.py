@@ -100,6 +100,8 @@ _SIGS = {
     "ora_trajectory": ([_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                         ctypes.c_float, ctypes.c_float, _P, ctypes.c_int, _P], ctypes.c_int),
     "ora_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "ora_mpnn": ([_P, _P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "ora_bf16": ([ctypes.c_float], ctypes.c_float),
 }
 
 _lib = None
@@ -329,3 +331,51 @@ class Oracle:
         done = ctypes.c_int()
         self._chk(load().ora_evaporate(self._h, ctypes.byref(opts), k, out, ctypes.byref(done)))
         return [out[i] for i in range(done.value)]
+
+
+# ---------------------------------------------------------------- warm-start network
+
+MPNN_IN, MPNN_HID, MPNN_LAYERS = 32, 64, 3
+MPNN_W_IN = 0
+MPNN_B_IN = MPNN_IN * MPNN_HID
+MPNN_LAYER0 = MPNN_B_IN + MPNN_HID
+MPNN_LAYER_STRIDE = 2 * MPNN_HID * MPNN_HID + MPNN_HID
+MPNN_W_OUT = MPNN_LAYER0 + MPNN_LAYERS * MPNN_LAYER_STRIDE
+MPNN_B_OUT = MPNN_W_OUT + MPNN_HID
+MPNN_NPARAM = MPNN_B_OUT + 1
+
+
+class _MpnnGraph(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ptr", ctypes.c_void_p), ("col", ctypes.c_void_p), ("a", ctypes.c_void_p),
+                ("x", ctypes.c_void_p), ("t", ctypes.c_void_p), ("mask", ctypes.c_void_p)]
+
+
+def mpnn(ptr, col, a, x, t, mask, params, emulate=True, grad=True):
+    """The warm-start network on one (or a block-diagonal batch of) graph(s):
+    returns (loss, y, grad or None) in double."""
+    lib = load()
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    a = np.ascontiguousarray(a, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    mask = np.ascontiguousarray(mask, np.float32)
+    params = np.ascontiguousarray(params, np.float32)
+    assert x.shape == (len(ptr) - 1, MPNN_IN) and params.shape == (MPNN_NPARAM,)
+    g = _MpnnGraph(len(ptr) - 1, _ptr(ptr), _ptr(col), _ptr(a), _ptr(x), _ptr(t), _ptr(mask))
+    loss = ctypes.c_double()
+    y = np.zeros(len(ptr) - 1)
+    gr = np.zeros(MPNN_NPARAM) if grad else None
+    rc = lib.ora_mpnn(ctypes.byref(g), _ptr(params), 1 if emulate else 0, ctypes.byref(loss), _ptr(y),
+                      _ptr(gr) if grad else None)
+    if rc:
+        raise OracleError("ora_mpnn failed (%d)" % rc)
+    return loss.value, y, gr
+
+
+def bf16(v):
+    """Round-to-nearest-even to bfloat16, as float32 values."""
+    v = np.ascontiguousarray(v, np.float32)
+    u = v.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)

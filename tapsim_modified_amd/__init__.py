@@ -129,6 +129,17 @@ _SIGS = {
     "tsg_accel_setup": ([_P, ctypes.c_double, _P], ctypes.c_int),
     "tsg_time_kernel": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_accel_relax": ([_P, ctypes.c_double, _P, _P, _P], ctypes.c_int),
+    "tsg_mpnn_create": ([ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_uint, _P], ctypes.c_int),
+    "tsg_mpnn_destroy": ([_P], None),
+    "tsg_mpnn_set_params": ([_P, _P], ctypes.c_int),
+    "tsg_mpnn_get_params": ([_P, _P], ctypes.c_int),
+    "tsg_mpnn_eval": ([_P, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "tsg_mpnn_add_graph": ([_P, ctypes.c_int, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "tsg_mpnn_dataset": ([_P, _P, _P, _P], ctypes.c_int),
+    "tsg_mpnn_clear": ([_P], ctypes.c_int),
+    "tsg_mpnn_train": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _P,
+                        _P], ctypes.c_int),
+    "tsg_mpnn_evaluate": ([_P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
 }
 
 
@@ -390,3 +401,80 @@ class Grid3D:
         done = ctypes.c_int()
         _chk(load().tsg_evaporate(self._h, ctypes.byref(options), k, out, ctypes.byref(done)))
         return [out[i] for i in range(done.value)]
+
+
+# ---------------------------------------------------------------- learned warm start
+
+MPNN_IN, MPNN_HID, MPNN_LAYERS = 32, 64, 3
+MPNN_NPARAM = MPNN_IN * MPNN_HID + MPNN_HID + MPNN_LAYERS * (2 * MPNN_HID * MPNN_HID + MPNN_HID) + MPNN_HID + 1
+
+
+class WarmStartNet:
+    """The warm-start network and its training store in device memory
+    (tsg_mpnn_*): forward/backward on host graphs, training on recorded
+    events."""
+
+    def __init__(self, max_batch_rows=65536, data_rows=1 << 22, data_edges=1 << 26, seed=1):
+        h = ctypes.c_void_p()
+        _chk(load().tsg_mpnn_create(max_batch_rows, data_rows, data_edges, seed, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().tsg_mpnn_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self):
+        out = np.zeros(MPNN_NPARAM, np.float32)
+        _chk(load().tsg_mpnn_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (MPNN_NPARAM,)
+        _chk(load().tsg_mpnn_set_params(self._h, _ptr(p)))
+
+    @staticmethod
+    def _graph(ptr, col, a, x, t, mask):
+        ptr = np.ascontiguousarray(ptr, np.int32)
+        n = len(ptr) - 1
+        return (n, ptr, np.ascontiguousarray(col, np.int32), np.ascontiguousarray(a, np.float32),
+                np.ascontiguousarray(x, np.float32).reshape(n, MPNN_IN), np.ascontiguousarray(t, np.float32),
+                np.ascontiguousarray(mask, np.float32))
+
+    def eval(self, ptr, col, a, x, t, mask, grad=True):
+        """(loss, y, grad) of one host graph."""
+        n, ptr, col, a, x, t, mask = self._graph(ptr, col, a, x, t, mask)
+        loss = ctypes.c_double()
+        y = np.zeros(n, np.float32)
+        g = np.zeros(MPNN_NPARAM, np.float32) if grad else None
+        _chk(load().tsg_mpnn_eval(self._h, n, _ptr(ptr), _ptr(col), _ptr(a), _ptr(x), _ptr(t), _ptr(mask),
+                                  ctypes.byref(loss), _ptr(y), _ptr(g) if grad else None))
+        return loss.value, y, g
+
+    def add_graph(self, ptr, col, a, x, t, mask):
+        n, ptr, col, a, x, t, mask = self._graph(ptr, col, a, x, t, mask)
+        _chk(load().tsg_mpnn_add_graph(self._h, n, _ptr(ptr), _ptr(col), _ptr(a), _ptr(x), _ptr(t), _ptr(mask)))
+
+    def dataset(self):
+        g, r, e = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()
+        _chk(load().tsg_mpnn_dataset(self._h, ctypes.byref(g), ctypes.byref(r), ctypes.byref(e)))
+        return g.value, r.value, e.value
+
+    def clear(self):
+        _chk(load().tsg_mpnn_clear(self._h))
+
+    def train(self, g_begin, g_end, epochs, batch_graphs=16, lr=1e-3, seed=1):
+        f, l = ctypes.c_double(), ctypes.c_double()
+        _chk(load().tsg_mpnn_train(self._h, g_begin, g_end, epochs, batch_graphs, lr, seed, ctypes.byref(f),
+                                   ctypes.byref(l)))
+        return f.value, l.value
+
+    def evaluate(self, g_begin, g_end):
+        v = ctypes.c_double()
+        _chk(load().tsg_mpnn_evaluate(self._h, g_begin, g_end, ctypes.byref(v)))
+        return v.value

@@ -12,6 +12,7 @@ namespace tsg {
 namespace {
 constexpr int kSlice = 64;
 constexpr int kSpareSlices = 64;  // 4096 rows for atoms that turn into vacuum
+constexpr int kSortWindow = 256;  // rows sorted by length within each window of this many
 constexpr double kEpsilon0 = 8.85418781762e-12;  // grid_3d.h:285
 }  // namespace
 
@@ -277,14 +278,32 @@ int Grid::build_rows()
 	std::vector<std::vector<int>> ecols(nrows_);
 	std::vector<std::vector<double>> ecoup(nrows_);
 	std::vector<double> ws(cap_slots_, 1.0), q(cap_slots_, 0.0);
-	for (int r = 0; r < nrows_; r++) {
-		double w;
-		row_entries(rows[r], &ecols[r], &ecoup[r], &w);
-		slot_node_[r] = rows[r];
-		slot_deg_[r] = (int) ecols[r].size();
-		slot_of_[rows[r]] = r;
-		ws[r] = w;
-		q[r] = charge_[rows[r]];
+	std::vector<double> wrow(nrows_);
+	for (int r = 0; r < nrows_; r++) row_entries(rows[r], &ecols[r], &ecoup[r], &wrow[r]);
+	// slot order: node order, except that each window of kSortWindow rows is
+	// sorted by row length (longest first, stable), so that the 64 rows of a
+	// slice have nearly equal length and a wavefront's trip count matches
+	// its work; the gathered potentials keep the locality of the node order
+	std::vector<int> order(nrows_);
+	for (int r = 0; r < nrows_; r++) order[r] = r;
+	for (int w0 = 0; w0 < nrows_; w0 += kSortWindow)
+		std::stable_sort(order.begin() + w0, order.begin() + std::min(nrows_, w0 + kSortWindow),
+		                 [&](int x, int y) { return ecols[x].size() > ecols[y].size(); });
+	{
+		std::vector<std::vector<int>> c2(nrows_);
+		std::vector<std::vector<double>> k2(nrows_);
+		for (int sl = 0; sl < nrows_; sl++) {
+			const int r = order[sl];
+			c2[sl].swap(ecols[r]);
+			k2[sl].swap(ecoup[r]);
+			slot_node_[sl] = rows[r];
+			slot_deg_[sl] = (int) c2[sl].size();
+			slot_of_[rows[r]] = sl;
+			ws[sl] = wrow[r];
+			q[sl] = charge_[rows[r]];
+		}
+		ecols.swap(c2);
+		ecoup.swap(k2);
 	}
 	for (int s = 0; s < nslices_main; s++) {
 		int w = 0;
@@ -313,8 +332,16 @@ int Grid::build_rows()
 	d_coup_.alloc(rs * (size_t) nent);
 	d_ws_.alloc(sizeof(double) * cap_slots_);
 	d_q_.alloc(sizeof(double) * cap_slots_);
-	d_block_max_.alloc(sizeof(double) * (cap_slots_ / 256 + 2));
-	d_block_flag_.alloc(sizeof(int) * (cap_slots_ / 256 + 2));
+	long_set_.clear();
+	for (int sl = 0; sl < nrows_; sl++)
+		if (slot_deg_[sl] > kLongRow) long_set_.insert(sl);
+	long_cap_ = (int) long_set_.size() + 4096;
+	d_long_.alloc(sizeof(int) * (size_t) long_cap_);
+	d_block_max_.alloc(sizeof(double) * (cap_slots_ / 256 + 2 + long_cap_ / 4 + 1));
+	d_block_flag_.alloc(sizeof(int) * (cap_slots_ / 256 + 2 + long_cap_ / 4 + 1));
+	if (!d_long_.ptr() || !d_block_max_.ptr()) return fail(-2, "device allocation failed (rows)");
+	long_dirty_ = true;
+	if (int rc = upload_long()) return rc;
 	if (!d_col_.ptr() || !d_coup_.ptr()) return fail(-2, "device allocation failed (rows)");
 	HIPCHK(hipMemcpy(d_slot_node_.ptr(), slot_node_.data(), d_slot_node_.bytes(), hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_slot_deg_.ptr(), slot_deg_.data(), d_slot_deg_.bytes(), hipMemcpyHostToDevice));
@@ -348,7 +375,27 @@ SweepArgs Grid::sweep_args()
 	a.block_max = d_block_max_.as<double>();
 	a.block_flag = d_block_flag_.as<int>();
 	a.threshold_mode = 1;
+	a.long_slots = d_long_.as<int>();
+	a.nlong = (int) long_set_.size();
 	return a;
+}
+
+void Grid::set_long(int slot, bool on)
+{
+	if (on) long_dirty_ |= long_set_.insert(slot).second;
+	else long_dirty_ |= long_set_.erase(slot) > 0;
+}
+
+// the long-row list to the device (ascending slots), when it changed
+int Grid::upload_long()
+{
+	if (!long_dirty_) return 0;
+	if ((int) long_set_.size() > long_cap_) return fail(-7, "long-row list full");
+	std::vector<int> v(long_set_.begin(), long_set_.end());
+	if (!v.empty()) HIPCHK(hipMemcpyAsync(d_long_.ptr(), v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	long_dirty_ = false;
+	return 0;
 }
 
 int Grid::arm_ctl(double threshold, int cycles, int queue)
@@ -646,6 +693,7 @@ int Grid::write_slot(int slot, int node, std::vector<long long>* ai, std::vector
 	slot_node_[slot] = node;
 	slot_deg_[slot] = deg;
 	slot_of_[node] = slot;
+	set_long(slot, deg > kLongRow);
 	// int patches are encoded as (array id << 60) | index
 	ai->push_back((0ll << 60) | slot); vi->push_back(node);
 	ai->push_back((1ll << 60) | slot); vi->push_back(deg);
@@ -695,6 +743,7 @@ int Grid::set_id_and_resync(int node, int id)
 		if (!unknown) {
 			if (slot >= 0) {  // a row that became Dirichlet is switched off
 				slot_node_[slot] = -1;
+				set_long(slot, false);
 				ai.push_back((0ll << 60) | slot); vi.push_back(-1);
 				slot_of_[r] = -1;
 			}
@@ -703,6 +752,7 @@ int Grid::set_id_and_resync(int node, int id)
 		if (slot >= 0 && write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw) == 0) continue;
 		if (slot >= 0) {  // does not fit its slice any more
 			slot_node_[slot] = -1;
+			set_long(slot, false);
 			ai.push_back((0ll << 60) | slot); vi.push_back(-1);
 		}
 		if (next_spare_ >= cap_slots_) return fail(-7, "row slots exhausted (more than 4096 new unknown nodes)");
@@ -800,7 +850,7 @@ int Grid::set_id_and_resync(int node, int id)
 	float evf = species_[id].evap_field;
 	HIPCHK(hipMemcpyAsync(d_evapF_.as<float>() + node, &evf, sizeof(float), hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
-	return 0;
+	return upload_long();
 }
 
 // ---------------------------------------------------------------- surface
@@ -1101,6 +1151,9 @@ AccelArgs Grid::accel_args() const
 	x.slice_off = d_slice_off_.as<long long>();
 	x.col = d_icol_.as<int>();
 	x.w = d_iw_.as<float>();
+	x.slot_deg = d_slot_deg_.as<int>();
+	x.long_slots = d_long_.as<int>();
+	x.nlong = (int) long_set_.size();
 	return x;
 }
 

@@ -171,6 +171,13 @@ private:
 	int nrows_ = 0, nslots_ = 0, cap_slots_ = 0, next_spare_ = 0, spare_width_ = 0;
 	std::vector<int> slot_of_, slot_node_, slot_deg_, slice_width_;
 	std::vector<long long> slice_off_;
+	// slots whose row is longer than kLongRow (one wavefront each in the row kernels)
+	std::set<int> long_set_;
+	bool long_dirty_ = false;
+	int long_cap_ = 0;
+	DevBuf d_long_;
+	void set_long(int slot, bool on);
+	int upload_long();
 
 	std::set<int> surface_;
 	int vacuum_id_ = -1;

@@ -12,6 +12,7 @@
 // for a spectral radius bound rho of G (Varga, Matrix Iterative Analysis, 5.1).
 #include "device_math.hpp"
 #include "kernels.hpp"
+#include "rows.hpp"
 
 namespace tsg {
 namespace dev {
@@ -26,36 +27,51 @@ __device__ __forceinline__ double wave_maxd(double v)
 // per-block max |r|; rows without a slot node are skipped
 __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __restrict__ phi, float* __restrict__ r32)
 {
-	const int slot = blockIdx.x * 256 + threadIdx.x;
+	const int main_blocks = (a.nslots + 255) >> 8;
+	int slot = -1, node = -1, deg = 0;
+	long long soff = 0;
+	double q = 0.0, ws = 1.0;
+	if ((int) blockIdx.x < main_blocks) {
+		slot = blockIdx.x * 256 + threadIdx.x;
+		if (slot < a.nslots) {
+			node = a.slot_node[slot];
+			deg = a.slot_deg[slot];
+			soff = a.slice_off[slot >> 6];
+			q = a.q[slot];
+			ws = a.ws[slot];
+		}
+	} else {
+		const int li = long_row_index(main_blocks);
+		if (li < a.nlong) {
+			slot = a.long_slots[li];
+			node = a.slot_node[slot];
+			deg = a.slot_deg[slot];
+			soff = a.slice_off[slot >> 6];
+			q = a.q[slot];
+			ws = a.ws[slot];
+		}
+	}
+	const long long base = soff + (slot & 63);
+	double lv = 0.0, old = 0.0;
+	if ((int) blockIdx.x < main_blocks) {
+		const bool act = node >= 0 && deg <= kLongRow;
+		if (act) old = phi[node];
+		else node = -1;
+		lv = short_row_dot_exact<double>(a.col, (const double*) a.coup, phi, act, deg, base);
+	} else if (node >= 0) {
+		old = phi[node];
+		lv = long_row_dot_exact<double>(a.col, (const double*) a.coup, phi, deg, base);
+		if (lane_id() != 0) node = -1;
+	}
 	double rmax = 0.0;
 	int nan = 0;
-	if (slot < a.nslots) {
-		const int node = a.slot_node[slot];
-		if (node >= 0) {
-			const int deg = a.slot_deg[slot];
-			const long long base = a.slice_off[slot >> 6] + (slot & 63);
-			const double* __restrict__ coup = (const double*) a.coup;
-			double lv = 0.0;
-			for (int k = 0; k < deg; k += 8) {
-				int j[8];
-				double c[8], p[8];
-#pragma unroll
-				for (int u = 0; u < 8; u++) j[u] = (k + u < deg) ? a.col[base + 64ll * (k + u)] : node;
-#pragma unroll
-				for (int u = 0; u < 8; u++) c[u] = (k + u < deg) ? coup[base + 64ll * (k + u)] : 0.0;
-#pragma unroll
-				for (int u = 0; u < 8; u++) p[u] = phi[j[u]];
-#pragma unroll
-				for (int u = 0; u < 8; u++)
-					if (k + u < deg) lv += p[u] * c[u];
-			}
-			lv -= a.q[slot];
-			lv /= a.ws[slot];
-			const double r = lv - phi[node];
-			if (r != r) nan = 1;
-			r32[node] = (float) r;
-			rmax = fabs(r);
-		}
+	if (node >= 0) {
+		lv -= q;
+		lv /= ws;
+		const double r = lv - old;
+		if (r != r) nan = 1;
+		r32[node] = (float) r;
+		rmax = fabs(r);
 	}
 	__shared__ double smax[4];
 	__shared__ int snan[4];
@@ -90,6 +106,52 @@ __global__ __launch_bounds__(256) void k_reduce_max(int nblocks, const double* b
 	}
 }
 
+// G x over the FP32 correction rows for this thread's slot (or long row):
+// returns the node (-1 when none) and the sum; the per-slot record and the
+// node's extra operands (e1[node], e2[node], either may be null) are loaded
+// in one round before the row
+__device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __restrict__ in, float* sum,
+                                         const float* __restrict__ e1, float* v1, const float* __restrict__ e2, float* v2)
+{
+	const int main_blocks = (x.nslots + 255) >> 8;
+	int slot = -1, node = -1, deg = 0, ideg = 0;
+	long long soff = 0;
+	if ((int) blockIdx.x < main_blocks) {
+		slot = blockIdx.x * 256 + threadIdx.x;
+		if (slot < x.nslots) {
+			node = x.slot_node[slot];
+			deg = x.slot_deg[slot];
+			ideg = x.inner_deg[slot];
+			soff = x.slice_off[slot >> 6];
+		}
+	} else {
+		const int li = long_row_index(main_blocks);
+		if (li < x.nlong) {
+			slot = x.long_slots[li];
+			node = x.slot_node[slot];
+			deg = x.slot_deg[slot];
+			ideg = x.inner_deg[slot];
+			soff = x.slice_off[slot >> 6];
+		}
+	}
+	const long long base = soff + (slot & 63);
+	*sum = 0.0f;
+	if ((int) blockIdx.x < main_blocks) {  // block-uniform; the row sum needs the whole wavefront
+		const bool act = node >= 0 && deg <= kLongRow;
+		if (act) {
+			if (e1) *v1 = e1[node];
+			if (e2) *v2 = e2[node];
+		}
+		*sum = short_row_fma_f32(x.col, x.w, in, act, ideg, base);
+		return act ? node : -1;
+	}
+	if (node < 0) return -1;
+	if (e1) *v1 = e1[node];
+	if (e2) *v2 = e2[node];
+	*sum = long_row_fma_f32(x.col, x.w, in, ideg, base);
+	return lane_id() == 0 ? node : -1;
+}
+
 // delta_prev = 0, delta_cur = r on the rows (the first Jacobi step from 0)
 __global__ void k_cheb_init(AccelArgs x, const float* __restrict__ r32, float* __restrict__ dprev, float* __restrict__ dcur)
 {
@@ -105,27 +167,10 @@ __global__ void k_cheb_init(AccelArgs x, const float* __restrict__ r32, float* _
 __global__ __launch_bounds__(256) void k_cheb32(AccelArgs x, const float* __restrict__ r32, const float* __restrict__ dcur,
                                               float* __restrict__ dprev_next, float omega)
 {
-	const int slot = blockIdx.x * 256 + threadIdx.x;
-	if (slot >= x.nslots) return;
-	const int node = x.slot_node[slot];
+	float s, r = 0.0f, prev = 0.0f;
+	const int node = accel_row(x, dcur, &s, r32, &r, dprev_next, &prev);
 	if (node < 0) return;
-	const int deg = x.inner_deg[slot];
-	const long long base = x.slice_off[slot >> 6] + (slot & 63);
-	float s = 0.0f;
-	for (int k = 0; k < deg; k += 8) {
-		int j[8];
-		float w[8], v[8];
-#pragma unroll
-		for (int u = 0; u < 8; u++) j[u] = (k + u < deg) ? x.col[base + 64ll * (k + u)] : node;
-#pragma unroll
-		for (int u = 0; u < 8; u++) w[u] = (k + u < deg) ? x.w[base + 64ll * (k + u)] : 0.0f;
-#pragma unroll
-		for (int u = 0; u < 8; u++) v[u] = dcur[j[u]];
-#pragma unroll
-		for (int u = 0; u < 8; u++) s = fmaf(w[u], v[u], s);
-	}
-	const float prev = dprev_next[node];
-	dprev_next[node] = fmaf(omega, s + r32[node] - prev, prev);
+	dprev_next[node] = fmaf(omega, s + r - prev, prev);
 }
 
 // phi += delta on the rows, in FP64; both potential buffers take the result
@@ -143,33 +188,16 @@ __global__ void k_apply64(AccelArgs x, const float* __restrict__ delta, double* 
 // y = G x (power iteration for the spectral radius)
 __global__ __launch_bounds__(256) void k_apply_g(AccelArgs x, const float* __restrict__ in, float* __restrict__ out)
 {
-	const int slot = blockIdx.x * 256 + threadIdx.x;
-	if (slot >= x.nslots) return;
-	const int node = x.slot_node[slot];
-	if (node < 0) return;
-	const int deg = x.inner_deg[slot];
-	const long long base = x.slice_off[slot >> 6] + (slot & 63);
-	float s = 0.0f;
-	for (int k = 0; k < deg; k += 8) {
-		int j[8];
-		float w[8], v[8];
-#pragma unroll
-		for (int u = 0; u < 8; u++) j[u] = (k + u < deg) ? x.col[base + 64ll * (k + u)] : node;
-#pragma unroll
-		for (int u = 0; u < 8; u++) w[u] = (k + u < deg) ? x.w[base + 64ll * (k + u)] : 0.0f;
-#pragma unroll
-		for (int u = 0; u < 8; u++) v[u] = in[j[u]];
-#pragma unroll
-		for (int u = 0; u < 8; u++) s = fmaf(w[u], v[u], s);
-	}
-	out[node] = s;
+	float s;
+	const int node = accel_row(x, in, &s, nullptr, nullptr, nullptr, nullptr);
+	if (node >= 0) out[node] = s;
 }
 
 }  // namespace dev
 
 int launch_resid64(hipStream_t s, const SweepArgs& a, const double* phi, float* r32, double* out)
 {
-	const int blocks = (a.nslots + 255) / 256;
+	const int blocks = row_launch_blocks(a.nslots, a.nlong);
 	if (blocks == 0) return 0;
 	hipLaunchKernelGGL(dev::k_resid64, dim3(blocks), dim3(256), 0, s, a, phi, r32);
 	hipLaunchKernelGGL(dev::k_reduce_max, dim3(1), dim3(256), 0, s, blocks, (const double*) a.block_max,
@@ -185,7 +213,7 @@ int launch_cheb_init(hipStream_t s, const AccelArgs& x, const float* r32, float*
 
 int launch_cheb_step(hipStream_t s, const AccelArgs& x, const float* r32, const float* dcur, float* dprev_next, float omega)
 {
-	hipLaunchKernelGGL(dev::k_cheb32, dim3((x.nslots + 255) / 256), dim3(256), 0, s, x, r32, dcur, dprev_next, omega);
+	hipLaunchKernelGGL(dev::k_cheb32, dim3(row_launch_blocks(x.nslots, x.nlong)), dim3(256), 0, s, x, r32, dcur, dprev_next, omega);
 	return (int) hipGetLastError();
 }
 
@@ -197,7 +225,7 @@ int launch_apply64(hipStream_t s, const AccelArgs& x, const float* delta, double
 
 int launch_apply_g(hipStream_t s, const AccelArgs& x, const float* in, float* out)
 {
-	hipLaunchKernelGGL(dev::k_apply_g, dim3((x.nslots + 255) / 256), dim3(256), 0, s, x, in, out);
+	hipLaunchKernelGGL(dev::k_apply_g, dim3(row_launch_blocks(x.nslots, x.nlong)), dim3(256), 0, s, x, in, out);
 	return (int) hipGetLastError();
 }
 

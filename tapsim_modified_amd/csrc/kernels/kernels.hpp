@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "mpnn.hpp"
+
 namespace tsg {
 
 // ---------------------------------------------------------------- geometry
@@ -15,6 +17,10 @@ int launch_edge_areas(hipStream_t, int n, long long nnz, const float* xyz, const
                       float* a_one32, float* dist32, double* dist64, int* status);
 
 // ---------------------------------------------------------------- relaxation
+
+// longest row taken by one lane in the row kernels; longer rows are listed
+// separately and each is taken by a whole wavefront
+constexpr int kLongRow = 24;
 
 // Device-resident control block of a threshold relaxation
 // (Grid_3d::Table::relax(threshold, cycleSize, queueSize), grid_3d.cpp:611-671).
@@ -50,7 +56,11 @@ struct SweepArgs {
 	double* block_max;        // per-block partial max |dphi| of the current cycle
 	int* block_flag;          // per-block NaN flag
 	int threshold_mode;       // 1: use ctl convergence/early exit; 0: plain fixed sweeps
+	const int* long_slots;    // slots whose row has more than 32 entries (one wavefront each)
+	int nlong;
 };
+// workgroups of a row launch: 256 slots each, then one wavefront per long row
+int row_launch_blocks(int nslots, int nlong);
 
 // One Jacobi cycle (read buffer `read`, write the other) followed by the
 // one-block decision kernel that reduces max|phi_new - phi_old| into ctl.
@@ -91,6 +101,9 @@ struct AccelArgs {
 	const long long* slice_off;
 	const int* col;
 	const float* w;
+	const int* slot_deg;      // full row length: the long-row test
+	const int* long_slots;
+	int nlong;
 };
 // FP64 scaled residual r = J(phi) - phi into r32, max |r| into out[0], NaN flag into out[1]
 int launch_resid64(hipStream_t, const SweepArgs&, const double* phi, float* r32, double* out);
@@ -98,6 +111,31 @@ int launch_cheb_init(hipStream_t, const AccelArgs&, const float* r32, float* dpr
 int launch_cheb_step(hipStream_t, const AccelArgs&, const float* r32, const float* dcur, float* dprev_next, float omega);
 int launch_apply64(hipStream_t, const AccelArgs&, const float* delta, double* phi, double* phi_other);
 int launch_apply_g(hipStream_t, const AccelArgs&, const float* in, float* out);
+
+// ---------------------------------------------------------------- warm start
+// the shell of an evaporated atom as the network's graph (rows of the
+// network's store) and the quantities that tie it to the grid
+struct WarmArgs {
+	int n;                    // shell nodes (ascending node index)
+	const int* nodes;
+	const int* slots;         // row slot, or -1 for a Dirichlet node
+	const signed char* hop;   // BFS distance from the evaporated atom
+	const float* wv;          // edge weight to the evaporated atom (0 if not adjacent)
+	const float* xyz;
+	float v[3];               // position of the evaporated atom
+	float ell;                // length scale (mean distance to its neighbours)
+	double phi_ref;           // its potential before the removal
+	double* phi;              // the potential buffer the relaxation works on
+	double* phi_pre;          // per shell node: potential before the warm start
+	double* resid;            // per shell node: residual before the warm start
+	double* scale;            // device scalar: max |residual| over the shell
+	bf16_t* x;                // features (network store rows)
+	float* t;                 // targets (network store rows)
+	float* mask;              // loss weights: 1 on unknown rows
+};
+int launch_warm_features(hipStream_t, const WarmArgs&, const SweepArgs&);
+int launch_warm_apply(hipStream_t, const WarmArgs&, const float* y);
+int launch_warm_targets(hipStream_t, const WarmArgs&);
 
 // ---------------------------------------------------------------- surface
 struct FieldArgs {

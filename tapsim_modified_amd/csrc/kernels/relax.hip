@@ -14,6 +14,7 @@
 // -ffp-contract=off so no product is fused into an add).
 #include "device_math.hpp"
 #include "kernels.hpp"
+#include "rows.hpp"
 
 namespace tsg {
 namespace dev {
@@ -110,24 +111,59 @@ __device__ __forceinline__ double row_value(const SweepArgs& a, int slot, const 
 template <typename R>
 __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 {
-	if (a.ctl->done) return;  // uniform per launch: converged earlier in this batch
 	const R* pin = (const R*) (read ? a.phi1 : a.phi0);
 	R* pout = (R*) (read ? a.phi0 : a.phi1);
-	const int slot = blockIdx.x * 256 + threadIdx.x;
+	const int main_blocks = (a.nslots + 255) >> 8;
+	// the per-slot record is loaded in one round before any branch on it
+	int slot = -1, node = -1, deg = 0;
+	long long soff = 0;
+	double q = 0.0, ws = 1.0;
+	if ((int) blockIdx.x < main_blocks) {
+		slot = blockIdx.x * 256 + threadIdx.x;
+		if (slot < a.nslots) {
+			node = a.slot_node[slot];
+			deg = a.slot_deg[slot];
+			soff = a.slice_off[slot >> 6];
+			q = a.q[slot];
+			ws = a.ws[slot];
+		}
+	} else {
+		const int li = long_row_index(main_blocks);
+		if (li < a.nlong) {
+			slot = a.long_slots[li];
+			node = a.slot_node[slot];
+			deg = a.slot_deg[slot];
+			soff = a.slice_off[slot >> 6];
+			q = a.q[slot];
+			ws = a.ws[slot];
+		}
+	}
+	if (a.ctl->done) return;  // uniform per launch: converged earlier in this batch
+	const long long base = soff + (slot & 63);
+	double lv = 0.0;
+	R old = (R) 0;
+	if ((int) blockIdx.x < main_blocks) {  // block-uniform
+		const bool act = node >= 0 && deg <= kLongRow;  // else empty slot, or on the long-row list
+		if (act) old = pin[node];
+		else node = -1;
+		lv = short_row_dot_exact<R>(a.col, (const R*) a.coup, pin, act, deg, base);
+	} else if (node >= 0) {  // wave-uniform
+		old = pin[node];
+		lv = long_row_dot_exact<R>(a.col, (const R*) a.coup, pin, deg, base);
+		if (lane_id() != 0) node = -1;
+	}
 	double dev = 0.0;
 	int nan = 0;
-	if (slot < a.nslots) {
-		const int node = a.slot_node[slot];
-		if (node >= 0) {
-			const double lv = row_value<R, false>(a, slot, pin);
-			if (lv != lv) nan = 1;
-			const R nv = (R) lv;
-			pout[node] = nv;
-			// relax(cycleSize): |phi(0) - phi(1)| in double (grid_3d.cpp:597-605)
-			double ld = (double) nv;
-			ld -= (double) pin[node];
-			dev = fabs(ld);
-		}
+	if (node >= 0) {
+		lv -= q;
+		lv /= ws;
+		if (lv != lv) nan = 1;
+		const R nv = (R) lv;
+		pout[node] = nv;
+		// relax(cycleSize): |phi(0) - phi(1)| in double (grid_3d.cpp:597-605)
+		double ld = (double) nv;
+		ld -= (double) old;
+		dev = fabs(ld);
 	}
 	__shared__ double smax[4];
 	__shared__ int snan[4];
@@ -146,13 +182,26 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 // deviation queue / zero-slope test and the threshold (grid_3d.cpp:611-671)
 __global__ __launch_bounds__(256) void k_finalize(int nblocks, RelaxCtl* ctl, const double* block_max, const int* block_flag)
 {
-	if (ctl->done) return;
+	// the partials are loaded four per thread per round (clamped indices,
+	// issued together) before the control block is consulted
 	double m = 0.0;
 	int nan = 0;
-	for (int b = threadIdx.x; b < nblocks; b += 256) {
-		m = fmax(m, block_max[b]);
-		nan |= block_flag[b];
+	for (int b0 = 0; b0 < nblocks; b0 += 1024) {
+		double v[4];
+		int f[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int b = min(b0 + u * 256 + (int) threadIdx.x, nblocks - 1);
+			v[u] = block_max[b];
+			f[u] = block_flag[b];
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			m = fmax(m, v[u]);
+			nan |= f[u];
+		}
 	}
+	if (ctl->done) return;
 	__shared__ double smax[4];
 	__shared__ int snan[4];
 	m = wave_max(m);
@@ -280,9 +329,11 @@ __global__ void k_scatter_i32(int n, const long long* addr, const int* val, int*
 
 }  // namespace dev
 
+int row_launch_blocks(int nslots, int nlong) { return (nslots + 255) / 256 + (nlong + 3) / 4; }
+
 int launch_jacobi_sweep(hipStream_t s, const SweepArgs& a, int read, bool f64)
 {
-	const int blocks = (a.nslots + 255) / 256;
+	const int blocks = row_launch_blocks(a.nslots, a.nlong);
 	if (blocks == 0) return 0;
 	if (f64) hipLaunchKernelGGL(dev::k_jacobi<double>, dim3(blocks), dim3(256), 0, s, a, read);
 	else hipLaunchKernelGGL(dev::k_jacobi<float>, dim3(blocks), dim3(256), 0, s, a, read);

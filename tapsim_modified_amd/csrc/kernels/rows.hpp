@@ -1,0 +1,189 @@
+// Row gathers over the SELL-64 row layout shared by the relaxation kernels.
+//
+// A row's entries are summed strictly in entry order (ascending neighbour
+// index), so the value does not depend on how the loads are scheduled.
+//
+// Short rows (at most kLongRow entries) are taken by their own lane, with
+// the loads of the whole row issued before the ordered additions (the batch
+// is 8, 16 or kLongRow entries, chosen per wavefront from its longest row;
+// rows are sorted by length within small windows so that a wavefront's
+// rows have similar lengths).  Rows
+// longer than that (the fan of a boundary-plane node under the tip base
+// reaches several hundred entries) are kept on a separate list and each is
+// taken by a whole wavefront in extra workgroups of the same launch: the
+// lanes load 256 entries at a time and every lane adds the products in entry
+// order through v_readlane.  A long row on the per-lane path would keep one
+// wavefront looping for the length of the row after the rest of the launch
+// has finished.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tsg {
+namespace dev {
+
+constexpr int kLongWindows = 4;  // 64-entry windows loaded per round on the long path
+
+__device__ __forceinline__ int lane_id() { return (int) (threadIdx.x & 63); }
+
+__device__ __forceinline__ int wave_max_int(int v)
+{
+	for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+	return v;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+	const unsigned long long b = (unsigned long long) __double_as_longlong(v);
+	const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) b, l);
+	const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (b >> 32), l);
+	return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int l)
+{
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// index of the long row handled by this wavefront in a launch whose first
+// `main_blocks` workgroups (256 threads) cover the row slots; wave-uniform
+__device__ __forceinline__ int long_row_index(int main_blocks)
+{
+	return __builtin_amdgcn_readfirstlane((int) ((blockIdx.x - main_blocks) * 4 + (threadIdx.x >> 6)));
+}
+
+// sum_k (double) p[col_k] * (double) c_k, added in entry order starting from
+// 0.0, each product rounded to double on its own (no fused multiply-add):
+// the reference's computePotential accumulation.  Per-lane path, deg <= kLongRow.
+template <typename R, int B>
+__device__ __forceinline__ double short_row_dot_exact_b(const int* __restrict__ col, const R* __restrict__ coup,
+                                                        const R* __restrict__ p, int deg, long long base)
+{
+	// loads are unconditional (entries past the end re-read the last one) so
+	// that the compiler issues the whole batch before the first wait
+	int j[B];
+	R c[B], v[B];
+	const int last = max(deg - 1, 0);
+#pragma unroll
+	for (int u = 0; u < B; u++) j[u] = col[base + 64ll * min(u, last)];
+#pragma unroll
+	for (int u = 0; u < B; u++) c[u] = coup[base + 64ll * min(u, last)];
+#pragma unroll
+	for (int u = 0; u < B; u++) v[u] = p[j[u]];
+	double lv = 0.0;
+#pragma unroll
+	for (int u = 0; u < B; u++) {
+		double adj = (double) v[u];
+		adj *= (double) c[u];
+		const double t = lv + adj;
+		lv = (u < deg) ? t : lv;
+	}
+	return lv;
+}
+
+// deg <= kLongRow on every active lane; `active` false lanes load nothing.
+// The batch is chosen per wavefront from its longest row (wave-uniform).
+template <typename R>
+__device__ __forceinline__ double short_row_dot_exact(const int* __restrict__ col, const R* __restrict__ coup,
+                                                      const R* __restrict__ p, bool active, int deg, long long base)
+{
+	const int wmax = wave_max_int(active ? deg : 0);
+	if (!active || wmax == 0) return 0.0;
+	if (wmax <= 8) return short_row_dot_exact_b<R, 8>(col, coup, p, deg, base);
+	if (wmax <= 16) return short_row_dot_exact_b<R, 16>(col, coup, p, deg, base);
+	return short_row_dot_exact_b<R, kLongRow>(col, coup, p, deg, base);
+}
+
+// the same sum for one long row taken by the whole wavefront (deg and base
+// wave-uniform); every lane returns the sum
+template <typename R>
+__device__ __forceinline__ double long_row_dot_exact(const int* __restrict__ col, const R* __restrict__ coup,
+                                                     const R* __restrict__ p, int deg, long long base)
+{
+	constexpr int W = kLongWindows;
+	double acc = 0.0;
+	for (int k0 = 0; k0 < deg; k0 += 64 * W) {
+		int j[W];
+		R c[W], v[W];
+#pragma unroll
+		for (int w = 0; w < W; w++) {
+			const long long e = base + 64ll * min(k0 + 64 * w + lane_id(), deg - 1);
+			j[w] = col[e];
+			c[w] = coup[e];
+		}
+#pragma unroll
+		for (int w = 0; w < W; w++) v[w] = p[j[w]];
+#pragma unroll
+		for (int w = 0; w < W; w++) {
+			double prod = (double) v[w];
+			prod *= (double) c[w];
+			const int cnt = min(64, deg - (k0 + 64 * w));
+			for (int u = 0; u < cnt; u++) acc += readlane_f64(prod, u);
+		}
+	}
+	return acc;
+}
+
+// sum_k w_k * x[col_k] in FP32 as a chain of fused multiply-adds in entry
+// order; per-lane path, deg <= kLongRow
+template <int B>
+__device__ __forceinline__ float short_row_fma_f32_b(const int* __restrict__ col, const float* __restrict__ w,
+                                                     const float* __restrict__ x, int deg, long long base)
+{
+	int j[B];
+	float c[B], v[B];
+	const int last = max(deg - 1, 0);
+#pragma unroll
+	for (int u = 0; u < B; u++) j[u] = col[base + 64ll * min(u, last)];
+#pragma unroll
+	for (int u = 0; u < B; u++) c[u] = w[base + 64ll * min(u, last)];
+#pragma unroll
+	for (int u = 0; u < B; u++) v[u] = x[j[u]];
+	float s = 0.0f;
+#pragma unroll
+	for (int u = 0; u < B; u++) {
+		const float t = fmaf(c[u], v[u], s);
+		s = (u < deg) ? t : s;
+	}
+	return s;
+}
+
+__device__ __forceinline__ float short_row_fma_f32(const int* __restrict__ col, const float* __restrict__ w,
+                                                   const float* __restrict__ x, bool active, int deg, long long base)
+{
+	const int wmax = wave_max_int(active ? deg : 0);
+	if (!active || wmax == 0) return 0.0f;
+	if (wmax <= 8) return short_row_fma_f32_b<8>(col, w, x, deg, base);
+	if (wmax <= 16) return short_row_fma_f32_b<16>(col, w, x, deg, base);
+	return short_row_fma_f32_b<kLongRow>(col, w, x, deg, base);
+}
+
+__device__ __forceinline__ float long_row_fma_f32(const int* __restrict__ col, const float* __restrict__ w,
+                                                  const float* __restrict__ x, int deg, long long base)
+{
+	constexpr int W = kLongWindows;
+	float acc = 0.0f;
+	for (int k0 = 0; k0 < deg; k0 += 64 * W) {
+		int j[W];
+		float c[W], v[W];
+#pragma unroll
+		for (int ww = 0; ww < W; ww++) {
+			const long long e = base + 64ll * min(k0 + 64 * ww + lane_id(), deg - 1);
+			j[ww] = col[e];
+			c[ww] = w[e];
+		}
+#pragma unroll
+		for (int ww = 0; ww < W; ww++) v[ww] = x[j[ww]];
+#pragma unroll
+		for (int ww = 0; ww < W; ww++) {
+			const int cnt = min(64, deg - (k0 + 64 * ww));
+			for (int u = 0; u < cnt; u++) acc = fmaf(readlane_f32(c[ww], u), readlane_f32(v[ww], u), acc);
+		}
+	}
+	return acc;
+}
+
+}  // namespace dev
+}  // namespace tsg
