@@ -159,7 +159,8 @@ typedef struct tsg_evap_options
 	double refresh_threshold;
 	int refresh_cycle, refresh_queue;
 	int exact_probability;
-	int method;               /* 0: the reference's relaxation (steps III as process.cpp:824-900), 1: accelerated */
+	int method;               /* 0: the reference's relaxation (steps III as process.cpp:824-900), 1: accelerated,
+	                             2: accelerated from the learned warm start (tsg_grid_attach_mpnn) */
 	double accel_tol;         /* accelerated: max |J(phi) - phi| target, V */
 } tsg_evap_options;
 
@@ -173,6 +174,9 @@ typedef struct tsg_event
 	int surface_size;
 	double scaling_ref;
 	int near_tie;
+	int warm_start;           /* 0 none, 1 prediction applied, 2 prediction rejected, 3 graph recorded only */
+	double residual_cold;     /* max |J(phi)-phi| over the shell after the removal */
+	double residual_warm;     /* max |J(phi)-phi| over the grid after the prediction */
 } tsg_event;
 
 /* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k
@@ -252,6 +256,11 @@ int tsg_mpnn_clear(tsg_mpnn*);
 int tsg_mpnn_train(tsg_mpnn*, int g_begin, int g_end, int epochs, int batch_graphs, float lr, unsigned seed,
                    double* first_loss, double* last_loss);
 int tsg_mpnn_evaluate(tsg_mpnn*, int g_begin, int g_end, double* loss);
+/* events of methods 1 and 2 on this grid record their 3-hop shell graph and
+ * relaxation target into the network's store (record != 0); method 2 starts
+ * each relaxation from the network's prediction (predict != 0).  net may be
+ * NULL to detach; the network must outlive the attachment. */
+int tsg_grid_attach_mpnn(tsg_grid*, tsg_mpnn* net, int record, int predict);
 
 #ifdef __cplusplus
 }

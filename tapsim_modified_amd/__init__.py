@@ -83,6 +83,7 @@ class Event(ctypes.Structure):
         ("pot_after", ctypes.c_double), ("field_after", ctypes.c_double * 3),
         ("local_iter", ctypes.c_longlong), ("global_iter", ctypes.c_longlong),
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_double), ("near_tie", ctypes.c_int),
+        ("warm_start", ctypes.c_int), ("residual_cold", ctypes.c_double), ("residual_warm", ctypes.c_double),
     ]
 
 
@@ -140,6 +141,7 @@ _SIGS = {
     "tsg_mpnn_train": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _P,
                         _P], ctypes.c_int),
     "tsg_mpnn_evaluate": ([_P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_grid_attach_mpnn": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
 }
 
 
@@ -395,6 +397,12 @@ class Grid3D:
         outer, inner, res = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_double()
         _chk(load().tsg_accel_relax(self._h, tol, ctypes.byref(outer), ctypes.byref(inner), ctypes.byref(res)))
         return outer.value, inner.value, res.value
+
+    def attach(self, net, record=True, predict=True):
+        """Attaches a WarmStartNet (None detaches): events of methods 1 and 2
+        record training graphs, method 2 uses the prediction."""
+        self._net = net
+        _chk(load().tsg_grid_attach_mpnn(self._h, net.handle if net is not None else None, int(record), int(predict)))
 
     def evaporate(self, options, k):
         out = (Event * max(1, k))()

@@ -11,6 +11,8 @@ struct tsg_grid {
 	std::unique_ptr<tsg::Grid> g;
 };
 
+tsg::Grid* tsg_grid_unwrap(tsg_grid* h) { return h->g.get(); }
+
 namespace {
 static_assert(sizeof(tsg_species) == sizeof(tsg::Species), "species layout");
 static_assert(sizeof(tsg_event) == sizeof(tsg::EventRecord), "event layout");

@@ -117,6 +117,15 @@ int tsg_mpnn_train(tsg_mpnn* h, int g_begin, int g_end, int epochs, int batch_gr
 	return check(h, h->net->train(g_begin, g_end, epochs, batch_graphs, lr, seed, first_loss, last_loss));
 }
 
+int tsg_grid_attach_mpnn(tsg_grid* g, tsg_mpnn* h, int record, int predict)
+{
+	if (!g) return tsg::fail(TSG_E_ARG, "null grid");
+	tsg::Grid& grid = *tsg_grid_unwrap(g);
+	const int rc = grid.attach_mpnn(h ? h->net.get() : nullptr, record != 0, predict != 0);
+	if (rc) tsg::set_error(grid.error());
+	return rc;
+}
+
 int tsg_mpnn_evaluate(tsg_mpnn* h, int g_begin, int g_end, double* loss)
 {
 	if (!h || !loss) return tsg::fail(TSG_E_ARG, "null argument");

@@ -1048,7 +1048,10 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 	*done = 0;
 	if (o.prob_mode != 0 || o.evap_mode != 0)
 		return fail(-1, "the device event loop implements the linear-field probability with the maximum choice");
-	if (o.method == 1 && !accel_)
+	if (o.method < 0 || o.method > 2) return fail(-1, "unknown relaxation method");
+	if (o.method == 2 && !(net_ && net_predict_))
+		return fail(-1, "method 2 needs a warm-start network attached with prediction on");
+	if (o.method >= 1 && !accel_)
 		if (int rc = accel_setup(0.5)) return rc;
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr)) return rc;
@@ -1081,11 +1084,17 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		if (int rc = potential(node, &r.pot_before)) return rc;
 		if (int rc = field_o1(1, &node, r.field_before)) return rc;
 		if (int rc = take_away(node)) return rc;
-		if (o.method == 1) {
+		if (o.method >= 1) {
+			const bool warm = net_ && (net_record_ || o.method == 2);
+			warm_.n = 0;
+			if (warm)
+				if (int rc = warm_begin(node, r.pot_before, o.method == 2, &r)) return rc;
 			int outer;
 			double fr;
 			if (int rc = accel_relax(o.accel_tol, &outer, &r.global_iter, &fr)) return rc;
 			r.local_iter = outer;
+			if (warm)
+				if (int rc = warm_end()) return rc;
 		} else {
 			if (int rc = local_relax(node, o.relax_shell, o.relax_threshold, o.relax_cycle, o.relax_queue, &r.local_iter)) return rc;
 			for (int c = 0; c < o.relax_global_cycles; c++)
@@ -1396,6 +1405,201 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 	HIPCHK(hipMemcpyAsync(d_phi1_.ptr(), save1.ptr(), rs * n, hipMemcpyDeviceToDevice, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	*avg_ms = ms / reps;
+	return 0;
+}
+
+}  // namespace tsg
+
+// ---------------------------------------------------------------- learned warm start
+
+#include "mpnn.hpp"
+
+namespace tsg {
+
+int Grid::attach_mpnn(MpnnNet* net, bool record, bool predict)
+{
+	if (net && !f64_) return fail(-1, "the learned warm start runs on a TSG_PLAIN_F64 grid");
+	net_ = net;
+	net_record_ = net && record;
+	net_predict_ = net && predict;
+	return 0;
+}
+
+// The 3-hop shell of the evaporated atom (every node within three grid
+// hops, Dirichlet ones included as context) becomes one graph of the
+// network's store: edges between shell nodes weighted c_ij / sqrt(W_i W_j)
+// (W = coupling sum of the node over all its neighbours), the features and
+// the pre-relaxation potentials computed on the device.  With predict the
+// network's output (times the shell's max residual) is added to the shell's
+// unknown rows, and taken back if it raises the grid's max residual.
+int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
+{
+	const int n = mesh_.n;
+	std::vector<int> shell;
+	std::vector<signed char> hop;
+	{
+		std::vector<int> frontier{node}, next;
+		std::vector<int> seen_at;
+		std::set<int> seen{node};
+		shell.push_back(node);
+		for (int h = 1; h <= 3; h++) {
+			next.clear();
+			for (int u : frontier)
+				for (int k = mesh_.ptr[u]; k < mesh_.ptr[u + 1]; k++) {
+					const int j = mesh_.idx[k];
+					if (seen.insert(j).second) {
+						next.push_back(j);
+						shell.push_back(j);
+					}
+				}
+			frontier.swap(next);
+		}
+		std::sort(shell.begin(), shell.end());
+		// hop of every shell node (BFS distance)
+		hop.assign(shell.size(), 3);
+		std::vector<int> d(1, node), nd;
+		auto local = [&](int g) { return (int) (std::lower_bound(shell.begin(), shell.end(), g) - shell.begin()); };
+		hop[local(node)] = 0;
+		for (int h = 1; h <= 2; h++) {
+			nd.clear();
+			for (int u : d)
+				for (int k = mesh_.ptr[u]; k < mesh_.ptr[u + 1]; k++) {
+					const int li = local(mesh_.idx[k]);
+					if (hop[li] > h) {
+						hop[li] = (signed char) h;
+						nd.push_back(mesh_.idx[k]);
+					}
+				}
+			d.swap(nd);
+		}
+	}
+	const int ns = (int) shell.size();
+	auto local = [&](int g) -> int {
+		auto it = std::lower_bound(shell.begin(), shell.end(), g);
+		return (it != shell.end() && *it == g) ? (int) (it - shell.begin()) : -1;
+	};
+	std::vector<double> wsum(ns, 0.0);
+	for (int i = 0; i < ns; i++)
+		for (int k = mesh_.ptr[shell[i]]; k < mesh_.ptr[shell[i] + 1]; k++) wsum[i] += coup_[k];
+	std::vector<int> lptr(ns + 1, 0), lcol, slots(ns);
+	std::vector<float> la, wv(ns, 0.0f);
+	const int lv = local(node);
+	int unknown = 0;
+	for (int i = 0; i < ns; i++) {
+		const int g = shell[i];
+		slots[i] = (flags_[g] & 2) ? -1 : slot_of_[g];
+		if (slots[i] >= 0) unknown++;
+		for (int k = mesh_.ptr[g]; k < mesh_.ptr[g + 1]; k++) {
+			const int lj = local(mesh_.idx[k]);
+			if (lj < 0) continue;
+			const double den = std::sqrt(wsum[i] * wsum[lj]);
+			const float w = den > 0.0 ? (float) (coup_[k] / den) : 0.0f;
+			lcol.push_back(lj);
+			la.push_back(w);
+			if (lj == lv) wv[i] = w;
+		}
+		lptr[i + 1] = (int) lcol.size();
+	}
+	const long long ne = (long long) lcol.size();
+	MpnnNet& net = *net_;
+	if (net.reserve(ns, ne)) {  // store full: stop recording, no prediction this event
+		net_record_ = false;
+		rec->warm_start = 0;
+		warm_.n = 0;
+		return 0;
+	}
+	const long long R = net.rows(), E = net.edges();
+	for (int i = 0; i <= ns; i++) lptr[i] = (int) (E + lptr[i]);
+	for (auto& c : lcol) c = (int) (R + c);
+	if (ns > warm_cap_) {
+		warm_cap_ = std::max(ns, 4096);
+		d_w_nodes_.alloc(sizeof(int) * warm_cap_);
+		d_w_slots_.alloc(sizeof(int) * warm_cap_);
+		d_w_hop_.alloc(sizeof(signed char) * warm_cap_);
+		d_w_wv_.alloc(sizeof(float) * warm_cap_);
+		d_w_pre_.alloc(sizeof(double) * warm_cap_);
+		d_w_res_.alloc(sizeof(double) * warm_cap_);
+		d_w_scale_.alloc(sizeof(double));
+		if (!d_w_nodes_.ptr() || !d_w_res_.ptr() || !d_w_scale_.ptr()) return fail(-2, "device allocation failed (warm start)");
+	}
+	HIPCHK(hipMemcpyAsync(net.ptr_at(R), lptr.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, stream_));
+	if (ne) {
+		HIPCHK(hipMemcpyAsync(net.col_at(E), lcol.data(), sizeof(int) * ne, hipMemcpyHostToDevice, stream_));
+		HIPCHK(hipMemcpyAsync(net.a_at(E), la.data(), sizeof(float) * ne, hipMemcpyHostToDevice, stream_));
+	}
+	HIPCHK(hipMemcpyAsync(d_w_nodes_.ptr(), shell.data(), sizeof(int) * ns, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_w_slots_.ptr(), slots.data(), sizeof(int) * ns, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_w_hop_.ptr(), hop.data(), sizeof(signed char) * ns, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_w_wv_.ptr(), wv.data(), sizeof(float) * ns, hipMemcpyHostToDevice, stream_));
+
+	WarmArgs& w = warm_;
+	w.n = ns;
+	w.nodes = d_w_nodes_.as<int>();
+	w.slots = d_w_slots_.as<int>();
+	w.hop = d_w_hop_.as<signed char>();
+	w.wv = d_w_wv_.as<float>();
+	w.xyz = d_xyz_.as<float>();
+	double ell = 0.0;
+	int cnt = 0;
+	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) {
+		const int j = mesh_.idx[k];
+		double d2 = 0.0;
+		for (int c = 0; c < 3; c++) {
+			const double d = (double) mesh_.xyz[3 * (size_t) j + c] - mesh_.xyz[3 * (size_t) node + c];
+			d2 += d * d;
+		}
+		ell += std::sqrt(d2);
+		cnt++;
+	}
+	for (int c = 0; c < 3; c++) w.v[c] = mesh_.xyz[3 * (size_t) node + c];
+	w.ell = cnt ? (float) (ell / cnt) : 1.0f;
+	w.phi_ref = phi_ref;
+	w.phi = (sw_ ? d_phi1_ : d_phi0_).as<double>();
+	w.phi_pre = d_w_pre_.as<double>();
+	w.resid = d_w_res_.as<double>();
+	w.scale = d_w_scale_.as<double>();
+	w.x = net.x_at(R);
+	w.t = net.t_at(R);
+	w.mask = net.mask_at(R);
+	warm_row0_ = R;
+	warm_edges_ = ne;
+	warm_unknown_ = unknown;
+	if (launch_warm_features(stream_, w, sweep_args())) return fail(-2, "warm-start feature launch failed");
+	double s = 0.0;
+	HIPCHK(hipMemcpyAsync(&s, w.scale, sizeof(double), hipMemcpyDeviceToHost, stream_));
+	rec->warm_start = 3;
+	if (predict) {
+		MpnnBatch b;
+		b.n = ns;
+		b.row0 = (int) R;
+		b.msum = (float) std::max(1, unknown);
+		if (int rc = net.forward(b, stream_)) return fail(rc, "warm start: " + net.error());
+		if (launch_warm_apply(stream_, w, net.y())) return fail(-2, "warm-start apply launch failed");
+		// the grid's max residual after the prediction
+		SweepArgs a = sweep_args();
+		double red[2];
+		if (launch_resid64(stream_, a, w.phi, d_r32_.as<float>(), d_redout_.as<double>())) return fail(-2, "residual launch failed");
+		HIPCHK(hipMemcpyAsync(red, d_redout_.ptr(), sizeof(red), hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipStreamSynchronize(stream_));
+		rec->residual_warm = red[0];
+		rec->warm_start = 1;
+		if (!(red[0] <= s)) {  // worse than no prediction (or NaN): start from the removal state
+			if (launch_warm_apply(stream_, w, nullptr)) return fail(-2, "warm-start apply launch failed");
+			rec->warm_start = 2;
+		}
+	} else
+		HIPCHK(hipStreamSynchronize(stream_));
+	rec->residual_cold = s;
+	return 0;
+}
+
+// the converged potentials give the graph's targets; the graph is kept
+int Grid::warm_end()
+{
+	if (!net_record_ || warm_.n == 0) return 0;
+	if (launch_warm_targets(stream_, warm_)) return fail(-2, "warm-start target launch failed");
+	HIPCHK(hipStreamSynchronize(stream_));
+	net_->commit_graph(warm_.n, warm_edges_, (float) std::max(1, warm_unknown_));
 	return 0;
 }
 

@@ -17,6 +17,8 @@
 
 namespace tsg {
 
+class MpnnNet;
+
 struct Species {
 	int defined = 0;
 	float charge_density = 0, epsilon = 1;
@@ -37,6 +39,11 @@ struct EventRecord {
 	int surface_size;
 	double scaling_ref;
 	int near_tie;
+	// learned warm start: 0 none, 1 prediction applied, 2 prediction rejected
+	// (it raised the residual), 3 graph recorded only; the max |residual|
+	// over the shell before and (when predicted) over the grid after it
+	int warm_start;
+	double residual_cold, residual_warm;
 };
 
 struct EvapOptions {
@@ -48,7 +55,8 @@ struct EvapOptions {
 	double refresh_threshold = 0.0;
 	int refresh_cycle = 1, refresh_queue = 0;
 	int exact_probability = 1;  // compute the sequential normaliser every event
-	int method = 0;             // 0: the reference's relaxation, 1: accelerated (mixed precision)
+	int method = 0;             // 0: the reference's relaxation, 1: accelerated (mixed precision),
+	                            // 2: accelerated from the learned warm start
 	double accel_tol = 1e-8;    // scaled-residual target of the accelerated relaxation
 };
 
@@ -128,6 +136,11 @@ public:
 	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
 	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done);
 
+	// learned warm start: events of methods 1 and 2 record their shell graph
+	// and relaxation target into the network's store (record), method 2
+	// starts the relaxation from the network's prediction (predict)
+	int attach_mpnn(MpnnNet* net, bool record, bool predict);
+
 	hipStream_t stream() const { return stream_; }
 	// algorithmic bytes per sweep and the accumulated sweep timing (HIP events
 	// around the threshold relaxations on the grid's stream)
@@ -202,6 +215,16 @@ private:
 	int write_inner(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* af,
 	                std::vector<float>* vf);
 	AccelArgs accel_args() const;
+	// warm start state of the current event
+	MpnnNet* net_ = nullptr;
+	bool net_record_ = false, net_predict_ = false;
+	DevBuf d_w_nodes_, d_w_slots_, d_w_hop_, d_w_wv_, d_w_pre_, d_w_res_, d_w_scale_;
+	int warm_cap_ = 0;
+	WarmArgs warm_{};
+	long long warm_row0_ = 0, warm_edges_ = 0;
+	int warm_unknown_ = 0;
+	int warm_begin(int node, double phi_ref, bool predict, EventRecord* rec);
+	int warm_end();
 	double stat_accel_ms_ = 0.0;
 	long long stat_cheb_ = 0, stat_resid_ = 0;
 	double stat_sweep_ms_ = 0.0;
