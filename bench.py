@@ -1,11 +1,13 @@
 """Benchmark: evaporation events/s of the per-event electrostatics on one
-MI355X (plain FP64 path), with the relaxation sweep's HBM roofline and the
-CPU oracle timed beside it.
+MI355X -- the learned warm start with the mixed FP32/FP64 relaxation (the
+value), with the mixed relaxation alone and the plain FP64 path (the
+reference's algorithm) timed beside it, the sweep kernels' HBM roofline and
+the CPU oracle on the host cores.
 
 A step is one evaporation event: choose the atom (first maximum of the
-surface probabilities), remove it and re-couple its cell, relax the 3-shell
-locally, relax globally to a Jacobi-scaled residual of 1e-8 V, and evaluate
-every surface field and probability.
+surface probabilities), remove it and re-couple its cell, relax to a
+Jacobi-scaled residual of 1e-8 V, and evaluate every surface field and
+probability.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
 """
@@ -68,39 +70,39 @@ def event_options(tsg):
 
 
 def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, sweeps_per_event, budget_s=20.0):
-    """The FP64 oracle (one thread) on a bounded sample of the same workload:
-    the Jacobi sweep and the surface evaluation are timed and combined with
-    the GPU's sweeps per event (the two are bit-identical iterations)."""
+    """The FP64 oracle on the host cores, on a bounded sample of the same
+    workload: relax(1) sweeps with the reference's threading (contiguous
+    chunks of N/T nodes per thread, cwrapper_threadedRelax) and one surface
+    evaluation, combined with the GPU's sweeps per event (the plain GPU path
+    and the oracle perform bit-identical iterations)."""
     from oracle import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
     o = O.Oracle(O.F64, xyz, ids, tets, neigh, O.species_table(SPECIES))
-    # couplings from the device precompute (the setup the reference's timer also excludes)
+    # couplings from the device precompute (setup, which the reference's timer also excludes)
     o.sync_from_couplings(grid.couplings())
-    t0 = time.perf_counter()
-    k = 0
-    while time.perf_counter() - t0 < budget_s * 0.75 or k < 2:
-        o.relax(1)
-        k += 1
-    t_sweep = (time.perf_counter() - t0) / k
+    _, t_probe = o.relax_threaded(4, threads)
+    k = max(4, int(0.75 * budget_s / max(t_probe / 4, 1e-6)))
+    _, t_total = o.relax_threaded(k, threads)
+    t_sweep = t_total / k
     o.surface_init(0)
     t1 = time.perf_counter()
     o.specific_fields()
     o.compute_probabilities(0)
     t_surf = time.perf_counter() - t1
     per_event = sweeps_per_event * t_sweep + t_surf
-    return dict(value=1.0 / per_event, unit="events/s", cores=1, kind="port",
-                sample="%d FP64 Jacobi sweeps (%.3f s each) + one surface evaluation (%.3f s) of the same mesh, "
-                       "combined with %.0f sweeps/event" % (k, t_sweep, t_surf, sweeps_per_event))
+    return dict(value=1.0 / per_event, unit="events/s", cores=threads, kind="port",
+                sample="%d FP64 Jacobi sweeps on %d threads (%.3f ms each) + one surface evaluation (%.3f s, one "
+                       "thread) of the same mesh, combined with %.0f sweeps/event of the plain path"
+                       % (k, threads, t_sweep * 1e3, t_surf, sweeps_per_event))
 
 
-def run_events(grid, opts, warmup, steps, tag):
-    for _ in range(warmup):
-        grid.evaporate(opts, 1)
+def run_events(grid, opts, steps, tag, quiet=False):
     s0 = grid.sweep_stats()
     t0 = time.perf_counter()
     events = []
     for k in range(steps):
         events += grid.evaporate(opts, 1)
-        if (k + 1) % max(1, steps // 4) == 0:
+        if not quiet and (k + 1) % max(1, steps // 4) == 0:
             log("%s event %d/%d" % (tag, k + 1, steps))
     dt = time.perf_counter() - t0
     s1 = grid.sweep_stats()
@@ -114,14 +116,21 @@ def roofline(grid, kind, kernel):
             "traffic": None, "kernel": kernel, "bytes_per_launch": nbytes, "launch_us": ms * 1e3}
 
 
+def first_difference(a, b):
+    n = min(len(a), len(b))
+    return next((i for i in range(n) if a[i] != b[i]), None)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20, help="timed events of the learned path")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed events before each timed run")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--train-events", type=int, default=200, help="events recorded for training the network")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 run (default: steps)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 comparison (default: steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)", file=sys.stderr)
@@ -136,69 +145,116 @@ def main():
     t0 = time.perf_counter()
     mesh = tsg.Mesh(xyz)
     log("mesh: %d tets (%.1fs)" % (mesh.m, time.perf_counter() - t0))
-    t0 = time.perf_counter()
     species = tsg.species_table(SPECIES)
-    grid = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
-    log("grid: nnz %d (%.1fs)" % (grid.nnz, time.perf_counter() - t0))
+
+    def new_grid(phi=None):
+        g = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
+        if phi is not None:
+            g.set_phi(*phi)
+        return g
+
     t0 = time.perf_counter()
-    rho = grid.accel_setup(0.5)
-    outer, inner, res = grid.accel_relax(1e-8)
-    log("initial relaxation to 1e-8 V: rho %.8f, %d refinements, %d correction steps (%.1fs)"
-        % (rho, outer, inner, time.perf_counter() - t0))
-    phi0, phi1 = grid.phi(0), grid.phi(1)
-    sw = grid.switches()
+    plain = new_grid()
+    rho = plain.accel_setup(0.5)
+    outer, inner, res = plain.accel_relax(1e-8)
+    state0 = (plain.phi(0), plain.phi(1)) + plain.switches()
+    log("grid nnz %d; initial relaxation to 1e-8 V: rho %.8f, %d refinements, %d correction steps (%.1fs)"
+        % (plain.nnz, rho, outer, inner, time.perf_counter() - t0))
 
-    # plain FP64: the reference's step III (local 3-shell relaxation, one
-    # global cycle) followed by a relaxation to a deviation of 1e-8 V
-    plain_steps = args.plain_steps if args.plain_steps is not None else args.steps
-    popts = event_options(tsg)
-    pev, pdt, ps0, ps1 = run_events(grid, popts, args.warmup, plain_steps, "plain")
-    psweeps = (ps1["sweeps"] - ps0["sweeps"]) / max(1, plain_steps)
-    roof_plain = roofline(grid, 0, "k_jacobi<double>")
-    log("plain FP64: %.3f events/s, %.0f sweeps/event" % (plain_steps / pdt, psweeps))
+    # Three grids start from the same relaxed state S0 (the device path is
+    # deterministic, so equal inputs give equal states):
+    #   learned: records `train_events` events (mixed method), the network
+    #            trains on them, then runs the timed events from its warm start
+    #   mixed:   the same recorded events plus the timed ones, without the network
+    #   plain:   the reference's relaxation (local 3-shell + relaxation to a
+    #            deviation of 1e-8 V) from S0
+    # Sequences: plain vs mixed on the events from S0, learned vs mixed on the
+    # timed events.
+    W, K = args.train_events, args.steps
+    Kp = args.plain_steps if args.plain_steps is not None else K
+    mixed_opts = event_options(tsg)
+    mixed_opts.method, mixed_opts.accel_tol = 1, 1e-8
+    learned_opts = event_options(tsg)
+    learned_opts.method, learned_opts.accel_tol = 2, 1e-8
+    plain_opts = event_options(tsg)
 
-    # accelerated: FP32 Chebyshev-Jacobi corrections in FP64 refinement, same start
-    grid2 = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
-    grid2.set_phi(phi0, phi1, sw[0], sw[1])
-    rho2 = grid2.accel_setup(0.5)
-    aopts = event_options(tsg)
-    aopts.method, aopts.accel_tol = 1, 1e-8
-    aev, adt, as0, as1 = run_events(grid2, aopts, args.warmup, args.steps, "mixed")
-    roof_acc = roofline(grid2, 1, "k_cheb32")
-    steps_per_event = sum(e.global_iter for e in aev) / max(1, len(aev))
-    refinements = sum(e.local_iter for e in aev) / max(1, len(aev))
-    pn = [e.node for e in pev]
-    an = [e.node for e in aev]
-    common = min(len(pn), len(an))
-    first_diff = next((i for i in range(common) if pn[i] != an[i]), None)
-    value = args.steps / adt
+    pev, pdt, ps0, ps1 = run_events(plain, plain_opts, Kp, "plain")
+    psweeps = (ps1["sweeps"] - ps0["sweeps"]) / max(1, Kp)
+    roof_plain = roofline(plain, 0, "k_jacobi<double>")
+    log("plain FP64: %.3f events/s, %.0f sweeps/event" % (Kp / pdt, psweeps))
+
+    mixed = new_grid(state0)
+    mixed.accel_setup(0.5)
+    learned = new_grid(state0)
+    learned.accel_setup(0.5)
+    net = tsg.WarmStartNet(max_batch_rows=65536, data_rows=max(1 << 20, 4000 * (W + 8)),
+                           data_edges=max(1 << 24, 60000 * (W + 8)), seed=1)
+    learned.attach(net, record=True, predict=False)
+    t0 = time.perf_counter()
+    rec_l, _, _, _ = run_events(learned, mixed_opts, W, "record", quiet=True)
+    rec_m, _, _, _ = run_events(mixed, mixed_opts, W, "record", quiet=True)
+    graphs, rows, edges = net.dataset()
+    log("recorded %d shell graphs (%d nodes, %d edges) in %.1fs" % (graphs, rows, edges, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    held = max(1, graphs // 10)
+    untrained = net.evaluate(graphs - held, graphs)
+    first, last = net.train(0, graphs - held, epochs=args.epochs, batch_graphs=16, lr=2e-3, seed=5)
+    heldout = net.evaluate(graphs - held, graphs)
+    t_train = time.perf_counter() - t0
+    log("trained on the GPU in %.1fs: loss %.3g -> %.3g, held-out %.3g (untrained %.3g)"
+        % (t_train, first, last, heldout, untrained))
+    learned.attach(net, record=False, predict=True)
+    for _ in range(args.warmup):
+        learned.evaporate(learned_opts, 1)
+        mixed.evaporate(mixed_opts, 1)
+    aev, adt, as0, as1 = run_events(learned, learned_opts, K, "learned")
+    mev, mdt, ms0, ms1 = run_events(mixed, mixed_opts, K, "mixed")
+    roof_acc = roofline(learned, 1, "k_cheb32")
+    steps_l = sum(e.global_iter for e in aev) / max(1, len(aev))
+    steps_m = sum(e.global_iter for e in mev) / max(1, len(mev))
+    kept = sum(1 for e in aev if e.warm_start == 1)
+    ratio = float(np.median([e.residual_warm / e.residual_cold for e in aev if e.residual_cold > 0])) if aev else None
+    value = K / adt
     res = {
-        "metric": "evaporation events/s at residual <=1e-8 V (mixed FP32/FP64 relaxation; plain FP64 alongside)",
+        "metric": "evaporation events/s at residual <=1e-8 V: learned+mixed vs plain FP64 on GPU",
         "value": value,
         "unit": "events/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": adt / args.steps * 1e3,
+        "ms_per_step": adt / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic (seeded BCC [110] tip, graded vacuum, own Delaunay mesh)",
+        "dtype": "f32+f64+bf16",
+        "data": "synthetic (seeded BCC [110] tip, graded vacuum, own Delaunay mesh); network trained on the GPU "
+                "from %d recorded events of the same run" % W,
         "config": {"workload": "%s: BCC [110] tip r %.0f nm shank %.0f nm, %d atoms + %d vacuum nodes, %d unknown rows"
                                % (args.workload, w["radius"] * 1e9, w["shank"] * 1e9, counts["atoms"],
                                   counts["vacuum_lattice"] + counts["vacuum_random"], ps1["rows"]),
-                   "rho": rho2, "refinements_per_event": refinements, "correction_steps_per_event": steps_per_event},
-        "plain_fp64": {"value": plain_steps / pdt, "unit": "events/s", "steps": plain_steps,
-                       "ms_per_step": pdt / plain_steps * 1e3, "sweeps_per_event": psweeps},
-        "speedup_vs_plain_fp64": value / (plain_steps / pdt),
-        "sequence": {"compared": common, "first_difference": first_diff},
+                   "rho": rho, "correction_steps_per_event": steps_l, "refinements_per_event":
+                   sum(e.local_iter for e in aev) / max(1, len(aev))},
+        "learned": {"train_events": W, "epochs": args.epochs, "train_s": t_train, "loss_first_epoch": first,
+                    "loss_last_epoch": last, "heldout_loss": heldout, "heldout_loss_untrained": untrained,
+                    "prediction_kept": kept, "median_residual_ratio": ratio},
+        "mixed_fp32_fp64": {"value": K / mdt, "unit": "events/s", "ms_per_step": mdt / K * 1e3,
+                            "correction_steps_per_event": steps_m},
+        "plain_fp64": {"value": Kp / pdt, "unit": "events/s", "steps": Kp, "ms_per_step": pdt / Kp * 1e3,
+                       "sweeps_per_event": psweeps},
+        "speedup_vs_plain_fp64": value / (Kp / pdt),
+        "speedup_vs_mixed": value / (K / mdt),
+        "sequence": {"plain_vs_mixed": {"compared": min(Kp, W),
+                                        "first_difference": first_difference([e.node for e in pev],
+                                                                             [e.node for e in rec_m])},
+                     "learned_vs_mixed": {"compared": K, "first_difference": first_difference(
+                         [e.node for e in aev], [e.node for e in mev])}},
         "roofline": roof_acc,
         "roofline_plain_fp64": roof_plain,
     }
     if not args.no_cpu:
         tv, tn = mesh.tets()
-        res["cpu_baseline"] = cpu_baseline(tsg, grid, xyz, ids, tv, tn, psweeps)
+        res["cpu_baseline"] = cpu_baseline(tsg, plain, xyz, ids, tv, tn, psweeps)
+        res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]
     print(json.dumps(res))
 
 

@@ -142,6 +142,13 @@ int tsg_surface_nodes(const tsg_grid*, int* out);
 /* evap_compute_specificFields + evap_compute_probabilities (linear field),
    surface_3d.cpp:194-219,266-297; exact_sum also forms the normaliser */
 int tsg_surface_evaluate(tsg_grid*, int exact_sum, double* scaling_ref);
+/* the same with the probability model of evap_compute_probabilities
+   (surface_3d.cpp:223-417): 0 probLinear_field, 1 probBoltzmann (needs the
+   grid's temperature), 2 probLinear_force (field_o2), 3 probVoronoiFluxForce */
+int tsg_surface_evaluate_model(tsg_grid*, int mode, int exact_sum, double* scaling_ref);
+/* seeds the process's rand(), which evapMonteCarlo draws from (the
+   reference seeds it with 99, tapsim.cpp:235) */
+void tsg_srand(unsigned int seed);
 int tsg_surface_values(tsg_grid*, double* out);
 int tsg_surface_probabilities(tsg_grid*, float* out);
 /* evapMaximum (surface_3d.cpp:423-431) */
@@ -205,6 +212,22 @@ int tsg_time_kernel(tsg_grid*, int kind, int reps, double* avg_ms, double* bytes
    counterpart: the reference has no instrumentation of its sweep. */
 int tsg_sweep_stats(const tsg_grid*, double* bytes, long long* rows, long long* entries, long long* referenced,
                     double* ms, long long* sweeps, long long* launched);
+
+/* ------------------------------------------------------------------ trajectories
+ * Trajectory_3d::init + integrate(delta, errorLimit) + extrapolate(Z, zmax)
+ * (tapsim/trajectory_3d.cpp:99-332, with Grid_3d::FastField::compute,
+ * grid_3d.cpp:1669-1721, and Geometry_3d::Table::findTetrahedron,
+ * geometry_3d.cpp:988-1044) for `count` ions, each starting at rest on
+ * nodes[i] with charge[i] (C) and mass[i] (kg), in the field of the current
+ * potentials.  integrator: 0 first order, 1 Cash-Karp (the reference's
+ * default); stepper: 0 none, 1 error-restricted (default), 2
+ * geometry-restricted, 3 adaptive.  out receives, per ion, the first cap
+ * phases (t, x, y, z, vx, vy, vz, tet) of its trajectory, the last one the
+ * extrapolation to the mesh's largest z; len[i] the number of phases and
+ * status[i] the Trajectory_3d status (trajectory_3d.h:89; 5 = left the
+ * mesh, 1 = no field), negative on failure (-4: work limit). */
+int tsg_trajectories(tsg_grid*, int count, const int* nodes, const float* charge, const float* mass, int integrator,
+                     int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status);
 
 /* ------------------------------------------------------------------ learned warm start
  * No counterpart in the reference (TAPSim starts every relaxation from the

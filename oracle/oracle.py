@@ -100,6 +100,7 @@ _SIGS = {
     "ora_trajectory": ([_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                         ctypes.c_float, ctypes.c_float, _P, ctypes.c_int, _P], ctypes.c_int),
     "ora_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "ora_relax_threaded": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_double),
     "ora_mpnn": ([_P, _P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "ora_bf16": ([ctypes.c_float], ctypes.c_float),
 }
@@ -243,6 +244,14 @@ class Oracle:
         d = load().ora_relax(self._h, cycles, ctypes.byref(err))
         self._chk(err.value)
         return d
+
+    def relax_threaded(self, sweeps, threads):
+        """`sweeps` relax(1) calls on `threads` workers (FP64): (deviation, seconds)."""
+        sec = ctypes.c_double()
+        err = ctypes.c_int()
+        dev = load().ora_relax_threaded(self._h, sweeps, threads, ctypes.byref(sec), ctypes.byref(err))
+        self._chk(err.value)
+        return dev, sec.value
 
     def relax_threshold(self, threshold, cycles=1, queue=0):
         err, fd, fs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()

@@ -22,6 +22,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <pthread.h>
+#include <time.h>
 
 #define ORA_EPSILON0 8.85418781762e-12 /* grid_3d.h:285 */
 #define ORA_BND 0x01                   /* grid_3d.h:84 */
@@ -379,6 +381,82 @@ double ora_relax(ora_sys* s, int cycles, int* err)
 	const double d = DISPATCH(s, relax, s, cycles, &e);
 	if (err) *err = e;
 	return d;
+}
+
+/* relax(1) calls timed for the CPU baseline, FP64 mode only, with the
+   reference's threading of a cycle (cwrapper_threadedRelax,
+   grid_3d.cpp:1458-1514): T workers each take a contiguous chunk of about
+   N/T nodes, a barrier ends the cycle.  Every call is one Jacobi cycle, so
+   the potentials equal those of ora_relax(s, 1) repeated. */
+typedef struct
+{
+	ora_sys* s;
+	int sweeps, threads, id, err;
+	double dev;
+	pthread_barrier_t* bar;
+	double* devs;
+} ora_relax_job;
+
+static void* relax_worker(void* arg)
+{
+	ora_relax_job* j = (ora_relax_job*) arg;
+	ora_sys* s = j->s;
+	const int n = s->n;
+	const int lo = (int) ((long long) n * j->id / j->threads), hi = (int) ((long long) n * (j->id + 1) / j->threads);
+	for (int it = 0; it < j->sweeps; it++)
+	{
+		const double* read = (const double*) (s->sw ? s->phi1 : s->phi0);
+		double* write = (double*) (s->sw ? s->phi0 : s->phi1);
+		int err = 0;
+		for (int i = lo; i < hi; i++) write[i] = compute_potential_f64(s, i, read, &err);
+		if (err) j->err = err;
+		double dev = 0.0;
+		for (int i = lo; i < hi; i++)
+		{
+			double ld = write[i];
+			ld -= read[i];
+			ld = fabs(ld);
+			if (ld > dev) dev = ld;
+		}
+		j->devs[j->id] = dev;
+		pthread_barrier_wait(j->bar);
+		if (j->id == 0)
+		{
+			double d = 0.0;
+			for (int t = 0; t < j->threads; t++) if (j->devs[t] > d) d = j->devs[t];
+			j->dev = d;
+			s->sw = (s->sw + 1) % 2;
+		}
+		pthread_barrier_wait(j->bar);
+	}
+	return NULL;
+}
+
+double ora_relax_threaded(ora_sys* s, int sweeps, int threads, double* seconds, int* err)
+{
+	if (s->mode != ORA_F64 || threads < 1 || threads > 256 || sweeps < 0) { if (err) *err = ORA_E_ARG; return 0.0; }
+	pthread_barrier_t bar;
+	pthread_barrier_init(&bar, NULL, (unsigned) threads);
+	ora_relax_job jobs[256];
+	double devs[256];
+	pthread_t tid[256];
+	struct timespec t0, t1;
+	clock_gettime(CLOCK_MONOTONIC, &t0);
+	for (int t = 0; t < threads; t++)
+	{
+		jobs[t].s = s; jobs[t].sweeps = sweeps; jobs[t].threads = threads; jobs[t].id = t; jobs[t].err = 0;
+		jobs[t].dev = 0.0; jobs[t].bar = &bar; jobs[t].devs = devs;
+		if (t > 0) pthread_create(&tid[t], NULL, relax_worker, &jobs[t]);
+	}
+	relax_worker(&jobs[0]);
+	for (int t = 1; t < threads; t++) pthread_join(tid[t], NULL);
+	clock_gettime(CLOCK_MONOTONIC, &t1);
+	pthread_barrier_destroy(&bar);
+	if (seconds) *seconds = (double) (t1.tv_sec - t0.tv_sec) + 1e-9 * (double) (t1.tv_nsec - t0.tv_nsec);
+	int e = 0;
+	for (int t = 0; t < threads; t++) if (jobs[t].err) e = jobs[t].err;
+	if (err) *err = e;
+	return jobs[0].dev;
 }
 
 /* the deviation queue and least-squares slope test of

@@ -142,6 +142,10 @@ _SIGS = {
                         _P], ctypes.c_int),
     "tsg_mpnn_evaluate": ([_P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "tsg_grid_attach_mpnn": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "tsg_surface_evaluate_model": ([_P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_srand": ([ctypes.c_uint], None),
+    "tsg_trajectories": ([_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_int, _P, _P, _P], ctypes.c_int),
 }
 
 
@@ -163,6 +167,11 @@ def load():
 
 def exported_symbols():
     return sorted(_SIGS)
+
+
+def srand(seed):
+    """Seeds the process's rand() (the Monte Carlo choice draws from it)."""
+    load().tsg_srand(seed)
 
 
 def _chk(rc):
@@ -352,9 +361,12 @@ class Grid3D:
         _chk(load().tsg_surface_nodes(self._h, _ptr(out)))
         return out
 
-    def surface_evaluate(self, exact_sum=True):
+    def surface_evaluate(self, exact_sum=True, mode=0):
+        """Specific fields and the probability model's values (mode 0 linear
+        field, 1 Boltzmann, 2 linear force, 3 Voronoi flux force); returns
+        the scaling reference max |E|/F."""
         ref = ctypes.c_double()
-        _chk(load().tsg_surface_evaluate(self._h, 1 if exact_sum else 0, ctypes.byref(ref)))
+        _chk(load().tsg_surface_evaluate_model(self._h, mode, 1 if exact_sum else 0, ctypes.byref(ref)))
         return ref.value
 
     def surface_values(self):
@@ -397,6 +409,21 @@ class Grid3D:
         outer, inner, res = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_double()
         _chk(load().tsg_accel_relax(self._h, tol, ctypes.byref(outer), ctypes.byref(inner), ctypes.byref(res)))
         return outer.value, inner.value, res.value
+
+    def trajectories(self, nodes, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6,
+                     cap=4096):
+        """Ion trajectories from the given nodes (tsg_trajectories): a list of
+        (status, phases[len, 8]) per ion, phases truncated to cap."""
+        nodes = np.ascontiguousarray(np.atleast_1d(nodes), np.int32)
+        k = len(nodes)
+        charge = np.ascontiguousarray(np.broadcast_to(np.float32(charge), (k,)), np.float32)
+        mass = np.ascontiguousarray(np.broadcast_to(np.float32(mass), (k,)), np.float32)
+        out = np.zeros((k, cap, 8), np.float32)
+        ln = np.zeros(k, np.int32)
+        st = np.zeros(k, np.int32)
+        _chk(load().tsg_trajectories(self._h, k, _ptr(nodes), _ptr(charge), _ptr(mass), integrator, stepper, dt0, dt_min,
+                                     err, cap, _ptr(out), _ptr(ln), _ptr(st)))
+        return [(int(st[i]), out[i, :min(int(ln[i]), cap)].copy()) for i in range(k)]
 
     def attach(self, net, record=True, predict=True):
         """Attaches a WarmStartNet (None detaches): events of methods 1 and 2

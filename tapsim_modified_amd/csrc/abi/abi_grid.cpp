@@ -1,4 +1,5 @@
 // C ABI over the device grid (include/tapsim_gpu.h).
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -162,6 +163,14 @@ int tsg_surface_nodes(const tsg_grid* h, int* out)
 	return h->g->surface_nodes(out);
 }
 
+int tsg_surface_evaluate_model(tsg_grid* h, int mode, int exact_sum, double* scaling_ref)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->surface_evaluate(exact_sum != 0, scaling_ref, mode));
+}
+
+void tsg_srand(unsigned int seed) { srand(seed); }
+
 int tsg_surface_evaluate(tsg_grid* h, int exact_sum, double* scaling_ref)
 {
 	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
@@ -205,6 +214,14 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.method = o->method;
 	e.accel_tol = o->accel_tol;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
+}
+
+int tsg_trajectories(tsg_grid* h, int count, const int* nodes, const float* charge, const float* mass, int integrator,
+                     int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->trajectories(count, nodes, charge, mass, integrator, stepper, dt0, dt_min, err_limit, cap, out,
+	                                   len, status));
 }
 
 int tsg_accel_setup(tsg_grid* h, double safety, double* rho)

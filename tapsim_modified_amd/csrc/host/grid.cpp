@@ -148,6 +148,8 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 	d_coupfull_.alloc(rs * (size_t) nnz);
 	d_flags_.alloc((size_t) n);
 	d_evapF_.alloc(sizeof(float) * (size_t) n);
+	d_evapE_.alloc(sizeof(float) * (size_t) n);
+	d_eps_.alloc(sizeof(float) * (size_t) n);
 	d_phi0_.alloc(rs * (size_t) n);
 	d_phi1_.alloc(rs * (size_t) n);
 	d_ctl_.alloc(sizeof(RelaxCtl));
@@ -161,9 +163,17 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 	std::vector<double> phi(n);
 	for (int i = 0; i < n; i++) phi[i] = species_[ids_[i]].phi;
 	if (set_phi(phi.data(), phi.data(), 0, 0)) return;
-	std::vector<float> evf(n);
-	for (int i = 0; i < n; i++) evf[i] = species_[ids_[i]].evap_field;
-	if (hip_check(hipMemcpy(d_evapF_.ptr(), evf.data(), d_evapF_.bytes(), hipMemcpyHostToDevice), "upload evapField")) return;
+	// per-node species data of the probability models
+	std::vector<float> evf(n), eve(n), eps(n);
+	for (int i = 0; i < n; i++) {
+		evf[i] = species_[ids_[i]].evap_field;
+		eve[i] = species_[ids_[i]].evap_energy;
+		eps[i] = species_[ids_[i]].epsilon;
+	}
+	if (hip_check(hipMemcpy(d_evapF_.ptr(), evf.data(), d_evapF_.bytes(), hipMemcpyHostToDevice), "upload evapField") ||
+	    hip_check(hipMemcpy(d_evapE_.ptr(), eve.data(), d_evapE_.bytes(), hipMemcpyHostToDevice), "upload evapEnergy") ||
+	    hip_check(hipMemcpy(d_eps_.ptr(), eps.data(), d_eps_.bytes(), hipMemcpyHostToDevice), "upload epsilon"))
+		return;
 }
 
 Grid::~Grid()
@@ -847,8 +857,10 @@ int Grid::set_id_and_resync(int node, int id)
 	    launch_scatter_f32(stream_, (int) fa.size(), (const long long*) (d + o_ia3), (const float*) (d + o_iw3), d_iw_.as<float>()))
 		return fail(-2, "patch launch failed");
 	HIPCHK(hipMemcpyAsync(d_flags_.as<unsigned char>() + node, &flags_[node], 1, hipMemcpyHostToDevice, stream_));
-	float evf = species_[id].evap_field;
-	HIPCHK(hipMemcpyAsync(d_evapF_.as<float>() + node, &evf, sizeof(float), hipMemcpyHostToDevice, stream_));
+	const float spec[3] = {species_[id].evap_field, species_[id].evap_energy, species_[id].epsilon};
+	HIPCHK(hipMemcpyAsync(d_evapF_.as<float>() + node, &spec[0], sizeof(float), hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_evapE_.as<float>() + node, &spec[1], sizeof(float), hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(d_eps_.as<float>() + node, &spec[2], sizeof(float), hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	return upload_long();
 }
@@ -893,11 +905,19 @@ int Grid::upload_surface()
 	return 0;
 }
 
-int Grid::surface_evaluate(bool exact_sum, double* scaling_ref)
+int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 {
+	if (mode < 0 || mode > 3) return fail(-1, "unknown probability mode");
+	if (mode == 1 && !(temperature_ > 0.0f)) return fail(-1, "Surface_3d::Table::probBoltzmann(): temperature must be positive");
 	if (int rc = upload_surface()) return rc;
 	const int cnt = (int) surface_vec_.size();
+	prob_mode_ = mode;
 	if (cnt == 0) return 0;
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	if (mode != 0 && d_surf_ef_.bytes() < rs * (size_t) cnt) {
+		d_surf_ef_.alloc(rs * (size_t) std::max(cnt, 1024) * 2);
+		if (!d_surf_ef_.ptr()) return fail(-2, "device allocation failed (surface)");
+	}
 	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
 	FieldArgs f;
 	f.count = cnt;
@@ -912,16 +932,28 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref)
 	f.value = d_surf_val_.ptr();
 	f.field = nullptr;
 	f.status = d_status_.as<int>();
+	f.mode = mode;
+	f.temperature = temperature_;
+	f.evap_energy = d_evapE_.as<float>();
+	f.epsilon = d_eps_.as<float>();
+	f.value_ef = mode != 0 ? d_surf_ef_.ptr() : nullptr;
 	if (launch_surface_fields(stream_, f, f64_)) return fail(-2, "field launch failed");
+	// evap_compute_specificFields: max |E|/F; the argmax of the model's values
+	if (mode != 0) {
+		if (launch_surface_argmax(stream_, cnt, d_surf_ef_.ptr(), f64_, d_argmax_.as<long long>(), d_maxval_.as<double>()))
+			return fail(-2, "argmax launch failed");
+		HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+	}
 	if (launch_surface_argmax(stream_, cnt, d_surf_val_.ptr(), f64_, d_argmax_.as<long long>(), d_maxval_.as<double>()))
 		return fail(-2, "argmax launch failed");
 	if (exact_sum && launch_surface_sum(stream_, cnt, d_surf_val_.ptr(), f64_, d_sum_.as<double>()))
 		return fail(-2, "sum launch failed");
 	int st = 0;
 	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
-	HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+	if (mode == 0) HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (st & 1) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	if (st & 4) return fail(-4, "Grid_3d::Table::field_o2(): singular least-squares system");
 	if (st & 2) return fail(-3, "Grid_3d::Table::field_o1(): NaN");
 	if (scaling_ref) *scaling_ref = scaling_ref_;
 	return 0;
@@ -993,6 +1025,27 @@ int Grid::resolve_candidate(int* node, int* near_tie)
 	return 0;
 }
 
+// evapMonteCarlo (surface_3d.cpp:433-450): threshold rand()/RAND_MAX on the
+// running float sum of the normalised probabilities in node order; draws
+// from the process's own rand() sequence like the reference
+int Grid::mc_candidate(int* node)
+{
+	const int cnt = (int) surface_vec_.size();
+	if (cnt == 0) return fail(-6, "empty surface");
+	std::vector<float> p(cnt);
+	if (int rc = surface_probabilities(p.data())) return rc;
+	double thr = rand();
+	thr /= RAND_MAX;
+	int k = 0;
+	double acc = p[0];
+	while (thr > acc && k + 1 < cnt) {
+		k++;
+		acc += p[k];
+	}
+	*node = surface_vec_[k];
+	return 0;
+}
+
 int Grid::surface_candidate(int* node, int* near_tie)
 {
 	if (surface_vec_.empty()) return fail(-6, "empty surface");
@@ -1046,21 +1099,24 @@ int Grid::take_away(int node)
 int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 {
 	*done = 0;
-	if (o.prob_mode != 0 || o.evap_mode != 0)
-		return fail(-1, "the device event loop implements the linear-field probability with the maximum choice");
+	if (o.prob_mode < 0 || o.prob_mode > 3 || o.evap_mode < 0 || o.evap_mode > 1)
+		return fail(-1, "unknown probability model or choice");
 	if (o.method < 0 || o.method > 2) return fail(-1, "unknown relaxation method");
 	if (o.method == 2 && !(net_ && net_predict_))
 		return fail(-1, "method 2 needs a warm-start network attached with prediction on");
 	if (o.method >= 1 && !accel_)
 		if (int rc = accel_setup(0.5)) return rc;
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
-	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr)) return rc;
+	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr, o.prob_mode)) return rc;
 	for (int ev = 0; ev < k; ev++) {
 		if (surface_.empty()) break;
 		EventRecord& r = out[ev];
 		std::memset(&r, 0, sizeof(r));
-		int node, nt;
-		if (int rc = surface_candidate(&node, &nt)) return rc;
+		int node, nt = 0;
+		if (o.evap_mode == 0) {
+			if (int rc = surface_candidate(&node, &nt)) return rc;
+		} else if (int rc = mc_candidate(&node))
+			return rc;
 		r.node = node;
 		r.id = ids_[node];
 		r.near_tie = nt;
@@ -1104,7 +1160,7 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 				if (int rc = relax_threshold(o.refresh_threshold, o.refresh_cycle, o.refresh_queue, &r.global_iter, &fd, &fs, &ld)) return rc;
 			}
 		}
-		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref)) return rc;
+		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode)) return rc;
 		r.surface_size = (int) surface_.size();
 		if (int rc = potential(node, &r.pot_after)) return rc;
 		if (int rc = field_o1(1, &node, r.field_after)) return rc;
@@ -1600,6 +1656,94 @@ int Grid::warm_end()
 	if (launch_warm_targets(stream_, warm_)) return fail(-2, "warm-start target launch failed");
 	HIPCHK(hipStreamSynchronize(stream_));
 	net_->commit_graph(warm_.n, warm_edges_, (float) std::max(1, warm_unknown_));
+	return 0;
+}
+
+}  // namespace tsg
+
+// ---------------------------------------------------------------- trajectories
+
+namespace tsg {
+
+// Trajectory_3d::init + integrate(delta, errorLimit) + extrapolate(Z, zmax)
+// (trajectory_3d.cpp:99-332) for a batch of ions, each starting at rest on
+// its node, in the field of the current potentials (the buffer potential()
+// reads); one device thread per ion
+int Grid::trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
+                       float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status)
+{
+	if (count <= 0) return 0;
+	if (cap < 0 || !nodes || !charge || !mass || !len || !status || (cap > 0 && !out)) return fail(-1, "trajectory: bad arguments");
+	if (integrator < 0 || integrator > 1 || stepper < 0 || stepper > 3) return fail(-1, "trajectory: unknown integrator or stepper");
+	for (int i = 0; i < count; i++)
+		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "trajectory: bad node");
+	const int n = mesh_.n, m = mesh_.m();
+	if (!d_tv_.ptr()) {  // the mesh arrays of the point location, kept from the first call on
+		d_tv_.alloc(sizeof(int) * 4 * (size_t) m);
+		d_tn_.alloc(sizeof(int) * 4 * (size_t) m);
+		d_assoc_.alloc(sizeof(int) * (size_t) n);
+		d_gbnd_.alloc((size_t) n);
+		d_nfield_.alloc(sizeof(float) * 3 * (size_t) n);
+		if (!d_tv_.ptr() || !d_tn_.ptr() || !d_assoc_.ptr() || !d_gbnd_.ptr() || !d_nfield_.ptr())
+			return fail(-2, "device allocation failed (trajectories)");
+		HIPCHK(hipMemcpy(d_tv_.ptr(), mesh_.tet_v.data(), d_tv_.bytes(), hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d_tn_.ptr(), mesh_.tet_n.data(), d_tn_.bytes(), hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d_assoc_.ptr(), mesh_.assoc.data(), d_assoc_.bytes(), hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d_gbnd_.ptr(), mesh_.boundary.data(), d_gbnd_.bytes(), hipMemcpyHostToDevice));
+		zmax_ = mesh_.xyz[2];
+		for (int i = 1; i < n; i++) zmax_ = std::max(zmax_, mesh_.xyz[3 * (size_t) i + 2]);
+	}
+	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	if (launch_field_snapshot(stream_, n, d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_coupfull_.ptr(),
+	                          d_flags_.as<unsigned char>(), phi, f64_, d_nfield_.as<float>(), d_status_.as<int>()))
+		return fail(-2, "field snapshot launch failed");
+	DevBuf d_in, d_out, d_res;
+	d_in.alloc(sizeof(int) * 2 * (size_t) count + sizeof(float) * 2 * (size_t) count);
+	d_out.alloc(sizeof(float) * 8 * (size_t) count * (size_t) std::max(cap, 1));
+	d_res.alloc(sizeof(int) * 2 * (size_t) count);
+	if (!d_in.ptr() || !d_out.ptr() || !d_res.ptr()) return fail(-2, "device allocation failed (trajectory batch)");
+	std::vector<int> snap(count, 0);
+	char* din = d_in.as<char>();
+	HIPCHK(hipMemcpyAsync(din, nodes, sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(din + sizeof(int) * count, snap.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(din + sizeof(int) * 2 * count, charge, sizeof(float) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(din + sizeof(int) * 2 * count + sizeof(float) * count, mass, sizeof(float) * count,
+	                      hipMemcpyHostToDevice, stream_));
+	TrajArgs a;
+	a.count = count;
+	a.nodes = (const int*) din;
+	a.snapshot = (const int*) (din + sizeof(int) * count);
+	a.charge = (const float*) (din + sizeof(int) * 2 * count);
+	a.mass = (const float*) (din + sizeof(int) * 2 * count + sizeof(float) * count);
+	a.integrator = integrator;
+	a.stepper = stepper;
+	a.dt0 = dt0;
+	a.dt_min = dt_min;
+	a.err_limit = err_limit;
+	a.zmax = zmax_;
+	a.max_steps = 1 << 24;
+	a.max_calls = 1ll << 22;
+	a.n = n;
+	a.m = m;
+	a.xyz = d_xyz_.as<float>();
+	a.tv = d_tv_.as<int>();
+	a.tn = d_tn_.as<int>();
+	a.assoc = d_assoc_.as<int>();
+	a.gbnd = d_gbnd_.as<unsigned char>();
+	a.nfield = d_nfield_.as<float>();
+	a.out = d_out.as<float>();
+	a.cap = cap;
+	a.len = d_res.as<int>();
+	a.status = d_res.as<int>() + count;
+	if (launch_trajectories(stream_, a)) return fail(-2, "trajectory launch failed");
+	int st = 0;
+	if (cap > 0) HIPCHK(hipMemcpyAsync(out, d_out.ptr(), sizeof(float) * 8 * (size_t) count * cap, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(len, d_res.ptr(), sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(status, d_res.as<int>() + count, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
 	return 0;
 }
 

@@ -118,11 +118,13 @@ public:
 	int surface_init(int vacuum_id);
 	int surface_size() const { return (int) surface_.size(); }
 	int surface_nodes(int* out) const;
-	// evap_compute_specificFields + evap_compute_probabilities(linear field)
-	int surface_evaluate(bool exact_sum, double* scaling_ref);
+	// evap_compute_specificFields + evap_compute_probabilities (mode 0 linear
+	// field, 1 Boltzmann, 2 linear force, 3 Voronoi flux force)
+	int surface_evaluate(bool exact_sum, double* scaling_ref, int mode = 0);
 	int surface_values(double* out);                  // |E|/F per site, site order
 	int surface_probabilities(float* out);            // normalised as the reference
 	int surface_candidate(int* node, int* near_tie);  // evapMaximum
+	int mc_candidate(int* node);                      // evapMonteCarlo
 	// evap_takeAway (surface_3d.cpp:507-564)
 	int take_away(int node);
 
@@ -135,6 +137,12 @@ public:
 
 	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
 	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done);
+
+	// ion trajectories (Trajectory_3d, trajectory_3d.cpp:99-332) of a batch of
+	// ions from their nodes in the current potentials; out holds cap phases
+	// (t, x, y, z, vx, vy, vz, tet) per ion
+	int trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
+	                 float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status);
 
 	// learned warm start: events of methods 1 and 2 record their shell graph
 	// and relaxation target into the network's store (record), method 2
@@ -199,7 +207,8 @@ private:
 	double scaling_ref_ = 1.0;
 
 	hipStream_t stream_ = nullptr;
-	DevBuf d_xyz_, d_ptr_, d_idx_, d_coupfull_, d_flags_, d_evapF_, d_phi0_, d_phi1_;
+	DevBuf d_xyz_, d_ptr_, d_idx_, d_coupfull_, d_flags_, d_evapF_, d_evapE_, d_eps_, d_phi0_, d_phi1_, d_surf_ef_;
+	int prob_mode_ = 0;
 	DevBuf d_slot_node_, d_slot_deg_, d_slice_off_, d_col_, d_coup_, d_ws_, d_q_, d_ctl_, d_block_max_, d_block_flag_;
 	DevBuf d_surf_, d_surf_val_, d_argmax_, d_maxval_, d_sum_, d_status_;
 	DevBuf d_shell_nodes_, d_shell_slots_, d_level_ptr_, d_order_, d_local_out_;
@@ -215,6 +224,9 @@ private:
 	int write_inner(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* af,
 	                std::vector<float>* vf);
 	AccelArgs accel_args() const;
+	// trajectories: mesh arrays for the point location, node-field snapshot
+	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_;
+	float zmax_ = 0.0f;
 	// warm start state of the current event
 	MpnnNet* net_ = nullptr;
 	bool net_record_ = false, net_predict_ = false;

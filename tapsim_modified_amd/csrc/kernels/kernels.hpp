@@ -137,6 +137,34 @@ int launch_warm_features(hipStream_t, const WarmArgs&, const SweepArgs&);
 int launch_warm_apply(hipStream_t, const WarmArgs&, const float* y);
 int launch_warm_targets(hipStream_t, const WarmArgs&);
 
+// ---------------------------------------------------------------- trajectories
+// a batch of ions, one thread each; node fields from per-ion snapshots
+struct TrajArgs {
+	int count;
+	const int* nodes;         // start node of each ion
+	const int* snapshot;      // index of its field snapshot
+	const float* charge;      // C
+	const float* mass;        // kg
+	int integrator, stepper;
+	float dt0, dt_min, err_limit, zmax;
+	int max_steps;            // accepted steps per ion
+	long long max_calls;      // integrator calls per ion
+	int n, m;
+	const float* xyz;
+	const int* tv;
+	const int* tn;
+	const int* assoc;
+	const unsigned char* gbnd;
+	const float* nfield;      // snapshots: 3 floats per node each
+	float* out;               // per ion: cap phases of (t, p[3], v[3], tet)
+	int cap;
+	int* len;                 // phases produced (the output holds the first cap)
+	int* status;              // Trajectory_3d status (trajectory_3d.h:89), negative on failure
+};
+int launch_trajectories(hipStream_t, const TrajArgs&);
+int launch_field_snapshot(hipStream_t, int n, const float* xyz, const int* ptr, const int* idx, const void* coup,
+                          const unsigned char* flags, const void* phi, bool f64, float* out, int* status);
+
 // ---------------------------------------------------------------- surface
 struct FieldArgs {
 	int count;
@@ -148,9 +176,16 @@ struct FieldArgs {
 	const unsigned char* flags;
 	const void* phi;          // potential() buffer
 	const float* evap_field;  // per node
-	void* value;              // R per surface slot: |E| / F
+	void* value;              // R per surface slot: the probability model's unnormalised value
 	float* field;             // optional 3 floats per slot (E vector) or null
 	int* status;
+	// probability model (Surface_3d::evap_compute_probabilities, surface_3d.cpp:223-417):
+	// 0 linear field |E|/F, 1 Boltzmann, 2 linear force (field_o2), 3 Voronoi flux force
+	int mode;
+	float temperature;        // K (mode 1)
+	const float* evap_energy; // per node, eV (mode 1)
+	const float* epsilon;     // per node (mode 3)
+	void* value_ef;           // R per slot: |E|/F (specific fields) when mode != 0, else null
 };
 int launch_surface_fields(hipStream_t, const FieldArgs&, bool f64);
 

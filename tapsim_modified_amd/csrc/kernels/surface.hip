@@ -5,45 +5,51 @@
 // (evapMaximum, surface_3d.cpp:423-431) and the sequential normaliser.
 #include "device_math.hpp"
 #include "kernels.hpp"
+#include "field.hpp"
 
 namespace tsg {
 namespace dev {
 
-// field_o1 in element type R; returns false on a singular system
+__device__ __forceinline__ float exp_r(float x) { return expf(x); }
+__device__ __forceinline__ double exp_r(double x) { return exp(x); }
+
+// probVoronoiFluxForce (surface_3d.cpp:337-400): Maxwell stress over the
+// node's Voronoi faces between media of different permittivity
 template <typename R>
-__device__ bool field_o1(int node, const float* __restrict__ xyz, const int* __restrict__ ptr,
-                         const int* __restrict__ idx, const R* __restrict__ coup, const unsigned char* __restrict__ flags,
-                         const R* __restrict__ phi, R E[3])
+__device__ R flux_force(const FieldArgs& f, int i)
 {
-	E[0] = E[1] = E[2] = (R) 0;
-	if (flags[node] & 0x01) return true;  // geometric boundary: zero field
-	const V3<R> pi = load_coord<R>(xyz, node);
-	const R p0 = phi[node];
-	R a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0;
-	R b[3] = {0, 0, 0};
-	// the reference forms each of the nine sums in its own loop over the same
-	// ascending neighbour list; the sums are independent, so one pass with
-	// nine accumulators performs exactly the same roundings
-	for (int k = ptr[node]; k < ptr[node + 1]; k++) {
-		if (coup[k] == (R) 0) continue;
-		const int j = idx[k];
-		const V3<R> c = vsub(load_coord<R>(xyz, j), pi);
-		a00 += c.x[0] * c.x[0];
-		a01 += c.x[0] * c.x[1];
-		a02 += c.x[0] * c.x[2];
-		a11 += c.x[1] * c.x[1];
-		a12 += c.x[1] * c.x[2];
-		a22 += c.x[2] * c.x[2];
-		const R dp = phi[j] - p0;
-		for (int i = 0; i < 3; i++) b[i] += dp * c.x[i];
+	const R* coup = (const R*) f.coup;
+	const R* phi = (const R*) f.phi;
+	const R eps0 = (R) 8.85418781762e-12;
+	R area = 0;
+	V3<R> force = v3<R>(0, 0, 0);
+	const R pi = phi[i];
+	const V3<R> ci = load_coord<R>(f.xyz, i);
+	for (int kk = f.ptr[i]; kk < f.ptr[i + 1]; kk++) {
+		const int j = f.idx[kk];
+		R v = phi[j];
+		v -= pi;
+		v *= v;
+		v = (R) (v * (-1.0 * eps0));
+		v *= coup[kk];
+		const R e1 = (R) (1.0 / f.epsilon[i]);
+		const R e2 = (R) (1.0 / f.epsilon[j]);
+		v *= e2 - e1;
+		const V3<R> cj = load_coord<R>(f.xyz, j);
+		R corr = coup[kk];
+		corr *= vlen(vsub(ci, cj));
+		corr = (R) (corr / 2.0);
+		corr *= e1 + e2;
+		area += corr;
+		V3<R> lf = vsub(cj, ci);
+		const R dd = vdot(lf, lf);
+		for (int c = 0; c < 3; c++) lf.x[c] /= dd;
+		for (int c = 0; c < 3; c++) lf.x[c] *= v;
+		for (int c = 0; c < 3; c++) force.x[c] += lf.x[c];
 	}
-	R a[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
-	R d;
-	int perm[3];
-	if (!lu_decmp<R, 3>(a, perm, &d)) return false;
-	lu_solve<R, 3>(a, perm, b);
-	for (int i = 0; i < 3; i++) E[i] = (R) (-1.0f * b[i]);
-	return true;
+	R value = vlen(force);
+	value /= area;
+	return value;
 }
 
 template <typename R>
@@ -58,10 +64,37 @@ __global__ __launch_bounds__(256) void k_surface_fields(FieldArgs f)
 		E[0] = E[1] = E[2] = (R) 0;
 	}
 	if (E[0] != E[0] || E[1] != E[1] || E[2] != E[2]) atomicOr(f.status, 2);
-	V3<R> v = v3<R>(E[0], E[1], E[2]);
-	R value = vlen(v);
-	value /= (R) f.evap_field[node];
+	const R flen = vlen(v3<R>(E[0], E[1], E[2]));
+	R ef = flen;
+	ef /= (R) f.evap_field[node];
+	R value = ef;
+	if (f.mode == 1) {  // probBoltzmann (surface_3d.cpp:223-262)
+		const R kB = (R) 8.6173431e-5f;
+		R pr = (R) -1.0;
+		pr *= flen;
+		pr /= (R) f.evap_field[node];
+		pr = (R) (pr + 1.0);
+		pr *= (R) f.evap_energy[node];
+		pr /= (R) f.temperature;
+		pr /= kB;
+		pr = (R) (pr / -2.0);
+		value = exp_r(pr);
+	} else if (f.mode == 2) {  // probLinear_force (surface_3d.cpp:300-333)
+		R e2[3], q;
+		const int rc = field_o2<R>(node, f.xyz, f.ptr, f.idx, f.flags, (const R*) f.phi, e2, &q);
+		if (rc) atomicOr(f.status, rc == 1 ? 4 : 2);
+		const R fs = vlen(v3<R>(e2[0], e2[1], e2[2]));
+		const R F = (R) f.evap_field[node];
+		value = q;
+		value *= fs;
+		value /= F * F;
+	} else if (f.mode == 3) {
+		const R F = (R) f.evap_field[node];
+		value = flux_force<R>(f, node);
+		value /= F * F;
+	}
 	((R*) f.value)[s] = value;
+	if (f.value_ef) ((R*) f.value_ef)[s] = ef;
 	if (f.field)
 		for (int i = 0; i < 3; i++) f.field[3 * (size_t) s + i] = (float) E[i];
 }

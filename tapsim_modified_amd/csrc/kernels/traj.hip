@@ -1,0 +1,570 @@
+// Ion trajectories through the tetrahedral mesh (Trajectory_3d,
+// tapsim/trajectory_3d.cpp:99-696) with the point location and field
+// interpolation they use (Geometry_3d::Table::findTetrahedron / tetWeights,
+// geometry_3d.cpp:872-1044; Grid_3d::FastField::compute, grid_3d.cpp:1669-1721).
+//
+// One thread integrates one ion, in float as the reference does.  The node
+// fields come from a snapshot (field_o1 of every node, taken after the
+// relaxation), the counterpart of FastField's per-node cache.  Point
+// location is the visibility walk with an exact orientation sign; the walk
+// starts at face 0 where the reference draws the first face from rand(),
+// which changes which tet is returned only for a point exactly on a face.
+#include "device_math.hpp"
+#include "field.hpp"
+#include "kernels.hpp"
+
+namespace tsg {
+namespace dev {
+
+// ---------------------------------------------------------------- exact orientation sign
+
+constexpr int kBn = 16;  // 1024-bit two's complement
+struct Bn {
+	unsigned long long w[kBn];
+};
+
+__device__ void bn_zero(Bn* r)
+{
+	for (int i = 0; i < kBn; i++) r->w[i] = 0ull;
+}
+
+__device__ void bn_neg(Bn* r, const Bn& a)
+{
+	unsigned long long carry = 1ull;
+	for (int i = 0; i < kBn; i++) {
+		const unsigned long long x = ~a.w[i] + carry;
+		carry = (carry && x == 0ull) ? 1ull : 0ull;
+		r->w[i] = x;
+	}
+}
+
+__device__ void bn_add(Bn* r, const Bn& a, const Bn& b)
+{
+	unsigned long long carry = 0ull;
+	for (int i = 0; i < kBn; i++) {
+		const unsigned long long s1 = a.w[i] + b.w[i];
+		const unsigned long long c1 = s1 < a.w[i] ? 1ull : 0ull;
+		const unsigned long long s2 = s1 + carry;
+		const unsigned long long c2 = s2 < s1 ? 1ull : 0ull;
+		r->w[i] = s2;
+		carry = c1 + c2;
+	}
+}
+
+__device__ void bn_sub(Bn* r, const Bn& a, const Bn& b)
+{
+	Bn nb;
+	bn_neg(&nb, b);
+	bn_add(r, a, nb);
+}
+
+__device__ int bn_sign(const Bn& a)
+{
+	if (a.w[kBn - 1] >> 63) return -1;
+	for (int i = 0; i < kBn; i++)
+		if (a.w[i]) return 1;
+	return 0;
+}
+
+__device__ void bn_mul(Bn* r, const Bn& a, const Bn& b)
+{
+	Bn x = a, y = b;
+	const int sa = bn_sign(x), sb = bn_sign(y);
+	if (sa < 0) bn_neg(&x, x);
+	if (sb < 0) bn_neg(&y, y);
+	unsigned long long out[kBn];
+	for (int i = 0; i < kBn; i++) out[i] = 0ull;
+	for (int i = 0; i < kBn; i++) {
+		if (x.w[i] == 0ull) continue;
+		unsigned long long carry = 0ull;
+		for (int j = 0; i + j < kBn; j++) {
+			const unsigned long long lo = x.w[i] * y.w[j];
+			const unsigned long long hi = __umul64hi(x.w[i], y.w[j]);
+			unsigned long long s = out[i + j] + lo;
+			unsigned long long c = s < lo ? 1ull : 0ull;
+			const unsigned long long s2 = s + carry;
+			c += s2 < s ? 1ull : 0ull;
+			out[i + j] = s2;
+			carry = hi + c;
+		}
+	}
+	for (int i = 0; i < kBn; i++) r->w[i] = out[i];
+	if ((sa < 0) != (sb < 0)) bn_neg(r, *r);
+}
+
+// v = m * 2^(e-24) with a 24-bit integer m, scaled to 2^emin
+__device__ void bn_from_float(Bn* r, float v, int emin)
+{
+	bn_zero(r);
+	if (v == 0.0f) return;
+	int e;
+	const float fm = frexpf(v, &e);
+	const long long m = (long long) ldexpf(fm, 24);
+	const int shift = (e - 24) - emin;
+	const unsigned long long um = (unsigned long long) (m < 0 ? -m : m);
+	const int limb = shift / 64, bit = shift % 64;
+	if (limb < kBn) r->w[limb] |= um << bit;
+	if (bit != 0 && limb + 1 < kBn) r->w[limb + 1] |= um >> (64 - bit);
+	if (m < 0) bn_neg(r, *r);
+}
+
+__device__ __noinline__ int orient_exact(const float* a, const float* b, const float* c, const float* d)
+{
+	const float* p[4] = {a, b, c, d};
+	int emin = 1 << 30;
+	for (int i = 0; i < 4; i++)
+		for (int k = 0; k < 3; k++)
+			if (p[i][k] != 0.0f) {
+				int e;
+				frexpf(p[i][k], &e);
+				if (e - 24 < emin) emin = e - 24;
+			}
+	Bn m[3][3];
+	for (int i = 0; i < 3; i++)
+		for (int k = 0; k < 3; k++) {
+			Bn x, y;
+			bn_from_float(&x, p[i][k], emin);
+			bn_from_float(&y, p[3][k], emin);
+			bn_sub(&m[i][k], x, y);
+		}
+	Bn t1, t2, minor, acc, term;
+	bn_zero(&acc);
+	bn_mul(&t1, m[1][1], m[2][2]); bn_mul(&t2, m[1][2], m[2][1]); bn_sub(&minor, t1, t2);
+	bn_mul(&term, m[0][0], minor); bn_add(&acc, acc, term);
+	bn_mul(&t1, m[1][0], m[2][2]); bn_mul(&t2, m[1][2], m[2][0]); bn_sub(&minor, t1, t2);
+	bn_mul(&term, m[0][1], minor); bn_sub(&acc, acc, term);
+	bn_mul(&t1, m[1][0], m[2][1]); bn_mul(&t2, m[1][1], m[2][0]); bn_sub(&minor, t1, t2);
+	bn_mul(&term, m[0][2], minor); bn_add(&acc, acc, term);
+	return bn_sign(acc);
+}
+
+// sign of det[a-d; b-d; c-d] (the orient3d convention of the reference's
+// predicates, used through Geometry_3d::Table::tetOrient): a double
+// evaluation with a forward error bound, exact big integers otherwise
+__device__ int orient_sign_f(const float* a, const float* b, const float* c, const float* d)
+{
+	const double adx = (double) a[0] - d[0], ady = (double) a[1] - d[1], adz = (double) a[2] - d[2];
+	const double bdx = (double) b[0] - d[0], bdy = (double) b[1] - d[1], bdz = (double) b[2] - d[2];
+	const double cdx = (double) c[0] - d[0], cdy = (double) c[1] - d[1], cdz = (double) c[2] - d[2];
+	const double det = adx * (bdy * cdz - bdz * cdy) - ady * (bdx * cdz - bdz * cdx) + adz * (bdx * cdy - bdy * cdx);
+	const double perm = fabs(adx) * (fabs(bdy * cdz) + fabs(bdz * cdy)) + fabs(ady) * (fabs(bdx * cdz) + fabs(bdz * cdx)) +
+	                    fabs(adz) * (fabs(bdx * cdy) + fabs(bdy * cdx));
+	const double bound = 1e-14 * perm;
+	if (det > bound) return 1;
+	if (det < -bound) return -1;
+	if (perm == 0.0) return 0;
+	return orient_exact(a, b, c, d);
+}
+
+// ---------------------------------------------------------------- point location and weights
+
+struct Mesh3 {
+	const float* xyz;
+	const int* tv;
+	const int* tn;
+	int m;
+	__device__ const float* P(int i) const { return xyz + 3 * (size_t) i; }
+};
+
+// findTetrahedron walk from t (geometry_3d.cpp:988-1044); -1 outside the
+// mesh, -2 when the walk does not end
+__device__ int find_tet(const Mesh3& g, const float* pos, int t)
+{
+	const int limit = min(g.m, 1 << 20);
+	for (int loop = 0; loop < limit; loop++) {
+		if (t < 0) return -1;
+		const int* v = g.tv + 4 * (size_t) t;
+		int index;
+		for (index = 0; index < 4; index++) {
+			// tetOrient(n1, n2, n3, pos) = orient3d(p2, p1, p3, pos)
+			const int n1 = v[index % 4], n2 = v[3 - (2 + index) % 4], n3 = v[(2 + index) % 4];
+			if (orient_sign_f(g.P(n2), g.P(n1), g.P(n3), pos) < 0) {
+				t = g.tn[4 * (size_t) t + 3 - index % 4];
+				break;
+			}
+		}
+		if (index == 4) return t;
+	}
+	return -2;
+}
+
+// tapsim/utils.h:212-226 (float): determinant from the LU diagonal by
+// mantissa products and an exponent sum
+__device__ float lu_det_f(const float* lu, float d)
+{
+	float x = d;
+	int y = 0;
+	for (int i = 0; i < 3; i++) {
+		int tmp;
+		x *= frexpf(lu[i * 3 + i], &tmp);
+		y += tmp;
+	}
+	return ldexpf(x, y);
+}
+
+__device__ float tet_volume(const Mesh3& g, int t)
+{
+	float lumat[9];
+	const int* v = g.tv + 4 * (size_t) t;
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) lumat[i * 3 + j] = g.P(v[i])[j] - g.P(v[3])[j];
+	float d;
+	int idx[3];
+	if (!lu_decmp<float, 3>(lumat, idx, &d)) return 0.0f;
+	float value = lu_det_f(lumat, d);
+	value /= 6.0f;
+	return fabsf(value);
+}
+
+__device__ void tet_weights(const Mesh3& g, int t, const float* p, float* w)
+{
+	const float volume = tet_volume(g, t);
+	if (volume == 0.0f) {
+		for (int i = 0; i < 4; i++) w[i] = 0.25f;
+		return;
+	}
+	const int* v = g.tv + 4 * (size_t) t;
+	for (int i = 0; i < 4; i++) {
+		float lumat[9];
+		for (int j = 0; j < 3; j++)
+			for (int k = 0; k < 3; k++) lumat[j * 3 + k] = g.P(v[(i + j) % 4])[k] - p[k];
+		float d;
+		int idx[3];
+		if (!lu_decmp<float, 3>(lumat, idx, &d)) {
+			w[(i + 3) % 4] = 0.0f;
+			continue;
+		}
+		float x = lu_det_f(lumat, d);
+		x /= 6.0f;
+		x = fabsf(x);
+		x /= volume;
+		w[(i + 3) % 4] = x;
+	}
+}
+
+// FastField::compute: field at pos from the node snapshot (zero outside the mesh)
+__device__ void fast_field(const Mesh3& g, const float* nfield, const float* pos, int guess, float* out)
+{
+	out[0] = out[1] = out[2] = 0.0f;
+	const int t = find_tet(g, pos, guess);
+	if (t < 0) return;
+	float w[4];
+	tet_weights(g, t, pos, w);
+	const int* v = g.tv + 4 * (size_t) t;
+	for (int k = 0; k < 4; k++)
+		for (int c = 0; c < 3; c++) out[c] += w[k] * nfield[3 * (size_t) v[k] + c];
+}
+
+// ---------------------------------------------------------------- integration
+
+struct Phase {
+	float t;
+	float p[3], v[3];
+	int tet;
+};
+
+struct Tr {
+	Mesh3 g;
+	const float* nfield;
+	float charge, mass;
+	int integrator, stepper;
+	float dt_min;
+	bool geom_error;       // the work limit ended the integration
+	long long calls, max_calls;
+};
+
+// Trajectory_3d::o5_integrator (trajectory_3d.cpp:583-676): Cash-Karp
+// embedded Runge-Kutta 4(5), tableau of Cash & Karp (1990)
+__device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
+{
+	const float b[6][5] = {{0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+	                       {1.0f / 5.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+	                       {3.0f / 40.0f, 9.0f / 40.0f, 0.0f, 0.0f, 0.0f},
+	                       {3.0f / 10.0f, -9.0f / 10.0f, 6.0f / 5.0f, 0.0f, 0.0f},
+	                       {-11.0f / 54.0f, 5.0f / 2.0f, -70.0f / 27.0f, 35.0f / 27.0f, 0.0f},
+	                       {1631.0f / 55296.0f, 175.0f / 512.0f, 575.0f / 13824.0f, 44275.0f / 110592.0f, 253.0f / 4096.0f}};
+	const float c1[6] = {37.0f / 378.0f, 0.0f, 250.0f / 621.0f, 125.0f / 594.0f, 0.0f, 512.0f / 1771.0f};
+	const float c2[6] = {2825.0f / 27648.0f, 0.0f, 18575.0f / 48384.0f, 13525.0f / 55296.0f, 277.0f / 14336.0f, 1.0f / 4.0f};
+	const float qm = tr.charge / tr.mass;
+	float dv[6][3], dp[6][3];
+	for (int i = 0; i < 6; i++) {
+		float vel[3], pos[3];
+		for (int l = 0; l < 3; l++) {
+			vel[l] = back.v[l];
+			pos[l] = back.p[l];
+		}
+		for (int j = 0; j < i; j++)
+			for (int l = 0; l < 3; l++) {
+				vel[l] += b[i][j] * dv[j][l];
+				pos[l] += b[i][j] * dp[j][l];
+			}
+		fast_field(tr.g, tr.nfield, pos, back.tet, dv[i]);
+		for (int l = 0; l < 3; l++) dv[i][l] *= qm;
+		for (int l = 0; l < 3; l++) dv[i][l] *= delta;
+		for (int l = 0; l < 3; l++) dp[i][l] = vel[l] * delta;
+	}
+	Phase next = back;
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 6; j++) {
+			next.p[i] += c1[j] * dp[j][i];
+			next.v[i] += c1[j] * dv[j][i];
+		}
+	next.t += delta;
+	error->t = 0.0f;
+	for (int i = 0; i < 3; i++) {
+		error->p[i] = 0.0f;
+		error->v[i] = 0.0f;
+		for (int j = 0; j < 6; j++) {
+			error->p[i] += (c1[j] - c2[j]) * dp[j][i];
+			error->v[i] += (c1[j] - c2[j]) * dv[j][i];
+		}
+	}
+	next.tet = find_tet(tr.g, next.p, back.tet);
+	return next;
+}
+
+// Trajectory_3d::o1_integrator (trajectory_3d.cpp:549-581)
+__device__ Phase o1_step(Tr& tr, const Phase& back, float delta, Phase* error)
+{
+	float f[3];
+	fast_field(tr.g, tr.nfield, back.p, back.tet, f);
+	for (int i = 0; i < 3; i++) f[i] *= tr.charge;
+	for (int i = 0; i < 3; i++) f[i] /= tr.mass;
+	Phase next = back;
+	for (int i = 0; i < 3; i++) {
+		next.v[i] += delta * f[i];
+		next.p[i] += delta * back.v[i];
+	}
+	next.t += delta;
+	error->t = 0.0f;
+	const float d2 = delta * delta;
+	for (int i = 0; i < 3; i++) {
+		error->p[i] = next.p[i] * d2;
+		error->v[i] = next.v[i] * d2;
+	}
+	next.tet = find_tet(tr.g, next.p, back.tet);
+	return next;
+}
+
+__device__ float rel_error(const Phase& value, const Phase& error)
+{
+	float rel = 0.0f;
+	for (int i = 0; i < 3; i++) {
+		float tmp;
+		if (value.p[i] != 0.0f) {
+			tmp = error.p[i];
+			tmp /= value.p[i];
+			tmp = fabsf(tmp);
+			if (rel < tmp) rel = tmp;
+		}
+		if (value.v[i] != 0.0f) {
+			tmp = error.v[i];
+			tmp /= value.v[i];
+			tmp = fabsf(tmp);
+			if (rel < tmp) rel = tmp;
+		}
+	}
+	return rel;
+}
+
+// a tet of adjacentTetrahedra(t) (geometry_3d.cpp:1046-1096): another tet
+// sharing a vertex with t
+__device__ bool adjacent(const Mesh3& g, int t, int u)
+{
+	if (u < 0 || u == t) return false;
+	const int* a = g.tv + 4 * (size_t) t;
+	const int* b = g.tv + 4 * (size_t) u;
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			if (a[i] == b[j]) return true;
+	return false;
+}
+
+__device__ bool boundary_tet(const Mesh3& g, const unsigned char* gbnd, int t)
+{
+	for (int q = 0; q < 4; q++)
+		if (gbnd[g.tv[4 * (size_t) t + q]]) return true;
+	return false;
+}
+
+// the steppers (trajectory_3d.cpp:358-532): 0 none, 1 error-restricted,
+// 2 geometry-restricted, 3 adaptive
+__device__ Phase stepper_step(Tr& tr, const unsigned char* gbnd, const Phase& back, float* delta, float err_limit,
+                              Phase* error)
+{
+	const float safety = 0.9f;
+	for (;;) {
+		if (++tr.calls > tr.max_calls) {  // bounds the work of one ion
+			tr.geom_error = true;
+			return back;
+		}
+		if (*delta < tr.dt_min) *delta = tr.dt_min;
+		const Phase value = tr.integrator == 0 ? o1_step(tr, back, *delta, error) : o5_step(tr, back, *delta, error);
+		if (tr.stepper == 0) return value;
+		if (tr.stepper == 2) {
+			if (value.tet < 0 && boundary_tet(tr.g, gbnd, back.tet)) return value;
+			if (adjacent(tr.g, back.tet, value.tet)) return value;
+			if (value.tet == back.tet)
+				*delta *= 1.1f;
+			else {
+				if (*delta == tr.dt_min) return value;
+				*delta *= 0.9f;
+			}
+			continue;
+		}
+		if (tr.stepper == 3) {
+			if (value.tet >= 0 && value.tet != back.tet && !adjacent(tr.g, back.tet, value.tet)) {
+				*delta *= 0.1f;
+				continue;
+			}
+		}
+		const float rel = rel_error(value, *error);
+		if (rel < err_limit || *delta == tr.dt_min) {
+			if (rel != 0.0f) {
+				float tmp = *delta;
+				tmp *= powf(err_limit / rel, 0.25f);
+				tmp *= safety;
+				if (tmp < *delta) *delta *= 1.1f;
+				else *delta = tmp;
+			}
+			return value;
+		}
+		float tmp = *delta;
+		tmp *= powf(err_limit / rel, 0.25f);
+		tmp *= safety;
+		if (tmp > *delta) *delta *= 0.9f;
+		else *delta = tmp;
+	}
+}
+
+__device__ void put(const TrajArgs& a, int ion, int k, const Phase& q)
+{
+	if (k >= a.cap) return;
+	float* o = a.out + ((size_t) ion * a.cap + k) * 8;
+	o[0] = q.t;
+	for (int c = 0; c < 3; c++) {
+		o[1 + c] = q.p[c];
+		o[4 + c] = q.v[c];
+	}
+	o[7] = (float) q.tet;
+}
+
+// Trajectory_3d::init / integrate(delta, errorLimit) / extrapolate(Z, zmax)
+// (trajectory_3d.cpp:99-126, 130-205, 303-332) for ion `i` of the batch
+__global__ void k_trajectory(TrajArgs a)
+{
+	const int ion = blockIdx.x * blockDim.x + threadIdx.x;
+	if (ion >= a.count) return;
+	Tr tr;
+	tr.g.xyz = a.xyz;
+	tr.g.tv = a.tv;
+	tr.g.tn = a.tn;
+	tr.g.m = a.m;
+	tr.nfield = a.nfield + (size_t) a.snapshot[ion] * 3 * (size_t) a.n;
+	tr.charge = a.charge[ion];
+	tr.mass = a.mass[ion];
+	tr.integrator = a.integrator;
+	tr.stepper = a.stepper;
+	tr.dt_min = a.dt_min;
+	tr.geom_error = false;
+	tr.calls = 0;
+	tr.max_calls = a.max_calls;
+	const int node = a.nodes[ion];
+	Phase back;
+	back.t = 0.0f;
+	for (int k = 0; k < 3; k++) {
+		back.p[k] = a.xyz[3 * (size_t) node + k];
+		back.v[k] = 0.0f;
+	}
+	back.tet = find_tet(tr.g, back.p, a.assoc[node]);
+	int status;
+	int len = 0;
+	if (back.tet < 0) {
+		a.len[ion] = 0;
+		a.status[ion] = -10;  // initial point outside the mesh
+		return;
+	}
+	put(a, ion, len++, back);
+	float f[3];
+	fast_field(tr.g, tr.nfield, back.p, back.tet, f);
+	if (f[0] == 0.0f && f[1] == 0.0f && f[2] == 0.0f)
+		status = 1;  // NO_FIELD
+	else {
+		float delta = a.dt0;
+		for (;;) {
+			Phase error;
+			const Phase next = stepper_step(tr, a.gbnd, back, &delta, a.err_limit, &error);
+			if (tr.geom_error) break;
+			back = next;
+			put(a, ion, len++, back);
+			if (back.tet < 0) {
+				status = 5;  // SYSTEM_LIMIT: the ion left the mesh
+				break;
+			}
+			if (len >= a.max_steps) {
+				status = 3;  // ITERATION_LIMIT
+				break;
+			}
+		}
+	}
+	if (tr.geom_error) {
+		a.len[ion] = len;
+		a.status[ion] = -4;  // work limit
+		return;
+	}
+	// extrapolate(Z, zmax)
+	{
+		float dt = a.zmax;
+		dt -= back.p[2];
+		dt /= back.v[2];
+		Phase item;
+		item.t = back.t;
+		item.t += dt;
+		for (int k = 0; k < 3; k++) item.p[k] = back.p[k];
+		for (int k = 0; k < 3; k++) item.p[k] += dt * back.v[k];
+		for (int k = 0; k < 3; k++) item.v[k] = back.v[k];
+		item.tet = find_tet(tr.g, item.p, back.tet);
+		put(a, ion, len++, item);
+	}
+	a.len[ion] = len;
+	a.status[ion] = status;
+}
+
+// field_o1 of every node into a float snapshot (the FastField cache)
+template <typename R>
+__global__ void k_field_snapshot(int n, const float* xyz, const int* ptr, const int* idx, const R* coup,
+                                 const unsigned char* flags, const R* phi, float* out, int* status)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n) return;
+	R E[3];
+	if (!field_o1<R>(s, xyz, ptr, idx, coup, flags, phi, E)) {
+		atomicOr(status, 1);
+		E[0] = E[1] = E[2] = (R) 0;
+	}
+	for (int i = 0; i < 3; i++) out[3 * (size_t) s + i] = (float) E[i];
+}
+
+}  // namespace dev
+
+int launch_trajectories(hipStream_t s, const TrajArgs& a)
+{
+	if (a.count <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_trajectory, dim3((a.count + 63) / 64), dim3(64), 0, s, a);
+	return (int) hipGetLastError();
+}
+
+int launch_field_snapshot(hipStream_t s, int n, const float* xyz, const int* ptr, const int* idx, const void* coup,
+                          const unsigned char* flags, const void* phi, bool f64, float* out, int* status)
+{
+	if (n <= 0) return 0;
+	if (f64)
+		hipLaunchKernelGGL(dev::k_field_snapshot<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, xyz, ptr, idx,
+		                   (const double*) coup, flags, (const double*) phi, out, status);
+	else
+		hipLaunchKernelGGL(dev::k_field_snapshot<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, xyz, ptr, idx,
+		                   (const float*) coup, flags, (const float*) phi, out, status);
+	return (int) hipGetLastError();
+}
+
+}  // namespace tsg

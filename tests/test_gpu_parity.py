@@ -223,3 +223,105 @@ def test_accelerated_events_track_plain_sequence(tip_small):
     np_, na = [e.node for e in ep], [e.node for e in ea]
     log("sequences plain %s accel %s" % (np_, na))
     assert np_ == na
+
+
+def test_trajectory_matches_reference(tip_small):
+    """Ion trajectory (Cash-Karp O5, error-restricted stepper) from the apex
+    atom after its removal, in the float-faithful grid, against the
+    reference's own trajectory.  The device's powf (the step-size control's
+    (tol/err)^0.25) is not glibc's, which differs from the correctly rounded
+    value for about one argument in 1500, so after such a step the time step
+    may differ in its last bit; the comparison is bit-exact up to the first
+    such step and within 1e-4 relative after it."""
+    node = int(tip_small["local_node"])
+    log("trajectory from node %d" % node)
+    g = grid(tip_small, tsg.FAITHFUL_F32)
+    for _ in range(300):
+        g.relax(1)
+    g.set_id(node, 0)
+    g.relax(1)
+    q = 2.0 * np.float32(1.60217733e-19)
+    m = np.float32(55.845) * np.float32(1.660538782e-27)
+    (st, tr), = g.trajectories([node], q, m, integrator=1, stepper=1, cap=8192)
+    ref = tip_small["ref_traj"]
+    same = 0
+    while same < min(len(tr), len(ref)) - 1 and np.array_equal(tr[same].view(np.uint32), ref[same].view(np.uint32)):
+        same += 1
+    log("status %d (ref %d), %d phases (ref %d), bit-identical for the first %d"
+        % (st, tip_small["ref_traj_status"][0], len(tr), len(ref), same))
+    assert st == tip_small["ref_traj_status"][0]
+    assert abs(len(tr) - len(ref)) <= max(2, len(ref) // 50)
+    assert same >= min(20, len(ref) - 1)
+    # the last in-mesh phase and the extrapolated point
+    span = np.max(np.abs(ref[:, 1:4]))
+    for k in (-2, -1):
+        assert np.max(np.abs(tr[k, 1:4] - ref[k, 1:4])) <= 1e-4 * span, (k, tr[k], ref[k])
+        assert np.max(np.abs(tr[k, 4:7] - ref[k, 4:7])) <= 1e-4 * np.max(np.abs(ref[k, 4:7]))
+
+
+def test_trajectory_batch_matches_single(tip_small):
+    """A batch of ions (one device thread each) gives the same trajectories
+    as one ion at a time."""
+    log("trajectory batch")
+    g = grid(tip_small, tsg.PLAIN_F64)
+    for _ in range(200):
+        g.relax(1)
+    g.surface_init(0)
+    nodes = g.surface_nodes()[::max(1, len(g.surface_nodes()) // 24)][:24]
+    q = 2.0 * np.float32(1.60217733e-19)
+    m = np.float32(55.845) * np.float32(1.660538782e-27)
+    batch = g.trajectories(nodes, q, m, cap=4096)
+    for k in (0, len(nodes) // 2, len(nodes) - 1):
+        (st, tr), = g.trajectories([nodes[k]], q, m, cap=4096)
+        assert st == batch[k][0]
+        assert np.array_equal(tr.view(np.uint32), batch[k][1].view(np.uint32))
+    log("statuses %s, lengths %s" % ([b[0] for b in batch], [len(b[1]) for b in batch]))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("precision", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_probability_models(tip_small, mode, precision):
+    """Surface probabilities of the four models of evap_compute_probabilities
+    against the oracle: bit-exact, except probBoltzmann's exponential (the
+    device exp is not glibc's: within 4 ulp)."""
+    log("probability model %d precision %d" % (mode, precision))
+    mesh = tsg.Mesh(tip_small["xyz"], tip_small["tets"], tip_small["ref_neigh"])
+    g = tsg.Grid3D(mesh, tip_small["ids"], tsg.species_table(FIXTURE_SPECIES), 300.0, precision)
+    o = O.Oracle(O.F32 if precision == tsg.FAITHFUL_F32 else O.F64, tip_small["xyz"], tip_small["ids"],
+                 tip_small["tets"], tip_small["ref_neigh"], O.species_table(FIXTURE_SPECIES), temperature=300.0)
+    o.sync()
+    for _ in range(120):
+        g.relax(1)
+        o.relax(1)
+    g.surface_init(0)
+    o.surface_init(0)
+    assert np.array_equal(g.surface_nodes(), o.surface_nodes())
+    ref_g = g.surface_evaluate(True, mode)
+    ref_o = o.specific_fields()
+    o.compute_probabilities(mode)
+    pg, po = g.surface_probabilities(), o.surface_probabilities()
+    assert np.float32(ref_g) == np.float32(ref_o)
+    if mode == 1:
+        np.testing.assert_allclose(pg, po, rtol=5e-7 if precision == tsg.FAITHFUL_F32 else 1e-14, atol=0)
+    else:
+        assert same_bits(pg, po, np.float32)
+
+
+def test_monte_carlo_choice(tip_small):
+    """evapMonteCarlo draws from the process's rand(): with the same seed the
+    device event loop picks the oracle's sequence."""
+    log("monte carlo choice")
+    g = grid(tip_small, tsg.PLAIN_F64)
+    o = oracle(tip_small, O.F64)
+    o.sync()
+    for _ in range(300):
+        g.relax(1)
+        o.relax(1)
+    eg, eo = evap_opts(), ora_opts()
+    eg.evap_mode, eo.evap_mode = 1, 1
+    tsg.srand(99)
+    evg = g.evaporate(eg, 12)
+    o.srand(99)
+    evo = o.evaporate(eo, 12)
+    assert [e.node for e in evg] == [e.node for e in evo]
+    assert same_bits(g.phi(0), o.phi(0), np.float64)

@@ -133,3 +133,20 @@ def test_trajectory(tip_small):
     assert bits_equal(tr[:-1], ref[:-1])
     assert bits_equal(tr[-1, :7], ref[-1, :7])
     assert ref[-1, 7] == -1 and tr[-1, 2] == ref[-1, 2]
+
+
+def test_threaded_relax_equals_serial(tip_small):
+    """The CPU baseline's threaded sweep (the reference's chunked scheme)
+    gives the serial relax(1)'s potentials and deviation bit for bit."""
+    a = O.Oracle(O.F64, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                 O.species_table(FIXTURE_SPECIES))
+    b = O.Oracle(O.F64, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                 O.species_table(FIXTURE_SPECIES))
+    a.sync()
+    b.sync()
+    for _ in range(7):
+        d = a.relax(1)
+    dt, sec = b.relax_threaded(7, 4)
+    assert d == dt and sec > 0
+    assert a.switches() == b.switches()
+    assert np.array_equal(a.phi(0), b.phi(0)) and np.array_equal(a.phi(1), b.phi(1))
