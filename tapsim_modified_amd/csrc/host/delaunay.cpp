@@ -21,12 +21,6 @@
 #include "predicates.hpp"
 
 namespace tsg {
-namespace {
-
-struct Tet {
-	int v[4];
-	int n[4];  // (tet << 2) | face of the neighbour, -1 outside
-};
 
 uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits)
 {
@@ -54,6 +48,13 @@ uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits)
 		for (int i = 0; i < 3; i++) key = (key << 1) | ((X[i] >> b) & 1u);
 	return key;
 }
+
+namespace {
+
+struct Tet {
+	int v[4];
+	int n[4];  // (tet << 2) | face of the neighbour, -1 outside
+};
 
 class Builder {
 public:

@@ -347,8 +347,11 @@ int Grid::build_rows()
 		if (slot_deg_[sl] > kLongRow) long_set_.insert(sl);
 	long_cap_ = (int) long_set_.size() + 4096;
 	d_long_.alloc(sizeof(int) * (size_t) long_cap_);
-	d_block_max_.alloc(sizeof(double) * (cap_slots_ / 256 + 2 + long_cap_ / 4 + 1));
-	d_block_flag_.alloc(sizeof(int) * (cap_slots_ / 256 + 2 + long_cap_ / 4 + 1));
+	// one partial per workgroup of the largest row launch (row_launch_blocks:
+	// slot groups + long-row groups, padded to a multiple of eight)
+	const size_t max_blocks = (size_t) row_launch_blocks(cap_slots_, long_cap_) + 8;
+	d_block_max_.alloc(sizeof(double) * max_blocks);
+	d_block_flag_.alloc(sizeof(int) * max_blocks);
 	if (!d_long_.ptr() || !d_block_max_.ptr()) return fail(-2, "device allocation failed (rows)");
 	long_dirty_ = true;
 	if (int rc = upload_long()) return rc;

@@ -40,6 +40,9 @@ struct Specimen {
 
 void generate_specimen(const SpecimenParams&, Specimen*);
 
+// key of a point on the 3-D Hilbert curve, coordinates quantised to `bits` bits
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits);
+
 // Tetrahedralisation result, in the reference's convention
 // (Geometry_3d::Table::TetrahedronData, geometry_3d.h:164-176): neighbours[i]
 // is the tet across the face opposite vertices[i] (-1 on the hull), every

@@ -182,6 +182,40 @@ void generate_specimen(const SpecimenParams& P, Specimen* out)
 				push(x, y, z, id);
 				out->n_box++;
 			}
+
+	// node numbering along a 3-D Hilbert curve: nodes close in space get
+	// close numbers, so a wavefront of consecutive rows gathers the
+	// potentials of a compact range of nodes (the device sweeps' locality)
+	const size_t n = out->ids.size();
+	float lo[3], hi[3];
+	for (int k = 0; k < 3; k++) lo[k] = hi[k] = out->xyz[k];
+	for (size_t i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++) {
+			lo[k] = std::min(lo[k], out->xyz[3 * i + k]);
+			hi[k] = std::max(hi[k], out->xyz[3 * i + k]);
+		}
+	double ext = 0.0;
+	for (int k = 0; k < 3; k++) ext = std::max(ext, (double) hi[k] - lo[k]);
+	const int bits = 21;
+	const double scale = ext > 0.0 ? ((1u << bits) - 1) / ext : 0.0;
+	std::vector<uint64_t> key(n);
+	for (size_t i = 0; i < n; i++) {
+		uint32_t q[3];
+		for (int k = 0; k < 3; k++)
+			q[k] = (uint32_t) std::min<double>((1u << bits) - 1, std::max(0.0, (out->xyz[3 * i + k] - lo[k]) * scale));
+		key[i] = hilbert3(q[0], q[1], q[2], bits);
+	}
+	std::vector<size_t> order(n);
+	for (size_t i = 0; i < n; i++) order[i] = i;
+	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+	std::vector<float> xyz(3 * n);
+	std::vector<int16_t> ids(n);
+	for (size_t i = 0; i < n; i++) {
+		for (int k = 0; k < 3; k++) xyz[3 * i + k] = out->xyz[3 * order[i] + k];
+		ids[i] = out->ids[order[i]];
+	}
+	out->xyz.swap(xyz);
+	out->ids.swap(ids);
 }
 
 }  // namespace tsg

@@ -28,12 +28,14 @@ __device__ __forceinline__ double wave_maxd(double v)
 __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __restrict__ phi, float* __restrict__ r32)
 {
 	const int main_blocks = (a.nslots + 255) >> 8;
+	const RowBlock rb = row_block(a.nlong);
+	const int lb = rb.is_long ? -1 : rb.index;
 	int slot = -1, node = -1, deg = 0;
 	long long soff = 0;
 	double q = 0.0, ws = 1.0;
-	if ((int) blockIdx.x < main_blocks) {
-		slot = blockIdx.x * 256 + threadIdx.x;
-		if (slot < a.nslots) {
+	if (!rb.is_long) {
+		if (lb < main_blocks) slot = lb * 256 + threadIdx.x;
+		if (slot >= 0 && slot < a.nslots) {
 			node = a.slot_node[slot];
 			deg = a.slot_deg[slot];
 			soff = a.slice_off[slot >> 6];
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __re
 			ws = a.ws[slot];
 		}
 	} else {
-		const int li = long_row_index(main_blocks);
+		const int li = long_row_index(rb);
 		if (li < a.nlong) {
 			slot = a.long_slots[li];
 			node = a.slot_node[slot];
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __re
 	}
 	const long long base = soff + (slot & 63);
 	double lv = 0.0, old = 0.0;
-	if ((int) blockIdx.x < main_blocks) {
+	if (!rb.is_long) {
 		const bool act = node >= 0 && deg <= kLongRow;
 		if (act) old = phi[node];
 		else node = -1;
@@ -114,18 +116,20 @@ __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __rest
                                          const float* __restrict__ e1, float* v1, const float* __restrict__ e2, float* v2)
 {
 	const int main_blocks = (x.nslots + 255) >> 8;
+	const RowBlock rb = row_block(x.nlong);
+	const int lb = rb.is_long ? -1 : rb.index;
 	int slot = -1, node = -1, deg = 0, ideg = 0;
 	long long soff = 0;
-	if ((int) blockIdx.x < main_blocks) {
-		slot = blockIdx.x * 256 + threadIdx.x;
-		if (slot < x.nslots) {
+	if (!rb.is_long) {
+		if (lb < main_blocks) slot = lb * 256 + threadIdx.x;
+		if (slot >= 0 && slot < x.nslots) {
 			node = x.slot_node[slot];
 			deg = x.slot_deg[slot];
 			ideg = x.inner_deg[slot];
 			soff = x.slice_off[slot >> 6];
 		}
 	} else {
-		const int li = long_row_index(main_blocks);
+		const int li = long_row_index(rb);
 		if (li < x.nlong) {
 			slot = x.long_slots[li];
 			node = x.slot_node[slot];
@@ -136,7 +140,7 @@ __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __rest
 	}
 	const long long base = soff + (slot & 63);
 	*sum = 0.0f;
-	if ((int) blockIdx.x < main_blocks) {  // block-uniform; the row sum needs the whole wavefront
+	if (!rb.is_long) {  // block-uniform; the row sum needs the whole wavefront
 		const bool act = node >= 0 && deg <= kLongRow;
 		if (act) {
 			if (e1) *v1 = e1[node];

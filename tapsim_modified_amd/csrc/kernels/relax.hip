@@ -114,13 +114,15 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 	const R* pin = (const R*) (read ? a.phi1 : a.phi0);
 	R* pout = (R*) (read ? a.phi0 : a.phi1);
 	const int main_blocks = (a.nslots + 255) >> 8;
+	const RowBlock rb = row_block(a.nlong);
+	const int lb = rb.is_long ? -1 : rb.index;
 	// the per-slot record is loaded in one round before any branch on it
 	int slot = -1, node = -1, deg = 0;
 	long long soff = 0;
 	double q = 0.0, ws = 1.0;
-	if ((int) blockIdx.x < main_blocks) {
-		slot = blockIdx.x * 256 + threadIdx.x;
-		if (slot < a.nslots) {
+	if (!rb.is_long) {
+		if (lb < main_blocks) slot = lb * 256 + threadIdx.x;
+		if (slot >= 0 && slot < a.nslots) {
 			node = a.slot_node[slot];
 			deg = a.slot_deg[slot];
 			soff = a.slice_off[slot >> 6];
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 			ws = a.ws[slot];
 		}
 	} else {
-		const int li = long_row_index(main_blocks);
+		const int li = long_row_index(rb);
 		if (li < a.nlong) {
 			slot = a.long_slots[li];
 			node = a.slot_node[slot];
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 	const long long base = soff + (slot & 63);
 	double lv = 0.0;
 	R old = (R) 0;
-	if ((int) blockIdx.x < main_blocks) {  // block-uniform
+	if (!rb.is_long) {  // block-uniform
 		const bool act = node >= 0 && deg <= kLongRow;  // else empty slot, or on the long-row list
 		if (act) old = pin[node];
 		else node = -1;
@@ -329,7 +331,14 @@ __global__ void k_scatter_i32(int n, const long long* addr, const int* val, int*
 
 }  // namespace dev
 
-int row_launch_blocks(int nslots, int nlong) { return (nslots + 255) / 256 + (nlong + 3) / 4; }
+// long-row workgroups (one wavefront per long row) then slot workgroups,
+// each part padded to a multiple of the XCD count (see row_block)
+int row_launch_blocks(int nslots, int nlong)
+{
+	const int lg = ((nlong + 3) / 4 + 7) / 8 * 8;
+	const int mg = ((nslots + 255) / 256 + 7) / 8 * 8;
+	return lg + mg;
+}
 
 int launch_jacobi_sweep(hipStream_t s, const SweepArgs& a, int read, bool f64)
 {

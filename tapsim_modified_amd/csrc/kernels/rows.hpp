@@ -47,11 +47,41 @@ __device__ __forceinline__ float readlane_f32(float v, int l)
 	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// index of the long row handled by this wavefront in a launch whose first
-// `main_blocks` workgroups (256 threads) cover the row slots; wave-uniform
-__device__ __forceinline__ int long_row_index(int main_blocks)
+// Workgroups of a row launch.  The long rows come first, one wavefront
+// each, in a multiple of eight workgroups: they are the slowest wavefronts,
+// so they start at once and spread over all XCDs.  The slot workgroups
+// follow.  The hardware deals workgroups to the eight XCDs in turn
+// (workgroup b runs on XCD b % 8), each XCD with its own L2; the slot
+// workgroups are padded to a multiple of eight and XCD x takes the
+// contiguous range [x q, (x+1) q) of them, so each L2 holds one contiguous
+// share of the rows and of the potentials they gather.
+constexpr int kXcds = 8;
+struct RowBlock {
+	bool is_long;
+	int index;  // long: first long row of the workgroup; slot: slot workgroup
+};
+__device__ __forceinline__ int long_groups(int nlong) { return ((nlong + 3) / 4 + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ RowBlock row_block(int nlong)
 {
-	return __builtin_amdgcn_readfirstlane((int) ((blockIdx.x - main_blocks) * 4 + (threadIdx.x >> 6)));
+	const int lg = long_groups(nlong);
+	RowBlock r;
+	const int b = (int) blockIdx.x;
+	if (b < lg) {
+		r.is_long = true;
+		r.index = b * 4;
+	} else {
+		const int bm = b - lg;
+		const int q = ((int) gridDim.x - lg) / kXcds;
+		r.is_long = false;
+		r.index = (bm % kXcds) * q + bm / kXcds;
+	}
+	return r;
+}
+
+// the long row handled by this wavefront (wave-uniform)
+__device__ __forceinline__ int long_row_index(const RowBlock& rb)
+{
+	return __builtin_amdgcn_readfirstlane(rb.index + (int) (threadIdx.x >> 6));
 }
 
 // sum_k (double) p[col_k] * (double) c_k, added in entry order starting from

@@ -346,6 +346,11 @@ __device__ Phase o1_step(Tr& tr, const Phase& back, float delta, Phase* error)
 	return next;
 }
 
+// (tol/err)^(1/4) of the step-size control: the reference calls glibc's
+// powf; the correctly rounded fourth root (two double square roots, one
+// rounding to float) agrees with it on all but about one argument in 1500
+__device__ __forceinline__ float fourth_root(float x) { return (float) sqrt(sqrt((double) x)); }
+
 __device__ float rel_error(const Phase& value, const Phase& error)
 {
 	float rel = 0.0f;
@@ -422,7 +427,7 @@ __device__ Phase stepper_step(Tr& tr, const unsigned char* gbnd, const Phase& ba
 		if (rel < err_limit || *delta == tr.dt_min) {
 			if (rel != 0.0f) {
 				float tmp = *delta;
-				tmp *= powf(err_limit / rel, 0.25f);
+				tmp *= fourth_root(err_limit / rel);
 				tmp *= safety;
 				if (tmp < *delta) *delta *= 1.1f;
 				else *delta = tmp;
@@ -430,7 +435,7 @@ __device__ Phase stepper_step(Tr& tr, const unsigned char* gbnd, const Phase& ba
 			return value;
 		}
 		float tmp = *delta;
-		tmp *= powf(err_limit / rel, 0.25f);
+		tmp *= fourth_root(err_limit / rel);
 		tmp *= safety;
 		if (tmp > *delta) *delta *= 0.9f;
 		else *delta = tmp;
