@@ -109,11 +109,43 @@ def run_events(grid, opts, steps, tag, quiet=False):
     return events, dt, s0, s1
 
 
-def roofline(grid, kind, kernel):
+def roofline(grid, kind, kernel, workload):
     ms, nbytes = grid.time_kernel(kind, 200)
     achieved = nbytes / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "kernel": kernel, "bytes_per_launch": nbytes, "launch_us": ms * 1e3}
+            "traffic": traffic_of(workload, kernel), "kernel": kernel, "bytes_per_launch": nbytes,
+            "launch_us": ms * 1e3}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+COPY_BYTES = 256 << 20
+
+
+def counters_run(tsg, grid):
+    """Launches the mixed path's correction sweep and the FP64 sweep 100 times
+    each on the relaxed grid, and one device-to-device copy of COPY_BYTES,
+    whose FETCH_SIZE / WRITE_SIZE calibrate the counters of the others."""
+    import torch
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ms1, by1 = grid.time_kernel(1, 100)
+    ms0, by0 = grid.time_kernel(0, 100)
+    print(json.dumps({"copy_bytes": COPY_BYTES, "k_cheb32": {"bytes": by1, "us": ms1 * 1e3},
+                      "k_jacobi<double>": {"bytes": by0, "us": ms0 * 1e3}}))
+
+
+def traffic_of(workload, kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    each scaled by the calibration copy), from the committed measurement."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)[workload][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def first_difference(a, b):
@@ -131,6 +163,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 run (default: steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="kernel-only run for rocprofv3 --pmc: the relaxed grid's sweep kernels launched back to back "
+                         "plus a device copy of known size that calibrates the counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)", file=sys.stderr)
@@ -155,7 +190,7 @@ def main():
 
     t0 = time.perf_counter()
     plain = new_grid()
-    rho = plain.accel_setup(0.5)
+    rho = plain.accel_setup(0.9)
     outer, inner, res = plain.accel_relax(1e-8)
     state0 = (plain.phi(0), plain.phi(1)) + plain.switches()
     log("grid nnz %d; initial relaxation to 1e-8 V: rho %.8f, %d refinements, %d correction steps (%.1fs)"
@@ -178,15 +213,19 @@ def main():
     learned_opts.method, learned_opts.accel_tol = 2, 1e-8
     plain_opts = event_options(tsg)
 
+    if args.counters:
+        counters_run(tsg, plain)
+        return
+
     pev, pdt, ps0, ps1 = run_events(plain, plain_opts, Kp, "plain")
     psweeps = (ps1["sweeps"] - ps0["sweeps"]) / max(1, Kp)
-    roof_plain = roofline(plain, 0, "k_jacobi<double>")
+    roof_plain = roofline(plain, 0, "k_jacobi<double>", args.workload)
     log("plain FP64: %.3f events/s, %.0f sweeps/event" % (Kp / pdt, psweeps))
 
     mixed = new_grid(state0)
-    mixed.accel_setup(0.5)
+    mixed.accel_setup(0.9)
     learned = new_grid(state0)
-    learned.accel_setup(0.5)
+    learned.accel_setup(0.9)
     net = tsg.WarmStartNet(max_batch_rows=65536, data_rows=max(1 << 20, 4000 * (W + 8)),
                            data_edges=max(1 << 24, 60000 * (W + 8)), seed=1)
     learned.attach(net, record=True, predict=False)
@@ -209,7 +248,7 @@ def main():
         mixed.evaporate(mixed_opts, 1)
     aev, adt, as0, as1 = run_events(learned, learned_opts, K, "learned")
     mev, mdt, ms0, ms1 = run_events(mixed, mixed_opts, K, "mixed")
-    roof_acc = roofline(learned, 1, "k_cheb32")
+    roof_acc = roofline(learned, 1, "k_cheb32", args.workload)
     steps_l = sum(e.global_iter for e in aev) / max(1, len(aev))
     steps_m = sum(e.global_iter for e in mev) / max(1, len(mev))
     kept = sum(1 for e in aev if e.warm_start == 1)

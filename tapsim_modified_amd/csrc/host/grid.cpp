@@ -1108,7 +1108,7 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 	if (o.method == 2 && !(net_ && net_predict_))
 		return fail(-1, "method 2 needs a warm-start network attached with prediction on");
 	if (o.method >= 1 && !accel_)
-		if (int rc = accel_setup(0.5)) return rc;
+		if (int rc = accel_setup(0.9)) return rc;
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr, o.prob_mode)) return rc;
 	for (int ev = 0; ev < k; ev++) {
@@ -1332,16 +1332,16 @@ int Grid::accel_setup(double safety)
 int Grid::accel_relax(double tol, int* outer, long long* inner, double* final_residual)
 {
 	if (!accel_)
-		if (int rc = accel_setup(0.5)) return rc;
+		if (int rc = accel_setup(0.9)) return rc;
 	const int n = mesh_.n;
 	double* phi = (sw_ ? d_phi1_ : d_phi0_).as<double>();
 	double* other = (sw_ ? d_phi0_ : d_phi1_).as<double>();
 	SweepArgs a = sweep_args();
 	AccelArgs x = accel_args();
-	const double sigma = rho_ / (1.0 + std::sqrt(1.0 - rho_ * rho_));
+	double sigma = rho_ / (1.0 + std::sqrt(1.0 - rho_ * rho_));
 	long long steps = 0;
 	int o = 0;
-	double rmax = 0.0;
+	double rmax = 0.0, rprev = 0.0;
 	if (!ev_start_) {
 		HIPCHK(hipEventCreate(&ev_start_));
 		HIPCHK(hipEventCreate(&ev_stop_));
@@ -1357,6 +1357,14 @@ int Grid::accel_relax(double tol, int* outer, long long* inner, double* final_re
 		if (red[1] != 0.0) return fail(-3, "accelerated relaxation: NaN residual");
 		if (rmax <= tol) break;
 		if (o >= 64) return fail(-5, "accelerated relaxation did not converge in 64 refinement steps");
+		// a refinement that did not halve the residual means the spectral
+		// bound is too low for the current operator: move it halfway to 1
+		if (o > 0 && rmax > 0.5 * rprev) {
+			rho_ = 1.0 - 0.5 * (1.0 - rho_);
+			sigma = rho_ / (1.0 + std::sqrt(1.0 - rho_ * rho_));
+			stat_rho_raised_++;
+		}
+		rprev = rmax;
 		// Chebyshev steps for a reduction by f (the FP32 correction carries ~6 digits)
 		const double f = std::max(1e-5, 0.5 * tol / rmax);
 		int m = (int) std::ceil(std::log(2.0 / f) / std::log(1.0 / sigma));
@@ -1425,7 +1433,7 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 		if (bytes) sweep_stats(bytes, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 	} else if (kind == 1) {
 		if (!accel_)
-			if (int rc = accel_setup(0.5)) return rc;
+			if (int rc = accel_setup(0.9)) return rc;
 		AccelArgs x = accel_args();
 		float* d0 = d_d0_.as<float>();
 		float* d1 = d_d1_.as<float>();

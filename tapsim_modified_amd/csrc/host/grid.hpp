@@ -238,7 +238,7 @@ private:
 	int warm_begin(int node, double phi_ref, bool predict, EventRecord* rec);
 	int warm_end();
 	double stat_accel_ms_ = 0.0;
-	long long stat_cheb_ = 0, stat_resid_ = 0;
+	long long stat_cheb_ = 0, stat_resid_ = 0, stat_rho_raised_ = 0;
 	double stat_sweep_ms_ = 0.0;
 	long long stat_sweeps_ = 0, stat_launched_ = 0, last_sweeps_ = 0;
 };

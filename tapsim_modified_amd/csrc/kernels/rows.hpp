@@ -89,41 +89,46 @@ __device__ __forceinline__ int long_row_index(const RowBlock& rb)
 // the reference's computePotential accumulation.  Per-lane path, deg <= kLongRow.
 template <typename R, int B>
 __device__ __forceinline__ double short_row_dot_exact_b(const int* __restrict__ col, const R* __restrict__ coup,
-                                                        const R* __restrict__ p, int deg, long long base)
+                                                        const R* __restrict__ p, int deg, long long base, int k0,
+                                                        double lv)
 {
-	// loads are unconditional (entries past the end re-read the last one) so
-	// that the compiler issues the whole batch before the first wait
+	// entries k0 .. k0+B-1 continuing the running sum lv; loads are
+	// unconditional (entries past the end re-read the last one) so that the
+	// compiler issues the whole batch before the first wait
 	int j[B];
 	R c[B], v[B];
 	const int last = max(deg - 1, 0);
 #pragma unroll
-	for (int u = 0; u < B; u++) j[u] = col[base + 64ll * min(u, last)];
+	for (int u = 0; u < B; u++) j[u] = col[base + 64ll * min(k0 + u, last)];
 #pragma unroll
-	for (int u = 0; u < B; u++) c[u] = coup[base + 64ll * min(u, last)];
+	for (int u = 0; u < B; u++) c[u] = coup[base + 64ll * min(k0 + u, last)];
 #pragma unroll
 	for (int u = 0; u < B; u++) v[u] = p[j[u]];
-	double lv = 0.0;
 #pragma unroll
 	for (int u = 0; u < B; u++) {
 		double adj = (double) v[u];
 		adj *= (double) c[u];
 		const double t = lv + adj;
-		lv = (u < deg) ? t : lv;
+		lv = (k0 + u < deg) ? t : lv;
 	}
 	return lv;
 }
 
 // deg <= kLongRow on every active lane; `active` false lanes load nothing.
-// The batch is chosen per wavefront from its longest row (wave-uniform).
+// The batch is chosen per wavefront from its longest row (wave-uniform);
+// double rows of more than 16 entries take a 16-entry batch and an 8-entry
+// one (a single 24-entry batch of doubles needs more registers than the
+// compiler keeps for one round of loads)
 template <typename R>
 __device__ __forceinline__ double short_row_dot_exact(const int* __restrict__ col, const R* __restrict__ coup,
                                                       const R* __restrict__ p, bool active, int deg, long long base)
 {
 	const int wmax = wave_max_int(active ? deg : 0);
 	if (!active || wmax == 0) return 0.0;
-	if (wmax <= 8) return short_row_dot_exact_b<R, 8>(col, coup, p, deg, base);
-	if (wmax <= 16) return short_row_dot_exact_b<R, 16>(col, coup, p, deg, base);
-	return short_row_dot_exact_b<R, kLongRow>(col, coup, p, deg, base);
+	if (wmax <= 8) return short_row_dot_exact_b<R, 8>(col, coup, p, deg, base, 0, 0.0);
+	const double lv = short_row_dot_exact_b<R, 16>(col, coup, p, deg, base, 0, 0.0);
+	if (wmax <= 16) return lv;
+	return short_row_dot_exact_b<R, kLongRow - 16>(col, coup, p, deg, base, 16, lv);
 }
 
 // the same sum for one long row taken by the whole wavefront (deg and base
