@@ -30,6 +30,8 @@ WORKLOADS = {
     "c1": dict(radius=5e-9, shank=10e-9, box=(60e-9, 60e-9, 120e-9), vacuum=150000, face=6e-9),
     # r 15 nm, shank 40 nm, ~3.0 M atoms + ~1.2 M vacuum
     "c2": dict(radius=15e-9, shank=40e-9, box=(150e-9, 150e-9, 300e-9), vacuum=1200000, face=15e-9),
+    # c1 geometry, two-phase: solute (25 V/nm) in 2 nm clusters, 1e-3 centres per nm^3 (seeded)
+    "c3": dict(radius=5e-9, shank=10e-9, box=(60e-9, 60e-9, 120e-9), vacuum=150000, face=6e-9, clusters=1e24),
     # small case for quick checks
     "s": dict(radius=2e-9, shank=3e-9, box=(20e-9, 20e-9, 40e-9), vacuum=15000, face=4e-9),
 }
@@ -54,7 +56,7 @@ def build_specimen(tsg, w, seed=20261203):
     for k in range(3):
         p.box[k] = w["box"][k]
     p.shell, p.vacuum_target, p.decay, p.face_spacing = 2.0, w["vacuum"], 3.0, w["face"]
-    p.cluster_density, p.cluster_radius, p.seed = 0.0, 2e-9, seed
+    p.cluster_density, p.cluster_radius, p.seed = w.get("clusters", 0.0), 2e-9, seed
     p.id_vacuum, p.id_top, p.id_bottom, p.id_sides, p.id_matrix, p.id_solute = 0, 1, 2, 3, 10, 11
     return tsg.generate_specimen(p)
 
@@ -123,15 +125,10 @@ COPY_BYTES = 256 << 20
 
 def counters_run(tsg, grid):
     """Launches the mixed path's correction sweep and the FP64 sweep 100 times
-    each on the relaxed grid, and one device-to-device copy of COPY_BYTES,
-    whose FETCH_SIZE / WRITE_SIZE calibrate the counters of the others."""
-    import torch
-    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
+    each on the relaxed grid, and one plain device copy of COPY_BYTES
+    (k_copy_probe), whose FETCH_SIZE / WRITE_SIZE calibrate the counters of
+    the others."""
+    tsg.copy_probe(COPY_BYTES, 1)
     ms1, by1 = grid.time_kernel(1, 100)
     ms0, by0 = grid.time_kernel(0, 100)
     print(json.dumps({"copy_bytes": COPY_BYTES, "k_cheb32": {"bytes": by1, "us": ms1 * 1e3},
@@ -249,6 +246,14 @@ def main():
     aev, adt, as0, as1 = run_events(learned, learned_opts, K, "learned")
     mev, mdt, ms0, ms1 = run_events(mixed, mixed_opts, K, "mixed")
     roof_acc = roofline(learned, 1, "k_cheb32", args.workload)
+    # potential error of the learned path's final state against the same
+    # system relaxed to an FP64 residual of 1e-12 V
+    sw = learned.switches()[0]
+    phi_run = learned.phi(sw)
+    learned.accel_relax(1e-12)
+    phi_tight = learned.phi(learned.switches()[0])
+    err = float(np.max(np.abs(phi_run - phi_tight)))
+    rel = float(np.max(np.abs(phi_run - phi_tight) / np.maximum(np.abs(phi_tight), 1e-300)))
     steps_l = sum(e.global_iter for e in aev) / max(1, len(aev))
     steps_m = sum(e.global_iter for e in mev) / max(1, len(mev))
     kept = sum(1 for e in aev if e.warm_start == 1)
@@ -268,9 +273,10 @@ def main():
         "dtype": "f32+f64+bf16",
         "data": "synthetic (seeded BCC [110] tip, graded vacuum, own Delaunay mesh); network trained on the GPU "
                 "from %d recorded events of the same run" % W,
-        "config": {"workload": "%s: BCC [110] tip r %.0f nm shank %.0f nm, %d atoms + %d vacuum nodes, %d unknown rows"
-                               % (args.workload, w["radius"] * 1e9, w["shank"] * 1e9, counts["atoms"],
-                                  counts["vacuum_lattice"] + counts["vacuum_random"], ps1["rows"]),
+        "config": {"workload": "%s: BCC [110] tip r %.0f nm shank %.0f nm, %d atoms (%d solute) + %d vacuum nodes, "
+                               "%d unknown rows" % (args.workload, w["radius"] * 1e9, w["shank"] * 1e9, counts["atoms"],
+                                                    counts["solute"], counts["vacuum_lattice"] + counts["vacuum_random"],
+                                                    ps1["rows"]),
                    "rho": rho, "correction_steps_per_event": steps_l, "refinements_per_event":
                    sum(e.local_iter for e in aev) / max(1, len(aev))},
         "learned": {"train_events": W, "epochs": args.epochs, "train_s": t_train, "loss_first_epoch": first,
@@ -280,6 +286,8 @@ def main():
                             "correction_steps_per_event": steps_m},
         "plain_fp64": {"value": Kp / pdt, "unit": "events/s", "steps": Kp, "ms_per_step": pdt / Kp * 1e3,
                        "sweeps_per_event": psweeps},
+        "potential_error": {"max_abs_V": err, "max_rel": rel,
+                            "against": "the same state relaxed to a max scaled FP64 residual of 1e-12 V"},
         "speedup_vs_plain_fp64": value / (Kp / pdt),
         "speedup_vs_mixed": value / (K / mdt),
         "sequence": {"plain_vs_mixed": {"compared": min(Kp, W),

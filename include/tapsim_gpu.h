@@ -212,6 +212,10 @@ int tsg_time_kernel(tsg_grid*, int kind, int reps, double* avg_ms, double* bytes
    counterpart: the reference has no instrumentation of its sweep. */
 int tsg_sweep_stats(const tsg_grid*, double* bytes, long long* rows, long long* entries, long long* referenced,
                     double* ms, long long* sweeps, long long* launched);
+/* a plain device-to-device copy of `bytes` (multiple of 16), launched reps
+   times: the known-size kernel that calibrates the HBM traffic counters;
+   no counterpart in the reference (measurement only) */
+int tsg_copy_probe(long long bytes, int reps, double* avg_ms);
 
 /* ------------------------------------------------------------------ trajectories
  * Trajectory_3d::init + integrate(delta, errorLimit) + extrapolate(Z, zmax)

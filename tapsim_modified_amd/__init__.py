@@ -144,6 +144,7 @@ _SIGS = {
     "tsg_grid_attach_mpnn": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "tsg_surface_evaluate_model": ([_P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "tsg_srand": ([ctypes.c_uint], None),
+    "tsg_copy_probe": ([ctypes.c_longlong, ctypes.c_int, _P], ctypes.c_int),
     "tsg_trajectories": ([_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_int, _P, _P, _P], ctypes.c_int),
 }
@@ -167,6 +168,13 @@ def load():
 
 def exported_symbols():
     return sorted(_SIGS)
+
+
+def copy_probe(nbytes, reps=1):
+    """Average time (ms) of a plain device copy of nbytes (the counter calibration kernel)."""
+    ms = ctypes.c_double()
+    _chk(load().tsg_copy_probe(nbytes, reps, ctypes.byref(ms)))
+    return ms.value
 
 
 def srand(seed):

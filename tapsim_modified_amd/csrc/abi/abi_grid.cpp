@@ -238,6 +238,33 @@ int tsg_accel_relax(tsg_grid* h, double tol, int* outer, long long* inner, doubl
 	return check(h, h->g->accel_relax(tol, outer, inner, final_residual));
 }
 
+int tsg_copy_probe(long long bytes, int reps, double* avg_ms)
+{
+	if (bytes <= 0 || bytes % 16 || reps <= 0 || !avg_ms) return tsg::fail(TSG_E_ARG, "tsg_copy_probe: bad arguments");
+	void *a = nullptr, *b = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int rc = TSG_OK;
+	if (hipMalloc(&a, (size_t) bytes) != hipSuccess || hipMalloc(&b, (size_t) bytes) != hipSuccess ||
+	    hipMemset(a, 0, (size_t) bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+	    hipEventCreate(&e1) != hipSuccess)
+		rc = tsg::fail(TSG_E_HIP, "tsg_copy_probe: allocation failed");
+	float ms = 0.0f;
+	if (!rc) {
+		hipEventRecord(e0, nullptr);
+		for (int r = 0; r < reps && !rc; r++)
+			if (tsg::launch_copy_probe(nullptr, bytes, a, b)) rc = tsg::fail(TSG_E_HIP, "tsg_copy_probe: launch failed");
+		hipEventRecord(e1, nullptr);
+		if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+			rc = tsg::fail(TSG_E_HIP, "tsg_copy_probe: timing failed");
+	}
+	if (e0) hipEventDestroy(e0);
+	if (e1) hipEventDestroy(e1);
+	if (a) hipFree(a);
+	if (b) hipFree(b);
+	if (!rc) *avg_ms = ms / reps;
+	return rc;
+}
+
 int tsg_time_kernel(tsg_grid* h, int kind, int reps, double* avg_ms, double* bytes)
 {
 	if (!h || !avg_ms) return tsg::fail(TSG_E_ARG, "null argument");

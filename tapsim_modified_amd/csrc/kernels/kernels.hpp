@@ -165,6 +165,10 @@ int launch_trajectories(hipStream_t, const TrajArgs&);
 int launch_field_snapshot(hipStream_t, int n, const float* xyz, const int* ptr, const int* idx, const void* coup,
                           const unsigned char* flags, const void* phi, bool f64, float* out, int* status);
 
+// ---------------------------------------------------------------- measurement
+// dst = src over `bytes` (a multiple of 16): the counter calibration copy
+int launch_copy_probe(hipStream_t, long long bytes, const void* src, void* dst);
+
 // ---------------------------------------------------------------- surface
 struct FieldArgs {
 	int count;
