@@ -247,10 +247,11 @@ def main():
     mev, mdt, ms0, ms1 = run_events(mixed, mixed_opts, K, "mixed")
     roof_acc = roofline(learned, 1, "k_cheb32", args.workload)
     # potential error of the learned path's final state against the same
-    # system relaxed to an FP64 residual of 1e-12 V
+    # system relaxed to a scaled FP64 residual of 1e-10 V (the residual's own
+    # rounding floor is ~1e-11 V at 5 kV)
     sw = learned.switches()[0]
     phi_run = learned.phi(sw)
-    learned.accel_relax(1e-12)
+    learned.accel_relax(1e-10)
     phi_tight = learned.phi(learned.switches()[0])
     err = float(np.max(np.abs(phi_run - phi_tight)))
     rel = float(np.max(np.abs(phi_run - phi_tight) / np.maximum(np.abs(phi_tight), 1e-300)))
@@ -287,7 +288,7 @@ def main():
         "plain_fp64": {"value": Kp / pdt, "unit": "events/s", "steps": Kp, "ms_per_step": pdt / Kp * 1e3,
                        "sweeps_per_event": psweeps},
         "potential_error": {"max_abs_V": err, "max_rel": rel,
-                            "against": "the same state relaxed to a max scaled FP64 residual of 1e-12 V"},
+                            "against": "the same state relaxed to a max scaled FP64 residual of 1e-10 V"},
         "speedup_vs_plain_fp64": value / (Kp / pdt),
         "speedup_vs_mixed": value / (K / mdt),
         "sequence": {"plain_vs_mixed": {"compared": min(Kp, W),

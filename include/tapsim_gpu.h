@@ -169,6 +169,12 @@ typedef struct tsg_evap_options
 	int method;               /* 0: the reference's relaxation (steps III as process.cpp:824-900), 1: accelerated,
 	                             2: accelerated from the learned warm start (tsg_grid_attach_mpnn) */
 	double accel_tol;         /* accelerated: max |J(phi) - phi| target, V */
+	/* step V, the ion trajectory (Process::EvaporationOptions, process.h:39-122) */
+	int trajectory;           /* 0 off, 1 on */
+	int integrator, stepper;  /* see tsg_trajectories */
+	float dt0, dt_min, err_limit;
+	int voltage_queue;        /* running mean length of the voltage (process.cpp:983-1010) */
+	float shrink_limit;       /* stop once the apex has receded this far, m */
 } tsg_evap_options;
 
 typedef struct tsg_event
@@ -184,6 +190,13 @@ typedef struct tsg_event
 	int warm_start;           /* 0 none, 1 prediction applied, 2 prediction rejected, 3 graph recorded only */
 	double residual_cold;     /* max |J(phi)-phi| over the shell after the removal */
 	double residual_warm;     /* max |J(phi)-phi| over the grid after the prediction */
+	/* steps V and VI (process.cpp:940-1037) */
+	float start[3], stop[3];  /* first and last (extrapolated) trajectory point */
+	float apex[3];            /* highest surface site before the event */
+	float voltage;            /* running mean of (phi_before - phi(exit)) F / |E_before| / scaling */
+	float time_scale;         /* sqrt((phi_after - phi(exit)) / voltage) */
+	float tof;                /* time of the last trajectory point times time_scale */
+	int traj_len, traj_status;
 } tsg_event;
 
 /* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k

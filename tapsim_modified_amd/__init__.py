@@ -73,7 +73,19 @@ class EvapOptions(ctypes.Structure):
         ("relax_queue", ctypes.c_int), ("relax_global_cycles", ctypes.c_int), ("refresh_interval", ctypes.c_int),
         ("refresh_threshold", ctypes.c_double), ("refresh_cycle", ctypes.c_int), ("refresh_queue", ctypes.c_int),
         ("exact_probability", ctypes.c_int), ("method", ctypes.c_int), ("accel_tol", ctypes.c_double),
+        ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int), ("stepper", ctypes.c_int),
+        ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
+        ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float),
     ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.integrator, self.stepper = 1, 1
+        self.dt0, self.dt_min, self.err_limit = 1e-14, 1e-16, 1e-6
+        self.voltage_queue, self.shrink_limit = 250, 1.0
+        self.accel_tol = 1e-8
+        for k, v in kw.items():
+            setattr(self, k, v)
 
 
 class Event(ctypes.Structure):
@@ -84,6 +96,9 @@ class Event(ctypes.Structure):
         ("local_iter", ctypes.c_longlong), ("global_iter", ctypes.c_longlong),
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_double), ("near_tie", ctypes.c_int),
         ("warm_start", ctypes.c_int), ("residual_cold", ctypes.c_double), ("residual_warm", ctypes.c_double),
+        ("start", ctypes.c_float * 3), ("stop", ctypes.c_float * 3), ("apex", ctypes.c_float * 3),
+        ("voltage", ctypes.c_float), ("time_scale", ctypes.c_float), ("tof", ctypes.c_float),
+        ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int),
     ]
 
 

@@ -213,6 +213,14 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.exact_probability = o->exact_probability;
 	e.method = o->method;
 	e.accel_tol = o->accel_tol;
+	e.trajectory = o->trajectory;
+	e.integrator = o->integrator;
+	e.stepper = o->stepper;
+	e.dt0 = o->dt0;
+	e.dt_min = o->dt_min;
+	e.err_limit = o->err_limit;
+	e.voltage_queue = o->voltage_queue;
+	e.shrink_limit = o->shrink_limit;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
 }
 
@@ -221,7 +229,7 @@ int tsg_trajectories(tsg_grid* h, int count, const int* nodes, const float* char
 {
 	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
 	return check(h, h->g->trajectories(count, nodes, charge, mass, integrator, stepper, dt0, dt_min, err_limit, cap, out,
-	                                   len, status));
+	                                   len, status, nullptr));
 }
 
 int tsg_accel_setup(tsg_grid* h, double safety, double* rho)

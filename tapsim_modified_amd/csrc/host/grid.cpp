@@ -1111,10 +1111,25 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		if (int rc = accel_setup(0.9)) return rc;
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr, o.prob_mode)) return rc;
+	if (o.trajectory && (o.voltage_queue < 0 || o.voltage_queue > 1 << 20)) return fail(-1, "bad voltage queue length");
+	auto apex_of_surface = [&](float* apex) {
+		int top = *surface_.begin();
+		for (int v : surface_)
+			if (mesh_.xyz[3 * (size_t) v + 2] > mesh_.xyz[3 * (size_t) top + 2]) top = v;
+		for (int c = 0; c < 3; c++) apex[c] = mesh_.xyz[3 * (size_t) top + c];
+	};
+	if (!apex0_set_ && !surface_.empty()) {
+		float a0[3];
+		apex_of_surface(a0);
+		apex0_ = a0[2];
+		apex0_set_ = true;
+	}
 	for (int ev = 0; ev < k; ev++) {
 		if (surface_.empty()) break;
 		EventRecord& r = out[ev];
 		std::memset(&r, 0, sizeof(r));
+		const float scaling_reference = (float) scaling_ref_;
+		apex_of_surface(r.apex);
 		int node, nt = 0;
 		if (o.evap_mode == 0) {
 			if (int rc = surface_candidate(&node, &nt)) return rc;
@@ -1165,9 +1180,68 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		}
 		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode)) return rc;
 		r.surface_size = (int) surface_.size();
+		// V) the ion's flight in the relaxed field (process.cpp:940-975)
+		bool have_back = false;
+		float backp[3] = {0.0f, 0.0f, 0.0f}, t_last = 0.0f;
+		int backt = -1;
+		const Species& sp = species_[r.id];
+		if (o.trajectory) {
+			float charge = (float) sp.evap_charge;
+			charge *= 1.60217733e-19f;
+			float mass = sp.mass;
+			mass *= 1.660538782e-27f;
+			float sm[12];
+			if (int rc = trajectories(1, &node, &charge, &mass, o.integrator, o.stepper, o.dt0, o.dt_min, o.err_limit, 0,
+			                          nullptr, &r.traj_len, &r.traj_status, sm))
+				return rc;
+			for (int c = 0; c < 3; c++) {
+				r.start[c] = sm[c];
+				r.stop[c] = sm[3 + c];
+			}
+			t_last = sm[6];
+			if (sm[11] != 0.0f) {
+				have_back = true;
+				for (int c = 0; c < 3; c++) backp[c] = sm[7 + c];
+				backt = (int) sm[10];
+			}
+		}
+		// VI) voltage and time scale (process.cpp:983-1037)
 		if (int rc = potential(node, &r.pot_after)) return rc;
 		if (int rc = field_o1(1, &node, r.field_after)) return rc;
+		float flen = 0.0f;
+		for (int c = 0; c < 3; c++) flen += (float) r.field_before[c] * (float) r.field_before[c];
+		flen = std::sqrt(flen);
+		if (have_back && flen != 0.0f) {
+			float pexit;
+			if (int rc = point_potential(backp, backt, &pexit)) return rc;
+			float v = (float) r.pot_before;
+			v -= pexit;
+			v *= sp.evap_field;
+			v /= flen;
+			v /= scaling_reference;
+			vq_.push_back(v);
+			vq_sum_ += v;
+			while (vq_.size() - vq_head_ > (size_t) o.voltage_queue) vq_sum_ -= vq_[vq_head_++];
+			if (vq_head_ > 4096) {
+				vq_.erase(vq_.begin(), vq_.begin() + (long) vq_head_);
+				vq_head_ = 0;
+			}
+			r.voltage = vq_sum_;
+			r.voltage /= (float) (vq_.size() - vq_head_);
+			float ts = (float) r.pot_after;
+			ts -= pexit;
+			ts /= r.voltage;
+			r.time_scale = std::sqrt(ts);
+		} else if (flen == 0.0f) {
+			r.voltage = 1.0f;
+			r.time_scale = 1.0f;
+		} else {
+			r.voltage = 0.0f;
+			r.time_scale = 1.0f;
+		}
+		r.tof = o.trajectory ? t_last * r.time_scale : 0.0f;
 		*done = ev + 1;
+		if (apex0_ - r.apex[2] >= o.shrink_limit) break;
 	}
 	return 0;
 }
@@ -1681,7 +1755,7 @@ namespace tsg {
 // its node, in the field of the current potentials (the buffer potential()
 // reads); one device thread per ion
 int Grid::trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
-                       float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status)
+                       float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status, float* summary)
 {
 	if (count <= 0) return 0;
 	if (cap < 0 || !nodes || !charge || !mass || !len || !status || (cap > 0 && !out)) return fail(-1, "trajectory: bad arguments");
@@ -1689,39 +1763,29 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	for (int i = 0; i < count; i++)
 		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "trajectory: bad node");
 	const int n = mesh_.n, m = mesh_.m();
-	if (!d_tv_.ptr()) {  // the mesh arrays of the point location, kept from the first call on
-		d_tv_.alloc(sizeof(int) * 4 * (size_t) m);
-		d_tn_.alloc(sizeof(int) * 4 * (size_t) m);
-		d_assoc_.alloc(sizeof(int) * (size_t) n);
-		d_gbnd_.alloc((size_t) n);
-		d_nfield_.alloc(sizeof(float) * 3 * (size_t) n);
-		if (!d_tv_.ptr() || !d_tn_.ptr() || !d_assoc_.ptr() || !d_gbnd_.ptr() || !d_nfield_.ptr())
-			return fail(-2, "device allocation failed (trajectories)");
-		HIPCHK(hipMemcpy(d_tv_.ptr(), mesh_.tet_v.data(), d_tv_.bytes(), hipMemcpyHostToDevice));
-		HIPCHK(hipMemcpy(d_tn_.ptr(), mesh_.tet_n.data(), d_tn_.bytes(), hipMemcpyHostToDevice));
-		HIPCHK(hipMemcpy(d_assoc_.ptr(), mesh_.assoc.data(), d_assoc_.bytes(), hipMemcpyHostToDevice));
-		HIPCHK(hipMemcpy(d_gbnd_.ptr(), mesh_.boundary.data(), d_gbnd_.bytes(), hipMemcpyHostToDevice));
-		zmax_ = mesh_.xyz[2];
-		for (int i = 1; i < n; i++) zmax_ = std::max(zmax_, mesh_.xyz[3 * (size_t) i + 2]);
-	}
+	if (int rc = ensure_point_location()) return rc;
 	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
 	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
 	if (launch_field_snapshot(stream_, n, d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_coupfull_.ptr(),
 	                          d_flags_.as<unsigned char>(), phi, f64_, d_nfield_.as<float>(), d_status_.as<int>()))
 		return fail(-2, "field snapshot launch failed");
-	DevBuf d_in, d_out, d_res;
-	d_in.alloc(sizeof(int) * 2 * (size_t) count + sizeof(float) * 2 * (size_t) count);
-	d_out.alloc(sizeof(float) * 8 * (size_t) count * (size_t) std::max(cap, 1));
-	d_res.alloc(sizeof(int) * 2 * (size_t) count);
-	if (!d_in.ptr() || !d_out.ptr() || !d_res.ptr()) return fail(-2, "device allocation failed (trajectory batch)");
+	// per ion: node, snapshot, charge, mass in; len, status, 12-float summary out
+	const size_t in_bytes = 16 * (size_t) count, res_bytes = 56 * (size_t) count;
+	const size_t out_bytes = sizeof(float) * 8 * (size_t) count * (size_t) std::max(cap, 1);
+	if (d_traj_in_.bytes() < in_bytes) d_traj_in_.alloc(in_bytes * 2);
+	if (d_traj_res_.bytes() < res_bytes) d_traj_res_.alloc(res_bytes * 2);
+	if (cap > 0 && d_traj_out_.bytes() < out_bytes) d_traj_out_.alloc(out_bytes);
+	if (!d_traj_in_.ptr() || !d_traj_res_.ptr() || (cap > 0 && !d_traj_out_.ptr()))
+		return fail(-2, "device allocation failed (trajectory batch)");
 	std::vector<int> snap(count, 0);
-	char* din = d_in.as<char>();
+	char* din = d_traj_in_.as<char>();
 	HIPCHK(hipMemcpyAsync(din, nodes, sizeof(int) * count, hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipMemcpyAsync(din + sizeof(int) * count, snap.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipMemcpyAsync(din + sizeof(int) * 2 * count, charge, sizeof(float) * count, hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipMemcpyAsync(din + sizeof(int) * 2 * count + sizeof(float) * count, mass, sizeof(float) * count,
 	                      hipMemcpyHostToDevice, stream_));
 	TrajArgs a;
+	std::memset(&a, 0, sizeof(a));
 	a.count = count;
 	a.nodes = (const int*) din;
 	a.snapshot = (const int*) (din + sizeof(int) * count);
@@ -1743,18 +1807,65 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	a.assoc = d_assoc_.as<int>();
 	a.gbnd = d_gbnd_.as<unsigned char>();
 	a.nfield = d_nfield_.as<float>();
-	a.out = d_out.as<float>();
+	a.out = cap > 0 ? d_traj_out_.as<float>() : nullptr;
 	a.cap = cap;
-	a.len = d_res.as<int>();
-	a.status = d_res.as<int>() + count;
+	int* res = d_traj_res_.as<int>();
+	a.len = res;
+	a.status = res + count;
+	a.summary = summary ? (float*) (res + 2 * count) : nullptr;
 	if (launch_trajectories(stream_, a)) return fail(-2, "trajectory launch failed");
 	int st = 0;
-	if (cap > 0) HIPCHK(hipMemcpyAsync(out, d_out.ptr(), sizeof(float) * 8 * (size_t) count * cap, hipMemcpyDeviceToHost, stream_));
-	HIPCHK(hipMemcpyAsync(len, d_res.ptr(), sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
-	HIPCHK(hipMemcpyAsync(status, d_res.as<int>() + count, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	if (cap > 0) HIPCHK(hipMemcpyAsync(out, d_traj_out_.ptr(), out_bytes, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(len, res, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(status, res + count, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	if (summary) HIPCHK(hipMemcpyAsync(summary, res + 2 * count, sizeof(float) * 12 * count, hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	return 0;
+}
+
+// the mesh arrays of the point location, kept on the device from the first use on
+int Grid::ensure_point_location()
+{
+	if (d_tv_.ptr()) return 0;
+	const int n = mesh_.n, m = mesh_.m();
+	d_tv_.alloc(sizeof(int) * 4 * (size_t) m);
+	d_tn_.alloc(sizeof(int) * 4 * (size_t) m);
+	d_assoc_.alloc(sizeof(int) * (size_t) n);
+	d_gbnd_.alloc((size_t) n);
+	d_nfield_.alloc(sizeof(float) * 3 * (size_t) n);
+	d_ppos_.alloc(sizeof(float) * 4 + sizeof(int) * 4);
+	if (!d_tv_.ptr() || !d_tn_.ptr() || !d_assoc_.ptr() || !d_gbnd_.ptr() || !d_nfield_.ptr() || !d_ppos_.ptr()) {
+		d_tv_.free();
+		return fail(-2, "device allocation failed (point location)");
+	}
+	HIPCHK(hipMemcpy(d_tv_.ptr(), mesh_.tet_v.data(), d_tv_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_tn_.ptr(), mesh_.tet_n.data(), d_tn_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_assoc_.ptr(), mesh_.assoc.data(), d_assoc_.bytes(), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_gbnd_.ptr(), mesh_.boundary.data(), d_gbnd_.bytes(), hipMemcpyHostToDevice));
+	zmax_ = mesh_.xyz[2];
+	for (int i = 1; i < n; i++) zmax_ = std::max(zmax_, mesh_.xyz[3 * (size_t) i + 2]);
+	return 0;
+}
+
+// Grid_3d::potential(pos, guess) (grid_3d.cpp:1585-1602) on the potential() buffer
+int Grid::point_potential(const float pos[3], int guess, float* out)
+{
+	if (int rc = ensure_point_location()) return rc;
+	float* dp = d_ppos_.as<float>();
+	int* dg = (int*) (dp + 3);
+	int g = guess;
+	float in[3] = {pos[0], pos[1], pos[2]};
+	HIPCHK(hipMemcpyAsync(dp, in, sizeof(float) * 3, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemcpyAsync(dg, &g, sizeof(int), hipMemcpyHostToDevice, stream_));
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	float* dout = (float*) (dg + 1);
+	if (launch_point_potential(stream_, 1, dp, dg, d_xyz_.as<float>(), d_tv_.as<int>(), d_tn_.as<int>(), mesh_.m(), phi,
+	                           f64_, dout))
+		return fail(-2, "point potential launch failed");
+	HIPCHK(hipMemcpyAsync(out, dout, sizeof(float), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
 	return 0;
 }
 

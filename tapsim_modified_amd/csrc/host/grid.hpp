@@ -44,6 +44,11 @@ struct EventRecord {
 	// over the shell before and (when predicted) over the grid after it
 	int warm_start;
 	double residual_cold, residual_warm;
+	// steps V and VI (process.cpp:940-1037): the ion's flight and the
+	// reconstruction quantities; apex = highest surface site before the event
+	float start[3], stop[3], apex[3];
+	float voltage, time_scale, tof;
+	int traj_len, traj_status;
 };
 
 struct EvapOptions {
@@ -58,6 +63,11 @@ struct EvapOptions {
 	int method = 0;             // 0: the reference's relaxation, 1: accelerated (mixed precision),
 	                            // 2: accelerated from the learned warm start
 	double accel_tol = 1e-8;    // scaled-residual target of the accelerated relaxation
+	// step V: the ion trajectory (EvaporationOptions, process.h:39-122)
+	int trajectory = 0, integrator = 1, stepper = 1;
+	float dt0 = 1e-14f, dt_min = 1e-16f, err_limit = 1e-6f;
+	int voltage_queue = 250;
+	float shrink_limit = 1.0f;  // stop once the apex has receded this far (m)
 };
 
 class DevBuf {
@@ -142,7 +152,10 @@ public:
 	// ions from their nodes in the current potentials; out holds cap phases
 	// (t, x, y, z, vx, vy, vz, tet) per ion
 	int trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
-	                 float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status);
+	                 float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status,
+	                 float* summary = nullptr);
+	// Grid_3d::potential(pos, guess): barycentric potential of a point
+	int point_potential(const float pos[3], int guess, float* out);
 
 	// learned warm start: events of methods 1 and 2 record their shell graph
 	// and relaxation target into the network's store (record), method 2
@@ -225,8 +238,15 @@ private:
 	                std::vector<float>* vf);
 	AccelArgs accel_args() const;
 	// trajectories: mesh arrays for the point location, node-field snapshot
-	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_;
+	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_;
 	float zmax_ = 0.0f;
+	int ensure_point_location();
+	// Process::evaporation's voltage queue (process.cpp:983-1037) and initial apex
+	std::vector<float> vq_;
+	size_t vq_head_ = 0;
+	float vq_sum_ = 0.0f;
+	float apex0_ = 0.0f;
+	bool apex0_set_ = false;
 	// warm start state of the current event
 	MpnnNet* net_ = nullptr;
 	bool net_record_ = false, net_predict_ = false;

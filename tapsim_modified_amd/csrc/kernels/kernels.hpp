@@ -160,7 +160,13 @@ struct TrajArgs {
 	int cap;
 	int* len;                 // phases produced (the output holds the first cap)
 	int* status;              // Trajectory_3d status (trajectory_3d.h:89), negative on failure
+	float* summary;           // optional, 12 floats per ion: start xyz, stop xyz, t of the last
+	                          // phase, last phase inside the mesh (xyz, tet), 1 if there is one
 };
+// Grid_3d::potential(pos, guess) (grid_3d.cpp:1585-1602) of `count` points:
+// barycentric interpolation of the potentials in float; 0 outside the mesh
+int launch_point_potential(hipStream_t, int count, const float* pos, const int* guess, const float* xyz, const int* tv,
+                           const int* tn, int m, const void* phi, bool f64, float* out);
 int launch_trajectories(hipStream_t, const TrajArgs&);
 int launch_field_snapshot(hipStream_t, int n, const float* xyz, const int* ptr, const int* idx, const void* coup,
                           const unsigned char* flags, const void* phi, bool f64, float* out, int* status);

@@ -444,6 +444,18 @@ __device__ Phase stepper_step(Tr& tr, const unsigned char* gbnd, const Phase& ba
 
 __device__ void put(const TrajArgs& a, int ion, int k, const Phase& q)
 {
+	if (a.summary) {
+		float* sm = a.summary + 12 * (size_t) ion;
+		if (k == 0)
+			for (int c = 0; c < 3; c++) sm[c] = q.p[c];
+		for (int c = 0; c < 3; c++) sm[3 + c] = q.p[c];
+		sm[6] = q.t;
+		if (q.tet >= 0) {
+			for (int c = 0; c < 3; c++) sm[7 + c] = q.p[c];
+			sm[10] = (float) q.tet;
+			sm[11] = 1.0f;
+		}
+	}
 	if (k >= a.cap) return;
 	float* o = a.out + ((size_t) ion * a.cap + k) * 8;
 	o[0] = q.t;
@@ -484,6 +496,8 @@ __global__ void k_trajectory(TrajArgs a)
 	back.tet = find_tet(tr.g, back.p, a.assoc[node]);
 	int status;
 	int len = 0;
+	if (a.summary)
+		for (int c = 0; c < 12; c++) a.summary[12 * (size_t) ion + c] = 0.0f;
 	if (back.tet < 0) {
 		a.len[ion] = 0;
 		a.status[ion] = -10;  // initial point outside the mesh
@@ -535,6 +549,24 @@ __global__ void k_trajectory(TrajArgs a)
 	a.status[ion] = status;
 }
 
+template <typename R>
+__global__ void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	const float* p = pos + 3 * (size_t) i;
+	const int t = find_tet(g, p, guess[i]);
+	if (t < 0) {
+		out[i] = 0.0f;
+		return;
+	}
+	float w[4];
+	tet_weights(g, t, p, w);
+	float value = 0.0f;
+	for (int k = 0; k < 4; k++) value += w[k] * (float) phi[g.tv[4 * (size_t) t + k]];
+	out[i] = value;
+}
+
 // field_o1 of every node into a float snapshot (the FastField cache)
 template <typename R>
 __global__ void k_field_snapshot(int n, const float* xyz, const int* ptr, const int* idx, const R* coup,
@@ -556,6 +588,24 @@ int launch_trajectories(hipStream_t s, const TrajArgs& a)
 {
 	if (a.count <= 0) return 0;
 	hipLaunchKernelGGL(dev::k_trajectory, dim3((a.count + 63) / 64), dim3(64), 0, s, a);
+	return (int) hipGetLastError();
+}
+
+int launch_point_potential(hipStream_t s, int count, const float* pos, const int* guess, const float* xyz, const int* tv,
+                           const int* tn, int m, const void* phi, bool f64, float* out)
+{
+	if (count <= 0) return 0;
+	dev::Mesh3 g;
+	g.xyz = xyz;
+	g.tv = tv;
+	g.tn = tn;
+	g.m = m;
+	if (f64)
+		hipLaunchKernelGGL(dev::k_point_potential<double>, dim3((count + 63) / 64), dim3(64), 0, s, count, pos, guess, g,
+		                   (const double*) phi, out);
+	else
+		hipLaunchKernelGGL(dev::k_point_potential<float>, dim3((count + 63) / 64), dim3(64), 0, s, count, pos, guess, g,
+		                   (const float*) phi, out);
 	return (int) hipGetLastError();
 }
 

@@ -326,3 +326,33 @@ def test_monte_carlo_choice(tip_small):
     evo = o.evaporate(eo, 12)
     assert [e.node for e in evg] == [e.node for e in evo]
     assert same_bits(g.phi(0), o.phi(0), np.float64)
+
+
+def test_evaporation_with_trajectories_matches_reference(tip_small):
+    """Steps I-VI with the ion flights on (float-faithful grid) against the
+    reference's 25 events: sequence, start points and apex bit-exact; flight
+    ends, lengths, voltage, time scale and time of flight within the
+    tolerances of the device's step-size control and point location (the
+    exit potential is located with exact signs, as in the oracle)."""
+    log("evaporation with trajectories")
+    g = grid(tip_small, tsg.FAITHFUL_F32)
+    for _ in range(300):
+        g.relax(1)
+    e = evap_opts()
+    e.trajectory = 1
+    ev = g.evaporate(e, 25)
+    F = tip_small["ref_evap_f"]
+    assert np.array_equal([x.node for x in ev], tip_small["ref_evap_node"])
+    assert same_bits([list(x.start) for x in ev], F[:, 11:14], np.float32)
+    assert same_bits([list(x.apex) for x in ev], F[:, 18:21], np.float32)
+    stop = np.array([list(x.stop) for x in ev])
+    span = np.max(np.abs(F[:, 14:17]))
+    log("max stop deviation %.3g of %.3g; lengths %s vs %s" % (np.max(np.abs(stop - F[:, 14:17])), span,
+                                                                [x.traj_len for x in ev][:8],
+                                                                list(tip_small["ref_evap_trajlen"][:8])))
+    assert np.max(np.abs(stop - F[:, 14:17])) <= 1e-4 * span
+    assert np.all(np.abs(np.array([x.traj_len for x in ev]) - tip_small["ref_evap_trajlen"])
+                  <= np.maximum(3, tip_small["ref_evap_trajlen"] // 20))
+    np.testing.assert_allclose([x.voltage for x in ev], F[:, 9], rtol=2e-3)
+    np.testing.assert_allclose([x.time_scale for x in ev], F[:, 10], rtol=2e-3)
+    np.testing.assert_allclose([x.tof for x in ev], F[:, 17], rtol=5e-3)
