@@ -121,6 +121,12 @@ int tsg_local_relax(tsg_grid*, int node, int order, double threshold, int cycles
 int tsg_get_phi(tsg_grid*, int buffer, double* out);
 int tsg_set_phi(tsg_grid*, const double* phi0, const double* phi1, int phi_switch, int local_switch);
 int tsg_phi_switches(const tsg_grid*, int* phi_switch, int* local_switch);
+/* initial potentials, both buffers: kind 0 the species presets
+   (Grid_3d::Table::reset, grid_3d.cpp:797-809), 1 random for the
+   non-Dirichlet nodes from the process's rand() (randomReset, :811-848),
+   2 the radial ramp of preset_sphericalPotential (system_3d.cpp:24-57) from
+   pmax at center */
+int tsg_reset_potentials(tsg_grid*, int kind, const float* center, float pmin, float pmax);
 /* Grid_3d::Table::potential(i) (grid_3d.h:184) */
 int tsg_potential(tsg_grid*, int node, double* out);
 /* Grid_3d::Table::field_o1(i) (grid_3d.cpp:909-991), 3 values per node */

@@ -100,6 +100,8 @@ _SIGS = {
     "ora_trajectory": ([_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                         ctypes.c_float, ctypes.c_float, _P, ctypes.c_int, _P], ctypes.c_int),
     "ora_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "ora_random_reset": ([_P], None),
+    "ora_spherical_preset": ([_P, _P, ctypes.c_float, ctypes.c_float], None),
     "ora_relax_threaded": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_double),
     "ora_mpnn": ([_P, _P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "ora_bf16": ([ctypes.c_float], ctypes.c_float),
@@ -244,6 +246,13 @@ class Oracle:
         d = load().ora_relax(self._h, cycles, ctypes.byref(err))
         self._chk(err.value)
         return d
+
+    def random_reset(self):
+        load().ora_random_reset(self._h)
+
+    def spherical_preset(self, center, pmin, pmax):
+        c = np.ascontiguousarray(center, np.float32)
+        load().ora_spherical_preset(self._h, _ptr(c), pmin, pmax)
 
     def relax_threaded(self, sweeps, threads):
         """`sweeps` relax(1) calls on `threads` workers (FP64): (deviation, seconds)."""

@@ -259,7 +259,10 @@ ora_sys* ora_create(int mode, int n, const float* xyz, const short* ids,
 	/* Geometry_3d::Table::adjacentNodes (geometry_3d.cpp:1098-1110): all
 	   vertices of the tets around the node, as an ascending std::set */
 	s->ptr = (int*) calloc((size_t) n + 1, sizeof(int));
-	int* tmp = (int*) malloc(sizeof(int) * 4096);
+	int maxinc = 0;
+	for (int i = 0; i < n; i++)
+		if (s->nt_ptr[i + 1] - s->nt_ptr[i] > maxinc) maxinc = s->nt_ptr[i + 1] - s->nt_ptr[i];
+	int* tmp = (int*) malloc(sizeof(int) * (3 * (size_t) maxinc + 1));
 	int cap = 1 << 20;
 	s->idx = (int*) malloc(sizeof(int) * (size_t) cap);
 	for (int i = 0; i < n; i++)
@@ -269,7 +272,7 @@ ora_sys* ora_create(int mode, int n, const float* xyz, const short* ids,
 			for (int q = 0; q < 4; q++)
 			{
 				const int v = s->tv[4 * (size_t) s->nt_idx[k] + q];
-				if (v != i && c < 4096) tmp[c++] = v;
+				if (v != i) tmp[c++] = v;
 			}
 		qsort(tmp, (size_t) c, sizeof(int), cmp_int);
 		int u = 0;
@@ -361,6 +364,12 @@ void ora_set_phi(ora_sys* s, const double* p0, const double* p1, int sw, int lsw
 	s->lsw = lsw;
 }
 int ora_phi_switch(const ora_sys* s) { return s->sw; }
+void ora_random_reset(ora_sys* s) { if (s->mode == ORA_F32) random_reset_f32(s); else random_reset_f64(s); }
+void ora_spherical_preset(ora_sys* s, const float center[3], float pmin, float pmax)
+{
+	if (s->mode == ORA_F32) spherical_preset_f32(s, center, pmin, pmax);
+	else spherical_preset_f64(s, center, pmin, pmax);
+}
 int ora_local_switch(const ora_sys* s) { return s->lsw; }
 double ora_potential(const ora_sys* s, int i) { return DISPATCH(s, potential, s, i); }
 void ora_set_inplace_order(ora_sys* s, int kind, const int* colour, int ncolours)

@@ -103,6 +103,9 @@ int ora_local_switch(const ora_sys*);
 double ora_potential(const ora_sys*, int node);
 void ora_set_inplace_order(ora_sys*, int kind, const int* colour, int ncolours);
 
+/* Grid_3d::Table::randomReset and preset_sphericalPotential (initial potentials) */
+void ora_random_reset(ora_sys* s);
+void ora_spherical_preset(ora_sys* s, const float center[3], float pmin, float pmax);
 double ora_relax(ora_sys*, int cycles, int* err);
 /* `sweeps` relax(1) calls with `threads` workers (FP64 mode; timing of the
    CPU baseline); wall time in *seconds, the last deviation returned */

@@ -159,6 +159,7 @@ _SIGS = {
     "tsg_grid_attach_mpnn": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "tsg_surface_evaluate_model": ([_P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "tsg_srand": ([ctypes.c_uint], None),
+    "tsg_reset_potentials": ([_P, ctypes.c_int, _P, ctypes.c_float, ctypes.c_float], ctypes.c_int),
     "tsg_copy_probe": ([ctypes.c_longlong, ctypes.c_int, _P], ctypes.c_int),
     "tsg_trajectories": ([_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_int, _P, _P, _P], ctypes.c_int),
@@ -447,6 +448,11 @@ class Grid3D:
         _chk(load().tsg_trajectories(self._h, k, _ptr(nodes), _ptr(charge), _ptr(mass), integrator, stepper, dt0, dt_min,
                                      err, cap, _ptr(out), _ptr(ln), _ptr(st)))
         return [(int(st[i]), out[i, :min(int(ln[i]), cap)].copy()) for i in range(k)]
+
+    def reset_potentials(self, kind=0, center=None, pmin=0.0, pmax=0.0):
+        """0 species presets, 1 random (randReset), 2 radial ramp (preset_sphericalPotential)."""
+        c = np.ascontiguousarray(center if center is not None else (0.0, 0.0, 0.0), np.float32)
+        _chk(load().tsg_reset_potentials(self._h, kind, _ptr(c), pmin, pmax))
 
     def attach(self, net, record=True, predict=True):
         """Attaches a WarmStartNet (None detaches): events of methods 1 and 2

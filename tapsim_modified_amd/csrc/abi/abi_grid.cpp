@@ -246,6 +246,14 @@ int tsg_accel_relax(tsg_grid* h, double tol, int* outer, long long* inner, doubl
 	return check(h, h->g->accel_relax(tol, outer, inner, final_residual));
 }
 
+int tsg_reset_potentials(tsg_grid* h, int kind, const float* center, float pmin, float pmax)
+{
+	if (!h || kind < 0 || kind > 2 || (kind == 2 && !center)) return tsg::fail(TSG_E_ARG, "tsg_reset_potentials: bad arguments");
+	if (kind == 0) return check(h, h->g->reset_potentials());
+	if (kind == 1) return check(h, h->g->random_potentials());
+	return check(h, h->g->spherical_potentials(center, pmin, pmax));
+}
+
 int tsg_copy_probe(long long bytes, int reps, double* avg_ms)
 {
 	if (bytes <= 0 || bytes % 16 || reps <= 0 || !avg_ms) return tsg::fail(TSG_E_ARG, "tsg_copy_probe: bad arguments");

@@ -1870,3 +1870,111 @@ int Grid::point_potential(const float pos[3], int guess, float* out)
 }
 
 }  // namespace tsg
+
+// ---------------------------------------------------------------- initial potentials
+
+namespace tsg {
+
+namespace {
+
+template <typename R>
+R length3(const R* a)
+{
+	R v = 0;
+	for (int k = 0; k < 3; k++) v += a[k] * a[k];
+	return std::sqrt(v);
+}
+
+}  // namespace
+
+// Grid_3d::Table::reset (grid_3d.cpp:797-809): every node takes its species preset
+int Grid::reset_potentials()
+{
+	std::vector<double> phi(mesh_.n);
+	for (int i = 0; i < mesh_.n; i++) phi[i] = species_[ids_[i]].phi;
+	return set_phi(phi.data(), phi.data(), sw_, lsw_);
+}
+
+// Grid_3d::Table::randomReset (grid_3d.cpp:811-848), in the grid's element
+// type, drawing from the process's rand()
+template <typename R>
+static void random_potentials(const std::vector<int16_t>& ids, const std::vector<uint8_t>& flags, const Species* sp,
+                              std::vector<double>* out)
+{
+	int first = -1;
+	for (int id = 0; id < 128; id++)
+		if (sp[id].defined) {
+			first = id;
+			break;
+		}
+	if (first < 0) return;
+	R minPhi = sp[first].phi, maxPhi = sp[first].phi;
+	for (int id = first; id < 128; id++) {
+		if (!sp[id].defined) continue;
+		const R v = sp[id].phi;
+		if (v < minPhi) minPhi = v;
+		else if (v > maxPhi) maxPhi = v;
+	}
+	for (size_t i = 0; i < ids.size(); i++) {
+		R phi;
+		if (flags[i] & 2) phi = sp[ids[i]].phi;
+		else {
+			phi = (R) rand();
+			phi /= RAND_MAX;
+			phi *= maxPhi - minPhi;
+			phi += minPhi;
+		}
+		(*out)[i] = phi;
+	}
+}
+
+int Grid::random_potentials()
+{
+	std::vector<double> phi(mesh_.n);
+	if (f64_) tsg::random_potentials<double>(ids_, flags_, species_, &phi);
+	else tsg::random_potentials<float>(ids_, flags_, species_, &phi);
+	return set_phi(phi.data(), phi.data(), sw_, lsw_);
+}
+
+// preset_sphericalPotential (system_3d.cpp:24-57): a radial ramp from pmax
+// at the centre, its slope set by the distance of the bounding box's lower
+// corner (the reference uses that corner for both extreme distances)
+template <typename R>
+static void spherical_potentials(const Mesh& mesh, const std::vector<int16_t>& ids, const std::vector<uint8_t>& flags,
+                                 const Species* sp, const float* center, float pmin, float pmax, std::vector<double>* out)
+{
+	R lo[3], c[3], d[3];
+	for (int k = 0; k < 3; k++) {
+		lo[k] = mesh.xyz[k];
+		c[k] = center[k];
+	}
+	for (int i = 1; i < mesh.n; i++)
+		for (int k = 0; k < 3; k++)
+			if (mesh.xyz[3 * (size_t) i + k] < lo[k]) lo[k] = mesh.xyz[3 * (size_t) i + k];
+	for (int k = 0; k < 3; k++) d[k] = lo[k] - c[k];
+	const R minDistance = length3(d), maxDistance = length3(d);
+	R phiSlope = pmax;
+	phiSlope -= pmin;
+	if (minDistance > maxDistance) phiSlope /= minDistance;
+	else phiSlope /= maxDistance;
+	for (int i = 0; i < mesh.n; i++) {
+		R phi;
+		if (flags[i] & 2) phi = sp[ids[i]].phi;
+		else {
+			for (int k = 0; k < 3; k++) d[k] = (R) mesh.xyz[3 * (size_t) i + k] - c[k];
+			phi = pmax;
+			phi -= phiSlope * length3(d);
+		}
+		(*out)[i] = phi;
+	}
+}
+
+int Grid::spherical_potentials(const float center[3], float pmin, float pmax)
+{
+	std::vector<double> phi(mesh_.n);
+	if (f64_) tsg::spherical_potentials<double>(mesh_, ids_, flags_, species_, center, pmin, pmax, &phi);
+	else tsg::spherical_potentials<float>(mesh_, ids_, flags_, species_, center, pmin, pmax, &phi);
+	return set_phi(phi.data(), phi.data(), sw_, lsw_);
+}
+
+}  // namespace tsg

@@ -112,6 +112,12 @@ public:
 	// Grid_3d::Table::localRelax(index, order, threshold, cycleSize, queueSize) (grid_3d.cpp:707-795)
 	int local_relax(int node, int order, double threshold, int cycles, int queue, long long* iterations);
 
+	// initial potentials: species presets (Grid_3d::Table::reset), random
+	// (randomReset, from the process's rand()), radial ramp
+	// (preset_sphericalPotential); both buffers, switches kept
+	int reset_potentials();
+	int random_potentials();
+	int spherical_potentials(const float center[3], float pmin, float pmax);
 	int get_phi(int buffer, double* out);
 	int set_phi(const double* phi0, const double* phi1, int phi_switch, int local_switch);
 	int phi_switch() const { return sw_; }

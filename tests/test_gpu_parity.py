@@ -356,3 +356,25 @@ def test_evaporation_with_trajectories_matches_reference(tip_small):
     np.testing.assert_allclose([x.voltage for x in ev], F[:, 9], rtol=2e-3)
     np.testing.assert_allclose([x.time_scale for x in ev], F[:, 10], rtol=2e-3)
     np.testing.assert_allclose([x.tof for x in ev], F[:, 17], rtol=5e-3)
+
+
+@pytest.mark.parametrize("mode", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_initial_potentials_match_oracle(tip_small, mode):
+    """randomReset (same rand() seed) and preset_sphericalPotential: device
+    grid and oracle bit for bit."""
+    log("initial potentials mode %d" % mode)
+    g = grid(tip_small, mode)
+    o = oracle(tip_small, O.F32 if mode == tsg.FAITHFUL_F32 else O.F64)
+    o.sync()
+    dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
+    tsg.srand(11)
+    g.reset_potentials(1)
+    o.srand(11)
+    o.random_reset()
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+    c = tip_small["xyz"].mean(0).astype(np.float32)
+    g.reset_potentials(2, c, 50.0, 4500.0)
+    o.spherical_preset(c, 50.0, 4500.0)
+    assert same_bits(g.phi(0), o.phi(0), dt)
+    g.reset_potentials(0)
+    assert same_bits(g.phi(0), np.array([FIXTURE_SPECIES[i].get("phi", 0.0) for i in tip_small["ids"]]), dt)
