@@ -161,6 +161,14 @@ int tsg_surface_probabilities(tsg_grid*, float* out);
 int tsg_surface_candidate(tsg_grid*, int* node, int* near_tie);
 /* evap_takeAway (surface_3d.cpp:507-564) */
 int tsg_take_away(tsg_grid*, int node);
+/* Surface_3d::Table::normal (surface_3d.cpp:123-169): best-plane normal of
+   the surface sites within three Delaunay hops, oriented toward the vacuum
+   nodes of that shell; zero for a node that is not a surface site */
+int tsg_surface_normal(const tsg_grid*, int node, float out[3]);
+/* Geometry_3d::find_best_plane (geometry_3d.cpp:1806-1870): unit
+   eigenvector of the smallest eigenvalue of the scatter matrix of count
+   offsets (x,y,z floats); host only, no device needed */
+int tsg_best_plane(const float* offsets, int count, float out[3]);
 
 typedef struct tsg_evap_options
 {
@@ -199,6 +207,7 @@ typedef struct tsg_event
 	/* steps V and VI (process.cpp:940-1037) */
 	float start[3], stop[3];  /* first and last (extrapolated) trajectory point */
 	float apex[3];            /* highest surface site before the event */
+	float normal[3];          /* surface normal at the site, surface_3d.cpp:123-169 */
 	float voltage;            /* running mean of (phi_before - phi(exit)) F / |E_before| / scaling */
 	float time_scale;         /* sqrt((phi_after - phi(exit)) / voltage) */
 	float tof;                /* time of the last trajectory point times time_scale */

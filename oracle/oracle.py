@@ -47,6 +47,7 @@ class OraEvent(ctypes.Structure):
         ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int),
         ("local_iter", ctypes.c_longlong), ("refresh_iter", ctypes.c_longlong),
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_float), ("apex", ctypes.c_float * 3),
+        ("normal", ctypes.c_float * 3),
     ]
 
 
@@ -92,6 +93,8 @@ _SIGS = {
     "ora_specific_fields": ([_P], ctypes.c_float),
     "ora_find_candidate": ([_P, ctypes.c_int], ctypes.c_int),
     "ora_take_away": ([_P, ctypes.c_int], ctypes.c_int),
+    "ora_surface_normal": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "ora_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "ora_srand": ([ctypes.c_uint], None),
     "ora_find_tetrahedron": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "ora_tet_weights": ([_P, ctypes.c_int, _P, _P], None),
@@ -318,6 +321,12 @@ class Oracle:
         load().ora_surface_probabilities(self._h, _ptr(out))
         return out
 
+    def surface_normal(self, node):
+        """Surface_3d::Table::normal (surface_3d.cpp:123-169)"""
+        out = np.zeros(3, np.float32)
+        self._chk(load().ora_surface_normal(self._h, node, _ptr(out)))
+        return out
+
     def compute_probabilities(self, mode=0):
         self._chk(load().ora_compute_probabilities(self._h, mode))
 
@@ -397,3 +406,14 @@ def bf16(v):
     u = v.view(np.uint32).astype(np.uint64)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
     return r.view(np.float32)
+
+
+def best_plane(offsets):
+    """Geometry_3d::find_best_plane (geometry_3d.cpp:1806-1870) of float
+    offsets (k, 3): unit eigenvector of the smallest scatter eigenvalue."""
+    off = np.ascontiguousarray(offsets, np.float32).reshape(-1, 3)
+    out = np.zeros(3, np.float32)
+    rc = load().ora_best_plane(_ptr(off), len(off), _ptr(out))
+    if rc:
+        raise RuntimeError(load().ora_last_error().decode())
+    return out

@@ -333,7 +333,7 @@ int main(int argc, char** argv)
 				Surface_3d::evap_compute_specificFields(&surf, system);
 				Surface_3d::evap_compute_probabilities(probMode, &surf, system);
 
-				std::vector<float> rec;
+				std::vector<float> rec, recNormal;
 				std::vector<int> recNode;
 				std::vector<long long> recIter;
 				std::queue<float> voltageQueue;
@@ -347,6 +347,7 @@ int main(int argc, char** argv)
 					const float potBefore = system.gridTable.potential(node);
 					const MathVector3d<float> fieldBefore = system.gridTable.field_o1(node, system.geomTable);
 					const Geometry_3d::Point apex = system.geomTable.nodeCoords(surf.apex(system)->index());
+					const Geometry_3d::Point normal = surf.normal(node, system);
 					const Configuration::NodeId nodeId = system.gridTable.id(node);
 
 					evap_takeAway(&system, &surf, node);
@@ -425,7 +426,9 @@ int main(int argc, char** argv)
 					rec.push_back(traj.data().back().time() * timeScale);
 					for (int c = 0; c < 3; c++) rec.push_back(apex[c]);
 					rec.push_back(surf.scalingReference());
+					for (int c = 0; c < 3; c++) recNormal.push_back(normal[c]);
 				}
+				writeArray("evap_normal.f32", recNormal);
 				writeArray("evap_int.i32", recNode);
 				writeArray("evap_iter.i64", recIter);
 				writeArray("evap_f.f32", rec);

@@ -682,6 +682,12 @@ static int surf_find(const ora_sys* s, int node)
 	return lo;
 }
 
+static int surf_has(const ora_sys* s, int node)
+{
+	const int pos = surf_find(s, node);
+	return pos < s->nsurf && s->surf[pos] == node;
+}
+
 static void surf_insert(ora_sys* s, int node)
 {
 	const int pos = surf_find(s, node);
@@ -1252,6 +1258,191 @@ static float flen3(const double e[3])
 	return sqrtf(v);
 }
 
+/* ---- surface normal: Surface_3d::Table::normal (surface_3d.cpp:123-169),
+   Geometry_3d::find_best_plane (geometry_3d.cpp:1806-1870), and utils.h's
+   Numerical Recipes householder (tred2, utils.h:245-330) and tri_eigen_qli
+   (tqli, utils.h:358-456) for n = 3, float with the reference's promotions */
+
+static float nr_hyp(float a, float b)
+{
+	float x = fabsf(a), y = fabsf(b);
+	if (x > y) { float t = y / x; return (float) (x * sqrt(1.0 + t * t)); }
+	if (y == 0.0f) return 0.0f;
+	float t = x / y;
+	return (float) (y * sqrt(1.0 + t * t));
+}
+
+static void nr_tred2(float a[3][3], float d[3], float e[3])
+{
+	for (int i = 2; i >= 1; i--)
+	{
+		int l = i - 1;
+		float h = 0.0f, scale = 0.0f;
+		if (l >= 1)
+		{
+			for (int k = 0; k <= l; k++) scale += fabsf(a[i][k]);
+			if (scale == 0.0f) e[i] = a[i][l];
+			else
+			{
+				for (int k = 0; k <= l; k++) { a[i][k] /= scale; h += a[i][k] * a[i][k]; }
+				float f = a[i][l];
+				float g = (f >= 0.0f) ? -sqrtf(h) : sqrtf(h);
+				e[i] = scale * g;
+				h -= f * g;
+				a[i][l] = f - g;
+				f = 0.0f;
+				for (int j = 0; j <= l; j++)
+				{
+					a[j][i] = a[i][j] / h;
+					g = 0.0f;
+					for (int k = 0; k <= j; k++) g += a[j][k] * a[i][k];
+					for (int k = j + 1; k <= l; k++) g += a[k][j] * a[i][k];
+					e[j] = g / h;
+					f += e[j] * a[i][j];
+				}
+				float hh = f / (h + h);
+				for (int j = 0; j <= l; j++)
+				{
+					f = a[i][j];
+					e[j] = g = e[j] - hh * f;
+					for (int k = 0; k <= j; k++) a[j][k] -= (f * e[k] + g * a[i][k]);
+				}
+			}
+		}
+		else e[i] = a[i][l];
+		d[i] = h;
+	}
+	d[0] = 0.0f;
+	e[0] = 0.0f;
+	for (int i = 0; i < 3; i++)
+	{
+		if (d[i] != 0.0f)
+			for (int j = 0; j < i; j++)
+			{
+				float g = 0.0f;
+				for (int k = 0; k < i; k++) g += a[i][k] * a[k][j];
+				for (int k = 0; k < i; k++) a[k][j] -= g * a[k][i];
+			}
+		d[i] = a[i][i];
+		a[i][i] = 1.0f;
+		for (int j = 0; j < i; j++) { a[j][i] = 0.0f; a[i][j] = 0.0f; }
+	}
+}
+
+/* returns 0, or ORA_E_NAN when an eigenvalue does not settle in 10^4 sweeps
+   (the reference never throws there and would keep iterating) */
+static int nr_tqli(float d[3], float e[3], float z[3][3])
+{
+	e[0] = e[1]; e[1] = e[2]; e[2] = 0.0f;
+	for (int l = 0; l < 3; l++)
+	{
+		int iter = 0, m;
+		for (;;)
+		{
+			for (m = l; m < 2; m++)
+			{
+				float dd = fabsf(d[m]) + fabsf(d[m + 1]);
+				if (fabsf(e[m]) + dd == dd) break;
+			}
+			if (m == l) break;
+			if (++iter > 10000) return ORA_E_NAN;
+			float g = (float) ((d[l + 1] - d[l]) / (2.0 * e[l]));
+			float r = nr_hyp(g, 1.0f);
+			g = d[m] - d[l] + e[l] / (g + (g >= 0.0f ? fabsf(r) : -fabsf(r)));
+			float s = 1.0f, c = 1.0f, p = 0.0f;
+			int i = m - 1, early = 0;
+			for (; i >= l; i--)
+			{
+				float f = s * e[i], b = c * e[i];
+				e[i + 1] = r = nr_hyp(f, g);
+				if (r == 0.0f) { d[i + 1] -= p; e[m] = 0.0f; early = 1; break; }
+				s = f / r;
+				c = g / r;
+				g = d[i + 1] - p;
+				r = (float) ((d[i] - g) * s + 2.0 * c * b);
+				d[i + 1] = g + (s * r);
+				p = s * r;
+				g = c * r - b;
+				for (int k = 0; k < 3; k++)
+				{
+					f = z[k][i + 1];
+					z[k][i + 1] = s * z[k][i] + c * f;
+					z[k][i] = c * z[k][i] - s * f;
+				}
+			}
+			if (early) continue;
+			d[l] -= p;
+			e[l] = g;
+			e[m] = 0.0f;
+		}
+	}
+	return ORA_OK;
+}
+
+int ora_best_plane(const float* off, int count, float out[3])
+{
+	float a[3][3], d[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+	for (int p = 0; p < 3; p++)
+		for (int q = p; q < 3; q++)
+		{
+			float acc = 0.0f;
+			for (int i = 0; i < count; i++) acc += off[3 * i + p] * off[3 * i + q];
+			a[p][q] = acc;
+			a[q][p] = acc;
+		}
+	nr_tred2(a, d, e);
+	if (nr_tqli(d, e, a)) return fail(ORA_E_NAN, "find_best_plane: no convergence");
+	int col = 0;
+	if (d[1] < d[col]) col = 1;
+	if (d[2] < d[col]) col = 2;
+	for (int i = 0; i < 3; i++)
+	{
+		out[i] = a[i][col];
+		if (out[i] != out[i]) return fail(ORA_E_NAN, "Geometry_3d::find_best_plane()");
+	}
+	if (out[0] != 0.0f || out[1] != 0.0f || out[2] != 0.0f)
+	{
+		float len = 0.0f;
+		for (int i = 0; i < 3; i++) len += out[i] * out[i];
+		len = sqrtf(len);
+		for (int i = 0; i < 3; i++) out[i] /= len;
+	}
+	return ORA_OK;
+}
+
+int ora_surface_normal(const ora_sys* s, int node, float out[3])
+{
+	out[0] = out[1] = out[2] = 0.0f;
+	if (!surf_has(s, node)) return ORA_OK;
+	int* shell = (int*) malloc(sizeof(int) * (size_t) s->n);
+	int c = ora_find_local_nodes(s, node, 3, shell, s->n);
+	float* off = (float*) malloc(sizeof(float) * 3 * (size_t) (c + 1));
+	float vac[3] = {0.0f, 0.0f, 0.0f};
+	int noff = 0;
+	const float* p0 = s->xyz + 3 * (size_t) node;
+	for (int k = 0; k < c; k++)
+	{
+		const int v = shell[k];
+		if (v == node) continue;  /* adjacentNodes(.., limit) erases the primary node */
+		const float* p = s->xyz + 3 * (size_t) v;
+		if (s->ids[v] == s->vacuum_id)
+			for (int q = 0; q < 3; q++) vac[q] += p[q] - p0[q];
+		if (surf_has(s, v))
+		{
+			for (int q = 0; q < 3; q++) off[3 * noff + q] = p0[q] - p[q];
+			noff++;
+		}
+	}
+	int e = ora_best_plane(off, noff, out);
+	free(off); free(shell);
+	if (e) return e;
+	float dot = 0.0f;
+	for (int q = 0; q < 3; q++) dot += out[q] * vac[q];
+	if (dot < 0.0f)
+		for (int q = 0; q < 3; q++) out[q] *= -1.0f;
+	return ORA_OK;
+}
+
 /* Process::evaporation, process.cpp:441-1124, steps I-VI (logging and dumps
    are not restated; update_coordination is off) */
 int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* out, int* done)
@@ -1291,6 +1482,8 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 				if (s->xyz[3 * (size_t) s->surf[k] + 2] > s->xyz[3 * (size_t) top + 2]) top = s->surf[k];
 			for (int k = 0; k < 3; k++) r->apex[k] = s->xyz[3 * (size_t) top + k];
 		}
+		e = ora_surface_normal(s, node, r->normal);
+		if (e) return e;
 		const int nodeId = s->ids[node];
 		/* II) take away */
 		e = ora_take_away(s, node);

@@ -72,6 +72,7 @@ typedef struct
 	int surface_size;
 	float scaling_ref;
 	float apex[3];
+	float normal[3];
 } ora_event;
 
 typedef struct ora_sys ora_sys;
@@ -132,6 +133,9 @@ int ora_compute_probabilities(ora_sys*, int mode);
 float ora_specific_fields(ora_sys*);
 int ora_find_candidate(const ora_sys*, int mode);
 int ora_take_away(ora_sys*, int node);
+/* Surface_3d::Table::normal / Geometry_3d::find_best_plane */
+int ora_surface_normal(const ora_sys*, int node, float out[3]);
+int ora_best_plane(const float* offsets, int count, float out[3]);
 
 /* trajectory (float, as trajectory_3d.cpp) */
 void ora_srand(unsigned int seed);

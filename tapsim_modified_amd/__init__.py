@@ -97,6 +97,7 @@ class Event(ctypes.Structure):
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_double), ("near_tie", ctypes.c_int),
         ("warm_start", ctypes.c_int), ("residual_cold", ctypes.c_double), ("residual_warm", ctypes.c_double),
         ("start", ctypes.c_float * 3), ("stop", ctypes.c_float * 3), ("apex", ctypes.c_float * 3),
+        ("normal", ctypes.c_float * 3),
         ("voltage", ctypes.c_float), ("time_scale", ctypes.c_float), ("tof", ctypes.c_float),
         ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int),
     ]
@@ -140,6 +141,8 @@ _SIGS = {
     "tsg_surface_probabilities": ([_P, _P], ctypes.c_int),
     "tsg_surface_candidate": ([_P, _P, _P], ctypes.c_int),
     "tsg_take_away": ([_P, ctypes.c_int], ctypes.c_int),
+    "tsg_surface_normal": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_sweep_stats": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "tsg_accel_setup": ([_P, ctypes.c_double, _P], ctypes.c_int),
@@ -191,6 +194,15 @@ def copy_probe(nbytes, reps=1):
     ms = ctypes.c_double()
     _chk(load().tsg_copy_probe(nbytes, reps, ctypes.byref(ms)))
     return ms.value
+
+
+def best_plane(offsets):
+    """Geometry_3d::find_best_plane (geometry_3d.cpp:1806-1870) of (k, 3)
+    float offsets: unit normal of the least-squares plane (host code)."""
+    off = np.ascontiguousarray(offsets, np.float32).reshape(-1, 3)
+    out = np.zeros(3, np.float32)
+    _chk(load().tsg_best_plane(_ptr(off), len(off), _ptr(out)))
+    return out
 
 
 def srand(seed):
@@ -410,6 +422,12 @@ class Grid3D:
 
     def take_away(self, node):
         _chk(load().tsg_take_away(self._h, node))
+
+    def surface_normal(self, node):
+        """Surface_3d::Table::normal(node) (surface_3d.cpp:123-169)"""
+        out = np.zeros(3, np.float32)
+        _chk(load().tsg_surface_normal(self._h, node, _ptr(out)))
+        return out
 
     def sweep_stats(self):
         b, ms = ctypes.c_double(), ctypes.c_double()

@@ -163,6 +163,12 @@ int tsg_surface_nodes(const tsg_grid* h, int* out)
 	return h->g->surface_nodes(out);
 }
 
+int tsg_surface_normal(const tsg_grid* h, int node, float out[3])
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->surface_normal(node, out);
+}
+
 int tsg_surface_evaluate_model(tsg_grid* h, int mode, int exact_sum, double* scaling_ref)
 {
 	if (!h) return tsg::fail(TSG_E_ARG, "null grid");

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 #include <unordered_set>
+#include <vector>
 
 #include "../../../include/tapsim_gpu.h"
 #include "../host/mesh.hpp"
@@ -124,5 +125,13 @@ int tsg_mesh_get_nodeinfo(const tsg_mesh* h, int* assoc, unsigned char* boundary
 }
 
 void tsg_mesh_destroy(tsg_mesh* h) { delete h; }
+
+int tsg_best_plane(const float* offsets, int count, float out[3])
+{
+	if (count < 0 || (count > 0 && !offsets) || !out) return tsg::fail(TSG_E_ARG, "bad arguments");
+	std::vector<float> off(offsets, offsets + 3 * (size_t) count);
+	if (!tsg::best_plane(off, out)) return tsg::fail(TSG_E_NAN, "Geometry_3d::find_best_plane(): NaN");
+	return TSG_OK;
+}
 
 }  // extern "C"

@@ -892,6 +892,14 @@ int Grid::surface_nodes(int* out) const
 	return 0;
 }
 
+int Grid::surface_normal(int node, float out[3])
+{
+	if (node < 0 || node >= mesh_.n) return fail(-1, "Surface_3d::normal: bad node index");
+	if (!tsg::surface_normal(mesh_, surface_, ids_.data(), vacuum_id_, node, out))
+		return fail(-3, "Geometry_3d::find_best_plane(): NaN");
+	return 0;
+}
+
 int Grid::upload_surface()
 {
 	if (!surface_dirty_) return 0;
@@ -1138,6 +1146,7 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		r.node = node;
 		r.id = ids_[node];
 		r.near_tie = nt;
+		if (int rc = surface_normal(node, r.normal)) return rc;
 		if (o.exact_probability) {
 			double sum;
 			HIPCHK(hipMemcpyAsync(&sum, d_sum_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));

@@ -45,8 +45,9 @@ struct EventRecord {
 	int warm_start;
 	double residual_cold, residual_warm;
 	// steps V and VI (process.cpp:940-1037): the ion's flight and the
-	// reconstruction quantities; apex = highest surface site before the event
-	float start[3], stop[3], apex[3];
+	// reconstruction quantities; apex = highest surface site before the event,
+	// normal = the surface normal at the evaporated site (Surface_3d::normal)
+	float start[3], stop[3], apex[3], normal[3];
 	float voltage, time_scale, tof;
 	int traj_len, traj_status;
 };
@@ -134,6 +135,7 @@ public:
 	int surface_init(int vacuum_id);
 	int surface_size() const { return (int) surface_.size(); }
 	int surface_nodes(int* out) const;
+	int surface_normal(int node, float out[3]);  // Surface_3d::Table::normal
 	// evap_compute_specificFields + evap_compute_probabilities (mode 0 linear
 	// field, 1 Boltzmann, 2 linear force, 3 Voronoi flux force)
 	int surface_evaluate(bool exact_sum, double* scaling_ref, int mode = 0);

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <string>
+#include <set>
 #include <vector>
 
 namespace tsg {
@@ -67,6 +68,12 @@ bool delaunay(const std::vector<float>& xyz, Mesh* out, std::string* err, int th
 
 // Builds assoc, boundary, node->tet and node->node tables from tet_v/tet_n.
 void build_topology(Mesh* m, int threads = 0);
+
+// Geometry_3d::find_best_plane over offsets (x,y,z triples) and
+// Surface_3d::Table::normal; both false on a NaN result (normal.cpp).
+bool best_plane(const std::vector<float>& off, float out[3]);
+bool surface_normal(const Mesh& mesh, const std::set<int>& surface, const int16_t* ids, int vacuum_id, int node,
+                    float out[3]);
 
 // Orientation sign of det[b-a, c-a, d-a] with exact fallback.
 int orient_sign(const float* a, const float* b, const float* c, const float* d);

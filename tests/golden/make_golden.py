@@ -147,6 +147,7 @@ def make_small():
         fx["ref_evap_trajlen"], fx["ref_evap_surface"] = ints[:, 2], ints[:, 3]
         fx["ref_evap_iter"] = rd(wd, "evap_iter.i64", np.int64).reshape(-1, 2)
         fx["ref_evap_f"] = rd(wd, "evap_f.f32", np.float32).reshape(-1, 22)
+        fx["ref_evap_normal"] = rd(wd, "evap_normal.f32", np.float32).reshape(-1, 3)
         fx["ref_phi0_evap"] = rd(wd, "phi0_evap.f32", np.float32)
         fx["ref_phi1_evap"] = rd(wd, "phi1_evap.f32", np.float32)
     with tempfile.TemporaryDirectory() as wd:

@@ -166,6 +166,8 @@ def test_evaporation_sequence(tip_small, mode):
     assert same_bits([e.pot_after for e in evg], [e.pot_after for e in evo], fdt)
     assert [e.local_iter for e in evg] == [e.local_iter for e in evo]
     assert [e.global_iter for e in evg] == [e.refresh_iter for e in evo]
+    assert same_bits([list(e.apex) for e in evg], [list(e.apex) for e in evo], fdt)
+    assert same_bits([list(e.normal) for e in evg], [list(e.normal) for e in evo], fdt)
     dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
     assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
     if mode == tsg.FAITHFUL_F32:
@@ -175,6 +177,7 @@ def test_evaporation_sequence(tip_small, mode):
         assert same_bits([e.pot_before for e in evg], F[:, 1], np.float32)
         assert same_bits([list(e.field_before) for e in evg], F[:, 2:5], np.float32)
         assert same_bits([e.pot_after for e in evg], F[:, 5], np.float32)
+        assert same_bits([list(e.normal) for e in evg], tip_small["ref_evap_normal"], np.float32)
         assert same_bits(g.phi(0), tip_small["ref_phi0_evap"], np.float32)
 
 
@@ -345,6 +348,7 @@ def test_evaporation_with_trajectories_matches_reference(tip_small):
     assert np.array_equal([x.node for x in ev], tip_small["ref_evap_node"])
     assert same_bits([list(x.start) for x in ev], F[:, 11:14], np.float32)
     assert same_bits([list(x.apex) for x in ev], F[:, 18:21], np.float32)
+    assert same_bits([list(x.normal) for x in ev], tip_small["ref_evap_normal"], np.float32)
     stop = np.array([list(x.stop) for x in ev])
     span = np.max(np.abs(F[:, 14:17]))
     log("max stop deviation %.3g of %.3g; lengths %s vs %s" % (np.max(np.abs(stop - F[:, 14:17])), span,

@@ -106,6 +106,8 @@ def test_evaporation_sequence(tip_small):
     assert bits_equal([list(e.stop) for e in ev], F[:, 14:17])
     assert np.array_equal([e.traj_len for e in ev], tip_small["ref_evap_trajlen"])
     assert np.array_equal([[e.local_iter, e.refresh_iter] for e in ev], tip_small["ref_evap_iter"])
+    assert bits_equal([list(e.apex) for e in ev], F[:, 18:21])
+    assert bits_equal([list(e.normal) for e in ev], tip_small["ref_evap_normal"])
     v = np.array([e.voltage for e in ev])
     np.testing.assert_allclose(v, F[:, 9], rtol=1e-3)
     np.testing.assert_allclose([e.time_scale for e in ev], F[:, 10], rtol=1e-3)
