@@ -152,9 +152,14 @@ int tsg_surface_evaluate(tsg_grid*, int exact_sum, double* scaling_ref);
    (surface_3d.cpp:223-417): 0 probLinear_field, 1 probBoltzmann (needs the
    grid's temperature), 2 probLinear_force (field_o2), 3 probVoronoiFluxForce */
 int tsg_surface_evaluate_model(tsg_grid*, int mode, int exact_sum, double* scaling_ref);
-/* seeds the process's rand(), which evapMonteCarlo draws from (the
-   reference seeds it with 99, tapsim.cpp:235) */
+/* seeds the process's rand() stream (the reference seeds the C library's
+   with 99, tapsim.cpp:235).  The library keeps this stream itself, so that
+   the point-location walks on the GPU can continue it: evapMonteCarlo, the
+   thermal start velocity, randomReset and every walk draw from it in the
+   reference's order.  For equal seeds it is glibc's rand() sequence. */
 void tsg_srand(unsigned int seed);
+/* the next count draws of that stream (std::rand() in the reference) */
+int tsg_rand(int count, int* out);
 int tsg_surface_values(tsg_grid*, double* out);
 int tsg_surface_probabilities(tsg_grid*, float* out);
 /* evapMaximum (surface_3d.cpp:423-431) */
@@ -189,6 +194,9 @@ typedef struct tsg_evap_options
 	float dt0, dt_min, err_limit;
 	int voltage_queue;        /* running mean length of the voltage (process.cpp:983-1010) */
 	float shrink_limit;       /* stop once the apex has receded this far, m */
+	int thermal_velocity;     /* 1: start velocity of evap_initialVelocity (surface_3d.cpp:472-502) from the
+	                             process's rand() and the grid's temperature; 0: at rest (the reference's
+	                             TRAJECTORY_NO_INITIAL_VELOCITY = 1) */
 } tsg_evap_options;
 
 typedef struct tsg_event

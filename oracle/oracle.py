@@ -34,6 +34,7 @@ class OraEvapOpts(ctypes.Structure):
         ("refresh_queue", ctypes.c_int), ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int),
         ("stepper", ctypes.c_int), ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_uint), ("shrink_limit", ctypes.c_float), ("inplace_order", ctypes.c_int),
+        ("thermal_velocity", ctypes.c_int),
     ]
 
 
@@ -47,7 +48,7 @@ class OraEvent(ctypes.Structure):
         ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int),
         ("local_iter", ctypes.c_longlong), ("refresh_iter", ctypes.c_longlong),
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_float), ("apex", ctypes.c_float * 3),
-        ("normal", ctypes.c_float * 3),
+        ("normal", ctypes.c_float * 3), ("traj_root_diff", ctypes.c_int),
     ]
 
 
@@ -97,6 +98,8 @@ _SIGS = {
     "ora_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "ora_srand": ([ctypes.c_uint], None),
     "ora_find_tetrahedron": ([_P, _P, ctypes.c_int], ctypes.c_int),
+    "ora_locate": ([_P, _P], ctypes.c_int),
+    "ora_set_root_mode": ([ctypes.c_int], None),
     "ora_tet_weights": ([_P, ctypes.c_int, _P, _P], None),
     "ora_point_potential": ([_P, _P, ctypes.c_int], ctypes.c_float),
     "ora_point_field": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
@@ -342,8 +345,12 @@ class Oracle:
     def srand(self, seed):
         load().ora_srand(seed)
 
-    def find_tetrahedron(self, pos, guess):
+    def find_tetrahedron(self, pos, guess=None):
+        """findTetrahedron(pos, guess); without a guess the one-argument form
+        (tetGuess's random sample, then the walk)"""
         p = np.ascontiguousarray(pos, dtype=np.float32)
+        if guess is None:
+            return load().ora_locate(self._h, _ptr(p))
         return load().ora_find_tetrahedron(self._h, _ptr(p), guess)
 
     def trajectory(self, node, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6, cap=200000):
@@ -417,3 +424,9 @@ def best_plane(offsets):
     if rc:
         raise RuntimeError(load().ora_last_error().decode())
     return out
+
+
+def set_root_mode(mode):
+    """The trajectories' step-size root: 0 glibc's powf(x, 0.25f) as the
+    reference, 1 the correctly rounded fourth root that the device uses."""
+    load().ora_set_root_mode(mode)

@@ -304,7 +304,7 @@ int main(int argc, char** argv)
 			}
 			else if (cmd == "evap")
 			{
-				// evap:EVENTS:PROB:CHOICE:LTHR:SHELL:LCYC:LQUE:GLOBAL:RINT:RTHR:RCYC:RQUE:INTEG:STEPPER
+				// evap:EVENTS:PROB:CHOICE:LTHR:SHELL:LCYC:LQUE:GLOBAL:RINT:RTHR:RCYC:RQUE:INTEG:STEPPER[:THERMAL]
 				// The steps I-VI of Process::evaporation (process.cpp:680-1037)
 				// composed of the reference's own functions.  The fork's
 				// update_coordination (process.cpp:647,770) is left out: it adds
@@ -326,6 +326,7 @@ int main(int argc, char** argv)
 				const unsigned int rque = std::atoi(tok.at(12).c_str());
 				const int integ = std::atoi(tok.at(13).c_str());
 				const int stepper = std::atoi(tok.at(14).c_str());
+				const int thermal = tok.size() > 15 ? std::atoi(tok.at(15).c_str()) : 0;
 
 				Surface_3d::Table surf;
 				surf.setVacuumId(system.configTable["Vacuum"].id());
@@ -378,7 +379,9 @@ int main(int argc, char** argv)
 					traj.setTimeStepLimit(1e-16f);
 					traj.setStepperLimit(std::numeric_limits<unsigned long>::max());
 					traj.setIterationLimit(std::numeric_limits<unsigned long>::max());
-					traj.init(system.geomTable.nodeCoords(node), MathVector3d<float>(0.0f), charge, mass,
+					MathVector3d<float> velocity(0.0f);
+					if (thermal) velocity = evap_initialVelocity(nodeId, system.configTable);
+					traj.init(system.geomTable.nodeCoords(node), velocity, charge, mass,
 					          system.geomTable.node(node).associatedTetrahedron);
 					traj.integrate(1e-14f, 1e-6f);
 					traj.extrapolate(Geometry_3d::Point::Z, system.geomTable.max().z());
@@ -429,6 +432,9 @@ int main(int argc, char** argv)
 					for (int c = 0; c < 3; c++) recNormal.push_back(normal[c]);
 				}
 				writeArray("evap_normal.f32", recNormal);
+				// where the events leave the process's rand() stream
+				std::vector<int> nextRand(1, std::rand());
+				writeArray("evap_next_rand.i32", nextRand);
 				writeArray("evap_int.i32", recNode);
 				writeArray("evap_iter.i64", recIter);
 				writeArray("evap_f.f32", rec);

@@ -854,11 +854,11 @@ static int tet_guess(const ora_sys* s, const float* pos)
 	return ref;
 }
 
-/* Geometry_3d::Table::findTetrahedron, geometry_3d.cpp:988-1044: stochastic
-   visibility walk, random first face from rand() % 4 */
+/* Geometry_3d::Table::findTetrahedron(pos, tetIndex), geometry_3d.cpp:988-1044:
+   stochastic visibility walk, random first face from rand() % 4; a negative
+   start tet returns -1 at once, without a draw */
 int ora_find_tetrahedron(const ora_sys* s, const float pos[3], int t)
 {
-	if (t < 0) t = tet_guess(s, pos);
 	for (int loop = 0; loop < s->m; loop++)
 	{
 		if (t < 0) return -1;
@@ -876,6 +876,13 @@ int ora_find_tetrahedron(const ora_sys* s, const float pos[3], int t)
 		if (index == maxI) return t;
 	}
 	return -2;
+}
+
+/* Geometry_3d::Table::findTetrahedron(pos), geometry_3d.cpp:982-986: the
+   walk from tetGuess's sample */
+int ora_locate(const ora_sys* s, const float pos[3])
+{
+	return ora_find_tetrahedron(s, pos, tet_guess(s, pos));
 }
 
 /* Geometry_3d::Table::tetVolume / tetWeights, geometry_3d.cpp:872-980 */
@@ -970,7 +977,22 @@ typedef struct
 	phase* data;
 	int len, cap;
 	phase err_est;
+	int root_diff;   /* steps whose glibc powf(x, 0.25f) differs from the correctly rounded fourth root */
 } traj;
+
+/* the step-size control's powf(x, 0.25f): glibc's, as the reference calls
+   it (mode 0), or the correctly rounded fourth root that the device computes
+   (mode 1); steps where the two differ are counted */
+static int g_root_mode = 0;
+void ora_set_root_mode(int mode) { g_root_mode = mode; }
+
+static float quarter_pow(traj* tr, float x)
+{
+	const float p = powf(x, 0.25f);
+	const float c = (float) sqrt(sqrt((double) x));
+	if (p != c) tr->root_diff++;
+	return g_root_mode ? c : p;
+}
 
 /* Trajectory_3d::o5_integrator, trajectory_3d.cpp:583-676 (Cash-Karp tableau
    in float, Numerical Recipes 16.2) */
@@ -1140,7 +1162,7 @@ static int stepper_step(traj* tr, float* delta, float err_limit, phase* error, p
 			if (rel != 0.0f)
 			{
 				float tmp = *delta;
-				tmp *= powf(err_limit / rel, 0.25f);
+				tmp *= quarter_pow(tr, err_limit / rel);
 				tmp *= safety;
 				if (tmp < *delta) *delta *= 1.1f; else *delta = tmp;
 			}
@@ -1148,7 +1170,7 @@ static int stepper_step(traj* tr, float* delta, float err_limit, phase* error, p
 			return 0;
 		}
 		float tmp = *delta;
-		tmp *= powf(err_limit / rel, 0.25f);
+		tmp *= quarter_pow(tr, err_limit / rel);
 		tmp *= safety;
 		if (tmp > *delta) *delta *= 0.9f; else *delta = tmp;
 	}
@@ -1166,11 +1188,12 @@ static void traj_push(traj* tr, phase p)
 
 /* Trajectory_3d::init / integrate(delta, errorLimit) / extrapolate,
    trajectory_3d.cpp:99-126,130-205,303-332 */
-static int traj_run(traj* tr, const float start[3], int guess, float dt0, float err_limit, float zmax)
+static int traj_run(traj* tr, const float start[3], const float* v0, int guess, float dt0, float err_limit, float zmax)
 {
 	phase p0;
 	memset(&p0, 0, sizeof(p0));
 	for (int k = 0; k < 3; k++) p0.p[k] = start[k];
+	if (v0) for (int k = 0; k < 3; k++) p0.v[k] = v0[k];
 	p0.tet = ora_find_tetrahedron(tr->s, start, guess);
 	if (p0.tet < 0) return fail(ORA_E_GEOM, "Trajectory_3d::init(): initial point is outside");
 	tr->len = 0;
@@ -1219,6 +1242,31 @@ static int traj_run(traj* tr, const float start[3], int guess, float dt0, float 
 	return status;
 }
 
+/* evap_initialVelocity, surface_3d.cpp:472-502 (two uniform splits of
+   2kT/m, then a random sign per axis; five rand() draws) */
+static void initial_velocity(const ora_sys* s, const ora_species* sp, float v[3])
+{
+	double emax = 2.0 * 1.3806504e-23;
+	emax *= s->temperature;
+	emax /= sp->mass;
+	emax /= 1.660538782e-27;
+	float x = (float) rand();
+	x /= (float) RAND_MAX;
+	x = (float) (x * emax);
+	float y = (float) rand();
+	y /= (float) RAND_MAX;
+	y = (float) (y * (emax - x));
+	float z = (float) emax;
+	z -= x;
+	z -= y;
+	v[0] = x; v[1] = y; v[2] = z;
+	for (int i = 0; i < 3; i++)
+	{
+		v[i] = sqrtf(v[i]);
+		if (rand() % 2) v[i] = -v[i];
+	}
+}
+
 static float zmax_of(const ora_sys* s)
 {
 	/* Geometry_3d::Table::setExtents, geometry_3d.cpp:1509-1528 */
@@ -1236,7 +1284,7 @@ int ora_trajectory(const ora_sys* s, int node, float charge, float mass, int int
 	memset(&tr, 0, sizeof(tr));
 	tr.s = s; tr.charge = charge; tr.mass = mass; tr.integrator = integrator; tr.stepper = stepper;
 	tr.dt_min = dt_min;
-	const int status = traj_run(&tr, nc(s, node), s->assoc[node], dt0, err_limit, zmax_of(s));
+	const int status = traj_run(&tr, nc(s, node), NULL, s->assoc[node], dt0, err_limit, zmax_of(s));
 	for (int k = 0; k < tr.len && k < cap; k++)
 	{
 		float* o = out + 8 * (size_t) k;
@@ -1521,8 +1569,11 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 			mass *= 1.660538782e-27f;
 			tr.s = s; tr.charge = charge; tr.mass = mass;
 			tr.integrator = o->integrator; tr.stepper = o->stepper; tr.dt_min = o->dt_min;
-			r->traj_status = traj_run(&tr, nc(s, node), s->assoc[node], o->dt0, o->err_limit, zmax);
+			float v0[3] = { 0.0f, 0.0f, 0.0f };
+			if (o->thermal_velocity) initial_velocity(s, sp, v0);
+			r->traj_status = traj_run(&tr, nc(s, node), v0, s->assoc[node], o->dt0, o->err_limit, zmax);
 			r->traj_len = tr.len;
+			r->traj_root_diff = tr.root_diff;
 			for (int k = 0; k < 3; k++) { r->start[k] = tr.data[0].p[k]; r->stop[k] = tr.data[tr.len - 1].p[k]; }
 			for (int k = tr.len - 1; k >= 0; k--)
 				if (tr.data[k].tet >= 0) { have_back = 1; backt = tr.data[k].tet; for (int c = 0; c < 3; c++) backp[c] = tr.data[k].p[c]; break; }

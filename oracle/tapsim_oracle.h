@@ -56,6 +56,7 @@ typedef struct
 	unsigned int voltage_queue;
 	float shrink_limit;
 	int inplace_order;     /* 0 ascending sequential (one thread), 1 colour classes */
+	int thermal_velocity;  /* 1: evap_initialVelocity from rand(), 0: at rest */
 } ora_evap_opts;
 
 typedef struct
@@ -73,6 +74,7 @@ typedef struct
 	float scaling_ref;
 	float apex[3];
 	float normal[3];
+	int traj_root_diff;    /* flight steps where powf(x, 0.25f) is not the correctly rounded root */
 } ora_event;
 
 typedef struct ora_sys ora_sys;
@@ -139,7 +141,11 @@ int ora_best_plane(const float* offsets, int count, float out[3]);
 
 /* trajectory (float, as trajectory_3d.cpp) */
 void ora_srand(unsigned int seed);
-int ora_find_tetrahedron(const ora_sys*, const float pos[3], int guess);
+int ora_find_tetrahedron(const ora_sys*, const float pos[3], int guess);  /* guess < 0: -1 */
+int ora_locate(const ora_sys*, const float pos[3]);                       /* tetGuess + walk */
+/* the trajectories' (tol/err)^(1/4): 0 glibc's powf as the reference, 1 the
+   correctly rounded root (the device's arithmetic) */
+void ora_set_root_mode(int mode);
 void ora_tet_weights(const ora_sys*, int tet, const float pos[3], float w[4]);
 float ora_point_potential(const ora_sys*, const float pos[3], int guess);
 int ora_point_field(const ora_sys*, const float pos[3], int guess, float out[3]);

@@ -75,7 +75,7 @@ class EvapOptions(ctypes.Structure):
         ("exact_probability", ctypes.c_int), ("method", ctypes.c_int), ("accel_tol", ctypes.c_double),
         ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int), ("stepper", ctypes.c_int),
         ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
-        ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float),
+        ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float), ("thermal_velocity", ctypes.c_int),
     ]
 
     def __init__(self, **kw):
@@ -83,6 +83,7 @@ class EvapOptions(ctypes.Structure):
         self.integrator, self.stepper = 1, 1
         self.dt0, self.dt_min, self.err_limit = 1e-14, 1e-16, 1e-6
         self.voltage_queue, self.shrink_limit = 250, 1.0
+        self.thermal_velocity = 0
         self.accel_tol = 1e-8
         for k, v in kw.items():
             setattr(self, k, v)
@@ -143,6 +144,7 @@ _SIGS = {
     "tsg_take_away": ([_P, ctypes.c_int], ctypes.c_int),
     "tsg_surface_normal": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_rand": ([ctypes.c_int, _P], ctypes.c_int),
     "tsg_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_sweep_stats": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "tsg_accel_setup": ([_P, ctypes.c_double, _P], ctypes.c_int),
@@ -206,8 +208,16 @@ def best_plane(offsets):
 
 
 def srand(seed):
-    """Seeds the process's rand() (the Monte Carlo choice draws from it)."""
+    """Seeds the process's rand() stream (the Monte Carlo choice, randomReset,
+    thermal start velocities and the point-location walks draw from it)."""
     load().tsg_srand(seed)
+
+
+def rand(count=1):
+    """The next draws of the process's rand() stream."""
+    out = np.zeros(count, np.int32)
+    _chk(load().tsg_rand(count, _ptr(out)))
+    return out
 
 
 def _chk(rc):

@@ -6,6 +6,7 @@
 
 #include "../../../include/tapsim_gpu.h"
 #include "../host/grid.hpp"
+#include "../host/prand.hpp"
 #include "errors.hpp"
 
 struct tsg_grid {
@@ -175,7 +176,14 @@ int tsg_surface_evaluate_model(tsg_grid* h, int mode, int exact_sum, double* sca
 	return check(h, h->g->surface_evaluate(exact_sum != 0, scaling_ref, mode));
 }
 
-void tsg_srand(unsigned int seed) { srand(seed); }
+void tsg_srand(unsigned int seed) { tsg::rand_seed(&tsg::process_rand(), seed); }
+
+int tsg_rand(int count, int* out)
+{
+	if (count < 0 || (count > 0 && !out)) return tsg::fail(TSG_E_ARG, "bad arguments");
+	for (int i = 0; i < count; i++) out[i] = tsg::prand();
+	return TSG_OK;
+}
 
 int tsg_surface_evaluate(tsg_grid* h, int exact_sum, double* scaling_ref)
 {
@@ -227,6 +235,7 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.err_limit = o->err_limit;
 	e.voltage_queue = o->voltage_queue;
 	e.shrink_limit = o->shrink_limit;
+	e.thermal_velocity = o->thermal_velocity;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
 }
 

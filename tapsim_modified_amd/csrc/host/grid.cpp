@@ -1,5 +1,6 @@
 // Host side of the device grid (see grid.hpp).
 #include "grid.hpp"
+#include "prand.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -1045,7 +1046,7 @@ int Grid::mc_candidate(int* node)
 	if (cnt == 0) return fail(-6, "empty surface");
 	std::vector<float> p(cnt);
 	if (int rc = surface_probabilities(p.data())) return rc;
-	double thr = rand();
+	double thr = prand();
 	thr /= RAND_MAX;
 	int k = 0;
 	double acc = p[0];
@@ -1105,6 +1106,30 @@ int Grid::take_away(int node)
 		if (int rc = update(j)) return rc;
 	}
 	return 0;
+}
+
+// evap_initialVelocity (surface_3d.cpp:472-502): kinetic energy kT split
+// over the axes by two uniform draws, a random sign per axis
+void Grid::thermal_velocity(const Species& sp, float v[3]) const
+{
+	double max = 2.0;
+	max *= 1.3806504e-23;
+	max *= temperature_;
+	max /= sp.mass;
+	max /= 1.660538782e-27;
+	v[0] = (float) prand();
+	v[0] /= RAND_MAX;
+	v[0] = (float) (v[0] * max);
+	v[1] = (float) prand();
+	v[1] /= RAND_MAX;
+	v[1] = (float) (v[1] * (max - v[0]));
+	v[2] = (float) max;
+	v[2] -= v[0];
+	v[2] -= v[1];
+	for (int i = 0; i < 3; i++) {
+		v[i] = std::sqrt(v[i]);
+		if (prand() % 2) v[i] *= -1.0f;
+	}
 }
 
 int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
@@ -1199,9 +1224,10 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 			charge *= 1.60217733e-19f;
 			float mass = sp.mass;
 			mass *= 1.660538782e-27f;
-			float sm[12];
+			float sm[12], v0[3] = {0.0f, 0.0f, 0.0f};
+			if (o.thermal_velocity) thermal_velocity(sp, v0);
 			if (int rc = trajectories(1, &node, &charge, &mass, o.integrator, o.stepper, o.dt0, o.dt_min, o.err_limit, 0,
-			                          nullptr, &r.traj_len, &r.traj_status, sm))
+			                          nullptr, &r.traj_len, &r.traj_status, sm, v0, true))
 				return rc;
 			for (int c = 0; c < 3; c++) {
 				r.start[c] = sm[c];
@@ -1222,7 +1248,8 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		flen = std::sqrt(flen);
 		if (have_back && flen != 0.0f) {
 			float pexit;
-			if (int rc = point_potential(backp, backt, &pexit)) return rc;
+			// the reference locates the exit point twice (voltage, time scale)
+			if (int rc = point_potential(backp, backt, &pexit, true, 2)) return rc;
 			float v = (float) r.pot_before;
 			v -= pexit;
 			v *= sp.evap_field;
@@ -1764,10 +1791,12 @@ namespace tsg {
 // its node, in the field of the current potentials (the buffer potential()
 // reads); one device thread per ion
 int Grid::trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
-                       float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status, float* summary)
+                       float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status, float* summary,
+                       const float* v0, bool process_rand)
 {
 	if (count <= 0) return 0;
 	if (cap < 0 || !nodes || !charge || !mass || !len || !status || (cap > 0 && !out)) return fail(-1, "trajectory: bad arguments");
+	if (process_rand && count != 1) return fail(-1, "trajectory: the rand() stream is continued for one ion at a time");
 	if (integrator < 0 || integrator > 1 || stepper < 0 || stepper > 3) return fail(-1, "trajectory: unknown integrator or stepper");
 	for (int i = 0; i < count; i++)
 		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "trajectory: bad node");
@@ -1778,8 +1807,9 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	if (launch_field_snapshot(stream_, n, d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_coupfull_.ptr(),
 	                          d_flags_.as<unsigned char>(), phi, f64_, d_nfield_.as<float>(), d_status_.as<int>()))
 		return fail(-2, "field snapshot launch failed");
-	// per ion: node, snapshot, charge, mass in; len, status, 12-float summary out
-	const size_t in_bytes = 16 * (size_t) count, res_bytes = 56 * (size_t) count;
+	// per ion: node, snapshot, charge, mass, start velocity in; len, status,
+	// 12-float summary out
+	const size_t in_bytes = 28 * (size_t) count, res_bytes = 56 * (size_t) count;
 	const size_t out_bytes = sizeof(float) * 8 * (size_t) count * (size_t) std::max(cap, 1);
 	if (d_traj_in_.bytes() < in_bytes) d_traj_in_.alloc(in_bytes * 2);
 	if (d_traj_res_.bytes() < res_bytes) d_traj_res_.alloc(res_bytes * 2);
@@ -1793,6 +1823,10 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	HIPCHK(hipMemcpyAsync(din + sizeof(int) * 2 * count, charge, sizeof(float) * count, hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipMemcpyAsync(din + sizeof(int) * 2 * count + sizeof(float) * count, mass, sizeof(float) * count,
 	                      hipMemcpyHostToDevice, stream_));
+	float* dv0 = (float*) (din + 16 * (size_t) count);
+	if (v0) HIPCHK(hipMemcpyAsync(dv0, v0, sizeof(float) * 3 * count, hipMemcpyHostToDevice, stream_));
+	RandState& prs = tsg::process_rand();
+	if (process_rand) HIPCHK(hipMemcpyAsync(d_rng_.ptr(), &prs, sizeof(RandState), hipMemcpyHostToDevice, stream_));
 	TrajArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.count = count;
@@ -1822,6 +1856,8 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	a.len = res;
 	a.status = res + count;
 	a.summary = summary ? (float*) (res + 2 * count) : nullptr;
+	a.v0 = v0 ? dv0 : nullptr;
+	a.rng = process_rand ? d_rng_.as<int>() : nullptr;
 	if (launch_trajectories(stream_, a)) return fail(-2, "trajectory launch failed");
 	int st = 0;
 	if (cap > 0) HIPCHK(hipMemcpyAsync(out, d_traj_out_.ptr(), out_bytes, hipMemcpyDeviceToHost, stream_));
@@ -1829,6 +1865,7 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	HIPCHK(hipMemcpyAsync(status, res + count, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
 	if (summary) HIPCHK(hipMemcpyAsync(summary, res + 2 * count, sizeof(float) * 12 * count, hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+	if (process_rand) HIPCHK(hipMemcpyAsync(&prs, d_rng_.ptr(), sizeof(RandState), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
 	return 0;
@@ -1845,7 +1882,9 @@ int Grid::ensure_point_location()
 	d_gbnd_.alloc((size_t) n);
 	d_nfield_.alloc(sizeof(float) * 3 * (size_t) n);
 	d_ppos_.alloc(sizeof(float) * 4 + sizeof(int) * 4);
-	if (!d_tv_.ptr() || !d_tn_.ptr() || !d_assoc_.ptr() || !d_gbnd_.ptr() || !d_nfield_.ptr() || !d_ppos_.ptr()) {
+	d_rng_.alloc(sizeof(RandState));
+	if (!d_tv_.ptr() || !d_tn_.ptr() || !d_assoc_.ptr() || !d_gbnd_.ptr() || !d_nfield_.ptr() || !d_ppos_.ptr() ||
+	    !d_rng_.ptr()) {
 		d_tv_.free();
 		return fail(-2, "device allocation failed (point location)");
 	}
@@ -1859,9 +1898,12 @@ int Grid::ensure_point_location()
 }
 
 // Grid_3d::potential(pos, guess) (grid_3d.cpp:1585-1602) on the potential() buffer
-int Grid::point_potential(const float pos[3], int guess, float* out)
+int Grid::point_potential(const float pos[3], int guess, float* out, bool process_rand, int walks)
 {
+	if (walks < 1) return fail(-1, "point potential: walks must be positive");
 	if (int rc = ensure_point_location()) return rc;
+	RandState& prs = tsg::process_rand();
+	if (process_rand) HIPCHK(hipMemcpyAsync(d_rng_.ptr(), &prs, sizeof(RandState), hipMemcpyHostToDevice, stream_));
 	float* dp = d_ppos_.as<float>();
 	int* dg = (int*) (dp + 3);
 	int g = guess;
@@ -1871,9 +1913,10 @@ int Grid::point_potential(const float pos[3], int guess, float* out)
 	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
 	float* dout = (float*) (dg + 1);
 	if (launch_point_potential(stream_, 1, dp, dg, d_xyz_.as<float>(), d_tv_.as<int>(), d_tn_.as<int>(), mesh_.m(), phi,
-	                           f64_, dout))
+	                           f64_, dout, process_rand ? d_rng_.as<int>() : nullptr, walks))
 		return fail(-2, "point potential launch failed");
 	HIPCHK(hipMemcpyAsync(out, dout, sizeof(float), hipMemcpyDeviceToHost, stream_));
+	if (process_rand) HIPCHK(hipMemcpyAsync(&prs, d_rng_.ptr(), sizeof(RandState), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	return 0;
 }
@@ -1928,7 +1971,7 @@ static void random_potentials(const std::vector<int16_t>& ids, const std::vector
 		R phi;
 		if (flags[i] & 2) phi = sp[ids[i]].phi;
 		else {
-			phi = (R) rand();
+			phi = (R) prand();
 			phi /= RAND_MAX;
 			phi *= maxPhi - minPhi;
 			phi += minPhi;

@@ -14,6 +14,7 @@
 
 #include "../kernels/kernels.hpp"
 #include "mesh.hpp"
+#include "prand.hpp"
 
 namespace tsg {
 
@@ -69,6 +70,7 @@ struct EvapOptions {
 	float dt0 = 1e-14f, dt_min = 1e-16f, err_limit = 1e-6f;
 	int voltage_queue = 250;
 	float shrink_limit = 1.0f;  // stop once the apex has receded this far (m)
+	int thermal_velocity = 0;   // 1: evap_initialVelocity from the process's rand()
 };
 
 class DevBuf {
@@ -159,11 +161,17 @@ public:
 	// ion trajectories (Trajectory_3d, trajectory_3d.cpp:99-332) of a batch of
 	// ions from their nodes in the current potentials; out holds cap phases
 	// (t, x, y, z, vx, vy, vz, tet) per ion
+	// v0: optional start velocities (3 per ion); process_rand: the walks
+	// continue the process's rand() stream (one ion only)
 	int trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
 	                 float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status,
-	                 float* summary = nullptr);
-	// Grid_3d::potential(pos, guess): barycentric potential of a point
-	int point_potential(const float pos[3], int guess, float* out);
+	                 float* summary = nullptr, const float* v0 = nullptr, bool process_rand = false);
+	// Grid_3d::potential(pos, guess): barycentric potential of a point; with
+	// process_rand the point is located `walks` times drawing from the
+	// process's rand() stream, as that many calls of the reference would
+	int point_potential(const float pos[3], int guess, float* out, bool process_rand = false, int walks = 1);
+	// evap_initialVelocity (surface_3d.cpp:472-502) from the process's rand()
+	void thermal_velocity(const Species& sp, float v[3]) const;
 
 	// learned warm start: events of methods 1 and 2 record their shell graph
 	// and relaxation target into the network's store (record), method 2
@@ -246,7 +254,7 @@ private:
 	                std::vector<float>* vf);
 	AccelArgs accel_args() const;
 	// trajectories: mesh arrays for the point location, node-field snapshot
-	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_;
+	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_, d_rng_;
 	float zmax_ = 0.0f;
 	int ensure_point_location();
 	// Process::evaporation's voltage queue (process.cpp:983-1037) and initial apex

@@ -162,11 +162,17 @@ struct TrajArgs {
 	int* status;              // Trajectory_3d status (trajectory_3d.h:89), negative on failure
 	float* summary;           // optional, 12 floats per ion: start xyz, stop xyz, t of the last
 	                          // phase, last phase inside the mesh (xyz, tet), 1 if there is one
+	const float* v0;          // optional start velocities, 3 per ion (m/s); at rest when null
+	int* rng;                 // optional rand() stream (31-word table + index, see host/prand.hpp):
+	                          // the walks draw their first face from it; count must be 1
 };
 // Grid_3d::potential(pos, guess) (grid_3d.cpp:1585-1602) of `count` points:
-// barycentric interpolation of the potentials in float; 0 outside the mesh
+// barycentric interpolation of the potentials in float; 0 outside the mesh.
+// With rng (count 1) the point is located `walks` times in a row, each walk
+// drawing from the stream as the reference's repeated calls do.
 int launch_point_potential(hipStream_t, int count, const float* pos, const int* guess, const float* xyz, const int* tv,
-                           const int* tn, int m, const void* phi, bool f64, float* out);
+                           const int* tn, int m, const void* phi, bool f64, float* out, int* rng = nullptr,
+                           int walks = 1);
 int launch_trajectories(hipStream_t, const TrajArgs&);
 int launch_field_snapshot(hipStream_t, int n, const float* xyz, const int* ptr, const int* idx, const void* coup,
                           const unsigned char* flags, const void* phi, bool f64, float* out, int* status);

@@ -6,9 +6,11 @@
 // One thread integrates one ion, in float as the reference does.  The node
 // fields come from a snapshot (field_o1 of every node, taken after the
 // relaxation), the counterpart of FastField's per-node cache.  Point
-// location is the visibility walk with an exact orientation sign; the walk
-// starts at face 0 where the reference draws the first face from rand(),
-// which changes which tet is returned only for a point exactly on a face.
+// location is the visibility walk with an exact orientation sign.  The
+// reference draws each walk step's first face from rand(); when the caller
+// hands over the process's rand() stream (one ion, as in the event loop) the
+// walk draws from it in the same order, otherwise it starts at face 0, which
+// changes which tet is returned only for a point exactly on a face.
 #include "device_math.hpp"
 #include "field.hpp"
 #include "kernels.hpp"
@@ -166,16 +168,46 @@ struct Mesh3 {
 	__device__ const float* P(int i) const { return xyz + 3 * (size_t) i; }
 };
 
+// the process's rand() stream continued on the device: glibc's 31-word
+// additive table (host/prand.hpp), staged in LDS by the one thread using it
+struct DevRand {
+	int* s;
+	int r;
+	__device__ int next()
+	{
+		const int f = r + 3 < 31 ? r + 3 : r + 3 - 31;
+		const unsigned v = (unsigned) s[f] + (unsigned) s[r];
+		s[f] = (int) v;
+		r = r + 1 < 31 ? r + 1 : 0;
+		return (int) (v >> 1);
+	}
+};
+
+__device__ void rand_stage(DevRand* d, int* lds, const int* src)
+{
+	for (int i = 0; i < 31; i++) lds[i] = src[i];
+	d->s = lds;
+	d->r = src[31];
+}
+
+__device__ void rand_unstage(const DevRand& d, int* dst)
+{
+	for (int i = 0; i < 31; i++) dst[i] = d.s[i];
+	dst[31] = d.r;
+}
+
 // findTetrahedron walk from t (geometry_3d.cpp:988-1044); -1 outside the
-// mesh, -2 when the walk does not end
-__device__ int find_tet(const Mesh3& g, const float* pos, int t)
+// mesh, -2 when the walk does not end.  Each step starts at face
+// rand() % 4 when a stream is given, else at face 0.
+__device__ int find_tet(const Mesh3& g, const float* pos, int t, DevRand* rng = nullptr)
 {
 	const int limit = min(g.m, 1 << 20);
 	for (int loop = 0; loop < limit; loop++) {
 		if (t < 0) return -1;
 		const int* v = g.tv + 4 * (size_t) t;
+		const int lo = rng ? rng->next() % 4 : 0, hi = lo + 4;
 		int index;
-		for (index = 0; index < 4; index++) {
+		for (index = lo; index < hi; index++) {
 			// tetOrient(n1, n2, n3, pos) = orient3d(p2, p1, p3, pos)
 			const int n1 = v[index % 4], n2 = v[3 - (2 + index) % 4], n3 = v[(2 + index) % 4];
 			if (orient_sign_f(g.P(n2), g.P(n1), g.P(n3), pos) < 0) {
@@ -183,7 +215,7 @@ __device__ int find_tet(const Mesh3& g, const float* pos, int t)
 				break;
 			}
 		}
-		if (index == 4) return t;
+		if (index == hi) return t;
 	}
 	return -2;
 }
@@ -243,10 +275,11 @@ __device__ void tet_weights(const Mesh3& g, int t, const float* p, float* w)
 }
 
 // FastField::compute: field at pos from the node snapshot (zero outside the mesh)
-__device__ void fast_field(const Mesh3& g, const float* nfield, const float* pos, int guess, float* out)
+__device__ void fast_field(const Mesh3& g, const float* nfield, const float* pos, int guess, float* out,
+                           DevRand* rng = nullptr)
 {
 	out[0] = out[1] = out[2] = 0.0f;
-	const int t = find_tet(g, pos, guess);
+	const int t = find_tet(g, pos, guess, rng);
 	if (t < 0) return;
 	float w[4];
 	tet_weights(g, t, pos, w);
@@ -271,6 +304,7 @@ struct Tr {
 	float dt_min;
 	bool geom_error;       // the work limit ended the integration
 	long long calls, max_calls;
+	DevRand* rng;          // the process's rand() stream, or null
 };
 
 // Trajectory_3d::o5_integrator (trajectory_3d.cpp:583-676): Cash-Karp
@@ -298,7 +332,7 @@ __device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
 				vel[l] += b[i][j] * dv[j][l];
 				pos[l] += b[i][j] * dp[j][l];
 			}
-		fast_field(tr.g, tr.nfield, pos, back.tet, dv[i]);
+		fast_field(tr.g, tr.nfield, pos, back.tet, dv[i], tr.rng);
 		for (int l = 0; l < 3; l++) dv[i][l] *= qm;
 		for (int l = 0; l < 3; l++) dv[i][l] *= delta;
 		for (int l = 0; l < 3; l++) dp[i][l] = vel[l] * delta;
@@ -319,7 +353,7 @@ __device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
 			error->v[i] += (c1[j] - c2[j]) * dv[j][i];
 		}
 	}
-	next.tet = find_tet(tr.g, next.p, back.tet);
+	next.tet = find_tet(tr.g, next.p, back.tet, tr.rng);
 	return next;
 }
 
@@ -327,7 +361,7 @@ __device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
 __device__ Phase o1_step(Tr& tr, const Phase& back, float delta, Phase* error)
 {
 	float f[3];
-	fast_field(tr.g, tr.nfield, back.p, back.tet, f);
+	fast_field(tr.g, tr.nfield, back.p, back.tet, f, tr.rng);
 	for (int i = 0; i < 3; i++) f[i] *= tr.charge;
 	for (int i = 0; i < 3; i++) f[i] /= tr.mass;
 	Phase next = back;
@@ -342,7 +376,7 @@ __device__ Phase o1_step(Tr& tr, const Phase& back, float delta, Phase* error)
 		error->p[i] = next.p[i] * d2;
 		error->v[i] = next.v[i] * d2;
 	}
-	next.tet = find_tet(tr.g, next.p, back.tet);
+	next.tet = find_tet(tr.g, next.p, back.tet, tr.rng);
 	return next;
 }
 
@@ -467,11 +501,9 @@ __device__ void put(const TrajArgs& a, int ion, int k, const Phase& q)
 }
 
 // Trajectory_3d::init / integrate(delta, errorLimit) / extrapolate(Z, zmax)
-// (trajectory_3d.cpp:99-126, 130-205, 303-332) for ion `i` of the batch
-__global__ void k_trajectory(TrajArgs a)
+// (trajectory_3d.cpp:99-126, 130-205, 303-332) for ion `ion` of the batch
+__device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng)
 {
-	const int ion = blockIdx.x * blockDim.x + threadIdx.x;
-	if (ion >= a.count) return;
 	Tr tr;
 	tr.g.xyz = a.xyz;
 	tr.g.tv = a.tv;
@@ -486,14 +518,15 @@ __global__ void k_trajectory(TrajArgs a)
 	tr.geom_error = false;
 	tr.calls = 0;
 	tr.max_calls = a.max_calls;
+	tr.rng = rng;
 	const int node = a.nodes[ion];
 	Phase back;
 	back.t = 0.0f;
 	for (int k = 0; k < 3; k++) {
 		back.p[k] = a.xyz[3 * (size_t) node + k];
-		back.v[k] = 0.0f;
+		back.v[k] = a.v0 ? a.v0[3 * (size_t) ion + k] : 0.0f;
 	}
-	back.tet = find_tet(tr.g, back.p, a.assoc[node]);
+	back.tet = find_tet(tr.g, back.p, a.assoc[node], rng);
 	int status;
 	int len = 0;
 	if (a.summary)
@@ -505,7 +538,7 @@ __global__ void k_trajectory(TrajArgs a)
 	}
 	put(a, ion, len++, back);
 	float f[3];
-	fast_field(tr.g, tr.nfield, back.p, back.tet, f);
+	fast_field(tr.g, tr.nfield, back.p, back.tet, f, rng);
 	if (f[0] == 0.0f && f[1] == 0.0f && f[2] == 0.0f)
 		status = 1;  // NO_FIELD
 	else {
@@ -542,20 +575,46 @@ __global__ void k_trajectory(TrajArgs a)
 		for (int k = 0; k < 3; k++) item.p[k] = back.p[k];
 		for (int k = 0; k < 3; k++) item.p[k] += dt * back.v[k];
 		for (int k = 0; k < 3; k++) item.v[k] = back.v[k];
-		item.tet = find_tet(tr.g, item.p, back.tet);
+		item.tet = find_tet(tr.g, item.p, back.tet, rng);
 		put(a, ion, len++, item);
 	}
 	a.len[ion] = len;
 	a.status[ion] = status;
 }
 
+__global__ void k_trajectory(TrajArgs a)
+{
+	const int ion = blockIdx.x * blockDim.x + threadIdx.x;
+	if (ion >= a.count) return;
+	if (a.rng) {  // one ion (checked on the host): the stream stays in order
+		__shared__ int table[31];
+		DevRand rng;
+		rand_stage(&rng, table, a.rng);
+		run_ion(a, ion, &rng);
+		rand_unstage(rng, a.rng);
+	} else {
+		run_ion(a, ion, nullptr);
+	}
+}
+
 template <typename R>
-__global__ void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out)
+__global__ void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out,
+                                  int* rng_state, int walks)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= count) return;
 	const float* p = pos + 3 * (size_t) i;
-	const int t = find_tet(g, p, guess[i]);
+	int t;
+	if (rng_state) {  // one point (checked on the host)
+		__shared__ int table[31];
+		DevRand rng;
+		rand_stage(&rng, table, rng_state);
+		t = find_tet(g, p, guess[i], &rng);
+		for (int w = 1; w < walks; w++) t = find_tet(g, p, guess[i], &rng);
+		rand_unstage(rng, rng_state);
+	} else {
+		t = find_tet(g, p, guess[i]);
+	}
 	if (t < 0) {
 		out[i] = 0.0f;
 		return;
@@ -587,14 +646,16 @@ __global__ void k_field_snapshot(int n, const float* xyz, const int* ptr, const 
 int launch_trajectories(hipStream_t s, const TrajArgs& a)
 {
 	if (a.count <= 0) return 0;
+	if (a.rng && a.count != 1) return (int) hipErrorInvalidValue;
 	hipLaunchKernelGGL(dev::k_trajectory, dim3((a.count + 63) / 64), dim3(64), 0, s, a);
 	return (int) hipGetLastError();
 }
 
 int launch_point_potential(hipStream_t s, int count, const float* pos, const int* guess, const float* xyz, const int* tv,
-                           const int* tn, int m, const void* phi, bool f64, float* out)
+                           const int* tn, int m, const void* phi, bool f64, float* out, int* rng, int walks)
 {
 	if (count <= 0) return 0;
+	if (rng && (count != 1 || walks < 1)) return (int) hipErrorInvalidValue;
 	dev::Mesh3 g;
 	g.xyz = xyz;
 	g.tv = tv;
@@ -602,10 +663,10 @@ int launch_point_potential(hipStream_t s, int count, const float* pos, const int
 	g.m = m;
 	if (f64)
 		hipLaunchKernelGGL(dev::k_point_potential<double>, dim3((count + 63) / 64), dim3(64), 0, s, count, pos, guess, g,
-		                   (const double*) phi, out);
+		                   (const double*) phi, out, rng, walks);
 	else
 		hipLaunchKernelGGL(dev::k_point_potential<float>, dim3((count + 63) / 64), dim3(64), 0, s, count, pos, guess, g,
-		                   (const float*) phi, out);
+		                   (const float*) phi, out, rng, walks);
 	return (int) hipGetLastError();
 }
 

@@ -34,6 +34,10 @@ SPECIES = {
 }
 
 
+# temperature of the thermal start velocity fixture, K
+THERMAL_T = 60.0
+
+
 def aly_text(species, temperature=0.0):
     out = ["ASCII", "", "TEMPERATURE = %.5e" % temperature]
     for sid in sorted(species):
@@ -148,8 +152,21 @@ def make_small():
         fx["ref_evap_iter"] = rd(wd, "evap_iter.i64", np.int64).reshape(-1, 2)
         fx["ref_evap_f"] = rd(wd, "evap_f.f32", np.float32).reshape(-1, 22)
         fx["ref_evap_normal"] = rd(wd, "evap_normal.f32", np.float32).reshape(-1, 3)
+        fx["ref_evap_next_rand"] = rd(wd, "evap_next_rand.i32", np.int32)
         fx["ref_phi0_evap"] = rd(wd, "phi0_evap.f32", np.float32)
         fx["ref_phi1_evap"] = rd(wd, "phi1_evap.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES, temperature=THERMAL_T))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        # D2) the same with the Monte Carlo choice and thermal start velocities
+        #     at THERMAL_T: every draw of rand() (choice, velocity, the walks'
+        #     first faces) in the reference's order
+        run(wd, ["mesh", "sync", "relax:300:1",
+                 "evap:25:0:1:1e-2:3:5:5:1:5:1e-3:1:0:1:1:1"])
+        ints = rd(wd, "evap_int.i32", np.int32).reshape(-1, 4)
+        fx["ref_mc_node"], fx["ref_mc_trajlen"] = ints[:, 0], ints[:, 2]
+        fx["ref_mc_f"] = rd(wd, "evap_f.f32", np.float32).reshape(-1, 22)
+        fx["ref_mc_next_rand"] = rd(wd, "evap_next_rand.i32", np.int32)
     with tempfile.TemporaryDirectory() as wd:
         open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
         write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)

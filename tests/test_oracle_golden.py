@@ -1,11 +1,17 @@
 """The oracle's float restatement against the reference's own outputs
 (tests/golden/tip_small.npz, written by the reference built from its sources).
 Bit-exact unless stated."""
+import ctypes
+
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from tests.conftest import FIXTURE_SPECIES
+
+# the C library's rand(), whose state the oracle's draws advance
+libc = ctypes.CDLL(None)
+libc.rand.restype = ctypes.c_int
 
 F1E2 = float(np.float32(1e-2))
 F1E3 = float(np.float32(1e-3))
@@ -86,9 +92,8 @@ def evap_opts():
 
 def test_evaporation_sequence(tip_small):
     """25 events of Process::evaporation (steps I-VI): sequence, selection
-    data and trajectory end points bit-exact; the voltage and time scale,
-    whose exit-point location the reference decides with its float predicates,
-    within 1e-3 relative (the oracle's point location uses exact signs)."""
+    data, trajectory end points, voltage and time scale bit-exact, and the
+    rand() stream left where the reference leaves it."""
     o = make(tip_small)
     o.sync()
     for _ in range(300):
@@ -108,10 +113,38 @@ def test_evaporation_sequence(tip_small):
     assert np.array_equal([[e.local_iter, e.refresh_iter] for e in ev], tip_small["ref_evap_iter"])
     assert bits_equal([list(e.apex) for e in ev], F[:, 18:21])
     assert bits_equal([list(e.normal) for e in ev], tip_small["ref_evap_normal"])
-    v = np.array([e.voltage for e in ev])
-    np.testing.assert_allclose(v, F[:, 9], rtol=1e-3)
-    np.testing.assert_allclose([e.time_scale for e in ev], F[:, 10], rtol=1e-3)
+    assert bits_equal([e.voltage for e in ev], F[:, 9])
+    assert bits_equal([e.time_scale for e in ev], F[:, 10])
+    assert bits_equal([e.tof for e in ev], F[:, 17])
     assert bits_equal(o.phi(0), tip_small["ref_phi0_evap"]) and bits_equal(o.phi(1), tip_small["ref_phi1_evap"])
+    # the walks' first faces drew from rand() exactly as often as the reference's
+    assert libc.rand() == tip_small["ref_evap_next_rand"][0]
+
+
+def test_monte_carlo_thermal_evaporation(tip_small):
+    """evapMonteCarlo with thermal start velocities at 60 K: the choice, the
+    velocity and every walk step draw from rand() in the reference's order, so
+    the sequence, the flights and the stream position after 25 events are the
+    reference's."""
+    o = O.Oracle(O.F32, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                 O.species_table(FIXTURE_SPECIES), temperature=60.0)
+    o.sync()
+    for _ in range(300):
+        o.relax(1)
+    o.srand(99)
+    opts = evap_opts()
+    opts.evap_mode = 1
+    opts.thermal_velocity = 1
+    ev = o.evaporate(opts, 25)
+    F = tip_small["ref_mc_f"]
+    assert np.array_equal([e.node for e in ev], tip_small["ref_mc_node"])
+    assert bits_equal([e.probability for e in ev], F[:, 0])
+    assert bits_equal([list(e.start) for e in ev], F[:, 11:14])
+    assert bits_equal([list(e.stop) for e in ev], F[:, 14:17])
+    assert np.array_equal([e.traj_len for e in ev], tip_small["ref_mc_trajlen"])
+    assert bits_equal([e.voltage for e in ev], F[:, 9])
+    assert bits_equal([e.time_scale for e in ev], F[:, 10])
+    assert libc.rand() == tip_small["ref_mc_next_rand"][0]
 
 
 def test_trajectory(tip_small):
@@ -128,13 +161,9 @@ def test_trajectory(tip_small):
     ref = tip_small["ref_traj"]
     assert st == tip_small["ref_traj_status"][0]
     assert tr.shape == ref.shape
-    # every integration step bit-exact; the final extrapolated point lies
-    # exactly on the top face of the box, which the reference's float
-    # orientation predicate places outside (tet -1) and the oracle's exact
-    # sign on the face (a hull tet) -- only that tet index may differ
-    assert bits_equal(tr[:-1], ref[:-1])
-    assert bits_equal(tr[-1, :7], ref[-1, :7])
-    assert ref[-1, 7] == -1 and tr[-1, 2] == ref[-1, 2]
+    # every phase bit-exact, the extrapolated one included (its walk starts
+    # from the last integrated point's tet, -1 outside the mesh)
+    assert bits_equal(tr, ref)
 
 
 def test_threaded_relax_equals_serial(tip_small):
