@@ -4,10 +4,12 @@ value), with the mixed relaxation alone and the plain FP64 path (the
 reference's algorithm) timed beside it, the sweep kernels' HBM roofline and
 the CPU oracle on the host cores.
 
-A step is one evaporation event: choose the atom (first maximum of the
-surface probabilities), remove it and re-couple its cell, relax to a
-Jacobi-scaled residual of 1e-8 V, and evaluate every surface field and
-probability.
+A step is one evaporation event, steps I-VI of the reference's
+Process::evaporation: choose the atom (first maximum of the surface
+probabilities), remove it and re-couple its cell, relax to a Jacobi-scaled
+residual of 1e-8 V, evaluate every surface field and probability, integrate
+the ion's flight (Cash-Karp, error-restricted steps) through the mesh and
+form the voltage and time scale from its exit point.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
 """
@@ -68,20 +70,23 @@ def event_options(tsg):
     e.relax_global_cycles = 1
     e.refresh_interval, e.refresh_threshold, e.refresh_cycle, e.refresh_queue = 1, 1e-8, 1, 0
     e.exact_probability = 1
+    e.trajectory, e.integrator, e.stepper = 1, 1, 1
     return e
 
 
 def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, sweeps_per_event, budget_s=20.0):
     """The FP64 oracle on the host cores, on a bounded sample of the same
     workload: relax(1) sweeps with the reference's threading (contiguous
-    chunks of N/T nodes per thread, cwrapper_threadedRelax) and one surface
-    evaluation, combined with the GPU's sweeps per event (the plain GPU path
-    and the oracle perform bit-identical iterations)."""
+    chunks of N/T nodes per thread, cwrapper_threadedRelax), one surface
+    evaluation and the flights of three ions from the top of the relaxed
+    tip, combined with the GPU's sweeps per event (the plain GPU path and the
+    oracle perform bit-identical iterations)."""
     from oracle import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     o = O.Oracle(O.F64, xyz, ids, tets, neigh, O.species_table(SPECIES))
     # couplings from the device precompute (setup, which the reference's timer also excludes)
     o.sync_from_couplings(grid.couplings())
+    phi_relaxed = (grid.phi(0), grid.phi(1)) + grid.switches()
     _, t_probe = o.relax_threaded(4, threads)
     k = max(4, int(0.75 * budget_s / max(t_probe / 4, 1e-6)))
     _, t_total = o.relax_threaded(k, threads)
@@ -91,11 +96,21 @@ def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, sweeps_per_event, budget_s=20
     o.specific_fields()
     o.compute_probabilities(0)
     t_surf = time.perf_counter() - t1
-    per_event = sweeps_per_event * t_sweep + t_surf
+    # flights in the relaxed field from the three highest surface atoms
+    o.set_phi(*phi_relaxed)
+    surf = o.surface_nodes()
+    top = surf[np.argsort(xyz[surf, 2])[-3:]]
+    q = np.float32(2.0) * np.float32(1.60217733e-19)
+    m = np.float32(55.845) * np.float32(1.660538782e-27)
+    t1 = time.perf_counter()
+    for node in top:
+        o.trajectory(int(node), q, m)
+    t_traj = (time.perf_counter() - t1) / len(top)
+    per_event = sweeps_per_event * t_sweep + t_surf + t_traj
     return dict(value=1.0 / per_event, unit="events/s", cores=threads, kind="port",
                 sample="%d FP64 Jacobi sweeps on %d threads (%.3f ms each) + one surface evaluation (%.3f s, one "
-                       "thread) of the same mesh, combined with %.0f sweeps/event of the plain path"
-                       % (k, threads, t_sweep * 1e3, t_surf, sweeps_per_event))
+                       "thread) + three ion flights (%.3f s each, one thread) on the same mesh, combined with %.0f "
+                       "sweeps/event of the plain path" % (k, threads, t_sweep * 1e3, t_surf, t_traj, sweeps_per_event))
 
 
 def run_events(grid, opts, steps, tag, quiet=False):
