@@ -269,6 +269,21 @@ int tsg_copy_probe(long long bytes, int reps, double* avg_ms);
 int tsg_trajectories(tsg_grid*, int count, const int* nodes, const float* charge, const float* mass, int integrator,
                      int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status);
 
+/* Fieldline_3d::init(start) + compute(distance) (tapsim/fieldline_3d.cpp:55-190)
+ * for `count` start points (3 floats each) in the field of the current
+ * potentials: steps of (start tet volume)^(1/3)/2 along the unit field,
+ * accepted when they land in a tet sharing a vertex with the current one,
+ * the step growing by 1.1 while it stays in the current tet and shrinking by
+ * 0.9 otherwise.  distance < 0: no distance limit.  The reference leaves its
+ * iteration and stepper limits uninitialised; here they are arguments.
+ * process_rand 1: the lines run in order and continue the process's rand()
+ * stream (tetGuess samples, walk faces) as the reference; 0: one device
+ * thread per line.  out: per line the first cap points (x, y, z, Ex, Ey, Ez,
+ * tet); status[i] the Fieldline_3d status (fieldline_3d.h:74; 5 = left the
+ * mesh, 4 = distance reached, 1 = no field), -10 for a start outside the mesh. */
+int tsg_fieldlines(tsg_grid*, int count, const float* start, float distance, long long iterations,
+                   long long stepper_limit, int process_rand, int cap, float* out, int* len, int* status);
+
 /* ------------------------------------------------------------------ learned warm start
  * No counterpart in the reference (TAPSim starts every relaxation from the
  * previous potentials).  A message-passing network over the 3-hop shell of

@@ -100,6 +100,7 @@ _SIGS = {
     "ora_find_tetrahedron": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "ora_locate": ([_P, _P], ctypes.c_int),
     "ora_set_root_mode": ([ctypes.c_int], None),
+    "ora_fieldline": ([_P, _P, ctypes.c_float, ctypes.c_ulong, ctypes.c_ulong, _P, ctypes.c_int, _P], ctypes.c_int),
     "ora_tet_weights": ([_P, ctypes.c_int, _P, _P], None),
     "ora_point_potential": ([_P, _P, ctypes.c_int], ctypes.c_float),
     "ora_point_field": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
@@ -352,6 +353,17 @@ class Oracle:
         if guess is None:
             return load().ora_locate(self._h, _ptr(p))
         return load().ora_find_tetrahedron(self._h, _ptr(p), guess)
+
+    def fieldline(self, start, distance=-1.0, iterations=5000, stepper_limit=1000, cap=100000):
+        """Fieldline_3d from `start`: (status, points (k, 7): x, y, z, Ex, Ey, Ez, tet)"""
+        p = np.ascontiguousarray(start, np.float32)
+        out = np.zeros((cap, 7), np.float32)
+        ln = ctypes.c_int()
+        st = load().ora_fieldline(self._h, _ptr(p), distance, iterations, stepper_limit, _ptr(out), cap,
+                                  ctypes.byref(ln))
+        if st < 0:
+            raise OracleError(load().ora_last_error().decode())
+        return st, out[:min(ln.value, cap)].copy()
 
     def trajectory(self, node, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6, cap=200000):
         out = np.zeros((cap, 8), np.float32)

@@ -50,6 +50,12 @@
 #include "process.h"
 #include "surface_3d.h"
 #include "trajectory_3d.h"
+// Fieldline_3d has no setters for its stepper and iteration limits and its
+// constructor leaves them uninitialised; the driver opens the class to set
+// them (the layout is unchanged).
+#define private public
+#include "fieldline_3d.h"
+#undef private
 
 namespace
 {
@@ -63,6 +69,18 @@ namespace
 		if (f == 0) throw std::runtime_error("cannot write " + path);
 		if (!v.empty()) std::fwrite(&v[0], sizeof(T), v.size(), f);
 		std::fclose(f);
+	}
+
+	std::vector<float> readFloats(const std::string& name)
+	{
+		const std::string path = g_out + "/" + name;
+		FILE* f = std::fopen(path.c_str(), "rb");
+		if (f == 0) throw std::runtime_error("cannot read " + path);
+		std::vector<float> v;
+		float x;
+		while (std::fread(&x, sizeof(float), 1, f) == 1) v.push_back(x);
+		std::fclose(f);
+		return v;
 	}
 
 	std::vector<std::string> split(const std::string& s, char c)
@@ -469,6 +487,40 @@ int main(int argc, char** argv)
 				std::vector<int> st(1, status);
 				writeArray("traj_" + tok.at(1) + ".f32", data);
 				writeArray("traj_status_" + tok.at(1) + ".i32", st);
+			}
+			else if (cmd == "fieldlines")
+			{
+				// fieldlines:TAG:DISTANCE:ITERATIONS:STEPPER with the start points
+				// (3 floats each) in TAG_start.f32 of the work directory; each line
+				// is init() + compute(distance) in turn (one rand() stream)
+				const std::string tag = tok.at(1);
+				const float distance = (float) std::atof(tok.at(2).c_str());
+				const unsigned long iters = std::strtoul(tok.at(3).c_str(), 0, 10);
+				const unsigned long steps = std::strtoul(tok.at(4).c_str(), 0, 10);
+				std::vector<float> start = readFloats(tag + "_start.f32");
+				std::vector<float> data;
+				std::vector<int> info;
+				for (size_t l = 0; l + 2 < start.size(); l += 3)
+				{
+					Fieldline_3d f(&system.geomTable, &system.gridTable);
+					f._stepperLimit = steps;
+					f._iterationLimit = iters;
+					f.init(MathVector3d<float>(start[l], start[l + 1], start[l + 2]));
+					const int status = f.compute(distance);
+					info.push_back(status);
+					info.push_back((int) f.data().size());
+					for (size_t k = 0; k < f.data().size(); k++)
+					{
+						const Fieldline_3d::phaseVector& p = f.data()[k];
+						for (int c = 0; c < 3; c++) data.push_back(p.position()[c]);
+						for (int c = 0; c < 3; c++) data.push_back(p.field()[c]);
+						data.push_back((float) p.tetIndex());
+					}
+				}
+				std::vector<int> nextRand(1, std::rand());
+				writeArray("fieldlines_" + tag + ".f32", data);
+				writeArray("fieldlines_" + tag + ".i32", info);
+				writeArray("fieldlines_next_rand_" + tag + ".i32", nextRand);
 			}
 			else
 				throw std::runtime_error("unknown step " + cmd);

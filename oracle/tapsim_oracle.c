@@ -1306,6 +1306,93 @@ static float flen3(const double e[3])
 	return sqrtf(v);
 }
 
+/* ------------------------------------------------------------ field lines */
+
+/* the field line's first step: powf(volume, 1/3) as the reference (mode 0)
+   or correctly rounded (mode 1, the device's) */
+static float cube_root(float x)
+{
+	return g_root_mode ? (float) pow((double) x, (double) (1.0f / 3.0f)) : powf(x, 1.0f / 3.0f);
+}
+
+/* Grid_3d::FastField::compute(pos) without a guess (grid_3d.cpp:1663-1667) */
+static void field_unguessed(const ora_sys* s, const float p[3], float out[3])
+{
+	const int g = ora_locate(s, p);
+	fast_field(s, p, g, out);
+}
+
+/* Fieldline_3d::init + compute(distance) (fieldline_3d.cpp:55-130) with the
+   stepper (:155-190); out holds 7 floats per point (position, field, tet).
+   Returns the Fieldline_3d status, or ORA_E_GEOM when the start lies
+   outside the mesh (init() throws there). */
+int ora_fieldline(const ora_sys* s, const float start[3], float distance, unsigned long iterations,
+                  unsigned long stepper_limit, float* out, int cap, int* len)
+{
+	enum { NO_FIELD = 1, STEPPER_LIMIT = 2, ITERATION_LIMIT = 3, GEOMETRY_LIMIT = 4, SYSTEM_LIMIT = 5 };
+	typedef struct { float p[3], f[3]; int tet; } pt;
+	int n = 0, ncap = 64;
+	pt* d = (pt*) malloc(sizeof(pt) * (size_t) ncap);
+	pt p0;
+	for (int c = 0; c < 3; c++) p0.p[c] = start[c];
+	field_unguessed(s, p0.p, p0.f);
+	p0.tet = ora_locate(s, p0.p);
+	if (p0.tet < 0) { free(d); *len = 0; return fail(ORA_E_GEOM, "Fieldline_3d::init()"); }
+	d[n++] = p0;
+	float delta = tet_volume(s, p0.tet);
+	delta = cube_root(delta);
+	delta /= 2.0f;
+	int status = ITERATION_LIMIT;
+	int* allowed = (int*) malloc(sizeof(int) * 4096);
+	unsigned long it = iterations;
+	while (it--)
+	{
+		const pt back = d[n - 1];
+		if (back.f[0] == 0.0f && back.f[1] == 0.0f && back.f[2] == 0.0f) { status = NO_FIELD; break; }
+		const int na = adjacent_tets(s, back.tet, allowed, 4096);
+		float len2 = 0.0f;
+		for (int c = 0; c < 3; c++) len2 += back.f[c] * back.f[c];
+		const float flen = sqrtf(len2);
+		float dir[3];
+		for (int c = 0; c < 3; c++) dir[c] = back.f[c] / flen;
+		int found = 0;
+		pt v;
+		unsigned long st = stepper_limit;
+		while (st--)
+		{
+			for (int c = 0; c < 3; c++) v.p[c] = back.p[c] + delta * dir[c];
+			v.tet = ora_find_tetrahedron(s, v.p, back.tet);
+			if ((v.tet < 0 && is_boundary_tet(s, back.tet)) || (v.tet >= 0 && in_list(allowed, na, v.tet)))
+			{
+				field_unguessed(s, v.p, v.f);
+				found = 1;
+				break;
+			}
+			if (v.tet == back.tet) delta *= 1.1f;
+			else delta *= 0.9f;
+		}
+		if (!found) { status = STEPPER_LIMIT; break; }
+		if (v.tet < 0) { status = SYSTEM_LIMIT; break; }
+		if (distance >= 0.0f)
+		{
+			float q2 = 0.0f;
+			for (int c = 0; c < 3; c++) { const float e = d[0].p[c] - v.p[c]; q2 += e * e; }
+			if (sqrtf(q2) > distance) { status = GEOMETRY_LIMIT; break; }
+		}
+		if (n == ncap) { ncap *= 2; d = (pt*) realloc(d, sizeof(pt) * (size_t) ncap); }
+		d[n++] = v;
+	}
+	for (int k = 0; k < n && k < cap; k++)
+	{
+		for (int c = 0; c < 3; c++) { out[7 * k + c] = d[k].p[c]; out[7 * k + 3 + c] = d[k].f[c]; }
+		out[7 * k + 6] = (float) d[k].tet;
+	}
+	*len = n;
+	free(allowed);
+	free(d);
+	return status;
+}
+
 /* ---- surface normal: Surface_3d::Table::normal (surface_3d.cpp:123-169),
    Geometry_3d::find_best_plane (geometry_3d.cpp:1806-1870), and utils.h's
    Numerical Recipes householder (tred2, utils.h:245-330) and tri_eigen_qli

@@ -146,6 +146,10 @@ int ora_locate(const ora_sys*, const float pos[3]);                       /* tet
 /* the trajectories' (tol/err)^(1/4): 0 glibc's powf as the reference, 1 the
    correctly rounded root (the device's arithmetic) */
 void ora_set_root_mode(int mode);
+/* Fieldline_3d::init(start) + compute(distance) with the given stepper and
+   iteration limits; out: 7 floats per point (x, y, z, Ex, Ey, Ez, tet) */
+int ora_fieldline(const ora_sys*, const float start[3], float distance, unsigned long iterations,
+                  unsigned long stepper_limit, float* out, int cap, int* len);
 void ora_tet_weights(const ora_sys*, int tet, const float pos[3], float w[4]);
 float ora_point_potential(const ora_sys*, const float pos[3], int guess);
 int ora_point_field(const ora_sys*, const float pos[3], int guess, float out[3]);

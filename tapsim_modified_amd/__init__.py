@@ -145,6 +145,8 @@ _SIGS = {
     "tsg_surface_normal": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_rand": ([ctypes.c_int, _P], ctypes.c_int),
+    "tsg_fieldlines": ([_P, ctypes.c_int, _P, ctypes.c_float, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
+                        ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "tsg_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_sweep_stats": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "tsg_accel_setup": ([_P, ctypes.c_double, _P], ctypes.c_int),
@@ -475,6 +477,18 @@ class Grid3D:
         st = np.zeros(k, np.int32)
         _chk(load().tsg_trajectories(self._h, k, _ptr(nodes), _ptr(charge), _ptr(mass), integrator, stepper, dt0, dt_min,
                                      err, cap, _ptr(out), _ptr(ln), _ptr(st)))
+        return [(int(st[i]), out[i, :min(int(ln[i]), cap)].copy()) for i in range(k)]
+
+    def fieldlines(self, starts, distance=-1.0, iterations=5000, stepper_limit=1000, process_rand=True, cap=4096):
+        """Field lines (tsg_fieldlines) from (k, 3) start points: a list of
+        (status, points[len, 7]) with points (x, y, z, Ex, Ey, Ez, tet)."""
+        starts = np.ascontiguousarray(np.asarray(starts, np.float32).reshape(-1, 3))
+        k = len(starts)
+        out = np.zeros((k, cap, 7), np.float32)
+        ln = np.zeros(k, np.int32)
+        st = np.zeros(k, np.int32)
+        _chk(load().tsg_fieldlines(self._h, k, _ptr(starts), distance, iterations, stepper_limit, int(process_rand), cap,
+                                   _ptr(out), _ptr(ln), _ptr(st)))
         return [(int(st[i]), out[i, :min(int(ln[i]), cap)].copy()) for i in range(k)]
 
     def reset_potentials(self, kind=0, center=None, pmin=0.0, pmax=0.0):

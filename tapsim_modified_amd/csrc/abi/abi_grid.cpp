@@ -164,6 +164,14 @@ int tsg_surface_nodes(const tsg_grid* h, int* out)
 	return h->g->surface_nodes(out);
 }
 
+int tsg_fieldlines(tsg_grid* h, int count, const float* start, float distance, long long iterations,
+                   long long stepper_limit, int process_rand, int cap, float* out, int* len, int* status)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->fieldlines(count, start, distance, iterations, stepper_limit, process_rand != 0, cap, out,
+	                                 len, status));
+}
+
 int tsg_surface_normal(const tsg_grid* h, int node, float out[3])
 {
 	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");

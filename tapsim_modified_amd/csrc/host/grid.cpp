@@ -1871,6 +1871,61 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	return 0;
 }
 
+int Grid::fieldlines(int count, const float* start, float distance, long long iterations, long long stepper_limit,
+                     bool process_rand, int cap, float* out, int* len, int* status)
+{
+	if (count <= 0) return 0;
+	if (!start || !len || !status || cap < 0 || (cap > 0 && !out) || iterations < 0 || stepper_limit < 0)
+		return fail(-1, "fieldlines: bad arguments");
+	const int n = mesh_.n, m = mesh_.m();
+	if (int rc = ensure_point_location()) return rc;
+	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	if (launch_field_snapshot(stream_, n, d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_coupfull_.ptr(),
+	                          d_flags_.as<unsigned char>(), phi, f64_, d_nfield_.as<float>(), d_status_.as<int>()))
+		return fail(-2, "field snapshot launch failed");
+	DevBuf din, dout;
+	const size_t in_bytes = sizeof(float) * 3 * (size_t) count + sizeof(int) * 2 * (size_t) count;
+	const size_t out_bytes = sizeof(float) * 7 * (size_t) count * (size_t) std::max(cap, 1);
+	din.alloc(in_bytes);
+	if (cap > 0) dout.alloc(out_bytes);
+	if (!din.ptr() || (cap > 0 && !dout.ptr())) return fail(-2, "device allocation failed (field lines)");
+	HIPCHK(hipMemcpyAsync(din.ptr(), start, sizeof(float) * 3 * count, hipMemcpyHostToDevice, stream_));
+	int* res = (int*) (din.as<char>() + sizeof(float) * 3 * (size_t) count);
+	RandState& prs = tsg::process_rand();
+	if (process_rand) HIPCHK(hipMemcpyAsync(d_rng_.ptr(), &prs, sizeof(RandState), hipMemcpyHostToDevice, stream_));
+	FieldlineArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.count = count;
+	a.start = din.as<float>();
+	a.distance = distance;
+	a.iterations = iterations;
+	a.stepper_limit = stepper_limit;
+	a.max_calls = 1ll << 22;
+	a.guess_samples = std::max(1, (int) std::pow((double) m, 0.25));
+	a.m = m;
+	a.xyz = d_xyz_.as<float>();
+	a.tv = d_tv_.as<int>();
+	a.tn = d_tn_.as<int>();
+	a.gbnd = d_gbnd_.as<unsigned char>();
+	a.nfield = d_nfield_.as<float>();
+	a.cap = cap;
+	a.out = cap > 0 ? dout.as<float>() : nullptr;
+	a.len = res;
+	a.status = res + count;
+	a.rng = process_rand ? d_rng_.as<int>() : nullptr;
+	if (launch_fieldlines(stream_, a)) return fail(-2, "field line launch failed");
+	int st = 0;
+	if (cap > 0) HIPCHK(hipMemcpyAsync(out, dout.ptr(), out_bytes, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(len, res, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(status, res + count, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+	if (process_rand) HIPCHK(hipMemcpyAsync(&prs, d_rng_.ptr(), sizeof(RandState), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	return 0;
+}
+
 // the mesh arrays of the point location, kept on the device from the first use on
 int Grid::ensure_point_location()
 {

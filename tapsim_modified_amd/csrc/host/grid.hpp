@@ -170,6 +170,11 @@ public:
 	// process_rand the point is located `walks` times drawing from the
 	// process's rand() stream, as that many calls of the reference would
 	int point_potential(const float pos[3], int guess, float* out, bool process_rand = false, int walks = 1);
+	// Fieldline_3d::init + compute(distance) (fieldline_3d.cpp:55-190) for
+	// `count` start points in the current potentials; out holds cap points of
+	// (x, y, z, Ex, Ey, Ez, tet) per line
+	int fieldlines(int count, const float* start, float distance, long long iterations, long long stepper_limit,
+	               bool process_rand, int cap, float* out, int* len, int* status);
 	// evap_initialVelocity (surface_3d.cpp:472-502) from the process's rand()
 	void thermal_velocity(const Species& sp, float v[3]) const;
 

@@ -174,6 +174,32 @@ int launch_point_potential(hipStream_t, int count, const float* pos, const int* 
                            const int* tn, int m, const void* phi, bool f64, float* out, int* rng = nullptr,
                            int walks = 1);
 int launch_trajectories(hipStream_t, const TrajArgs&);
+
+// Fieldline_3d::init + compute(distance) (fieldline_3d.cpp:55-190) of
+// `count` field lines through the node-field snapshot.  With rng the lines
+// run one after another in one thread, continuing the process's rand()
+// stream (tetGuess samples and walk faces) in the reference's order; without
+// it one thread per line, tetGuess sampling evenly spaced tets and the walks
+// starting at face 0 (the located tet differs only on a face).
+struct FieldlineArgs {
+	int count;
+	const float* start;       // 3 floats per line
+	float distance;           // < 0: no distance limit
+	long long iterations, stepper_limit, max_calls;
+	int guess_samples;        // tetGuess's sample count, (int) pow(m, 0.25)
+	int m;
+	const float* xyz;
+	const int* tv;
+	const int* tn;
+	const unsigned char* gbnd;
+	const float* nfield;      // field_o1 snapshot, 3 floats per node
+	int cap;
+	float* out;               // per line: cap points of (x, y, z, Ex, Ey, Ez, tet)
+	int* len;                 // points of each line (the output holds the first cap)
+	int* status;              // Fieldline_3d status; -10 start outside the mesh, -4 work limit
+	int* rng;
+};
+int launch_fieldlines(hipStream_t, const FieldlineArgs&);
 int launch_field_snapshot(hipStream_t, int n, const float* xyz, const int* ptr, const int* idx, const void* coup,
                           const unsigned char* flags, const void* phi, bool f64, float* out, int* status);
 

@@ -597,6 +597,152 @@ __global__ void k_trajectory(TrajArgs a)
 	}
 }
 
+// ---------------------------------------------------------------- field lines
+
+// Geometry_3d::Table::tetGuess (geometry_3d.cpp:1712-1748): the sampled tet
+// with the vertex nearest to pos (samples from the stream, or evenly spaced)
+__device__ int tet_guess(const Mesh3& g, const float* pos, int cnt, DevRand* rng)
+{
+	int ref = -1;
+	float refd = 3.402823466e+38f;
+	for (int k = 0; k < cnt; k++) {
+		const int t = rng ? rng->next() % g.m : (int) (((long long) k * g.m) / cnt);
+		for (int v = 0; v < 4; v++) {
+			const float* c = g.P(g.tv[4 * (size_t) t + v]);
+			float d = 0.0f;
+			for (int i = 0; i < 3; i++) {
+				float tmp = c[i] - pos[i];
+				tmp *= tmp;
+				d += tmp;
+			}
+			if (d < refd) {
+				refd = d;
+				ref = t;
+			}
+		}
+	}
+	return ref;
+}
+
+// FastField::compute(pos) without a guess (grid_3d.cpp:1663-1667): locate
+// from tetGuess, then the guessed evaluation (a second walk)
+__device__ void field_unguessed(const Mesh3& g, const float* nf, const float* p, int cnt, DevRand* rng, float* out)
+{
+	const int t = find_tet(g, p, tet_guess(g, p, cnt, rng), rng);
+	fast_field(g, nf, p, t, out, rng);
+}
+
+__device__ void put_point(const FieldlineArgs& a, int line, int k, const float* p, const float* f, int tet)
+{
+	if (k >= a.cap) return;
+	float* o = a.out + ((size_t) line * a.cap + k) * 7;
+	for (int c = 0; c < 3; c++) {
+		o[c] = p[c];
+		o[3 + c] = f[c];
+	}
+	o[6] = (float) tet;
+}
+
+__device__ void run_fieldline(const FieldlineArgs& a, int line, DevRand* rng)
+{
+	Mesh3 g;
+	g.xyz = a.xyz;
+	g.tv = a.tv;
+	g.tn = a.tn;
+	g.m = a.m;
+	float p0[3], bp[3], bf[3];
+	for (int c = 0; c < 3; c++) p0[c] = a.start[3 * (size_t) line + c];
+	field_unguessed(g, a.nfield, p0, a.guess_samples, rng, bf);
+	int bt = find_tet(g, p0, tet_guess(g, p0, a.guess_samples, rng), rng);
+	if (bt < 0) {
+		a.len[line] = 0;
+		a.status[line] = -10;  // init() throws: the start is outside the mesh
+		return;
+	}
+	for (int c = 0; c < 3; c++) bp[c] = p0[c];
+	int n = 0;
+	put_point(a, line, n++, bp, bf, bt);
+	// the first step: half the edge of a cube of the start tet's volume
+	float delta = tet_volume(g, bt);
+	delta = (float) pow((double) delta, (double) (1.0f / 3.0f));
+	delta /= 2.0f;
+	int status = 3;  // ITERATION_LIMIT
+	long long calls = 0;
+	for (long long it = 0; it < a.iterations; it++) {
+		if (bf[0] == 0.0f && bf[1] == 0.0f && bf[2] == 0.0f) {
+			status = 1;  // NO_FIELD
+			break;
+		}
+		float len2 = 0.0f;
+		for (int c = 0; c < 3; c++) len2 += bf[c] * bf[c];
+		const float flen = sqrtf(len2);
+		float dir[3];
+		for (int c = 0; c < 3; c++) dir[c] = bf[c] / flen;
+		bool found = false;
+		float vp[3], vf[3];
+		int vt = -1;
+		for (long long st = 0; st < a.stepper_limit; st++) {
+			if (++calls > a.max_calls) break;
+			for (int c = 0; c < 3; c++) vp[c] = bp[c] + delta * dir[c];
+			vt = find_tet(g, vp, bt, rng);
+			// adjacentTetrahedra(back) excludes the back tet itself
+			if ((vt < 0 && boundary_tet(g, a.gbnd, bt)) || adjacent(g, bt, vt)) {
+				field_unguessed(g, a.nfield, vp, a.guess_samples, rng, vf);
+				found = true;
+				break;
+			}
+			if (vt == bt) delta *= 1.1f;
+			else delta *= 0.9f;
+		}
+		if (calls > a.max_calls) {
+			status = -4;  // work limit
+			break;
+		}
+		if (!found) {
+			status = 2;  // STEPPER_LIMIT
+			break;
+		}
+		if (vt < 0) {
+			status = 5;  // SYSTEM_LIMIT: the line left the mesh
+			break;
+		}
+		if (a.distance >= 0.0f) {
+			float q2 = 0.0f;
+			for (int c = 0; c < 3; c++) {
+				const float e = p0[c] - vp[c];
+				q2 += e * e;
+			}
+			if (sqrtf(q2) > a.distance) {
+				status = 4;  // GEOMETRY_LIMIT
+				break;
+			}
+		}
+		for (int c = 0; c < 3; c++) {
+			bp[c] = vp[c];
+			bf[c] = vf[c];
+		}
+		bt = vt;
+		put_point(a, line, n++, bp, bf, bt);
+	}
+	a.len[line] = n;
+	a.status[line] = status;
+}
+
+__global__ void k_fieldlines(FieldlineArgs a)
+{
+	const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (a.rng) {  // the lines in order on one stream
+		if (tid != 0) return;
+		__shared__ int table[31];
+		DevRand rng;
+		rand_stage(&rng, table, a.rng);
+		for (int line = 0; line < a.count; line++) run_fieldline(a, line, &rng);
+		rand_unstage(rng, a.rng);
+	} else if (tid < a.count) {
+		run_fieldline(a, tid, nullptr);
+	}
+}
+
 template <typename R>
 __global__ void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out,
                                   int* rng_state, int walks)
@@ -648,6 +794,15 @@ int launch_trajectories(hipStream_t s, const TrajArgs& a)
 	if (a.count <= 0) return 0;
 	if (a.rng && a.count != 1) return (int) hipErrorInvalidValue;
 	hipLaunchKernelGGL(dev::k_trajectory, dim3((a.count + 63) / 64), dim3(64), 0, s, a);
+	return (int) hipGetLastError();
+}
+
+int launch_fieldlines(hipStream_t s, const FieldlineArgs& a)
+{
+	if (a.count <= 0) return 0;
+	if (a.m <= 0 || a.guess_samples < 1 || a.cap < 0) return (int) hipErrorInvalidValue;
+	const int blocks = a.rng ? 1 : (a.count + 63) / 64;
+	hipLaunchKernelGGL(dev::k_fieldlines, dim3(blocks), dim3(64), 0, s, a);
 	return (int) hipGetLastError();
 }
 

@@ -174,6 +174,27 @@ def make_small():
         run(wd, ["mesh", "sync", "relax:300:1", "evapnode:%d" % node, "relax:1:1", "traj:t:%d:1:1" % node])
         fx["ref_traj"] = rd(wd, "traj_t.f32", np.float32).reshape(-1, 8)
         fx["ref_traj_status"] = rd(wd, "traj_status_t.i32", np.int32)
+    # F) field lines (Fieldline_3d, debug.cpp's surface field lines) from the
+    #    centroids of the highest vacuum-side tets of the six highest atoms:
+    #    to the mesh's end, and limited to 2 nm
+    top = rem[np.argsort(xyz[rem, 2])[-6:]]
+    starts = []
+    for a in top:
+        inc = np.where((tv == a).any(1))[0]
+        cz = xyz[tv[inc]].astype(np.float32).mean(1)
+        starts.append(cz[np.argmax(cz[:, 2])])
+    starts = np.array(starts, np.float32)
+    fx["fieldline_start"] = starts
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        starts.tofile(os.path.join(wd, "a_start.f32"))
+        starts.tofile(os.path.join(wd, "b_start.f32"))
+        run(wd, ["mesh", "sync", "relax:300:1", "fieldlines:a:-1:5000:1000", "fieldlines:b:2e-9:5000:1000"])
+        for t in "ab":
+            fx["ref_fieldline_%s_info" % t] = rd(wd, "fieldlines_%s.i32" % t, np.int32).reshape(-1, 2)
+            fx["ref_fieldline_%s" % t] = rd(wd, "fieldlines_%s.f32" % t, np.float32).reshape(-1, 7)
+            fx["ref_fieldline_%s_next_rand" % t] = rd(wd, "fieldlines_next_rand_%s.i32" % t, np.int32)
     np.savez_compressed(os.path.join(OUT, "tip_small.npz"), **fx)
     print("tip_small: nodes %d tets %d counts %s" % (len(xyz), len(tv), counts))
 

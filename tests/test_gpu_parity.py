@@ -470,3 +470,48 @@ def test_initial_potentials_match_oracle(tip_small, mode):
     assert same_bits(g.phi(0), o.phi(0), dt)
     g.reset_potentials(0)
     assert same_bits(g.phi(0), np.array([FIXTURE_SPECIES[i].get("phi", 0.0) for i in tip_small["ids"]]), dt)
+
+
+def test_fieldlines_match_reference(tip_small):
+    """Field lines (Fieldline_3d) from six points above the tip, one rand()
+    stream: to the mesh's end, then limited to 2 nm.  Device = oracle with
+    the device's cube root (correctly rounded) bit for bit, the stream
+    position included; device = reference wherever the oracle's two root
+    modes agree; the parallel mode (one thread per line, no stream) gives
+    the same lines."""
+    log("field lines")
+    g = grid(tip_small, tsg.FAITHFUL_F32)
+    o = oracle(tip_small, O.F32)
+    o.sync()
+    for _ in range(300):
+        g.relax(1)
+        o.relax(1)
+    starts = tip_small["fieldline_start"]
+    tsg.srand(99)
+    dev = {t: g.fieldlines(starts, np.float32(d), 5000, 1000) for t, d in (("a", -1.0), ("b", 2e-9))}
+    dev_next = int(tsg.rand(1)[0])
+    ora = {}
+    for mode in (1, 0):
+        o.srand(99)
+        try:
+            O.set_root_mode(mode)
+            ora[mode] = {t: [o.fieldline(s, np.float32(d), 5000, 1000) for s in starts]
+                         for t, d in (("a", -1.0), ("b", 2e-9))}
+        finally:
+            O.set_root_mode(0)
+        if mode == 1:
+            ora_next = libc.rand()
+    assert dev_next == ora_next
+    for t in "ab":
+        info = tip_small["ref_fieldline_%s_info" % t]
+        ref = tip_small["ref_fieldline_%s" % t]
+        k = 0
+        for i, ((st, pts), (sto, pto), (st0, pt0)) in enumerate(zip(dev[t], ora[1][t], ora[0][t])):
+            assert st == sto and same_bits(pts, pto, np.float32), (t, i)
+            if st0 == sto and same_bits(pt0, pto, np.float32):
+                assert st == info[i, 0] and same_bits(pts, ref[k:k + info[i, 1]], np.float32), (t, i)
+            k += info[i, 1]
+        par = g.fieldlines(starts, np.float32(-1.0 if t == "a" else 2e-9), 5000, 1000, process_rand=False)
+        for (st, pts), (stp, ptp) in zip(dev[t], par):
+            assert st == stp and same_bits(pts, ptp, np.float32)
+    log("field line statuses %s / %s" % ([s for s, _ in dev["a"]], [s for s, _ in dev["b"]]))

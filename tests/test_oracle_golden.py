@@ -181,3 +181,26 @@ def test_threaded_relax_equals_serial(tip_small):
     assert d == dt and sec > 0
     assert a.switches() == b.switches()
     assert np.array_equal(a.phi(0), b.phi(0)) and np.array_equal(a.phi(1), b.phi(1))
+
+
+def test_fieldlines(tip_small):
+    """Fieldline_3d (init + compute) from six points above the tip, one after
+    the other on one rand() stream: to the mesh's end (a), then limited to
+    2 nm (b).  Every point, field and tet bit-exact, the statuses, and the
+    stream position after each set."""
+    o = make(tip_small)
+    o.sync()
+    for _ in range(300):
+        o.relax(1)
+    o.srand(99)
+    for tag, distance in (("a", -1.0), ("b", 2e-9)):
+        info = tip_small["ref_fieldline_%s_info" % tag]
+        ref = tip_small["ref_fieldline_%s" % tag]
+        k = 0
+        for i, start in enumerate(tip_small["fieldline_start"]):
+            st, pts = o.fieldline(start, np.float32(distance), 5000, 1000)
+            assert st == info[i, 0] and len(pts) == info[i, 1]
+            assert bits_equal(pts, ref[k:k + len(pts)])
+            k += len(pts)
+        # the reference driver draws once to record the position
+        assert libc.rand() == tip_small["ref_fieldline_%s_next_rand" % tag][0]
