@@ -133,6 +133,10 @@ int tsg_potential(tsg_grid*, int node, double* out);
 int tsg_field_o1(tsg_grid*, int count, const int* nodes, double* out);
 /* the host mirror of Node::coupling for every directed entry of the CSR */
 int tsg_get_couplings(const tsg_grid*, double* out);
+/* Node::charge of every node (computeCharge, grid_3d.cpp:1373-1390: Voronoi
+   cell volume closed at half the mean neighbour distance x charge density /
+   epsilon0), as the grid stores it */
+int tsg_get_charges(const tsg_grid*, double* out);
 int tsg_get_flags(const tsg_grid*, unsigned char* out);
 int tsg_get_ids(const tsg_grid*, short* out);
 /* per directed entry: Voronoi face area closed at |pi-pj|/2 (float and

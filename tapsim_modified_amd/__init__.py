@@ -130,6 +130,7 @@ _SIGS = {
     "tsg_potential": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_get_couplings": ([_P, _P], ctypes.c_int),
+    "tsg_get_charges": ([_P, _P], ctypes.c_int),
     "tsg_get_flags": ([_P, _P], ctypes.c_int),
     "tsg_get_ids": ([_P, _P], ctypes.c_int),
     "tsg_get_areas": ([_P, _P, _P, _P], ctypes.c_int),
@@ -371,6 +372,12 @@ class Grid3D:
         nodes = np.ascontiguousarray(np.atleast_1d(nodes), dtype=np.int32)
         out = np.zeros((len(nodes), 3), np.float64)
         _chk(load().tsg_field_o1(self._h, len(nodes), _ptr(nodes), _ptr(out)))
+        return out
+
+    def charges(self):
+        """Node::charge of every node (computeCharge)"""
+        out = np.zeros(self.n, np.float64)
+        _chk(load().tsg_get_charges(self._h, _ptr(out)))
         return out
 
     def couplings(self):

@@ -121,6 +121,12 @@ int tsg_get_couplings(const tsg_grid* h, double* out)
 	return h->g->get_couplings(out);
 }
 
+int tsg_get_charges(const tsg_grid* h, double* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	return h->g->get_charges(out);
+}
+
 int tsg_get_flags(const tsg_grid* h, unsigned char* out)
 {
 	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");

@@ -131,6 +131,20 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 		fail(-2, "geometry kernel launch failed");
 		return;
 	}
+	DevBuf d_v32, d_v64;
+	d_v32.alloc(sizeof(float) * (size_t) n);
+	d_v64.alloc(sizeof(double) * (size_t) n);
+	if (!d_v32.ptr() || !d_v64.ptr()) {
+		fail(-2, "device allocation failed (cell volumes)");
+		return;
+	}
+	if (launch_cell_volumes(stream_, n, d_xyz_.as<float>(), d_tv.as<int>(), d_tn.as<int>(), d_assoc.as<int>(),
+	                        d_ntp.as<int>(), d_nti.as<int>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_cc64.as<double>(),
+	                        d_cc32.as<float>(), d_d32.as<float>(), d_d64.as<double>(), d_v32.as<float>(),
+	                        d_v64.as<double>(), d_status_.as<int>())) {
+		fail(-2, "cell volume kernel launch failed");
+		return;
+	}
 	int gst = 0;
 	a_half32_.resize(nnz); a_half64_.resize(nnz); a_one32_.resize(nnz); dist32_.resize(nnz); dist64_.resize(nnz);
 	if (hip_check(hipStreamSynchronize(stream_), "geometry kernels") ||
@@ -140,6 +154,11 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 	    hip_check(hipMemcpy(a_one32_.data(), d_a132.ptr(), d_a132.bytes(), hipMemcpyDeviceToHost), "areas") ||
 	    hip_check(hipMemcpy(dist32_.data(), d_d32.ptr(), d_d32.bytes(), hipMemcpyDeviceToHost), "distances") ||
 	    hip_check(hipMemcpy(dist64_.data(), d_d64.ptr(), d_d64.bytes(), hipMemcpyDeviceToHost), "distances"))
+		return;
+	cell_vol32_.resize(n);
+	cell_vol64_.resize(n);
+	if (hip_check(hipMemcpy(cell_vol32_.data(), d_v32.ptr(), d_v32.bytes(), hipMemcpyDeviceToHost), "volumes") ||
+	    hip_check(hipMemcpy(cell_vol64_.data(), d_v64.ptr(), d_v64.bytes(), hipMemcpyDeviceToHost), "volumes"))
 		return;
 	if (gst & 1) { fail(-4, "degenerate tetrahedron (circumcentre)"); return; }
 	if (gst & 2) { fail(-4, "Voronoi face walk failed"); return; }
@@ -209,15 +228,35 @@ double Grid::compute_coupling(int n1, int n2, long long entry) const
 	return value;
 }
 
+// Grid_3d::Table::computeCharge (grid_3d.cpp:1373-1390): cell volume (closed
+// at half the mean neighbour distance) times the species' charge density
+// over epsilon0; stored as the grid's element type
+int Grid::get_charges(double* out) const
+{
+	std::copy(charge_.begin(), charge_.end(), out);
+	return 0;
+}
+
+double Grid::charge_of(int node) const
+{
+	const float rho = species_[ids_[node]].charge_density;
+	if (rho == 0.0f) return 0.0;  // the reference's product is exactly 0 for the finite volume
+	double value = f64_ ? cell_vol64_[node] : (double) cell_vol32_[node];
+	value *= (double) rho;
+	value /= 8.85418781762e-12;  // grid_3d.h:285
+	return f64_ ? value : (double) (float) value;
+}
+
 int Grid::sync()
 {
 	const int n = mesh_.n;
 	const long long nnz = (long long) mesh_.idx.size();
 	coup_.assign(nnz, std::numeric_limits<double>::quiet_NaN());
 	charge_.assign(n, 0.0);
-	for (int i = 0; i < n; i++)
-		if (species_[ids_[i]].charge_density != 0.0f)
-			return fail(-7, "non-zero charge density (Voronoi cell volumes) is not supported on the device path");
+	for (int i = 0; i < n; i++) {
+		charge_[i] = charge_of(i);
+		if (charge_[i] != charge_[i]) return fail(-3, "Grid_3d::Table::computeCharge(): NaN");
+	}
 	// couplings once per pair as computeCoupling(lower, higher), mirrored
 	for (int i = 0; i < n; i++)
 		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++) {
@@ -725,7 +764,6 @@ int Grid::set_id_and_resync(int node, int id)
 {
 	if (node < 0 || node >= mesh_.n || id < 0 || id > 127 || !species_[id].defined)
 		return fail(-1, "setId: bad node or species");
-	if (species_[id].charge_density != 0.0f) return fail(-7, "non-zero charge density is not supported on the device path");
 	ids_[node] = (int16_t) id;
 	const Species& sp = species_[id];
 	flags_[node] = (mesh_.boundary[node] ? 1 : 0) | (sp.dirichlet ? 2 : 0) | (sp.neumann ? 4 : 0);
@@ -742,7 +780,8 @@ int Grid::set_id_and_resync(int node, int id)
 		full_addr.push_back(k); full_val.push_back(c);
 		full_addr.push_back(mirror_[k]); full_val.push_back(c);
 	}
-	charge_[node] = 0.0;
+	charge_[node] = charge_of(node);
+	if (charge_[node] != charge_[node]) return fail(-3, "Grid_3d::Table::computeCharge(): NaN");
 
 	// rows touched: the node itself and every neighbour that owns a row
 	std::vector<long long> ai, ar, aw;

@@ -129,6 +129,7 @@ public:
 
 	int field_o1(int count, const int* nodes, double* out);
 	int get_couplings(double* out) const;
+	int get_charges(double* out) const;
 	int get_flags(unsigned char* out) const;
 	int get_ids(int16_t* out) const;
 	int get_areas(float* a_half32, double* a_half64, float* a_one32) const;
@@ -220,6 +221,9 @@ private:
 	std::vector<double> a_half64_, dist64_;
 	std::vector<double> coup_;             // host mirror (values exact in R)
 	std::vector<double> charge_;
+	std::vector<float> cell_vol32_;   // Voronoi cell volume closed at half the mean neighbour distance
+	std::vector<double> cell_vol64_;
+	double charge_of(int node) const;
 	int sw_ = 0, lsw_ = 0;
 
 	// rows (SELL-64)

@@ -230,7 +230,72 @@ __global__ void k_edge_areas(int n, long long nnz, const float* __restrict__ xyz
 	a_one32[k] = ring_area<float>(r, xyz, cc64, cc32, i, j, 1.0f);
 }
 
+// Geometry_3d::Table::voronoiVolume(i, inf) with computeCharge's closing
+// factor (grid_3d.cpp:1373-1390): inf = half the mean distance to the
+// neighbours; VoronoiCell::finiteVolume (geometry_3d.cpp:520-535) sums
+// finiteArea(inf) * |pi - pj| / 6 over the faces in ascending neighbour order.
+// One thread per node, float and double (the plain FP64 grid) variants.
+__global__ void k_cell_volumes(int n, const float* __restrict__ xyz, const int* __restrict__ tv,
+                               const int* __restrict__ tn, const int* __restrict__ assoc,
+                               const int* __restrict__ nt_ptr, const int* __restrict__ nt_idx,
+                               const int* __restrict__ ptr, const int* __restrict__ idx,
+                               const double* __restrict__ cc64, const float* __restrict__ cc32,
+                               const float* __restrict__ dist32, const double* __restrict__ dist64,
+                               float* __restrict__ vol32, double* __restrict__ vol64, int* __restrict__ status)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int k0 = ptr[i], k1 = ptr[i + 1];
+	if (k1 == k0) {
+		vol32[i] = 0.0f;
+		vol64[i] = 0.0;
+		return;
+	}
+	float m32 = 0.0f;
+	double m64 = 0.0;
+	for (int k = k0; k < k1; k++) {
+		m32 += dist32[k];
+		m64 += dist64[k];
+	}
+	m32 /= (float) (k1 - k0);
+	m32 /= 2.0f;
+	m64 /= (double) (k1 - k0);
+	m64 /= 2.0;
+	float v32 = 0.0f;
+	double v64 = 0.0;
+	for (int k = k0; k < k1; k++) {
+		Ring r;
+		if (!build_ring(tv, tn, assoc, nt_ptr, nt_idx, i, idx[k], r)) {
+			atomicOr(status, 2);
+			vol32[i] = 0.0f;
+			vol64[i] = 0.0;
+			return;
+		}
+		float t32 = ring_area<float>(r, xyz, cc64, cc32, i, idx[k], m32);
+		t32 *= dist32[k];
+		t32 /= 3.0f * 2.0f;
+		v32 += t32;
+		double t64 = ring_area<double>(r, xyz, cc64, cc32, i, idx[k], m64);
+		t64 *= dist64[k];
+		t64 /= 3.0 * 2.0;
+		v64 += t64;
+	}
+	vol32[i] = v32;
+	vol64[i] = v64;
+}
+
 }  // namespace dev
+
+int launch_cell_volumes(hipStream_t s, int n, const float* xyz, const int* tv, const int* tn, const int* assoc,
+                        const int* nt_ptr, const int* nt_idx, const int* ptr, const int* idx, const double* cc64,
+                        const float* cc32, const float* dist32, const double* dist64, float* vol32, double* vol64,
+                        int* status)
+{
+	if (n <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_cell_volumes, dim3((n + 127) / 128), dim3(128), 0, s, n, xyz, tv, tn, assoc, nt_ptr, nt_idx,
+	                   ptr, idx, cc64, cc32, dist32, dist64, vol32, vol64, status);
+	return (int) hipGetLastError();
+}
 
 int launch_circumcentres(hipStream_t s, int m, const float* xyz, const int* tv, double* cc64, float* cc32, int* status)
 {

@@ -11,6 +11,13 @@ namespace tsg {
 
 // ---------------------------------------------------------------- geometry
 int launch_circumcentres(hipStream_t, int m, const float* xyz, const int* tv, double* cc64, float* cc32, int* status);
+// per node: the Voronoi cell volume closed at half the mean neighbour
+// distance (computeCharge), float and double; needs dist32/dist64 of
+// launch_edge_areas
+int launch_cell_volumes(hipStream_t, int n, const float* xyz, const int* tv, const int* tn, const int* assoc,
+                        const int* nt_ptr, const int* nt_idx, const int* ptr, const int* idx, const double* cc64,
+                        const float* cc32, const float* dist32, const double* dist64, float* vol32, double* vol64,
+                        int* status);
 int launch_edge_areas(hipStream_t, int n, long long nnz, const float* xyz, const int* tv, const int* tn,
                       const int* assoc, const int* nt_ptr, const int* nt_idx, const int* ptr, const int* idx,
                       const int* row_of, const double* cc64, const float* cc32, float* a_half32, double* a_half64,

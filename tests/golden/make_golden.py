@@ -36,13 +36,15 @@ SPECIES = {
 
 # temperature of the thermal start velocity fixture, K
 THERMAL_T = 60.0
+# charge density of the vacuum in the space-charge fixture, C/m^3
+RHO_VACUUM = 2.0e6
 
 
 def aly_text(species, temperature=0.0):
     out = ["ASCII", "", "TEMPERATURE = %.5e" % temperature]
     for sid in sorted(species):
         s = species[sid]
-        out += ["", "ID = %d" % sid, "NAME = %s" % s["name"], "CHARGE_DENSITY = 0.00000e+00",
+        out += ["", "ID = %d" % sid, "NAME = %s" % s["name"], "CHARGE_DENSITY = %.8e" % s.get("rho", 0.0),
                 "DIELECTRICITY = %.8e" % s["eps"], "REMOVABLE = %d" % s["rem"], "NEUMANN_BOUNDARY = %d" % s["neu"],
                 "DIRICHLET_BOUNDARY = %d" % s["dir"], "POTENTIAL = %.8e" % s["phi"],
                 "MASS = %.8e" % s.get("mass", 0.0), "EVAPORATION_CHARGE_STATE = %d" % s.get("charge", 0),
@@ -174,6 +176,22 @@ def make_small():
         run(wd, ["mesh", "sync", "relax:300:1", "evapnode:%d" % node, "relax:1:1", "traj:t:%d:1:1" % node])
         fx["ref_traj"] = rd(wd, "traj_t.f32", np.float32).reshape(-1, 8)
         fx["ref_traj_status"] = rd(wd, "traj_status_t.i32", np.int32)
+    # G) space charge: the vacuum species with a charge density (computeCharge's
+    #    Voronoi cell volumes in fastSync and fastResync), then Jacobi cycles
+    sq = {k: dict(v) for k, v in SPECIES.items()}
+    sq[0]["rho"] = RHO_VACUUM
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(sq))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        run(wd, ["mesh", "sync", "dumpgrid", "relax:6:1", "dumpphi:q6"])
+        fx["ref_charge_q"] = rd(wd, "charge.f32", np.float32)
+        fx["ref_phi0_q6"] = rd(wd, "phi0_q6.f32", np.float32)
+        fx["ref_phi1_q6"] = rd(wd, "phi1_q6.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(sq))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        run(wd, ["mesh", "sync", "relax:40:1", "evapnode:%d" % node, "dumpgrid"])
+        fx["ref_charge_q_resync"] = rd(wd, "charge.f32", np.float32)
     # F) field lines (Fieldline_3d, debug.cpp's surface field lines) from the
     #    centroids of the highest vacuum-side tets of the six highest atoms:
     #    to the mesh's end, and limited to 2 nm

@@ -515,3 +515,36 @@ def test_fieldlines_match_reference(tip_small):
         for (st, pts), (stp, ptp) in zip(dev[t], par):
             assert st == stp and same_bits(pts, ptp, np.float32)
     log("field line statuses %s / %s" % ([s for s, _ in dev["a"]], [s for s, _ in dev["b"]]))
+
+
+@pytest.mark.parametrize("mode", [tsg.FAITHFUL_F32, tsg.PLAIN_F64])
+def test_space_charge(tip_small, mode):
+    """computeCharge with a charged vacuum (2e6 C/m^3): the device's cell
+    volumes give every charge after fastSync and after the fastResync of an
+    evaporated atom, and six Jacobi cycles with them, bit-exact against the
+    oracle (both precisions) and the reference (float)."""
+    log("space charge mode %d" % mode)
+    sp = {k: dict(v) for k, v in FIXTURE_SPECIES.items()}
+    sp[0]["charge_density"] = 2.0e6
+    mesh = tsg.Mesh(tip_small["xyz"], tip_small["tets"], tip_small["ref_neigh"])
+    g = tsg.Grid3D(mesh, tip_small["ids"], tsg.species_table(sp), 0.0, mode)
+    om = O.F32 if mode == tsg.FAITHFUL_F32 else O.F64
+    o = O.Oracle(om, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"], O.species_table(sp))
+    o.sync()
+    dt = np.float32 if mode == tsg.FAITHFUL_F32 else np.float64
+    assert same_bits(g.charges(), o.charges(), dt)
+    for _ in range(6):
+        g.relax(1)
+        o.relax(1)
+    assert same_bits(g.phi(0), o.phi(0), dt) and same_bits(g.phi(1), o.phi(1), dt)
+    node = int(tip_small["local_node"])
+    g.set_id(node, 0)
+    o.set_id(node, 0)
+    o.resync(node)
+    assert same_bits(g.charges(), o.charges(), dt) and g.charges()[node] != 0.0
+    if mode == tsg.FAITHFUL_F32:
+        g2 = tsg.Grid3D(mesh, tip_small["ids"], tsg.species_table(sp), 0.0, mode)
+        assert same_bits(g2.charges(), tip_small["ref_charge_q"], np.float32)
+        for _ in range(6):
+            g2.relax(1)
+        assert same_bits(g2.phi(0), tip_small["ref_phi0_q6"], np.float32)

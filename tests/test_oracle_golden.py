@@ -204,3 +204,32 @@ def test_fieldlines(tip_small):
             k += len(pts)
         # the reference driver draws once to record the position
         assert libc.rand() == tip_small["ref_fieldline_%s_next_rand" % tag][0]
+
+
+def space_charge_species():
+    sp = {k: dict(v) for k, v in FIXTURE_SPECIES.items()}
+    sp[0]["charge_density"] = 2.0e6   # C/m^3, the vacuum of the space-charge fixture
+    return sp
+
+
+def test_space_charge(tip_small):
+    """computeCharge with a charged vacuum: every node's charge after fastSync
+    and after fastResync of an evaporated atom, and six Jacobi cycles with
+    the charges, bit-exact."""
+    o = O.Oracle(O.F32, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                 O.species_table(space_charge_species()))
+    o.sync()
+    assert bits_equal(o.charges(), tip_small["ref_charge_q"])
+    for _ in range(6):
+        o.relax(1)
+    assert bits_equal(o.phi(0), tip_small["ref_phi0_q6"]) and bits_equal(o.phi(1), tip_small["ref_phi1_q6"])
+    o2 = O.Oracle(O.F32, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                  O.species_table(space_charge_species()))
+    o2.sync()
+    for _ in range(40):
+        o2.relax(1)
+    node = int(tip_small["local_node"])
+    o2.set_id(node, 0)
+    o2.resync(node)
+    assert bits_equal(o2.charges(), tip_small["ref_charge_q_resync"])
+    assert o2.charges()[node] != 0.0
