@@ -26,6 +26,7 @@ extern "C" {
 #define TSG_E_RELAX_FAULT (-5)  /* Grid_3d::Table::RelaxationFault */
 #define TSG_E_SURFACE (-6)
 #define TSG_E_STATE (-7)
+#define TSG_E_CONFIG (-8)       /* Configuration::Table::operator[]: an id without a species */
 
 /* precision / method of a grid (selected at creation) */
 #define TSG_PLAIN_F64 0         /* reference algorithm promoted to FP64 (drop-in, bit-exact to the FP64 oracle) */
@@ -170,6 +171,10 @@ int tsg_surface_probabilities(tsg_grid*, float* out);
 int tsg_surface_candidate(tsg_grid*, int* node, int* near_tie);
 /* evap_takeAway (surface_3d.cpp:507-564) */
 int tsg_take_away(tsg_grid*, int node);
+/* the fork's update_coordination (surface_3d.cpp:568-593): every id >= 10
+   grows by its number of neighbours with id >= 10, in node order, wrapping
+   as a signed char; only the ids change */
+int tsg_update_coordination(tsg_grid*);
 /* Surface_3d::Table::normal (surface_3d.cpp:123-169): best-plane normal of
    the surface sites within three Delaunay hops, oriented toward the vacuum
    nodes of that shell; zero for a node that is not a surface site */
@@ -198,6 +203,9 @@ typedef struct tsg_evap_options
 	float dt0, dt_min, err_limit;
 	int voltage_queue;        /* running mean length of the voltage (process.cpp:983-1010) */
 	float shrink_limit;       /* stop once the apex has receded this far, m */
+	int update_coordination;  /* 1: the fork's update_coordination (surface_3d.cpp:568-593) after the initial
+	                             surface evaluation and after every take-away (process.cpp:647,770); a later
+	                             lookup of an id without a species fails with TSG_E_CONFIG, as the reference throws */
 	int thermal_velocity;     /* 1: start velocity of evap_initialVelocity (surface_3d.cpp:472-502) from the
 	                             process's rand() and the grid's temperature; 0: at rest (the reference's
 	                             TRAJECTORY_NO_INITIAL_VELOCITY = 1) */

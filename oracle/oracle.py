@@ -34,7 +34,7 @@ class OraEvapOpts(ctypes.Structure):
         ("refresh_queue", ctypes.c_int), ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int),
         ("stepper", ctypes.c_int), ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_uint), ("shrink_limit", ctypes.c_float), ("inplace_order", ctypes.c_int),
-        ("thermal_velocity", ctypes.c_int),
+        ("thermal_velocity", ctypes.c_int), ("update_coordination", ctypes.c_int),
     ]
 
 
@@ -97,6 +97,7 @@ _SIGS = {
     "ora_surface_normal": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "ora_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "ora_srand": ([ctypes.c_uint], None),
+    "ora_update_coordination": ([_P], None),
     "ora_find_tetrahedron": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "ora_locate": ([_P, _P], ctypes.c_int),
     "ora_set_root_mode": ([ctypes.c_int], None),
@@ -342,6 +343,14 @@ class Oracle:
 
     def take_away(self, node):
         self._chk(load().ora_take_away(self._h, node))
+
+    def ids(self):
+        out = np.zeros(self.n, np.int16)
+        load().ora_get_ids(self._h, _ptr(out))
+        return out
+
+    def update_coordination(self):
+        load().ora_update_coordination(self._h)
 
     def srand(self, seed):
         load().ora_srand(seed)

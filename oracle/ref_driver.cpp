@@ -322,7 +322,7 @@ int main(int argc, char** argv)
 			}
 			else if (cmd == "evap")
 			{
-				// evap:EVENTS:PROB:CHOICE:LTHR:SHELL:LCYC:LQUE:GLOBAL:RINT:RTHR:RCYC:RQUE:INTEG:STEPPER[:THERMAL]
+				// evap:EVENTS:PROB:CHOICE:LTHR:SHELL:LCYC:LQUE:GLOBAL:RINT:RTHR:RCYC:RQUE:INTEG:STEPPER[:THERMAL[:COORD]]
 				// The steps I-VI of Process::evaporation (process.cpp:680-1037)
 				// composed of the reference's own functions.  The fork's
 				// update_coordination (process.cpp:647,770) is left out: it adds
@@ -345,12 +345,17 @@ int main(int argc, char** argv)
 				const int integ = std::atoi(tok.at(13).c_str());
 				const int stepper = std::atoi(tok.at(14).c_str());
 				const int thermal = tok.size() > 15 ? std::atoi(tok.at(15).c_str()) : 0;
+				// COORD 1: the fork's update_coordination where its Process calls it
+				// (process.cpp:647 after the initial surface evaluation, :770 after
+				// every take-away)
+				const int coord = tok.size() > 16 ? std::atoi(tok.at(16).c_str()) : 0;
 
 				Surface_3d::Table surf;
 				surf.setVacuumId(system.configTable["Vacuum"].id());
 				surf.init(system);
 				Surface_3d::evap_compute_specificFields(&surf, system);
 				Surface_3d::evap_compute_probabilities(probMode, &surf, system);
+				if (coord) update_coordination(&system);
 
 				std::vector<float> rec, recNormal;
 				std::vector<int> recNode;
@@ -370,6 +375,7 @@ int main(int argc, char** argv)
 					const Configuration::NodeId nodeId = system.gridTable.id(node);
 
 					evap_takeAway(&system, &surf, node);
+					if (coord) update_coordination(&system);
 
 					long long localIter = 0, refreshIter = 0;
 					try { localIter = system.gridTable.localRelax(node, shell, lthr, lcyc, lque); }
@@ -487,6 +493,12 @@ int main(int argc, char** argv)
 				std::vector<int> st(1, status);
 				writeArray("traj_" + tok.at(1) + ".f32", data);
 				writeArray("traj_status_" + tok.at(1) + ".i32", st);
+			}
+			else if (cmd == "coord")
+			{
+				// the fork's update_coordination (surface_3d.cpp:568-593); it
+				// prints every new id, which goes to the driver's stdout
+				update_coordination(&system);
 			}
 			else if (cmd == "fieldlines")
 			{

@@ -64,9 +64,15 @@ struct ora_sys
 	int* surf;
 	double* prob;
 	int nsurf, capsurf;
+	int coord_done;        /* the run-start update_coordination has been applied */
 	int vacuum_id;
 	double scaling_ref;
 };
+
+/* Configuration::Table::operator[] (configuration.cpp:213-231) throws for an
+   id without a species; ids can leave the table through the fork's
+   update_coordination */
+static int sp_bad(const ora_sys* s, int id) { return id < 0 || id > 127 || !s->species[id].defined; }
 
 static void ora_flags_from_config(ora_sys* s, int i)
 {
@@ -747,6 +753,7 @@ static int surface_update(ora_sys* s, int node)
 	{
 		const int j = s->idx[k];
 		if (s->ids[j] == s->vacuum_id) continue;
+		if (sp_bad(s, s->ids[j])) return fail(ORA_E_CONFIG, "Configuration::Table::operator[]()");
 		if (!s->species[s->ids[j]].removable) continue;
 		if (has_vacuum_face(s, j)) surf_insert(s, j);
 	}
@@ -823,6 +830,21 @@ int ora_take_away(ora_sys* s, int node)
 /* ------------------------------------------------------------ point location */
 
 void ora_srand(unsigned int seed) { srand(seed); }
+
+/* update_coordination, the fork's patch (surface_3d.cpp:568-593): in node
+   order, every id >= 10 grows by the number of neighbours whose id (already
+   updated for lower indices) is >= 10; ids are a signed char and wrap */
+void ora_update_coordination(ora_sys* s)
+{
+	for (int i = 0; i < s->n; i++)
+	{
+		if (s->ids[i] < 10) continue;
+		int count = 0;
+		for (int k = s->ptr[i]; k < s->ptr[i + 1]; k++)
+			if (s->ids[s->idx[k]] >= 10) count++;
+		s->ids[i] = (short) (signed char) (s->ids[i] + count);
+	}
+}
 
 static const float* nc(const ora_sys* s, int i) { return s->xyz + 3 * (size_t) i; }
 
@@ -1578,6 +1600,13 @@ int ora_surface_normal(const ora_sys* s, int node, float out[3])
 	return ORA_OK;
 }
 
+static int surface_species_bad(const ora_sys* s)
+{
+	for (int k = 0; k < s->nsurf; k++)
+		if (sp_bad(s, s->ids[s->surf[k]])) return 1;
+	return 0;
+}
+
 /* Process::evaporation, process.cpp:441-1124, steps I-VI (logging and dumps
    are not restated; update_coordination is off) */
 int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* out, int* done)
@@ -1585,6 +1614,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 	*done = 0;
 	int e;
 	if (s->vacuum_id != o->vacuum_id || s->nsurf == 0) ora_surface_init(s, o->vacuum_id);
+	if (surface_species_bad(s)) return fail(ORA_E_CONFIG, "Configuration::Table::operator[]()");
 	ora_specific_fields(s);
 	e = ora_compute_probabilities(s, o->prob_mode);
 	if (e) return e;
@@ -1596,6 +1626,9 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 	for (int k = 0; k < s->nsurf; k++)
 		if (s->xyz[3 * (size_t) s->surf[k] + 2] > apex0) apex0 = s->xyz[3 * (size_t) s->surf[k] + 2];
 	if (o->voltage_queue > 4096) return fail(ORA_E_ARG, "voltage queue too long");
+	/* the fork applies update_coordination once after the initial surface
+	   evaluation (process.cpp:645-647) and after every take-away (:770) */
+	if (o->update_coordination && !s->coord_done) { ora_update_coordination(s); s->coord_done = 1; }
 	for (int ev = 0; ev < nevents && s->nsurf > 0; ev++)
 	{
 		ora_event* r = &out[ev];
@@ -1623,6 +1656,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 		/* II) take away */
 		e = ora_take_away(s, node);
 		if (e) return e;
+		if (o->update_coordination) ora_update_coordination(s);
 		/* III) relaxation */
 		r->local_iter = ora_local_relax_threshold(s, node, o->relax_shell, (double) o->relax_threshold,
 		                                          o->relax_cycle, o->relax_queue, &e);
@@ -1636,6 +1670,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 			if (e) return e;
 		}
 		/* IV) fields and probabilities */
+		if (surface_species_bad(s)) return fail(ORA_E_CONFIG, "Configuration::Table::operator[]()");
 		ora_specific_fields(s);
 		e = ora_compute_probabilities(s, o->prob_mode);
 		if (e) return e;
@@ -1649,6 +1684,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 		int backt = -1;
 		if (o->trajectory)
 		{
+			if (sp_bad(s, nodeId)) return fail(ORA_E_CONFIG, "Configuration::Table::operator[]()");
 			const ora_species* sp = &s->species[nodeId];
 			float charge = (float) sp->evap_charge;
 			charge *= 1.60217733e-19f;

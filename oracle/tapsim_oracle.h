@@ -25,7 +25,7 @@ extern "C" {
 
 enum { ORA_F32 = 0, ORA_F64 = 1 };
 
-enum { ORA_OK = 0, ORA_E_ARG = -1, ORA_E_NAN = -2, ORA_E_GEOM = -3, ORA_E_SURF = -4 };
+enum { ORA_OK = 0, ORA_E_ARG = -1, ORA_E_NAN = -2, ORA_E_GEOM = -3, ORA_E_SURF = -4, ORA_E_CONFIG = -5 };
 
 typedef struct
 {
@@ -57,6 +57,7 @@ typedef struct
 	float shrink_limit;
 	int inplace_order;     /* 0 ascending sequential (one thread), 1 colour classes */
 	int thermal_velocity;  /* 1: evap_initialVelocity from rand(), 0: at rest */
+	int update_coordination; /* 1: the fork's update_coordination at the run's start and after each take-away */
 } ora_evap_opts;
 
 typedef struct
@@ -89,6 +90,7 @@ int ora_sync(ora_sys*);                                   /* fastSync */
 int ora_set_couplings(ora_sys*, const double* coupling);
 int ora_resync(ora_sys*, int node);                       /* fastResync */
 int ora_set_id(ora_sys*, int node, int id);
+void ora_update_coordination(ora_sys*);                  /* the fork's patch, surface_3d.cpp:568-593 */
 
 int ora_nnz(const ora_sys*);
 void ora_get_csr(const ora_sys*, int* ptr, int* idx);

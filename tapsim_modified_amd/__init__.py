@@ -75,7 +75,8 @@ class EvapOptions(ctypes.Structure):
         ("exact_probability", ctypes.c_int), ("method", ctypes.c_int), ("accel_tol", ctypes.c_double),
         ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int), ("stepper", ctypes.c_int),
         ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
-        ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float), ("thermal_velocity", ctypes.c_int),
+        ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float), ("update_coordination", ctypes.c_int),
+        ("thermal_velocity", ctypes.c_int),
     ]
 
     def __init__(self, **kw):
@@ -84,6 +85,7 @@ class EvapOptions(ctypes.Structure):
         self.dt0, self.dt_min, self.err_limit = 1e-14, 1e-16, 1e-6
         self.voltage_queue, self.shrink_limit = 250, 1.0
         self.thermal_velocity = 0
+        self.update_coordination = 0
         self.accel_tol = 1e-8
         for k, v in kw.items():
             setattr(self, k, v)
@@ -144,6 +146,7 @@ _SIGS = {
     "tsg_surface_candidate": ([_P, _P, _P], ctypes.c_int),
     "tsg_take_away": ([_P, ctypes.c_int], ctypes.c_int),
     "tsg_surface_normal": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_update_coordination": ([_P], ctypes.c_int),
     "tsg_best_plane": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_rand": ([ctypes.c_int, _P], ctypes.c_int),
     "tsg_fieldlines": ([_P, ctypes.c_int, _P, ctypes.c_float, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
@@ -441,6 +444,10 @@ class Grid3D:
 
     def take_away(self, node):
         _chk(load().tsg_take_away(self._h, node))
+
+    def update_coordination(self):
+        """the fork's update_coordination on the grid's ids"""
+        _chk(load().tsg_update_coordination(self._h))
 
     def surface_normal(self, node):
         """Surface_3d::Table::normal(node) (surface_3d.cpp:123-169)"""

@@ -178,6 +178,12 @@ int tsg_fieldlines(tsg_grid* h, int count, const float* start, float distance, l
 	                                 len, status));
 }
 
+int tsg_update_coordination(tsg_grid* h)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->update_coordination());
+}
+
 int tsg_surface_normal(const tsg_grid* h, int node, float out[3])
 {
 	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
@@ -250,6 +256,7 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.voltage_queue = o->voltage_queue;
 	e.shrink_limit = o->shrink_limit;
 	e.thermal_velocity = o->thermal_velocity;
+	e.update_coordination = o->update_coordination;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
 }
 

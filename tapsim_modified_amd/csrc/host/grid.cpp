@@ -773,6 +773,8 @@ int Grid::set_id_and_resync(int node, int id)
 	std::vector<double> full_val;
 	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) {
 		const int j = mesh_.idx[k];
+		if (!(!(flags_[node] & 4) && (flags_[j] & 4)) && !species_ok(j))
+			return fail(-8, "Configuration::Table::operator[]()");
 		const double c = compute_coupling(node, j, k);
 		if (c != c) return fail(-3, "Grid_3d::Table::computeCoupling(): NaN");
 		coup_[k] = c;
@@ -962,6 +964,9 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 	if (mode == 1 && !(temperature_ > 0.0f)) return fail(-1, "Surface_3d::Table::probBoltzmann(): temperature must be positive");
 	if (int rc = upload_surface()) return rc;
 	const int cnt = (int) surface_vec_.size();
+	if (coord_done_)  // ids may have left the species table (update_coordination)
+		for (int v : surface_vec_)
+			if (!species_ok(v)) return fail(-8, "Configuration::Table::operator[]()");
 	prob_mode_ = mode;
 	if (cnt == 0) return 0;
 	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
@@ -1114,6 +1119,39 @@ static bool has_vacuum_face(const Mesh& m, const std::vector<int16_t>& ids, cons
 	return false;
 }
 
+// the fork's update_coordination (surface_3d.cpp:568-593): in node order,
+// every id >= 10 grows by the number of neighbours whose id (already updated
+// for lower indices) is >= 10; the id is a signed char (configuration.h:62)
+// and wraps.  Only the ids change, as in the fork (no flags or couplings are
+// recomputed); the per-node species data of the device follow them.
+int Grid::update_coordination()
+{
+	const int n = mesh_.n;
+	bool changed = false;
+	for (int i = 0; i < n; i++) {
+		if (ids_[i] < 10) continue;
+		int count = 0;
+		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++)
+			if (ids_[mesh_.idx[k]] >= 10) count++;
+		if (count == 0) continue;
+		ids_[i] = (int16_t) (signed char) (ids_[i] + count);
+		changed = true;
+	}
+	if (!changed) return 0;
+	std::vector<float> spec(3 * (size_t) n);
+	for (int i = 0; i < n; i++) {
+		const bool ok = species_ok(i);
+		spec[i] = ok ? species_[ids_[i]].evap_field : 0.0f;
+		spec[n + (size_t) i] = ok ? species_[ids_[i]].evap_energy : 0.0f;
+		spec[2 * (size_t) n + i] = ok ? species_[ids_[i]].epsilon : 0.0f;
+	}
+	HIPCHK(hipMemcpy(d_evapF_.ptr(), spec.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_evapE_.ptr(), spec.data() + n, sizeof(float) * n, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d_eps_.ptr(), spec.data() + 2 * (size_t) n, sizeof(float) * n, hipMemcpyHostToDevice));
+	surface_dirty_ = true;
+	return 0;
+}
+
 int Grid::take_away(int node)
 {
 	if (vacuum_id_ < 0) return fail(-6, "surface not initialised");
@@ -1122,6 +1160,7 @@ int Grid::take_away(int node)
 		for (int k = mesh_.ptr[v]; k < mesh_.ptr[v + 1]; k++) {
 			const int j = mesh_.idx[k];
 			if (ids_[j] == vacuum_id_) continue;
+			if (!species_ok(j)) return fail(-8, "Configuration::Table::operator[]()");
 			if (!species_[ids_[j]].removable) continue;
 			if (has_vacuum_face(mesh_, ids_, a_one32_, vacuum_id_, j)) surface_.insert(j);
 		}
@@ -1196,6 +1235,12 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		apex0_ = a0[2];
 		apex0_set_ = true;
 	}
+	// the fork applies update_coordination once after the initial surface
+	// evaluation (process.cpp:645-647) and after every take-away (:770)
+	if (o.update_coordination && !coord_done_) {
+		if (int rc = update_coordination()) return rc;
+		coord_done_ = true;
+	}
 	for (int ev = 0; ev < k; ev++) {
 		if (surface_.empty()) break;
 		EventRecord& r = out[ev];
@@ -1231,6 +1276,8 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		if (int rc = potential(node, &r.pot_before)) return rc;
 		if (int rc = field_o1(1, &node, r.field_before)) return rc;
 		if (int rc = take_away(node)) return rc;
+		if (o.update_coordination)
+			if (int rc = update_coordination()) return rc;
 		if (o.method >= 1) {
 			const bool warm = net_ && (net_record_ || o.method == 2);
 			warm_.n = 0;
@@ -1257,7 +1304,9 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		bool have_back = false;
 		float backp[3] = {0.0f, 0.0f, 0.0f}, t_last = 0.0f;
 		int backt = -1;
-		const Species& sp = species_[r.id];
+		if (o.trajectory && !(r.id >= 0 && r.id < 128 && species_[r.id].defined))
+			return fail(-8, "Configuration::Table::operator[]()");
+		const Species& sp = species_[o.trajectory ? r.id : 0];
 		if (o.trajectory) {
 			float charge = (float) sp.evap_charge;
 			charge *= 1.60217733e-19f;

@@ -71,6 +71,7 @@ struct EvapOptions {
 	int voltage_queue = 250;
 	float shrink_limit = 1.0f;  // stop once the apex has receded this far (m)
 	int thermal_velocity = 0;   // 1: evap_initialVelocity from the process's rand()
+	int update_coordination = 0;  // 1: the fork's update_coordination (surface_3d.cpp:568-593)
 };
 
 class DevBuf {
@@ -130,6 +131,15 @@ public:
 	int field_o1(int count, const int* nodes, double* out);
 	int get_couplings(double* out) const;
 	int get_charges(double* out) const;
+	// the fork's update_coordination (surface_3d.cpp:568-593) on the ids
+	int update_coordination();
+	// the node's id names a defined species (Configuration::Table::operator[]
+	// throws otherwise)
+	bool species_ok(int node) const
+	{
+		const int id = ids_[node];
+		return id >= 0 && id < 128 && species_[id].defined;
+	}
 	int get_flags(unsigned char* out) const;
 	int get_ids(int16_t* out) const;
 	int get_areas(float* a_half32, double* a_half64, float* a_one32) const;
@@ -221,6 +231,7 @@ private:
 	std::vector<double> a_half64_, dist64_;
 	std::vector<double> coup_;             // host mirror (values exact in R)
 	std::vector<double> charge_;
+	bool coord_done_ = false;         // the run-start update_coordination has been applied
 	std::vector<float> cell_vol32_;   // Voronoi cell volume closed at half the mean neighbour distance
 	std::vector<double> cell_vol64_;
 	double charge_of(int node) const;

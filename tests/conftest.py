@@ -26,6 +26,18 @@ FIXTURE_SPECIES = {
 }
 
 
+def coordination_species():
+    """FIXTURE_SPECIES plus ids 12..127 as matrix atoms whose evaporation field
+    grows with the id (the species table of the update_coordination fixture;
+    values as the fixture's configuration text gives them)"""
+    sp = {k: dict(v) for k, v in FIXTURE_SPECIES.items()}
+    for k in range(12, 128):
+        field = float(np.float32(float("%.8e" % (3.0e10 * (1.0 + 0.002 * (k - 10))))))
+        sp[k] = dict(epsilon=1e6, removable=1, dirichlet=1, phi=5000.0, mass=55.845, evap_charge=2, evap_field=field,
+                     evap_energy=1.0)
+    return sp
+
+
 @pytest.fixture(scope="session")
 def tip_small():
     return dict(np.load(os.path.join(ROOT, "tests", "golden", "tip_small.npz")))

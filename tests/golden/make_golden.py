@@ -40,6 +40,16 @@ THERMAL_T = 60.0
 RHO_VACUUM = 2.0e6
 
 
+def coordination_species():
+    """SPECIES plus ids 12..127 as matrix atoms whose evaporation field grows
+    with the id (the ids that the fork's update_coordination produces)"""
+    sp = {k: dict(v) for k, v in SPECIES.items()}
+    for k in range(12, 128):
+        sp[k] = dict(name="Coord%d" % k, eps=1.0e6, rem=1, neu=0, dir=1, phi=5000.0, mass=55.845, charge=2,
+                     field=3.0e10 * (1.0 + 0.002 * (k - 10)), energy=1.0)
+    return sp
+
+
 def aly_text(species, temperature=0.0):
     out = ["ASCII", "", "TEMPERATURE = %.5e" % temperature]
     for sid in sorted(species):
@@ -192,6 +202,26 @@ def make_small():
         write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
         run(wd, ["mesh", "sync", "relax:40:1", "evapnode:%d" % node, "dumpgrid"])
         fx["ref_charge_q_resync"] = rd(wd, "charge.f32", np.float32)
+    # H) the fork's update_coordination applied once and twelve times (ids of every
+    #    atom grow by their atom-neighbour counts and wrap as a signed char)
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        run(wd, ["mesh", "sync", "coord", "dumpgrid"])
+        fx["ref_coord_ids1"] = rd(wd, "ids.i16", np.int16)
+        run(wd, ["mesh", "sync"] + ["coord"] * 12 + ["dumpgrid"])
+        fx["ref_coord_ids12"] = rd(wd, "ids.i16", np.int16)
+    # H2) Process::evaporation with the fork's update_coordination (6 events),
+    #     every id from 10 to 127 defined, the evaporation field growing with
+    #     the id, so that the coordination ids steer the sequence
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(coordination_species()))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+        run(wd, ["mesh", "sync", "relax:300:1", "evap:6:0:0:1e-2:3:5:5:1:5:1e-3:1:0:1:1:0:1", "dumpgrid"])
+        ints = rd(wd, "evap_int.i32", np.int32).reshape(-1, 4)
+        fx["ref_coord_evap_node"], fx["ref_coord_evap_id"] = ints[:, 0], ints[:, 1]
+        fx["ref_coord_evap_f"] = rd(wd, "evap_f.f32", np.float32).reshape(-1, 22)
+        fx["ref_coord_evap_ids"] = rd(wd, "ids.i16", np.int16)
     # F) field lines (Fieldline_3d, debug.cpp's surface field lines) from the
     #    centroids of the highest vacuum-side tets of the six highest atoms:
     #    to the mesh's end, and limited to 2 nm

@@ -548,3 +548,53 @@ def test_space_charge(tip_small, mode):
         for _ in range(6):
             g2.relax(1)
         assert same_bits(g2.phi(0), tip_small["ref_phi0_q6"], np.float32)
+
+
+def test_update_coordination(tip_small):
+    """The fork's update_coordination on the device grid's ids, once and
+    twelve times (signed-char wrap): the reference's ids.  Events with it (ids
+    10..127 defined): sequence, ids, probabilities and flights as the oracle
+    with the device's root and the reference's sequence and ids; with only
+    the standard species the first event fails at the configuration lookup."""
+    from tests.conftest import coordination_species
+    log("update_coordination")
+    g = grid(tip_small, tsg.FAITHFUL_F32)
+    g.update_coordination()
+    assert np.array_equal(g.ids(), tip_small["ref_coord_ids1"])
+    for _ in range(11):
+        g.update_coordination()
+    assert np.array_equal(g.ids(), tip_small["ref_coord_ids12"])
+    sp = coordination_species()
+    mesh = tsg.Mesh(tip_small["xyz"], tip_small["tets"], tip_small["ref_neigh"])
+    g = tsg.Grid3D(mesh, tip_small["ids"], tsg.species_table(sp), 0.0, tsg.FAITHFUL_F32)
+    for _ in range(300):
+        g.relax(1)
+    tsg.srand(99)
+    e = flight_opts()
+    e.update_coordination = 1
+    ev = g.evaporate(e, 6)
+    o = O.Oracle(O.F32, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                 O.species_table(sp))
+    o.sync()
+    for _ in range(300):
+        o.relax(1)
+    o.srand(99)
+    oo = flight_ora_opts()
+    oo.update_coordination = 1
+    try:
+        O.set_root_mode(1)
+        evo = o.evaporate(oo, 6)
+    finally:
+        O.set_root_mode(0)
+    d, o1 = flight_fields(ev), flight_fields(evo)
+    assert d["node"] == o1["node"] and [x.id for x in ev] == [x.id for x in evo]
+    for k in ("start", "stop", "voltage", "ts", "tof"):
+        assert same_bits(d[k], o1[k], np.float32), k
+    assert np.array_equal(g.ids(), o.ids())
+    assert np.array_equal(d["node"], tip_small["ref_coord_evap_node"])
+    assert np.array_equal([x.id for x in ev], tip_small["ref_coord_evap_id"])
+    assert same_bits([x.probability for x in ev], tip_small["ref_coord_evap_f"][:, 0], np.float32)
+    assert np.array_equal(g.ids(), tip_small["ref_coord_evap_ids"])
+    g2 = grid(tip_small, tsg.FAITHFUL_F32)
+    with pytest.raises(RuntimeError):
+        g2.evaporate(e, 1)

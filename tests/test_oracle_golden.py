@@ -233,3 +233,43 @@ def test_space_charge(tip_small):
     o2.resync(node)
     assert bits_equal(o2.charges(), tip_small["ref_charge_q_resync"])
     assert o2.charges()[node] != 0.0
+
+
+def test_update_coordination(tip_small):
+    """The fork's update_coordination once and twelve times (ids wrap as a
+    signed char): every id as the reference's."""
+    o = make(tip_small)
+    o.sync()
+    o.update_coordination()
+    assert np.array_equal(o.ids(), tip_small["ref_coord_ids1"])
+    for _ in range(11):
+        o.update_coordination()
+    assert np.array_equal(o.ids(), tip_small["ref_coord_ids12"])
+
+
+def test_evaporation_with_update_coordination(tip_small):
+    """Process::evaporation with the fork's update_coordination after the
+    initial surface evaluation and after every take-away, ids 10..127
+    defined: six events' sequence, ids, selection data and flights, and every
+    node's id afterwards, bit-exact; with only the standard species the first
+    event fails at the configuration lookup, as the reference's does."""
+    from tests.conftest import coordination_species
+    o = O.Oracle(O.F32, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                 O.species_table(coordination_species()))
+    o.sync()
+    for _ in range(300):
+        o.relax(1)
+    o.srand(99)
+    opts = evap_opts()
+    opts.update_coordination = 1
+    ev = o.evaporate(opts, 6)
+    F = tip_small["ref_coord_evap_f"]
+    assert np.array_equal([e.node for e in ev], tip_small["ref_coord_evap_node"])
+    assert np.array_equal([e.id for e in ev], tip_small["ref_coord_evap_id"])
+    assert bits_equal([e.probability for e in ev], F[:, 0])
+    assert bits_equal([list(e.stop) for e in ev], F[:, 14:17])
+    assert np.array_equal(o.ids(), tip_small["ref_coord_evap_ids"])
+    o2 = make(tip_small)
+    o2.sync()
+    with pytest.raises(O.OracleError):
+        o2.evaporate(opts, 1)
