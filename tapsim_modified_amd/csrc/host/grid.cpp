@@ -131,19 +131,25 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 		fail(-2, "geometry kernel launch failed");
 		return;
 	}
+	// cell volumes of computeCharge, needed only when a species carries charge
+	bool charged = false;
+	for (int id = 0; id < 128; id++)
+		if (species_[id].defined && species_[id].charge_density != 0.0f) charged = true;
 	DevBuf d_v32, d_v64;
-	d_v32.alloc(sizeof(float) * (size_t) n);
-	d_v64.alloc(sizeof(double) * (size_t) n);
-	if (!d_v32.ptr() || !d_v64.ptr()) {
-		fail(-2, "device allocation failed (cell volumes)");
-		return;
-	}
-	if (launch_cell_volumes(stream_, n, d_xyz_.as<float>(), d_tv.as<int>(), d_tn.as<int>(), d_assoc.as<int>(),
-	                        d_ntp.as<int>(), d_nti.as<int>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_cc64.as<double>(),
-	                        d_cc32.as<float>(), d_d32.as<float>(), d_d64.as<double>(), d_v32.as<float>(),
-	                        d_v64.as<double>(), d_status_.as<int>())) {
-		fail(-2, "cell volume kernel launch failed");
-		return;
+	if (charged) {
+		d_v32.alloc(sizeof(float) * (size_t) n);
+		d_v64.alloc(sizeof(double) * (size_t) n);
+		if (!d_v32.ptr() || !d_v64.ptr()) {
+			fail(-2, "device allocation failed (cell volumes)");
+			return;
+		}
+		if (launch_cell_volumes(stream_, n, d_xyz_.as<float>(), d_tv.as<int>(), d_tn.as<int>(), d_assoc.as<int>(),
+		                        d_ntp.as<int>(), d_nti.as<int>(), d_ptr_.as<int>(), d_idx_.as<int>(),
+		                        d_cc64.as<double>(), d_cc32.as<float>(), d_d32.as<float>(), d_d64.as<double>(),
+		                        d_v32.as<float>(), d_v64.as<double>(), d_status_.as<int>())) {
+			fail(-2, "cell volume kernel launch failed");
+			return;
+		}
 	}
 	int gst = 0;
 	a_half32_.resize(nnz); a_half64_.resize(nnz); a_one32_.resize(nnz); dist32_.resize(nnz); dist64_.resize(nnz);
@@ -155,11 +161,13 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 	    hip_check(hipMemcpy(dist32_.data(), d_d32.ptr(), d_d32.bytes(), hipMemcpyDeviceToHost), "distances") ||
 	    hip_check(hipMemcpy(dist64_.data(), d_d64.ptr(), d_d64.bytes(), hipMemcpyDeviceToHost), "distances"))
 		return;
-	cell_vol32_.resize(n);
-	cell_vol64_.resize(n);
-	if (hip_check(hipMemcpy(cell_vol32_.data(), d_v32.ptr(), d_v32.bytes(), hipMemcpyDeviceToHost), "volumes") ||
-	    hip_check(hipMemcpy(cell_vol64_.data(), d_v64.ptr(), d_v64.bytes(), hipMemcpyDeviceToHost), "volumes"))
-		return;
+	if (charged) {
+		cell_vol32_.resize(n);
+		cell_vol64_.resize(n);
+		if (hip_check(hipMemcpy(cell_vol32_.data(), d_v32.ptr(), d_v32.bytes(), hipMemcpyDeviceToHost), "volumes") ||
+		    hip_check(hipMemcpy(cell_vol64_.data(), d_v64.ptr(), d_v64.bytes(), hipMemcpyDeviceToHost), "volumes"))
+			return;
+	}
 	if (gst & 1) { fail(-4, "degenerate tetrahedron (circumcentre)"); return; }
 	if (gst & 2) { fail(-4, "Voronoi face walk failed"); return; }
 

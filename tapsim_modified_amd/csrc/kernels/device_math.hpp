@@ -84,13 +84,20 @@ __device__ __forceinline__ V3<R> vcross(const V3<R>& a, const V3<R>& b)
 
 // Numerical Recipes LU decomposition with partial pivoting as restated in
 // tapsim/utils.h:52-140 (including the double reciprocal of the pivot).
+// Every loop is unrolled and the row exchange is written as conditional
+// swaps with fixed indices, so that the matrix stays in registers (an
+// exchange through a computed row index would place it in scratch memory);
+// the arithmetic and its order are those of the reference.
 template <typename T, int n>
-__device__ bool lu_decmp(T* lu, int* index, T* d)
+__device__ __forceinline__ bool lu_decmp(T* lu, int* index, T* d)
 {
 	T parity = (T) 1.0;
-	T ii[n], vv[n];
+	T vv[n];
+	int ii[n];
+#pragma unroll
 	for (int i = 0; i < n; i++) {
 		T big = (T) 0.0;
+#pragma unroll
 		for (int j = 0; j < n; j++) {
 			const T tmp = fabs(lu[i * n + j]);
 			if (tmp > big) big = tmp;
@@ -100,57 +107,79 @@ __device__ bool lu_decmp(T* lu, int* index, T* d)
 		vv[i] /= big;
 	}
 	int imax = 0;
+#pragma unroll
 	for (int j = 0; j < n; j++) {
+#pragma unroll
 		for (int i = 0; i < j; i++) {
 			T sum = lu[i * n + j];
+#pragma unroll
 			for (int k = 0; k < i; k++) sum -= lu[i * n + k] * lu[k * n + j];
 			lu[i * n + j] = sum;
 		}
 		T big = (T) 0.0;
+#pragma unroll
 		for (int i = j; i < n; i++) {
 			T sum = lu[i * n + j];
+#pragma unroll
 			for (int k = 0; k < j; k++) sum -= lu[i * n + k] * lu[k * n + j];
 			lu[i * n + j] = sum;
 			const T dum = vv[i] * fabs(sum);
 			if (dum >= big) { big = dum; imax = i; }
 		}
-		if (j != imax) {
-			for (int k = 0; k < n; k++) {
-				const T t = lu[imax * n + k];
-				lu[imax * n + k] = lu[j * n + k];
-				lu[j * n + k] = t;
+#pragma unroll
+		for (int r = j + 1; r < n; r++) {
+			if (r == imax) {
+#pragma unroll
+				for (int k = 0; k < n; k++) {
+					const T t = lu[r * n + k];
+					lu[r * n + k] = lu[j * n + k];
+					lu[j * n + k] = t;
+				}
+				parity *= (T) -1.0;
+				vv[r] = vv[j];
 			}
-			parity *= (T) -1.0;
-			vv[imax] = vv[j];
 		}
-		ii[j] = (T) imax;
+		ii[j] = imax;
 		if (lu[j * n + j] == (T) 0.0) return false;
 		const T dum = (T) (1.0 / (double) lu[j * n + j]);
+#pragma unroll
 		for (int i = j + 1; i < n; i++) lu[i * n + j] *= dum;
 	}
 	if (index)
-		for (int i = 0; i < n; i++) index[i] = (int) ii[i];
+#pragma unroll
+		for (int i = 0; i < n; i++) index[i] = ii[i];
 	if (d) *d = parity;
 	return true;
 }
 
-// tapsim/utils.h:163-195
+// tapsim/utils.h:163-195, with the permuted load and store of b and the
+// first-nonzero start of the forward sum as fixed-index selects
 template <typename T, int n>
-__device__ void lu_solve(const T* lu, const int* index, T* b)
+__device__ __forceinline__ void lu_solve(const T* lu, const int* index, T* b)
 {
 	int ii = 0;
+#pragma unroll
 	for (int i = 0; i < n; i++) {
 		const int ip = index[i];
-		T sum = b[ip];
-		b[ip] = b[i];
+		T sum = b[0];
+#pragma unroll
+		for (int r = 1; r < n; r++)
+			if (r == ip) sum = b[r];
+#pragma unroll
+		for (int r = 0; r < n; r++)
+			if (r == ip) b[r] = b[i];
 		if (ii) {
-			for (int j = ii - 1; j < i; j++) sum -= lu[i * n + j] * b[j];
+#pragma unroll
+			for (int j = 0; j < i; j++)
+				if (j >= ii - 1) sum -= lu[i * n + j] * b[j];
 		} else if (sum != (T) 0)
 			ii = i + 1;
 		b[i] = sum;
 	}
+#pragma unroll
 	for (int i = n - 1; i >= 0; i--) {
 		T sum = b[i];
+#pragma unroll
 		for (int j = i + 1; j < n; j++) sum -= lu[i * n + j] * b[j];
 		b[i] = sum / lu[i * n + i];
 	}

@@ -582,7 +582,7 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng)
 	a.status[ion] = status;
 }
 
-__global__ void k_trajectory(TrajArgs a)
+__global__ __launch_bounds__(64) void k_trajectory(TrajArgs a)
 {
 	const int ion = blockIdx.x * blockDim.x + threadIdx.x;
 	if (ion >= a.count) return;
@@ -728,7 +728,7 @@ __device__ void run_fieldline(const FieldlineArgs& a, int line, DevRand* rng)
 	a.status[line] = status;
 }
 
-__global__ void k_fieldlines(FieldlineArgs a)
+__global__ __launch_bounds__(64) void k_fieldlines(FieldlineArgs a)
 {
 	const int tid = blockIdx.x * blockDim.x + threadIdx.x;
 	if (a.rng) {  // the lines in order on one stream
@@ -744,7 +744,7 @@ __global__ void k_fieldlines(FieldlineArgs a)
 }
 
 template <typename R>
-__global__ void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out,
+__global__ __launch_bounds__(64) void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out,
                                   int* rng_state, int walks)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
