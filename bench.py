@@ -114,13 +114,16 @@ def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, sweeps_per_event, budget_s=20
 
 
 def run_events(grid, opts, steps, tag, quiet=False):
+    """steps events in four calls (the flights of consecutive events overlap
+    within one call)"""
     s0 = grid.sweep_stats()
     t0 = time.perf_counter()
     events = []
-    for k in range(steps):
-        events += grid.evaporate(opts, 1)
-        if not quiet and (k + 1) % max(1, steps // 4) == 0:
-            log("%s event %d/%d" % (tag, k + 1, steps))
+    chunk = max(1, (steps + 3) // 4)
+    while len(events) < steps:
+        events += grid.evaporate(opts, min(chunk, steps - len(events)))
+        if not quiet:
+            log("%s event %d/%d" % (tag, len(events), steps))
     dt = time.perf_counter() - t0
     s1 = grid.sweep_stats()
     return events, dt, s0, s1

@@ -206,6 +206,13 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 
 Grid::~Grid()
 {
+	if (fstream_) (void) hipStreamSynchronize(fstream_);
+	for (FlightSlot& f : fslot_) {
+		if (f.ready) (void) hipEventDestroy(f.ready);
+		if (f.done) (void) hipEventDestroy(f.done);
+		if (f.host) (void) hipHostFree(f.host);
+	}
+	if (fstream_) (void) hipStreamDestroy(fstream_);
 	if (stream_) (void) hipStreamSynchronize(stream_);
 	if (ev_start_) (void) hipEventDestroy(ev_start_);
 	if (ev_stop_) (void) hipEventDestroy(ev_stop_);
@@ -1218,7 +1225,190 @@ void Grid::thermal_velocity(const Species& sp, float v[3]) const
 	}
 }
 
+// Steps V-VI of one event on the flight stream: the field snapshot and a
+// copy of the potentials go into the event's slot on the main stream, then
+// the flight kernel (thermal start velocity, trajectory, exit potential on
+// the rand() stream) waits for them on the flight stream.
+int Grid::flight_start(const EvapOptions& o, int ev, int node, const Species& sp, float flen, float pot_before,
+                       float pot_after, float scaling)
+{
+	if (int rc = ensure_point_location()) return rc;
+	const int n = mesh_.n;
+	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	if (!fstream_) HIPCHK(hipStreamCreateWithFlags(&fstream_, hipStreamNonBlocking));
+	FlightSlot& f = fslot_[flight_seq_ % 2];
+	if (f.active)
+		if (int rc = flight_finish(f, nullptr, o)) return rc;
+	if (!f.host) {
+		HIPCHK(hipHostMalloc((void**) &f.host, 24 * sizeof(float)));
+		HIPCHK(hipEventCreateWithFlags(&f.ready, hipEventDisableTiming));
+		HIPCHK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+		f.nfield.alloc(sizeof(float) * 3 * (size_t) n);
+		f.phi.alloc(rs * (size_t) n);
+		f.io.alloc(24 * sizeof(float));
+		if (!f.nfield.ptr() || !f.phi.ptr() || !f.io.ptr()) return fail(-2, "device allocation failed (flight slot)");
+	}
+	int* io = f.io.as<int>();
+	// main stream: the event's field snapshot and potentials
+	HIPCHK(hipMemsetAsync(io + 23, 0, sizeof(int), stream_));
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	if (launch_field_snapshot(stream_, n, d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_coupfull_.ptr(),
+	                          d_flags_.as<unsigned char>(), phi, f64_, f.nfield.as<float>(), io + 23))
+		return fail(-2, "field snapshot launch failed");
+	HIPCHK(hipMemcpyAsync(f.phi.ptr(), phi, rs * (size_t) n, hipMemcpyDeviceToDevice, stream_));
+	HIPCHK(hipEventRecord(f.ready, stream_));
+	// flight stream
+	if (!rng_on_device_) {
+		HIPCHK(hipMemcpyAsync(d_rng_.ptr(), &tsg::process_rand(), sizeof(RandState), hipMemcpyHostToDevice, fstream_));
+		HIPCHK(hipStreamSynchronize(fstream_));
+		rng_on_device_ = true;
+	}
+	float charge = (float) sp.evap_charge;
+	charge *= 1.60217733e-19f;
+	float mass = sp.mass;
+	mass *= 1.660538782e-27f;
+	int* hi = (int*) f.host;
+	hi[0] = node;
+	hi[1] = 0;
+	f.host[2] = charge;
+	f.host[3] = mass;
+	HIPCHK(hipMemcpyAsync(io, f.host, 4 * sizeof(float), hipMemcpyHostToDevice, fstream_));
+	HIPCHK(hipStreamWaitEvent(fstream_, f.ready, 0));
+	FlightArgs a;
+	std::memset(&a, 0, sizeof(a));
+	TrajArgs& t = a.t;
+	t.count = 1;
+	t.nodes = io;
+	t.snapshot = io + 1;
+	t.charge = (const float*) (io + 2);
+	t.mass = (const float*) (io + 3);
+	t.integrator = o.integrator;
+	t.stepper = o.stepper;
+	t.dt0 = o.dt0;
+	t.dt_min = o.dt_min;
+	t.err_limit = o.err_limit;
+	t.zmax = zmax_;
+	t.max_steps = 1 << 24;
+	t.max_calls = 1ll << 22;
+	t.n = n;
+	t.m = mesh_.m();
+	t.xyz = d_xyz_.as<float>();
+	t.tv = d_tv_.as<int>();
+	t.tn = d_tn_.as<int>();
+	t.assoc = d_assoc_.as<int>();
+	t.gbnd = d_gbnd_.as<unsigned char>();
+	t.nfield = f.nfield.as<float>();
+	t.cap = 0;
+	t.len = io + 4;
+	t.status = io + 5;
+	t.summary = (float*) (io + 6);
+	t.rng = d_rng_.as<int>();
+	a.thermal = o.thermal_velocity;
+	a.temperature = temperature_;
+	a.mass_amu = sp.mass;
+	a.v0 = (float*) (io + 20);
+	// the reference locates the exit point twice (voltage, time scale), and
+	// only when the field before was not zero
+	a.exit_walks = flen != 0.0f ? 2 : 0;
+	a.phi = f.phi.ptr();
+	a.f64 = f64_;
+	a.exit = (float*) (io + 18);
+	if (launch_flight(fstream_, a)) return fail(-2, "flight launch failed");
+	HIPCHK(hipMemcpyAsync(f.host + 4, io + 4, 20 * sizeof(float), hipMemcpyDeviceToHost, fstream_));
+	HIPCHK(hipEventRecord(f.done, fstream_));
+	f.active = true;
+	f.seq = flight_seq_++;
+	f.ev = ev;
+	f.flen = flen;
+	f.pot_before = pot_before;
+	f.pot_after = pot_after;
+	f.evap_field = sp.evap_field;
+	f.scaling = scaling;
+	return 0;
+}
+
+// collects one flight: the record's flight fields, and the voltage running
+// mean and time scale of process.cpp:983-1037 in event order
+int Grid::flight_finish(FlightSlot& f, EventRecord* out, const EvapOptions& o)
+{
+	HIPCHK(hipEventSynchronize(f.done));
+	f.active = false;
+	EventRecord* recs = out ? out : flight_out_;
+	EventRecord& r = recs[f.ev];
+	const int* hi = (const int*) f.host;
+	if (hi[23]) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	r.traj_len = hi[4];
+	r.traj_status = hi[5];
+	const float* sm = f.host + 6;
+	for (int c = 0; c < 3; c++) {
+		r.start[c] = sm[c];
+		r.stop[c] = sm[3 + c];
+	}
+	const float t_last = sm[6];
+	const bool have_back = sm[11] != 0.0f;
+	if (have_back && f.flen != 0.0f) {
+		const float pexit = f.host[18];
+		float v = f.pot_before;
+		v -= pexit;
+		v *= f.evap_field;
+		v /= f.flen;
+		v /= f.scaling;
+		vq_.push_back(v);
+		vq_sum_ += v;
+		while (vq_.size() - vq_head_ > (size_t) o.voltage_queue) vq_sum_ -= vq_[vq_head_++];
+		if (vq_head_ > 4096) {
+			vq_.erase(vq_.begin(), vq_.begin() + (long) vq_head_);
+			vq_head_ = 0;
+		}
+		r.voltage = vq_sum_;
+		r.voltage /= (float) (vq_.size() - vq_head_);
+		float ts = f.pot_after;
+		ts -= pexit;
+		ts /= r.voltage;
+		r.time_scale = std::sqrt(ts);
+	} else if (f.flen == 0.0f) {
+		r.voltage = 1.0f;
+		r.time_scale = 1.0f;
+	} else {
+		r.voltage = 0.0f;
+		r.time_scale = 1.0f;
+	}
+	r.tof = t_last * r.time_scale;
+	return 0;
+}
+
+// collects every flight in the air, oldest first, and hands the rand()
+// stream back to the host
+int Grid::flight_drain(EventRecord* out, const EvapOptions& o)
+{
+	int rc = 0;
+	for (;;) {
+		FlightSlot* oldest = nullptr;
+		for (FlightSlot& f : fslot_)
+			if (f.active && (!oldest || f.seq < oldest->seq)) oldest = &f;
+		if (!oldest) break;
+		const int r = flight_finish(*oldest, out, o);
+		if (r && !rc) rc = r;
+	}
+	if (rng_on_device_) {
+		HIPCHK(hipMemcpyAsync(&tsg::process_rand(), d_rng_.ptr(), sizeof(RandState), hipMemcpyDeviceToHost, fstream_));
+		HIPCHK(hipStreamSynchronize(fstream_));
+		rng_on_device_ = false;
+	}
+	return rc;
+}
+
 int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
+{
+	flight_out_ = out;
+	const int rc = evaporate_events(o, k, out, done);
+	// collect the flights still in the air (their records are in out)
+	const int rd = flight_drain(out, o);
+	flight_out_ = nullptr;
+	return rc ? rc : rd;
+}
+
+int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* done)
 {
 	*done = 0;
 	if (o.prob_mode < 0 || o.prob_mode > 3 || o.evap_mode < 0 || o.evap_mode > 1)
@@ -1258,8 +1448,11 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		int node, nt = 0;
 		if (o.evap_mode == 0) {
 			if (int rc = surface_candidate(&node, &nt)) return rc;
-		} else if (int rc = mc_candidate(&node))
-			return rc;
+		} else {
+			// the choice draws from the rand() stream the flights advance
+			if (int rc = flight_drain(out, o)) return rc;
+			if (int rc = mc_candidate(&node)) return rc;
+		}
 		r.node = node;
 		r.id = ids_[node];
 		r.near_tie = nt;
@@ -1308,62 +1501,21 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 		}
 		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode)) return rc;
 		r.surface_size = (int) surface_.size();
-		// V) the ion's flight in the relaxed field (process.cpp:940-975)
-		bool have_back = false;
-		float backp[3] = {0.0f, 0.0f, 0.0f}, t_last = 0.0f;
-		int backt = -1;
+		// V) the ion's flight in the relaxed field (process.cpp:940-975) and
+		// VI) voltage and time scale (process.cpp:983-1037)
 		if (o.trajectory && !(r.id >= 0 && r.id < 128 && species_[r.id].defined))
 			return fail(-8, "Configuration::Table::operator[]()");
-		const Species& sp = species_[o.trajectory ? r.id : 0];
-		if (o.trajectory) {
-			float charge = (float) sp.evap_charge;
-			charge *= 1.60217733e-19f;
-			float mass = sp.mass;
-			mass *= 1.660538782e-27f;
-			float sm[12], v0[3] = {0.0f, 0.0f, 0.0f};
-			if (o.thermal_velocity) thermal_velocity(sp, v0);
-			if (int rc = trajectories(1, &node, &charge, &mass, o.integrator, o.stepper, o.dt0, o.dt_min, o.err_limit, 0,
-			                          nullptr, &r.traj_len, &r.traj_status, sm, v0, true))
-				return rc;
-			for (int c = 0; c < 3; c++) {
-				r.start[c] = sm[c];
-				r.stop[c] = sm[3 + c];
-			}
-			t_last = sm[6];
-			if (sm[11] != 0.0f) {
-				have_back = true;
-				for (int c = 0; c < 3; c++) backp[c] = sm[7 + c];
-				backt = (int) sm[10];
-			}
-		}
-		// VI) voltage and time scale (process.cpp:983-1037)
 		if (int rc = potential(node, &r.pot_after)) return rc;
 		if (int rc = field_o1(1, &node, r.field_after)) return rc;
 		float flen = 0.0f;
 		for (int c = 0; c < 3; c++) flen += (float) r.field_before[c] * (float) r.field_before[c];
 		flen = std::sqrt(flen);
-		if (have_back && flen != 0.0f) {
-			float pexit;
-			// the reference locates the exit point twice (voltage, time scale)
-			if (int rc = point_potential(backp, backt, &pexit, true, 2)) return rc;
-			float v = (float) r.pot_before;
-			v -= pexit;
-			v *= sp.evap_field;
-			v /= flen;
-			v /= scaling_reference;
-			vq_.push_back(v);
-			vq_sum_ += v;
-			while (vq_.size() - vq_head_ > (size_t) o.voltage_queue) vq_sum_ -= vq_[vq_head_++];
-			if (vq_head_ > 4096) {
-				vq_.erase(vq_.begin(), vq_.begin() + (long) vq_head_);
-				vq_head_ = 0;
-			}
-			r.voltage = vq_sum_;
-			r.voltage /= (float) (vq_.size() - vq_head_);
-			float ts = (float) r.pot_after;
-			ts -= pexit;
-			ts /= r.voltage;
-			r.time_scale = std::sqrt(ts);
+		if (o.trajectory) {
+			// on the flight stream, overlapping the next event; the flight's
+			// part of the record is filled when it is collected, in event order
+			if (int rc = flight_start(o, ev, node, species_[r.id], flen, (float) r.pot_before, (float) r.pot_after,
+			                          scaling_reference))
+				return rc;
 		} else if (flen == 0.0f) {
 			r.voltage = 1.0f;
 			r.time_scale = 1.0f;
@@ -1371,7 +1523,6 @@ int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
 			r.voltage = 0.0f;
 			r.time_scale = 1.0f;
 		}
-		r.tof = o.trajectory ? t_last * r.time_scale : 0.0f;
 		*done = ev + 1;
 		if (apex0_ - r.apex[2] >= o.shrink_limit) break;
 	}

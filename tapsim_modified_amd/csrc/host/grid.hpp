@@ -232,6 +232,28 @@ private:
 	std::vector<double> coup_;             // host mirror (values exact in R)
 	std::vector<double> charge_;
 	bool coord_done_ = false;         // the run-start update_coordination has been applied
+	// steps V-VI pipelined on a second stream: the flight of event e runs
+	// while event e+1 relaxes; two slots, each with the field snapshot and
+	// the potentials of its event, its inputs and outputs and two events
+	struct FlightSlot {
+		bool active = false;
+		long long seq = 0;
+		int ev = 0;
+		float flen = 0.0f, pot_before = 0.0f, pot_after = 0.0f, evap_field = 0.0f, scaling = 0.0f;
+		DevBuf nfield, phi, io;
+		float* host = nullptr;  // pinned: inputs [0, 4), outputs [4, 24)
+		hipEvent_t ready = nullptr, done = nullptr;
+	};
+	FlightSlot fslot_[2];
+	hipStream_t fstream_ = nullptr;
+	bool rng_on_device_ = false;      // the flight stream holds the process's rand() stream
+	long long flight_seq_ = 0;
+	EventRecord* flight_out_ = nullptr;  // the records of the evaporate call in progress
+	int flight_start(const EvapOptions& o, int ev, int node, const Species& sp, float flen, float pot_before,
+	                 float pot_after, float scaling);
+	int flight_finish(FlightSlot& f, EventRecord* out, const EvapOptions& o);
+	int flight_drain(EventRecord* out, const EvapOptions& o);
+	int evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* done);
 	std::vector<float> cell_vol32_;   // Voronoi cell volume closed at half the mean neighbour distance
 	std::vector<double> cell_vol64_;
 	double charge_of(int node) const;

@@ -173,6 +173,23 @@ struct TrajArgs {
 	int* rng;                 // optional rand() stream (31-word table + index, see host/prand.hpp):
 	                          // the walks draw their first face from it; count must be 1
 };
+// One event's flight (steps V-VI of Process::evaporation, process.cpp:940-1037)
+// in one thread, on the process's rand() stream: the thermal start velocity
+// (evap_initialVelocity, surface_3d.cpp:472-502) when asked, the trajectory,
+// and the potential at its last point inside the mesh, located `exit_walks`
+// times as the reference's repeated Grid_3d::potential calls do.
+struct FlightArgs {
+	TrajArgs t;               // count 1; t.rng required; t.v0 is set by the kernel when thermal
+	int thermal;
+	float temperature, mass_amu;
+	float* v0;                // 3 floats: the start velocity (written when thermal)
+	int exit_walks;           // 0: no exit potential
+	const void* phi;          // potentials the exit point is interpolated from
+	bool f64;
+	float* exit;              // [0] the exit potential, [1] 1 when it was formed
+};
+int launch_flight(hipStream_t, const FlightArgs&);
+
 // Grid_3d::potential(pos, guess) (grid_3d.cpp:1585-1602) of `count` points:
 // barycentric interpolation of the potentials in float; 0 outside the mesh.
 // With rng (count 1) the point is located `walks` times in a row, each walk

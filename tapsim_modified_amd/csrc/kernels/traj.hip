@@ -743,6 +743,80 @@ __global__ __launch_bounds__(64) void k_fieldlines(FieldlineArgs a)
 	}
 }
 
+// evap_initialVelocity (surface_3d.cpp:472-502) on the stream: 2kT/m split
+// by two uniform draws, then a random sign per axis (five draws)
+__device__ void thermal_velocity(DevRand* rng, float temperature, float mass_amu, float* v)
+{
+	double emax = 2.0;
+	emax *= 1.3806504e-23;
+	emax *= (double) temperature;
+	emax /= (double) mass_amu;
+	emax /= 1.660538782e-27;
+	const float rmax = (float) 2147483647;
+	float x = (float) rng->next();
+	x /= rmax;
+	x = (float) ((double) x * emax);
+	float y = (float) rng->next();
+	y /= rmax;
+	y = (float) ((double) y * (emax - (double) x));
+	float z = (float) emax;
+	z -= x;
+	z -= y;
+	v[0] = x;
+	v[1] = y;
+	v[2] = z;
+	for (int i = 0; i < 3; i++) {
+		v[i] = sqrtf(v[i]);
+		if (rng->next() % 2) v[i] *= -1.0f;
+	}
+}
+
+// Grid_3d::potential(pos, guess) (grid_3d.cpp:1585-1602) after `walks`
+// locations of the point, each drawing from the stream
+template <typename R>
+__device__ float exit_potential(const Mesh3& g, const float* p, int guess, int walks, DevRand* rng, const R* phi)
+{
+	int t = -1;
+	for (int w = 0; w < walks; w++) t = find_tet(g, p, guess, rng);
+	if (t < 0) return 0.0f;
+	float wt[4];
+	tet_weights(g, t, p, wt);
+	float value = 0.0f;
+	for (int k = 0; k < 4; k++) value += wt[k] * (float) phi[g.tv[4 * (size_t) t + k]];
+	return value;
+}
+
+__global__ __launch_bounds__(64) void k_flight(FlightArgs f)
+{
+	if (blockIdx.x != 0 || threadIdx.x != 0) return;
+	__shared__ int table[31];
+	DevRand rng;
+	rand_stage(&rng, table, f.t.rng);
+	TrajArgs a = f.t;
+	if (f.thermal) {
+		float v[3];
+		thermal_velocity(&rng, f.temperature, f.mass_amu, v);
+		for (int c = 0; c < 3; c++) f.v0[c] = v[c];
+		a.v0 = f.v0;
+	}
+	run_ion(a, 0, &rng);
+	f.exit[0] = 0.0f;
+	f.exit[1] = 0.0f;
+	if (f.exit_walks > 0 && a.summary && a.summary[11] != 0.0f) {
+		const float p[3] = {a.summary[7], a.summary[8], a.summary[9]};
+		const int guess = (int) a.summary[10];
+		Mesh3 g;
+		g.xyz = a.xyz;
+		g.tv = a.tv;
+		g.tn = a.tn;
+		g.m = a.m;
+		f.exit[0] = f.f64 ? exit_potential<double>(g, p, guess, f.exit_walks, &rng, (const double*) f.phi)
+		                  : exit_potential<float>(g, p, guess, f.exit_walks, &rng, (const float*) f.phi);
+		f.exit[1] = 1.0f;
+	}
+	rand_unstage(rng, f.t.rng);
+}
+
 template <typename R>
 __global__ __launch_bounds__(64) void k_point_potential(int count, const float* pos, const int* guess, Mesh3 g, const R* phi, float* out,
                                   int* rng_state, int walks)
@@ -803,6 +877,13 @@ int launch_fieldlines(hipStream_t s, const FieldlineArgs& a)
 	if (a.m <= 0 || a.guess_samples < 1 || a.cap < 0) return (int) hipErrorInvalidValue;
 	const int blocks = a.rng ? 1 : (a.count + 63) / 64;
 	hipLaunchKernelGGL(dev::k_fieldlines, dim3(blocks), dim3(64), 0, s, a);
+	return (int) hipGetLastError();
+}
+
+int launch_flight(hipStream_t s, const FlightArgs& f)
+{
+	if (f.t.count != 1 || !f.t.rng || !f.exit || (f.thermal && !f.v0)) return (int) hipErrorInvalidValue;
+	hipLaunchKernelGGL(dev::k_flight, dim3(1), dim3(64), 0, s, f);
 	return (int) hipGetLastError();
 }
 
