@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--train-events", type=int, default=200, help="events recorded for training the network")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 run (default: steps)")
+    ap.add_argument("--solver", default="chebyshev", choices=("chebyshev", "multigrid"),
+                    help="FP32 correction solve of the accelerated paths")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--counters", action="store_true",
                     help="kernel-only run for rocprofv3 --pmc: the relaxed grid's sweep kernels launched back to back "
@@ -226,6 +228,7 @@ def main():
     mixed_opts.method, mixed_opts.accel_tol = 1, 1e-8
     learned_opts = event_options(tsg)
     learned_opts.method, learned_opts.accel_tol = 2, 1e-8
+    mixed_opts.accel_solver = learned_opts.accel_solver = 1 if args.solver == "multigrid" else 0
     plain_opts = event_options(tsg)
 
     if args.counters:
@@ -296,7 +299,7 @@ def main():
                                "%d unknown rows" % (args.workload, w["radius"] * 1e9, w["shank"] * 1e9, counts["atoms"],
                                                     counts["solute"], counts["vacuum_lattice"] + counts["vacuum_random"],
                                                     ps1["rows"]),
-                   "rho": rho, "correction_steps_per_event": steps_l, "refinements_per_event":
+                   "rho": rho, "correction_solver": args.solver, "correction_steps_per_event": steps_l, "refinements_per_event":
                    sum(e.local_iter for e in aev) / max(1, len(aev))},
         "learned": {"train_events": W, "epochs": args.epochs, "train_s": t_train, "loss_first_epoch": first,
                     "loss_last_epoch": last, "heldout_loss": heldout, "heldout_loss_untrained": untrained,

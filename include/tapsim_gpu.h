@@ -209,6 +209,9 @@ typedef struct tsg_evap_options
 	int thermal_velocity;     /* 1: start velocity of evap_initialVelocity (surface_3d.cpp:472-502) from the
 	                             process's rand() and the grid's temperature; 0: at rest (the reference's
 	                             TRAJECTORY_NO_INITIAL_VELOCITY = 1) */
+	int accel_solver;         /* accelerated methods' FP32 correction solve (no reference counterpart):
+	                             0 Chebyshev-Jacobi, 1 conjugate gradients preconditioned by one V-cycle of
+	                             aggregation multigrid; the FP64 refinement and its target are the same */
 } tsg_evap_options;
 
 typedef struct tsg_event
@@ -246,6 +249,12 @@ int tsg_evaporate(tsg_grid*, const tsg_evap_options*, int k, tsg_event* out, int
    radius (power iteration) with the given safety factor (0 < safety <= 1). */
 int tsg_accel_setup(tsg_grid*, double safety, double* rho);
 int tsg_accel_relax(tsg_grid*, double tol, int* outer, long long* inner, double* final_residual);
+/* Selects the correction solve of tsg_accel_relax (no reference counterpart):
+   on = 1 conjugate gradients preconditioned by one V-cycle of aggregation
+   multigrid (hierarchy built from the current rows on the first solve; inner
+   then counts CG iterations), on = 0 the Chebyshev-Jacobi steps.  *levels
+   receives the number of coarse levels built so far (0 before the first solve). */
+int tsg_accel_multigrid(tsg_grid*, int on, int* levels);
 
 /* Measurement: average device time of one launch (HIP events on the grid's
    stream around reps launches) of kind 0 = the Jacobi cycle, kind 1 = the

@@ -76,7 +76,7 @@ class EvapOptions(ctypes.Structure):
         ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int), ("stepper", ctypes.c_int),
         ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float), ("update_coordination", ctypes.c_int),
-        ("thermal_velocity", ctypes.c_int),
+        ("thermal_velocity", ctypes.c_int), ("accel_solver", ctypes.c_int),
     ]
 
     def __init__(self, **kw):
@@ -156,6 +156,7 @@ _SIGS = {
     "tsg_accel_setup": ([_P, ctypes.c_double, _P], ctypes.c_int),
     "tsg_time_kernel": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_accel_relax": ([_P, ctypes.c_double, _P, _P, _P], ctypes.c_int),
+    "tsg_accel_multigrid": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_mpnn_create": ([ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_uint, _P], ctypes.c_int),
     "tsg_mpnn_destroy": ([_P], None),
     "tsg_mpnn_set_params": ([_P, _P], ctypes.c_int),
@@ -477,6 +478,13 @@ class Grid3D:
         outer, inner, res = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_double()
         _chk(load().tsg_accel_relax(self._h, tol, ctypes.byref(outer), ctypes.byref(inner), ctypes.byref(res)))
         return outer.value, inner.value, res.value
+
+    def accel_multigrid(self, on=True):
+        """select the multigrid-preconditioned CG correction solve (True) or
+        the Chebyshev-Jacobi steps (False); returns the coarse levels built"""
+        lv = ctypes.c_int()
+        _chk(load().tsg_accel_multigrid(self._h, 1 if on else 0, ctypes.byref(lv)))
+        return lv.value
 
     def trajectories(self, nodes, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6,
                      cap=4096):

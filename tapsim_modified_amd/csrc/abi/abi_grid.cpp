@@ -257,6 +257,7 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.shrink_limit = o->shrink_limit;
 	e.thermal_velocity = o->thermal_velocity;
 	e.update_coordination = o->update_coordination;
+	e.accel_solver = o->accel_solver;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
 }
 
@@ -280,6 +281,14 @@ int tsg_accel_relax(tsg_grid* h, double tol, int* outer, long long* inner, doubl
 {
 	if (!h || !(tol > 0.0)) return tsg::fail(TSG_E_ARG, "tsg_accel_relax: bad arguments");
 	return check(h, h->g->accel_relax(tol, outer, inner, final_residual));
+}
+
+int tsg_accel_multigrid(tsg_grid* h, int on, int* levels)
+{
+	if (!h || (on != 0 && on != 1)) return tsg::fail(TSG_E_ARG, "tsg_accel_multigrid: bad arguments");
+	h->g->amg_select(on == 1);
+	if (levels) *levels = h->g->amg_levels();
+	return 0;
 }
 
 int tsg_reset_potentials(tsg_grid* h, int kind, const float* center, float pmin, float pmax)

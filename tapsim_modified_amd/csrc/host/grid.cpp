@@ -837,6 +837,8 @@ int Grid::set_id_and_resync(int node, int id)
 	if (accel_)
 		for (int r : touched)
 			if (slot_of_[r] >= 0 && write_inner(slot_of_[r], r, &ai, &vi, &fa, &fv)) return fail(-7, "inner row too wide");
+	if (amg_on_)
+		if (int rc = amg_touch(touched)) return rc;
 
 	// ship all patches through the pinned staging area
 	const size_t nr = ar.size(), nw = aw.size(), nf = full_addr.size();
@@ -1416,8 +1418,10 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 	if (o.method < 0 || o.method > 2) return fail(-1, "unknown relaxation method");
 	if (o.method == 2 && !(net_ && net_predict_))
 		return fail(-1, "method 2 needs a warm-start network attached with prediction on");
+	if (o.accel_solver < 0 || o.accel_solver > 1) return fail(-1, "unknown correction solver");
 	if (o.method >= 1 && !accel_)
 		if (int rc = accel_setup(0.9)) return rc;
+	amg_select(o.accel_solver == 1);
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr, o.prob_mode)) return rc;
 	if (o.trajectory && (o.voltage_queue < 0 || o.voltage_queue > 1 << 20)) return fail(-1, "bad voltage queue length");
@@ -1690,6 +1694,9 @@ int Grid::accel_relax(double tol, int* outer, long long* inner, double* final_re
 {
 	if (!accel_)
 		if (int rc = accel_setup(0.9)) return rc;
+	if (amg_want_ && !amg_on_)
+		if (int rc = amg_setup()) return rc;
+	if (!amg_want_) amg_on_ = false;
 	const int n = mesh_.n;
 	double* phi = (sw_ ? d_phi1_ : d_phi0_).as<double>();
 	double* other = (sw_ ? d_phi0_ : d_phi1_).as<double>();
@@ -1716,14 +1723,22 @@ int Grid::accel_relax(double tol, int* outer, long long* inner, double* final_re
 		if (o >= 64) return fail(-5, "accelerated relaxation did not converge in 64 refinement steps");
 		// a refinement that did not halve the residual means the spectral
 		// bound is too low for the current operator: move it halfway to 1
-		if (o > 0 && rmax > 0.5 * rprev) {
+		if (o > 0 && rmax > 0.5 * rprev && !amg_on_) {
 			rho_ = 1.0 - 0.5 * (1.0 - rho_);
 			sigma = rho_ / (1.0 + std::sqrt(1.0 - rho_ * rho_));
 			stat_rho_raised_++;
 		}
 		rprev = rmax;
-		// Chebyshev steps for a reduction by f (the FP32 correction carries ~6 digits)
+		// the correction solve for a reduction by f (the FP32 correction carries ~6 digits)
 		const double f = std::max(1e-5, 0.5 * tol / rmax);
+		if (amg_on_) {
+			int it = 0;
+			float* dl = nullptr;
+			if (int rc = amg_solve(x, (float) f, &it, &dl)) return rc;
+			steps += it;
+			if (launch_apply64(stream_, x, dl, phi, nullptr)) return fail(-2, "update launch failed");
+			continue;
+		}
 		int m = (int) std::ceil(std::log(2.0 / f) / std::log(1.0 / sigma));
 		m = std::max(2, std::min(m, 200000));
 		float* dprev = d_d0_.as<float>();

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <set>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../kernels/kernels.hpp"
@@ -65,6 +66,7 @@ struct EvapOptions {
 	int method = 0;             // 0: the reference's relaxation, 1: accelerated (mixed precision),
 	                            // 2: accelerated from the learned warm start
 	double accel_tol = 1e-8;    // scaled-residual target of the accelerated relaxation
+	int accel_solver = 0;       // correction solve: 0 Chebyshev-Jacobi, 1 multigrid-preconditioned CG
 	// step V: the ion trajectory (EvaporationOptions, process.h:39-122)
 	int trajectory = 0, integrator = 1, stepper = 1;
 	float dt0 = 1e-14f, dt_min = 1e-16f, err_limit = 1e-6f;
@@ -164,6 +166,11 @@ public:
 	int accel_setup(double safety);
 	int accel_relax(double tol, int* outer, long long* inner, double* final_residual);
 	double accel_rho() const { return rho_; }
+	// aggregation multigrid for the correction solve (built on first use;
+	// rows created later are smoothed but not aggregated)
+	int amg_setup();
+	void amg_select(bool on) { amg_want_ = on; }
+	int amg_levels() const { return (int) amg_levels_.size(); }
 	int time_kernel(int kind, int reps, double* avg_ms, double* bytes);
 
 	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
@@ -295,6 +302,21 @@ private:
 	int write_inner(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* af,
 	                std::vector<float>* vf);
 	AccelArgs accel_args() const;
+	// multigrid state: coarse levels (CSR), level-0 row sums, aggregate of
+	// every level-0 node, the coarsest level's dense inverse, PCG vectors
+	struct AmgLevelHost {
+		int n = 0;
+		float w = 0.0f;
+		DevBuf ptr, col, val, dinv, rptr, ridx, agg, b, x, r;
+	};
+	static constexpr int kAmgCoarsest = 512;
+	bool amg_want_ = false, amg_on_ = false;
+	float amg_w0_ = 0.0f;
+	std::vector<std::unique_ptr<AmgLevelHost>> amg_levels_;
+	DevBuf amg_dn_, amg_agg0_, amg_minv_, amg_part_, amg_st_;
+	DevBuf amg_r_, amg_z_, amg_p_, amg_q_, amg_x0_, amg_res_, amg_delta_;
+	int amg_touch(const std::vector<int>& nodes);
+	int amg_solve(const AccelArgs& x, float f, int* iters, float** delta);
 	// trajectories: mesh arrays for the point location, node-field snapshot
 	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_, d_rng_;
 	float zmax_ = 0.0f;

@@ -119,6 +119,40 @@ int launch_cheb_step(hipStream_t, const AccelArgs&, const float* r32, const floa
 int launch_apply64(hipStream_t, const AccelArgs&, const float* delta, double* phi, double* phi_other);
 int launch_apply_g(hipStream_t, const AccelArgs&, const float* in, float* out);
 
+// ---------------------------------------------------------------- multigrid correction solve
+// A coarse level of the aggregation multigrid (CSR, float): the operator, its
+// inverse diagonal, the members of each row among the finer level's indices
+// (restriction) and the map of the finer level's indices onto this level
+// (prolongation; -1 outside every aggregate)
+struct AmgLevel {
+	int n;
+	const int* ptr;
+	const int* col;
+	const float* val;
+	const float* dinv;
+	const int* rptr;
+	const int* ridx;
+	const int* agg;
+};
+// slots of the PCG scalars in device memory
+enum { kAmgAlpha = 0, kAmgBeta = 1, kAmgRz = 2, kAmgRr = 3, kAmgRr0 = 4, kAmgScalars = 8 };
+int amg_fine_blocks(const AccelArgs&);
+int amg_slot_blocks(const AccelArgs&);
+// r = D r32, delta = 0, rr0 = rr = r.r
+int launch_amg_start(hipStream_t, const AccelArgs&, const float* dn, const float* r32, float* r, float* delta,
+                     double* part, double* st);
+// z = V(b) (part receives the block partials of b.z)
+int launch_amg_vcycle(hipStream_t, const AccelArgs&, const float* dn, const int* agg0, float w0, const AmgLevel* lv,
+                      const float* w, int nlev, const float* b, float* x0, float* res, float* z, float* const* bl,
+                      float* const* xl, float* const* rl, const float* minv, double* part);
+// mode 3: beta = b.z / rz, rz = b.z; mode 4: rz = b.z
+int launch_amg_scalars(hipStream_t, int nblocks, const double* part, double* st, int mode);
+// p = z + beta p (first: p = z)
+int launch_amg_dir(hipStream_t, const AccelArgs&, const double* st, const float* z, float* p, int first);
+// q = A p, alpha = rz / p.q, delta += alpha p, r -= alpha q, rr = r.r
+int launch_amg_matvec_update(hipStream_t, const AccelArgs&, const float* dn, const float* p, float* q, float* delta,
+                             float* r, double* part, double* st);
+
 // ---------------------------------------------------------------- warm start
 // the shell of an evaporated atom as the network's graph (rows of the
 // network's store) and the quantities that tie it to the grid

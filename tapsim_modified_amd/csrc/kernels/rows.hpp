@@ -220,5 +220,53 @@ __device__ __forceinline__ float long_row_fma_f32(const int* __restrict__ col, c
 	return acc;
 }
 
+// G x over the FP32 correction rows for this thread's slot (or long row):
+// returns the node (-1 when none) and the sum; the per-slot record and the
+// node's extra operands (e1[node], e2[node], either may be null) are loaded
+// in one round before the row
+__device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __restrict__ in, float* sum,
+                                         const float* __restrict__ e1, float* v1, const float* __restrict__ e2, float* v2)
+{
+	const int main_blocks = (x.nslots + 255) >> 8;
+	const RowBlock rb = row_block(x.nlong);
+	const int lb = rb.is_long ? -1 : rb.index;
+	int slot = -1, node = -1, deg = 0, ideg = 0;
+	long long soff = 0;
+	if (!rb.is_long) {
+		if (lb < main_blocks) slot = lb * 256 + threadIdx.x;
+		if (slot >= 0 && slot < x.nslots) {
+			node = x.slot_node[slot];
+			deg = x.slot_deg[slot];
+			ideg = x.inner_deg[slot];
+			soff = x.slice_off[slot >> 6];
+		}
+	} else {
+		const int li = long_row_index(rb);
+		if (li < x.nlong) {
+			slot = x.long_slots[li];
+			node = x.slot_node[slot];
+			deg = x.slot_deg[slot];
+			ideg = x.inner_deg[slot];
+			soff = x.slice_off[slot >> 6];
+		}
+	}
+	const long long base = soff + (slot & 63);
+	*sum = 0.0f;
+	if (!rb.is_long) {  // block-uniform; the row sum needs the whole wavefront
+		const bool act = node >= 0 && deg <= kLongRow;
+		if (act) {
+			if (e1) *v1 = e1[node];
+			if (e2) *v2 = e2[node];
+		}
+		*sum = short_row_fma_f32(x.col, x.w, in, act, ideg, base);
+		return act ? node : -1;
+	}
+	if (node < 0) return -1;
+	if (e1) *v1 = e1[node];
+	if (e2) *v2 = e2[node];
+	*sum = long_row_fma_f32(x.col, x.w, in, ideg, base);
+	return lane_id() == 0 ? node : -1;
+}
+
 }  // namespace dev
 }  // namespace tsg

@@ -1,0 +1,112 @@
+"""GPU: the multigrid-preconditioned CG correction solve of the accelerated
+relaxation (tsg_accel_multigrid).  It has no reference counterpart; the bar is
+the accelerated relaxation's own: the FP64 residual target, checked by the
+oracle's FP64 Jacobi sweep on the device result, the distance to the
+oracle's tightly relaxed solution, and the evaporation sequence of the plain
+FP64 path."""
+import sys
+
+import numpy as np
+import pytest
+
+import bench
+import tapsim_modified_amd as tsg
+from oracle import oracle as O
+from tests.conftest import FIXTURE_SPECIES
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print("[gpu-test] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def fixture_grid(fx):
+    mesh = tsg.Mesh(fx["xyz"], fx["tets"], fx["ref_neigh"])
+    return tsg.Grid3D(mesh, fx["ids"], tsg.species_table(FIXTURE_SPECIES), 0.0, tsg.PLAIN_F64)
+
+
+def fixture_oracle(fx):
+    return O.Oracle(O.F64, fx["xyz"], fx["ids"], fx["tets"], fx["ref_neigh"], O.species_table(FIXTURE_SPECIES))
+
+
+@pytest.fixture(scope="module")
+def small_tip():
+    xyz, ids, _ = bench.build_specimen(tsg, bench.WORKLOADS["s"])
+    mesh = tsg.Mesh(xyz)
+    return mesh, ids
+
+
+def test_multigrid_relaxation_residual_and_error(tip_small):
+    """max|J(phi)-phi| <= 1e-8 V by the oracle's own FP64 sweep, and within
+    1e-6 V of the oracle's solution relaxed to 1e-13 V (the Chebyshev
+    solve's bar)."""
+    log("multigrid relaxation tip_small")
+    g = fixture_grid(tip_small)
+    g.accel_setup(0.9)
+    assert g.accel_multigrid(True) == 0
+    outer, iters, res = g.accel_relax(1e-8)
+    levels = g.accel_multigrid(True)
+    log("multigrid: levels %d outer %d CG iterations %d residual %.2e" % (levels, outer, iters, res))
+    assert levels >= 1 and res <= 1e-8 and outer >= 1
+    phi = g.phi(0)
+    assert np.array_equal(phi, g.phi(1))
+    o = fixture_oracle(tip_small)
+    o.sync()
+    o.set_phi(phi, phi, 0, 0)
+    assert o.relax(1) <= 1e-8
+    ex = fixture_oracle(tip_small)
+    ex.sync()
+    ex.relax_threshold(1e-13, 1, 0)
+    err = np.max(np.abs(phi - ex.phi(ex.switches()[0])))
+    log("multigrid error %.2e" % err)
+    assert err < 1e-6
+
+
+def test_multigrid_matches_chebyshev_and_is_deterministic(small_tip):
+    """On a 24 k-node tip (several coarse levels): both correction solves reach
+    1e-8 V and agree within 2e-8 V / (1 - rho) everywhere (two solutions whose
+    Jacobi residuals are below 1e-8 V, rho the spectral radius of the Jacobi
+    operator); two multigrid runs are bit-identical (fixed-order reductions)."""
+    mesh, ids = small_tip
+    species = tsg.species_table(bench.SPECIES)
+    runs = []
+    for solver in (0, 1, 1):
+        log("small tip relaxation, solver %d" % solver)
+        g = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
+        rho = g.accel_setup(0.9)
+        g.accel_multigrid(solver == 1)
+        outer, inner, res = g.accel_relax(1e-8)
+        log("solver %d: levels %d outer %d inner %d residual %.2e"
+            % (solver, g.accel_multigrid(solver == 1), outer, inner, res))
+        assert res <= 1e-8
+        runs.append(g.phi(0))
+    diff = np.max(np.abs(runs[0] - runs[1]))
+    log("chebyshev vs multigrid: max difference %.2e V (bound %.2e)" % (diff, 2e-8 / (1.0 - rho)))
+    assert diff < 2e-8 / (1.0 - rho)
+    assert np.array_equal(runs[1].view(np.uint64), runs[2].view(np.uint64))
+
+
+def test_multigrid_events_follow_plain_sequence(small_tip):
+    """Events (steps I-VI, flights on) with the multigrid correction solve:
+    the evaporation sequence equals the plain FP64 path's and the Chebyshev
+    solve's from the same relaxed start."""
+    mesh, ids = small_tip
+    species = tsg.species_table(bench.SPECIES)
+    g0 = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
+    g0.accel_setup(0.9)
+    g0.accel_relax(1e-10)
+    state = (g0.phi(0), g0.phi(1)) + g0.switches()
+    seqs = {}
+    for name, method, solver in (("plain", 0, 0), ("chebyshev", 1, 0), ("multigrid", 1, 1)):
+        log("small tip events: %s" % name)
+        g = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
+        g.set_phi(*state)
+        o = bench.event_options(tsg)
+        o.method, o.accel_tol, o.accel_solver = method, 1e-8, solver
+        ev = g.evaporate(o, 12)
+        seqs[name] = [e.node for e in ev]
+        log("%s: %s" % (name, seqs[name]))
+    assert seqs["multigrid"] == seqs["plain"]
+    assert seqs["multigrid"] == seqs["chebyshev"]
