@@ -117,6 +117,7 @@ def run_events(grid, opts, steps, tag, quiet=False):
     """steps events in four calls (the flights of consecutive events overlap
     within one call)"""
     s0 = grid.sweep_stats()
+    p0, e0 = grid.phase_times()
     t0 = time.perf_counter()
     events = []
     chunk = max(1, (steps + 3) // 4)
@@ -126,6 +127,8 @@ def run_events(grid, opts, steps, tag, quiet=False):
             log("%s event %d/%d" % (tag, len(events), steps))
     dt = time.perf_counter() - t0
     s1 = grid.sweep_stats()
+    p1, e1 = grid.phase_times()
+    s1["phases_ms_per_event"] = {k: (p1[k] - p0[k]) / max(1, e1 - e0) for k in p1}
     return events, dt, s0, s1
 
 
@@ -142,14 +145,16 @@ COPY_BYTES = 256 << 20
 
 
 def counters_run(tsg, grid):
-    """Launches the mixed path's correction sweep and the FP64 sweep 100 times
-    each on the relaxed grid, and one plain device copy of COPY_BYTES
-    (k_copy_probe), whose FETCH_SIZE / WRITE_SIZE calibrate the counters of
-    the others."""
+    """Launches the mixed path's correction sweeps (Chebyshev step, multigrid
+    operator product) and the FP64 sweep 100 times each on the relaxed grid,
+    and one plain device copy of COPY_BYTES (k_copy_probe), whose FETCH_SIZE /
+    WRITE_SIZE calibrate the counters of the others."""
     tsg.copy_probe(COPY_BYTES, 1)
     ms1, by1 = grid.time_kernel(1, 100)
+    ms2, by2 = grid.time_kernel(2, 100)
     ms0, by0 = grid.time_kernel(0, 100)
     print(json.dumps({"copy_bytes": COPY_BYTES, "k_cheb32": {"bytes": by1, "us": ms1 * 1e3},
+                      "k_amg_fine_matvec": {"bytes": by2, "us": ms2 * 1e3},
                       "k_jacobi<double>": {"bytes": by0, "us": ms0 * 1e3}}))
 
 
@@ -177,8 +182,9 @@ def main():
     ap.add_argument("--train-events", type=int, default=200, help="events recorded for training the network")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 run (default: steps)")
-    ap.add_argument("--solver", default="chebyshev", choices=("chebyshev", "multigrid"),
-                    help="FP32 correction solve of the accelerated paths")
+    ap.add_argument("--solver", default="chebyshev", choices=("chebyshev", "multigrid", "multigrid-eager"),
+                    help="FP32 correction solve of the accelerated paths (multigrid-eager: every kernel "
+                         "launched on its own instead of replaying the captured iteration graph)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--counters", action="store_true",
                     help="kernel-only run for rocprofv3 --pmc: the relaxed grid's sweep kernels launched back to back "
@@ -228,7 +234,8 @@ def main():
     mixed_opts.method, mixed_opts.accel_tol = 1, 1e-8
     learned_opts = event_options(tsg)
     learned_opts.method, learned_opts.accel_tol = 2, 1e-8
-    mixed_opts.accel_solver = learned_opts.accel_solver = 1 if args.solver == "multigrid" else 0
+    mixed_opts.accel_solver = learned_opts.accel_solver = {"chebyshev": 0, "multigrid": 1, "multigrid-eager": 2}[
+        args.solver]
     plain_opts = event_options(tsg)
 
     if args.counters:
@@ -266,7 +273,10 @@ def main():
         mixed.evaporate(mixed_opts, 1)
     aev, adt, as0, as1 = run_events(learned, learned_opts, K, "learned")
     mev, mdt, ms0, ms1 = run_events(mixed, mixed_opts, K, "mixed")
-    roof_acc = roofline(learned, 1, "k_cheb32", args.workload)
+    if args.solver == "chebyshev":
+        roof_acc = roofline(learned, 1, "k_cheb32", args.workload)
+    else:  # the solve's three row sweeps per iteration share this kernel's shape
+        roof_acc = roofline(learned, 2, "k_amg_fine_matvec", args.workload)
     # potential error of the learned path's final state against the same
     # system relaxed to a scaled FP64 residual of 1e-10 V (the residual's own
     # rounding floor is ~1e-11 V at 5 kV)
@@ -299,13 +309,15 @@ def main():
                                "%d unknown rows" % (args.workload, w["radius"] * 1e9, w["shank"] * 1e9, counts["atoms"],
                                                     counts["solute"], counts["vacuum_lattice"] + counts["vacuum_random"],
                                                     ps1["rows"]),
+                   "phases_ms_per_event": as1["phases_ms_per_event"],
                    "rho": rho, "correction_solver": args.solver, "correction_steps_per_event": steps_l, "refinements_per_event":
                    sum(e.local_iter for e in aev) / max(1, len(aev))},
         "learned": {"train_events": W, "epochs": args.epochs, "train_s": t_train, "loss_first_epoch": first,
                     "loss_last_epoch": last, "heldout_loss": heldout, "heldout_loss_untrained": untrained,
                     "prediction_kept": kept, "median_residual_ratio": ratio},
         "mixed_fp32_fp64": {"value": K / mdt, "unit": "events/s", "ms_per_step": mdt / K * 1e3,
-                            "correction_steps_per_event": steps_m},
+                            "correction_steps_per_event": steps_m,
+                            "phases_ms_per_event": ms1["phases_ms_per_event"]},
         "plain_fp64": {"value": Kp / pdt, "unit": "events/s", "steps": Kp, "ms_per_step": pdt / Kp * 1e3,
                        "sweeps_per_event": psweeps},
         "potential_error": {"max_abs_V": err, "max_rel": rel,

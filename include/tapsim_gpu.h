@@ -211,7 +211,8 @@ typedef struct tsg_evap_options
 	                             TRAJECTORY_NO_INITIAL_VELOCITY = 1) */
 	int accel_solver;         /* accelerated methods' FP32 correction solve (no reference counterpart):
 	                             0 Chebyshev-Jacobi, 1 conjugate gradients preconditioned by one V-cycle of
-	                             aggregation multigrid; the FP64 refinement and its target are the same */
+	                             aggregation multigrid (iterations replayed from a captured HIP graph), 2 the
+	                             same launched kernel by kernel; the FP64 refinement and its target are the same */
 } tsg_evap_options;
 
 typedef struct tsg_event
@@ -252,14 +253,25 @@ int tsg_accel_relax(tsg_grid*, double tol, int* outer, long long* inner, double*
 /* Selects the correction solve of tsg_accel_relax (no reference counterpart):
    on = 1 conjugate gradients preconditioned by one V-cycle of aggregation
    multigrid (hierarchy built from the current rows on the first solve; inner
-   then counts CG iterations), on = 0 the Chebyshev-Jacobi steps.  *levels
-   receives the number of coarse levels built so far (0 before the first solve). */
+   then counts CG iterations; the iterations replay a captured HIP graph),
+   on = 2 the same with every kernel launched on its own (identical results),
+   on = 0 the Chebyshev-Jacobi steps.  *levels receives the number of coarse
+   levels built so far (0 before the first solve). */
 int tsg_accel_multigrid(tsg_grid*, int on, int* levels);
+
+/* Measurement (no reference counterpart): host wall time of tsg_evaporate's
+   phases, accumulated over every event so far, in ms: ms[0] the choice (with
+   the fields before), [1] the take-away, [2] the warm start, [3] the
+   relaxation, [4] the surface evaluation (with the fields after), [5] the
+   flight hand-off and collection; n <= 6 entries are written; *events the
+   number of events timed. */
+int tsg_phase_times(tsg_grid*, double* ms, int n, long long* events);
 
 /* Measurement: average device time of one launch (HIP events on the grid's
    stream around reps launches) of kind 0 = the Jacobi cycle, kind 1 = the
-   FP32 Chebyshev correction step, and that kernel's algorithmic bytes per
-   launch.  Potentials are restored afterwards. */
+   FP32 Chebyshev correction step, kind 2 = the multigrid solve's level-0
+   operator product q = A p (hierarchy built if needed), and that kernel's
+   algorithmic bytes per launch.  Potentials are restored afterwards. */
 int tsg_time_kernel(tsg_grid*, int kind, int reps, double* avg_ms, double* bytes);
 
 /* Measurement: algorithmic bytes of one sweep (column index + coupling per

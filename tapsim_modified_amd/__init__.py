@@ -157,6 +157,7 @@ _SIGS = {
     "tsg_time_kernel": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_accel_relax": ([_P, ctypes.c_double, _P, _P, _P], ctypes.c_int),
     "tsg_accel_multigrid": ([_P, ctypes.c_int, _P], ctypes.c_int),
+    "tsg_phase_times": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_mpnn_create": ([ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_uint, _P], ctypes.c_int),
     "tsg_mpnn_destroy": ([_P], None),
     "tsg_mpnn_set_params": ([_P, _P], ctypes.c_int),
@@ -479,11 +480,24 @@ class Grid3D:
         _chk(load().tsg_accel_relax(self._h, tol, ctypes.byref(outer), ctypes.byref(inner), ctypes.byref(res)))
         return outer.value, inner.value, res.value
 
-    def accel_multigrid(self, on=True):
+    PHASES = ("choice", "take_away", "warm_start", "relaxation", "surface", "flight")
+
+    def phase_times(self):
+        """host wall time (ms) of the event loop's phases, accumulated, and the
+        number of events timed"""
+        ms = np.zeros(6, np.float64)
+        ev = ctypes.c_longlong()
+        _chk(load().tsg_phase_times(self._h, ms.ctypes.data_as(_P), 6, ctypes.byref(ev)))
+        return dict(zip(self.PHASES, ms.tolist())), ev.value
+
+    def accel_multigrid(self, on=True, eager=False):
         """select the multigrid-preconditioned CG correction solve (True) or
-        the Chebyshev-Jacobi steps (False); returns the coarse levels built"""
+        the Chebyshev-Jacobi steps (False); eager launches every kernel on its
+        own instead of replaying the captured iteration graph; returns the
+        coarse levels built"""
         lv = ctypes.c_int()
-        _chk(load().tsg_accel_multigrid(self._h, 1 if on else 0, ctypes.byref(lv)))
+        mode = (2 if eager else 1) if on else 0
+        _chk(load().tsg_accel_multigrid(self._h, mode, ctypes.byref(lv)))
         return lv.value
 
     def trajectories(self, nodes, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6,

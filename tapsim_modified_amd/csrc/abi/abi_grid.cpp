@@ -283,10 +283,16 @@ int tsg_accel_relax(tsg_grid* h, double tol, int* outer, long long* inner, doubl
 	return check(h, h->g->accel_relax(tol, outer, inner, final_residual));
 }
 
+int tsg_phase_times(tsg_grid* h, double* ms, int n, long long* events)
+{
+	if (!h || !ms || n < 0) return tsg::fail(TSG_E_ARG, "tsg_phase_times: bad arguments");
+	return h->g->phase_times(ms, n, events);
+}
+
 int tsg_accel_multigrid(tsg_grid* h, int on, int* levels)
 {
-	if (!h || (on != 0 && on != 1)) return tsg::fail(TSG_E_ARG, "tsg_accel_multigrid: bad arguments");
-	h->g->amg_select(on == 1);
+	if (!h || on < 0 || on > 2) return tsg::fail(TSG_E_ARG, "tsg_accel_multigrid: bad arguments");
+	h->g->amg_select(on >= 1, on == 2);
 	if (levels) *levels = h->g->amg_levels();
 	return 0;
 }

@@ -162,6 +162,14 @@ bool dense_inverse(const Csr& A, std::vector<float>* out)
 	return true;
 }
 
+// the row arguments a captured graph holds by value
+bool same_rows(const AccelArgs& a, const AccelArgs& b)
+{
+	return a.nslots == b.nslots && a.nlong == b.nlong && a.slot_node == b.slot_node && a.inner_deg == b.inner_deg &&
+	       a.slice_off == b.slice_off && a.col == b.col && a.w == b.w && a.slot_deg == b.slot_deg &&
+	       a.long_slots == b.long_slots;
+}
+
 template <typename T>
 int upload(DevBuf* b, const std::vector<T>& v)
 {
@@ -214,7 +222,8 @@ int Grid::amg_setup()
 		A.ptr[u + 1] = (int) A.col.size();
 	}
 	amg_w0_ = (float) (4.0 / (3.0 * jacobi_lmax(A)));
-	// coarse levels
+	// coarse levels (a captured iteration graph holds the old ones' pointers)
+	amg_graph_drop();
 	amg_levels_.clear();
 	std::vector<int> agg0_node(n, -1);
 	Csr fine = A;
@@ -270,6 +279,7 @@ int Grid::amg_setup()
 	amg_part_.alloc(sizeof(double) * nb);
 	amg_st_.alloc(sizeof(double) * kAmgScalars);
 	if (!amg_part_.ptr() || !amg_st_.ptr()) return fail(-2, "device allocation failed (multigrid)");
+	HIPCHK(hipMemset(amg_st_.ptr(), 0, sizeof(double) * kAmgScalars));
 	amg_on_ = true;
 	return 0;
 }
@@ -325,38 +335,90 @@ int Grid::amg_solve(const AccelArgs& x, float f, int* iters, float** delta)
 		xl[l] = H.x.as<float>();
 		rl[l] = H.r.as<float>();
 	}
-	float* dn = amg_dn_.as<float>();
-	float *r = amg_r_.as<float>(), *z = amg_z_.as<float>(), *p = amg_p_.as<float>(), *q = amg_q_.as<float>();
-	float *x0 = amg_x0_.as<float>(), *res = amg_res_.as<float>(), *dl = amg_delta_.as<float>();
-	double* part = amg_part_.as<double>();
-	double* st = amg_st_.as<double>();
-	const int fb = amg_fine_blocks(x);
-	auto vcycle = [&](int mode) -> int {
-		if (launch_amg_vcycle(stream_, x, dn, amg_agg0_.as<int>(), amg_w0_, lv.data(), w.data(), nlev, r, x0, res, z,
-		                      bl.data(), xl.data(), rl.data(), amg_minv_.as<float>(), part) ||
-		    launch_amg_scalars(stream_, fb, part, st, mode))
+	AmgSolve S;
+	S.x = x;
+	S.dn = amg_dn_.as<float>();
+	S.agg0 = amg_agg0_.as<int>();
+	S.w0 = amg_w0_;
+	S.lv = lv.data();
+	S.w = w.data();
+	S.nlev = nlev;
+	S.r = amg_r_.as<float>();
+	S.z = amg_z_.as<float>();
+	S.p = amg_p_.as<float>();
+	S.q = amg_q_.as<float>();
+	S.x0 = amg_x0_.as<float>();
+	S.res = amg_res_.as<float>();
+	S.delta = amg_delta_.as<float>();
+	S.bl = bl.data();
+	S.xl = xl.data();
+	S.rl = rl.data();
+	S.minv = amg_minv_.as<float>();
+	S.part = amg_part_.as<double>();
+	S.st = amg_st_.as<double>();
+	const double f2 = (double) f * (double) f;
+	if (launch_amg_start(stream_, S, d_r32_.as<float>(), f2) || launch_amg_vcycle(stream_, S, 4) ||
+	    launch_amg_dir(stream_, S, 1))
+		return fail(-2, "multigrid launch failed");
+	auto iteration = [&]() -> int {
+		if (launch_amg_matvec_update(stream_, S) || launch_amg_vcycle(stream_, S, 3) || launch_amg_dir(stream_, S, 0))
 			return -1;
 		return 0;
 	};
-	if (launch_amg_start(stream_, x, dn, d_r32_.as<float>(), r, dl, part, st)) return fail(-2, "multigrid launch failed");
-	if (vcycle(4) || launch_amg_dir(stream_, x, st, z, p, 1)) return fail(-2, "multigrid launch failed");
-	const double f2 = (double) f * (double) f;
-	int it = 0;
-	for (;;) {
-		it++;
-		if (launch_amg_matvec_update(stream_, x, dn, p, q, dl, r, part, st)) return fail(-2, "multigrid launch failed");
-		if (it % 2 == 0 || it >= 400) {
-			double s[kAmgScalars];
-			HIPCHK(hipMemcpyAsync(s, st, sizeof(s), hipMemcpyDeviceToHost, stream_));
-			HIPCHK(hipStreamSynchronize(stream_));
-			if (!(s[kAmgRr] == s[kAmgRr])) return fail(-3, "multigrid: NaN in the correction solve");
-			if (s[kAmgRr] <= f2 * s[kAmgRr0] || it >= 400) break;
+	// the iterations of one batch, captured once per row layout and replayed
+	if (!amg_eager_ && (!amg_exec_ || !same_rows(amg_graph_args_, x))) {
+		amg_graph_drop();
+		HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+		int bad = 0;
+		for (int b = 0; b < kAmgBatch; b++) bad |= iteration();
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(stream_, &g);
+		if (bad || e != hipSuccess) {
+			if (g) (void) hipGraphDestroy(g);
+			return fail(-2, "multigrid: graph capture failed");
 		}
-		if (vcycle(3) || launch_amg_dir(stream_, x, st, z, p, 0)) return fail(-2, "multigrid launch failed");
+		amg_graph_ = g;
+		HIPCHK(hipGraphInstantiate(&amg_exec_, amg_graph_, nullptr, nullptr, 0));
+		amg_graph_args_ = x;
 	}
-	*iters = it;
-	*delta = dl;
+	if (!amg_host_) {
+		HIPCHK(hipHostMalloc((void**) &amg_host_, sizeof(double) * 2 * kAmgScalars, hipHostMallocDefault));
+		for (hipEvent_t& e : amg_ev_) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	}
+	// batches run ahead of the check: batch k's scalars are read back while
+	// batch k+1 runs (once the target is met every launch returns at once)
+	const int max_batches = 400 / kAmgBatch;
+	double* last = nullptr;
+	for (int k = 0; k < max_batches; k++) {
+		if (amg_eager_) {
+			for (int b = 0; b < kAmgBatch; b++)
+				if (iteration()) return fail(-2, "multigrid launch failed");
+		} else {
+			HIPCHK(hipGraphLaunch(amg_exec_, stream_));
+		}
+		double* slot = amg_host_ + (k & 1) * kAmgScalars;
+		HIPCHK(hipMemcpyAsync(slot, S.st, sizeof(double) * kAmgScalars, hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipEventRecord(amg_ev_[k & 1], stream_));
+		if (k == 0) continue;
+		HIPCHK(hipEventSynchronize(amg_ev_[(k - 1) & 1]));
+		last = amg_host_ + ((k - 1) & 1) * kAmgScalars;
+		if (last[kAmgDone] != 0.0) break;
+	}
+	HIPCHK(hipStreamSynchronize(stream_));
+	double fin[kAmgScalars];
+	HIPCHK(hipMemcpy(fin, S.st, sizeof(fin), hipMemcpyDeviceToHost));
+	if (!(fin[kAmgRr] == fin[kAmgRr])) return fail(-3, "multigrid: NaN in the correction solve");
+	*iters = (int) fin[kAmgIters];
+	*delta = S.delta;
 	return 0;
+}
+
+void Grid::amg_graph_drop()
+{
+	if (amg_exec_) (void) hipGraphExecDestroy(amg_exec_);
+	if (amg_graph_) (void) hipGraphDestroy(amg_graph_);
+	amg_exec_ = nullptr;
+	amg_graph_ = nullptr;
 }
 
 }  // namespace tsg

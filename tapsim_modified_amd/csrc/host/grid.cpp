@@ -3,6 +3,7 @@
 #include "prand.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -217,6 +218,10 @@ Grid::~Grid()
 	if (ev_start_) (void) hipEventDestroy(ev_start_);
 	if (ev_stop_) (void) hipEventDestroy(ev_stop_);
 	if (h_pinned_) (void) hipHostFree(h_pinned_);
+	amg_graph_drop();
+	for (hipEvent_t& e : amg_ev_)
+		if (e) (void) hipEventDestroy(e);
+	if (amg_host_) (void) hipHostFree(amg_host_);
 	if (stream_) (void) hipStreamDestroy(stream_);
 }
 
@@ -1418,10 +1423,10 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 	if (o.method < 0 || o.method > 2) return fail(-1, "unknown relaxation method");
 	if (o.method == 2 && !(net_ && net_predict_))
 		return fail(-1, "method 2 needs a warm-start network attached with prediction on");
-	if (o.accel_solver < 0 || o.accel_solver > 1) return fail(-1, "unknown correction solver");
+	if (o.accel_solver < 0 || o.accel_solver > 2) return fail(-1, "unknown correction solver");
 	if (o.method >= 1 && !accel_)
 		if (int rc = accel_setup(0.9)) return rc;
-	amg_select(o.accel_solver == 1);
+	amg_select(o.accel_solver >= 1, o.accel_solver == 2);
 	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr, o.prob_mode)) return rc;
 	if (o.trajectory && (o.voltage_queue < 0 || o.voltage_queue > 1 << 20)) return fail(-1, "bad voltage queue length");
@@ -1443,10 +1448,20 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 		if (int rc = update_coordination()) return rc;
 		coord_done_ = true;
 	}
+	using clk = std::chrono::steady_clock;
+	auto t_last = clk::now();
+	// host wall time since the previous mark into phase p (the phases end
+	// with a stream synchronisation or a blocking readback, except where noted)
+	auto mark = [&](int p) {
+		const auto t = clk::now();
+		stat_phase_ms_[p] += std::chrono::duration<double, std::milli>(t - t_last).count();
+		t_last = t;
+	};
 	for (int ev = 0; ev < k; ev++) {
 		if (surface_.empty()) break;
 		EventRecord& r = out[ev];
 		std::memset(&r, 0, sizeof(r));
+		stat_phase_events_++;
 		const float scaling_reference = (float) scaling_ref_;
 		apex_of_surface(r.apex);
 		int node, nt = 0;
@@ -1480,20 +1495,25 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 		}
 		if (int rc = potential(node, &r.pot_before)) return rc;
 		if (int rc = field_o1(1, &node, r.field_before)) return rc;
+		mark(kPhaseChoice);
 		if (int rc = take_away(node)) return rc;
 		if (o.update_coordination)
 			if (int rc = update_coordination()) return rc;
+		mark(kPhaseTakeAway);
 		if (o.method >= 1) {
 			const bool warm = net_ && (net_record_ || o.method == 2);
 			warm_.n = 0;
 			if (warm)
 				if (int rc = warm_begin(node, r.pot_before, o.method == 2, &r)) return rc;
+			mark(kPhaseWarm);
 			int outer;
 			double fr;
 			if (int rc = accel_relax(o.accel_tol, &outer, &r.global_iter, &fr)) return rc;
 			r.local_iter = outer;
+			mark(kPhaseRelax);
 			if (warm)
 				if (int rc = warm_end()) return rc;
+			mark(kPhaseWarm);
 		} else {
 			if (int rc = local_relax(node, o.relax_shell, o.relax_threshold, o.relax_cycle, o.relax_queue, &r.local_iter)) return rc;
 			for (int c = 0; c < o.relax_global_cycles; c++)
@@ -1503,6 +1523,7 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 				if (int rc = relax_threshold(o.refresh_threshold, o.refresh_cycle, o.refresh_queue, &r.global_iter, &fd, &fs, &ld)) return rc;
 			}
 		}
+		mark(kPhaseRelax);
 		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode)) return rc;
 		r.surface_size = (int) surface_.size();
 		// V) the ion's flight in the relaxed field (process.cpp:940-975) and
@@ -1511,6 +1532,7 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 			return fail(-8, "Configuration::Table::operator[]()");
 		if (int rc = potential(node, &r.pot_after)) return rc;
 		if (int rc = field_o1(1, &node, r.field_after)) return rc;
+		mark(kPhaseSurface);
 		float flen = 0.0f;
 		for (int c = 0; c < 3; c++) flen += (float) r.field_before[c] * (float) r.field_before[c];
 		flen = std::sqrt(flen);
@@ -1527,6 +1549,7 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 			r.voltage = 0.0f;
 			r.time_scale = 1.0f;
 		}
+		mark(kPhaseFlight);
 		*done = ev + 1;
 		if (apex0_ - r.apex[2] >= o.shrink_limit) break;
 	}
@@ -1713,10 +1736,17 @@ int Grid::accel_relax(double tol, int* outer, long long* inner, double* final_re
 	HIPCHK(hipEventRecord(ev_start_, stream_));
 	for (;; o++) {
 		double red[2];
-		if (launch_resid64(stream_, a, phi, d_r32_.as<float>(), d_redout_.as<double>())) return fail(-2, "residual launch failed");
-		HIPCHK(hipMemcpyAsync(red, d_redout_.ptr(), sizeof(red), hipMemcpyDeviceToHost, stream_));
-		HIPCHK(hipStreamSynchronize(stream_));
-		stat_resid_++;
+		if (o == 0 && r32_ready_) {  // formed by the warm start's check on these potentials
+			red[0] = r32_rmax_;
+			red[1] = 0.0;
+		} else {
+			if (launch_resid64(stream_, a, phi, d_r32_.as<float>(), d_redout_.as<double>()))
+				return fail(-2, "residual launch failed");
+			HIPCHK(hipMemcpyAsync(red, d_redout_.ptr(), sizeof(red), hipMemcpyDeviceToHost, stream_));
+			HIPCHK(hipStreamSynchronize(stream_));
+			stat_resid_++;
+		}
+		r32_ready_ = false;
 		rmax = red[0];
 		if (red[1] != 0.0) return fail(-3, "accelerated relaxation: NaN residual");
 		if (rmax <= tol) break;
@@ -1838,6 +1868,47 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 			for (uint8_t v : ref) nref += v;
 			*bytes = (double) ne * 8 + (double) nr * (4 + 4 + 4 + 4 + 4) + (double) nref * 4;
 		}
+	} else if (kind == 2) {
+		if (!accel_)
+			if (int rc = accel_setup(0.9)) return rc;
+		if (!amg_on_)
+			if (int rc = amg_setup()) return rc;
+		AmgSolve S{};
+		S.x = accel_args();
+		S.dn = amg_dn_.as<float>();
+		S.p = amg_p_.as<float>();
+		S.q = amg_q_.as<float>();
+		S.part = amg_part_.as<double>();
+		S.st = amg_st_.as<double>();
+		// a live solve state: done cleared, p = the current FP32 residual
+		const double zero = 0.0;
+		HIPCHK(hipMemcpyAsync(S.st + kAmgDone, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
+		HIPCHK(hipMemcpyAsync(S.p, d_r32_.ptr(), sizeof(float) * n, hipMemcpyDeviceToDevice, stream_));
+		if (launch_amg_fine_matvec(stream_, S)) return fail(-2, "multigrid launch failed");
+		HIPCHK(hipEventRecord(ev_start_, stream_));
+		for (int r = 0; r < reps; r++)
+			if (launch_amg_fine_matvec(stream_, S)) return fail(-2, "multigrid launch failed");
+		HIPCHK(hipEventRecord(ev_stop_, stream_));
+		HIPCHK(hipEventSynchronize(ev_stop_));
+		HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+		if (bytes) {
+			// per entry column + weight; per row node, degree, inner degree,
+			// row sum, q written; every referenced p once
+			long long nr = 0, ne = 0;
+			std::vector<uint8_t> ref(n, 0);
+			for (int sl = 0; sl < next_spare_; sl++) {
+				const int node = slot_node_[sl];
+				if (node < 0) continue;
+				nr++;
+				ne += inner_deg_[sl];
+				ref[node] = 1;
+				for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++)
+					if (participates(node, mesh_.idx[k]) && !(flags_[mesh_.idx[k]] & 2)) ref[mesh_.idx[k]] = 1;
+			}
+			long long nref = 0;
+			for (uint8_t v : ref) nref += v;
+			*bytes = (double) ne * 8 + (double) nr * (4 + 4 + 4 + 4 + 4) + (double) nref * 4;
+		}
 	} else
 		return fail(-1, "time_kernel: kind");
 	HIPCHK(hipMemcpyAsync(d_phi0_.ptr(), save0.ptr(), rs * n, hipMemcpyDeviceToDevice, stream_));
@@ -1874,49 +1945,45 @@ int Grid::attach_mpnn(MpnnNet* net, bool record, bool predict)
 int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 {
 	const int n = mesh_.n;
+	// the shell: every node within three hops (BFS), in ascending node order,
+	// with its hop; warm_local_ maps a node to its shell index while the
+	// shell is built (-1 elsewhere; cleared again below)
+	if ((int) warm_local_.size() != n) warm_local_.assign(n, -1);
 	std::vector<int> shell;
 	std::vector<signed char> hop;
 	{
-		std::vector<int> frontier{node}, next;
-		std::vector<int> seen_at;
-		std::set<int> seen{node};
-		shell.push_back(node);
+		std::vector<std::pair<int, signed char>> found{{node, 0}};
+		warm_local_[node] = 0;
+		size_t lo = 0;
 		for (int h = 1; h <= 3; h++) {
-			next.clear();
-			for (int u : frontier)
+			const size_t hi = found.size();
+			for (size_t f = lo; f < hi; f++) {
+				const int u = found[f].first;
 				for (int k = mesh_.ptr[u]; k < mesh_.ptr[u + 1]; k++) {
 					const int j = mesh_.idx[k];
-					if (seen.insert(j).second) {
-						next.push_back(j);
-						shell.push_back(j);
-					}
+					if (warm_local_[j] >= 0) continue;
+					warm_local_[j] = 0;
+					found.push_back({j, (signed char) h});
 				}
-			frontier.swap(next);
+			}
+			lo = hi;
 		}
-		std::sort(shell.begin(), shell.end());
-		// hop of every shell node (BFS distance)
-		hop.assign(shell.size(), 3);
-		std::vector<int> d(1, node), nd;
-		auto local = [&](int g) { return (int) (std::lower_bound(shell.begin(), shell.end(), g) - shell.begin()); };
-		hop[local(node)] = 0;
-		for (int h = 1; h <= 2; h++) {
-			nd.clear();
-			for (int u : d)
-				for (int k = mesh_.ptr[u]; k < mesh_.ptr[u + 1]; k++) {
-					const int li = local(mesh_.idx[k]);
-					if (hop[li] > h) {
-						hop[li] = (signed char) h;
-						nd.push_back(mesh_.idx[k]);
-					}
-				}
-			d.swap(nd);
+		std::sort(found.begin(), found.end());
+		shell.resize(found.size());
+		hop.resize(found.size());
+		for (size_t i = 0; i < found.size(); i++) {
+			shell[i] = found[i].first;
+			hop[i] = found[i].second;
+			warm_local_[shell[i]] = (int) i;
 		}
 	}
 	const int ns = (int) shell.size();
-	auto local = [&](int g) -> int {
-		auto it = std::lower_bound(shell.begin(), shell.end(), g);
-		return (it != shell.end() && *it == g) ? (int) (it - shell.begin()) : -1;
-	};
+	auto local = [&](int g) -> int { return warm_local_[g]; };
+	struct Clear {
+		std::vector<int>& m;
+		const std::vector<int>& s;
+		~Clear() { for (int g : s) m[g] = -1; }
+	} clear_marks{warm_local_, shell};
 	std::vector<double> wsum(ns, 0.0);
 	for (int i = 0; i < ns; i++)
 		for (int k = mesh_.ptr[shell[i]]; k < mesh_.ptr[shell[i] + 1]; k++) wsum[i] += coup_[k];
@@ -2025,6 +2092,9 @@ int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 		if (!(red[0] <= s)) {  // worse than no prediction (or NaN): start from the removal state
 			if (launch_warm_apply(stream_, w, nullptr)) return fail(-2, "warm-start apply launch failed");
 			rec->warm_start = 2;
+		} else if (red[1] == 0.0) {  // the relaxation's first residual is this one
+			r32_ready_ = true;
+			r32_rmax_ = red[0];
 		}
 	} else
 		HIPCHK(hipStreamSynchronize(stream_));

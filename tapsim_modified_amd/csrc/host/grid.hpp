@@ -169,9 +169,22 @@ public:
 	// aggregation multigrid for the correction solve (built on first use;
 	// rows created later are smoothed but not aggregated)
 	int amg_setup();
-	void amg_select(bool on) { amg_want_ = on; }
+	// (eager: every kernel launched on its own instead of replaying the
+	// captured iteration graph; the same kernels in the same order)
+	void amg_select(bool on, bool eager = false) { amg_want_ = on; amg_eager_ = eager; }
 	int amg_levels() const { return (int) amg_levels_.size(); }
 	int time_kernel(int kind, int reps, double* avg_ms, double* bytes);
+	// host wall time of the event loop's phases, accumulated over all events
+	// (ms): choice (with the record's fields before), take-away, warm start,
+	// relaxation, surface evaluation (with the fields after), flight hand-off
+	// and collection
+	enum { kPhaseChoice, kPhaseTakeAway, kPhaseWarm, kPhaseRelax, kPhaseSurface, kPhaseFlight, kPhases };
+	int phase_times(double* ms, int n, long long* events) const
+	{
+		for (int p = 0; p < n && p < kPhases; p++) ms[p] = stat_phase_ms_[p];
+		if (events) *events = stat_phase_events_;
+		return 0;
+	}
 
 	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
 	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done);
@@ -309,12 +322,21 @@ private:
 		float w = 0.0f;
 		DevBuf ptr, col, val, dinv, rptr, ridx, agg, b, x, r;
 	};
-	static constexpr int kAmgCoarsest = 512;
+	static constexpr int kAmgCoarsest = 1024;
 	bool amg_want_ = false, amg_on_ = false;
 	float amg_w0_ = 0.0f;
 	std::vector<std::unique_ptr<AmgLevelHost>> amg_levels_;
 	DevBuf amg_dn_, amg_agg0_, amg_minv_, amg_part_, amg_st_;
 	DevBuf amg_r_, amg_z_, amg_p_, amg_q_, amg_x0_, amg_res_, amg_delta_;
+	bool amg_eager_ = false;
+	// the captured PCG iterations (kAmgBatch per replay) for the current rows
+	static constexpr int kAmgBatch = 2;
+	hipGraph_t amg_graph_ = nullptr;
+	hipGraphExec_t amg_exec_ = nullptr;
+	AccelArgs amg_graph_args_{};  // the row arguments the graph was captured with
+	double* amg_host_ = nullptr;  // pinned readback of the scalars, two batches
+	hipEvent_t amg_ev_[2] = {nullptr, nullptr};
+	void amg_graph_drop();
 	int amg_touch(const std::vector<int>& nodes);
 	int amg_solve(const AccelArgs& x, float f, int* iters, float** delta);
 	// trajectories: mesh arrays for the point location, node-field snapshot
@@ -332,11 +354,18 @@ private:
 	bool net_record_ = false, net_predict_ = false;
 	DevBuf d_w_nodes_, d_w_slots_, d_w_hop_, d_w_wv_, d_w_pre_, d_w_res_, d_w_scale_;
 	int warm_cap_ = 0;
+	std::vector<int> warm_local_;
+	// the FP64 residual of the current potentials is in d_r32_ (max r32_rmax_):
+	// the warm start's check formed it, the next relaxation starts from it
+	bool r32_ready_ = false;
+	double r32_rmax_ = 0.0;  // node -> shell index while a shell is built, else -1
 	WarmArgs warm_{};
 	long long warm_row0_ = 0, warm_edges_ = 0;
 	int warm_unknown_ = 0;
 	int warm_begin(int node, double phi_ref, bool predict, EventRecord* rec);
 	int warm_end();
+	double stat_phase_ms_[kPhases] = {};
+	long long stat_phase_events_ = 0;
 	double stat_accel_ms_ = 0.0;
 	long long stat_cheb_ = 0, stat_resid_ = 0, stat_rho_raised_ = 0;
 	double stat_sweep_ms_ = 0.0;

@@ -14,7 +14,11 @@
 //
 // Level 0 is the grid's own FP32 correction rows (SELL-64, node-indexed
 // vectors); the coarse levels are CSR.  Every reduction is summed in a fixed
-// order (block partials, then one block), so the solve is deterministic.
+// order (block partials, then one block sums them in block order), so the
+// solve is deterministic.  Smoothing steps are fused into the
+// neighbouring kernels: the pre-smoothing from zero (x0 = w b / D) is formed
+// where b is formed and recomputed on the fly at the neighbours, the
+// prolongation is applied on the fly inside the post-smoothing.
 #include "kernels.hpp"
 #include "rows.hpp"
 
@@ -27,7 +31,7 @@ __device__ __forceinline__ double wave_sumd(double v)
 	return v;
 }
 
-// sum of v over the 256-thread block into part[blockIdx.x] (fixed order)
+// the sum of v over the 256-thread block goes to part[blockIdx.x] (fixed order)
 __device__ __forceinline__ void block_sum(double v, double* part)
 {
 	__shared__ double s[4];
@@ -37,187 +41,237 @@ __device__ __forceinline__ void block_sum(double v, double* part)
 	if (threadIdx.x == 0) part[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-// the row operand of this thread on level 0 (G in over the correction rows),
-// as the Chebyshev kernels read it
+// one block sums the partials in block order and applies `mode` to the
+// scalars: 0 p.q -> alpha = rz / p.q; 1 r.r -> rr, one more iteration,
+// done once rr <= tol2 rr0; 2 r.r -> rr, rr0 (the start: tol2 stored, done
+// and the iteration count cleared); 3 r.z -> beta = r.z / rz, rz; 4 r.z -> rz.
+// (A last-block-done fusion into the producing kernel needs a device-scope
+// fence per block, which writes back the L2 and costs more than this launch.)
+__global__ __launch_bounds__(1024) void k_amg_scalars(int nblocks, const double* __restrict__ part, double* __restrict__ st,
+                                                      int mode, double tol2)
+{
+	if (mode != 2 && st[kAmgDone] != 0.0) return;
+	double t = 0.0;
+	for (int b = threadIdx.x; b < nblocks; b += 1024) t += part[b];
+	__shared__ double s[16];
+	t = wave_sumd(t);
+	if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = t;
+	__syncthreads();
+	if (threadIdx.x != 0) return;
+	t = 0.0;
+	for (int k = 0; k < 16; k++) t += s[k];
+	if (mode == 0) {
+		st[kAmgAlpha] = t != 0.0 ? st[kAmgRz] / t : 0.0;
+	} else if (mode == 1) {
+		st[kAmgRr] = t;
+		st[kAmgIters] += 1.0;
+		if (t <= st[kAmgTol2] * st[kAmgRr0] || t != t) st[kAmgDone] = 1.0;
+	} else if (mode == 2) {
+		st[kAmgRr] = t;
+		st[kAmgRr0] = t;
+		st[kAmgTol2] = tol2;
+		st[kAmgIters] = 0.0;
+		st[kAmgDone] = 0.0;
+	} else {
+		if (mode == 3) st[kAmgBeta] = st[kAmgRz] != 0.0 ? t / st[kAmgRz] : 0.0;
+		st[kAmgRz] = t;
+	}
+}
+
+// the solve has met its target: the remaining launches return at once
+__device__ __forceinline__ bool amg_done(const double* st) { return st[kAmgDone] != 0.0; }
+
+// level-0 slot loop: node of this thread's slot (or -1)
+__device__ __forceinline__ int slot_node_of(const AccelArgs& x)
+{
+	const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+	if (slot >= x.nslots) return -1;
+	return x.slot_node[slot];
+}
+
+// the row operand of this thread on level 0 (G in over the correction rows)
 __device__ __forceinline__ int fine_row(const AccelArgs& x, const float* __restrict__ in, float* gin)
 {
 	return accel_row(x, in, gin, nullptr, nullptr, nullptr, nullptr);
 }
 
-// q = A p = D (p - G p); part += p.q
-__global__ __launch_bounds__(256) void k_amg_fine_matvec(AccelArgs x, const float* __restrict__ dn,
-                                                         const float* __restrict__ p, float* __restrict__ q,
-                                                         double* __restrict__ part)
+// r = D r32, delta = 0, x0 = w0 r / D; partials of r.r
+__global__ __launch_bounds__(256) void k_amg_fine_start(AmgSolve S, const float* __restrict__ r32)
 {
-	float g;
-	const int node = fine_row(x, p, &g);
+	const int node = slot_node_of(S.x);
 	double c = 0.0;
 	if (node >= 0) {
-		const float pi = p[node];
-		const float v = dn[node] * (pi - g);
-		q[node] = v;
-		c = (double) pi * (double) v;
-	}
-	block_sum(c, part);
-}
-
-// level-0 slot loop: node of slot i of this thread (or -1)
-__device__ __forceinline__ int slot_node_of(const AccelArgs& x, int* slot_out)
-{
-	const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-	*slot_out = slot;
-	if (slot >= x.nslots) return -1;
-	return x.slot_node[slot];
-}
-
-// delta += alpha p, r -= alpha q; part += r.r
-__global__ __launch_bounds__(256) void k_amg_fine_update(AccelArgs x, const double* __restrict__ st,
-                                                         const float* __restrict__ p, const float* __restrict__ q,
-                                                         float* __restrict__ delta, float* __restrict__ r,
-                                                         double* __restrict__ part)
-{
-	int slot;
-	const int node = slot_node_of(x, &slot);
-	double c = 0.0;
-	if (node >= 0) {
-		const float alpha = (float) st[kAmgAlpha];
-		delta[node] += alpha * p[node];
-		const float rv = r[node] - alpha * q[node];
-		r[node] = rv;
-		c = (double) rv * (double) rv;
-	}
-	block_sum(c, part);
-}
-
-// r = D r32, delta = 0; part += r.r
-__global__ __launch_bounds__(256) void k_amg_fine_start(AccelArgs x, const float* __restrict__ dn,
-                                                        const float* __restrict__ r32, float* __restrict__ r,
-                                                        float* __restrict__ delta, double* __restrict__ part)
-{
-	int slot;
-	const int node = slot_node_of(x, &slot);
-	double c = 0.0;
-	if (node >= 0) {
-		const float v = dn[node] * r32[node];
-		r[node] = v;
-		delta[node] = 0.0f;
+		const float d = S.dn[node];
+		const float v = d * r32[node];
+		S.r[node] = v;
+		S.delta[node] = 0.0f;
+		S.x0[node] = S.w0 * v / d;
 		c = (double) v * (double) v;
 	}
-	block_sum(c, part);
+	block_sum(c, S.part);
 }
 
-// V-cycle on level 0: pre-smoothing from zero, x0 = w b / D
-__global__ __launch_bounds__(256) void k_amg_fine_pre(AccelArgs x, const float* __restrict__ dn, const float* __restrict__ b,
-                                                      float* __restrict__ x0, float w)
+// q = A p = D (p - G p); partials of p.q
+__global__ __launch_bounds__(256) void k_amg_fine_matvec(AmgSolve S)
 {
-	int slot;
-	const int node = slot_node_of(x, &slot);
-	if (node >= 0) x0[node] = w * b[node] / dn[node];
-}
-
-// res = b - D (x0 - G x0)
-__global__ __launch_bounds__(256) void k_amg_fine_resid(AccelArgs x, const float* __restrict__ dn,
-                                                        const float* __restrict__ b, const float* __restrict__ x0,
-                                                        float* __restrict__ res)
-{
+	if (amg_done(S.st)) return;
 	float g;
-	const int node = fine_row(x, x0, &g);
-	if (node >= 0) res[node] = b[node] - dn[node] * (x0[node] - g);
-}
-
-// x += e[agg] (prolongation from level 1; rows outside every aggregate keep x)
-__global__ __launch_bounds__(256) void k_amg_fine_prolong(AccelArgs x, const int* __restrict__ agg,
-                                                          const float* __restrict__ e, float* __restrict__ xv)
-{
-	int slot;
-	const int node = slot_node_of(x, &slot);
-	if (node < 0) return;
-	const int a = agg[node];
-	if (a >= 0) xv[node] += e[a];
-}
-
-// z = xin + w (b / D - xin + G xin); part += b.z (b is the PCG residual)
-__global__ __launch_bounds__(256) void k_amg_fine_post(AccelArgs x, const float* __restrict__ dn,
-                                                       const float* __restrict__ b, const float* __restrict__ xin,
-                                                       float* __restrict__ z, float w, double* __restrict__ part)
-{
-	float g;
-	const int node = fine_row(x, xin, &g);
+	const int node = fine_row(S.x, S.p, &g);
 	double c = 0.0;
 	if (node >= 0) {
-		const float bi = b[node];
-		const float xi = xin[node];
-		const float v = xi + w * (bi / dn[node] - xi + g);
-		z[node] = v;
+		const float pi = S.p[node];
+		const float v = S.dn[node] * (pi - g);
+		S.q[node] = v;
+		c = (double) pi * (double) v;
+	}
+	block_sum(c, S.part);
+}
+
+// delta += alpha p, r -= alpha q, x0 = w0 r / D; partials of r.r
+__global__ __launch_bounds__(256) void k_amg_fine_update(AmgSolve S)
+{
+	if (amg_done(S.st)) return;
+	const int node = slot_node_of(S.x);
+	double c = 0.0;
+	if (node >= 0) {
+		const float alpha = (float) S.st[kAmgAlpha];
+		S.delta[node] += alpha * S.p[node];
+		const float rv = S.r[node] - alpha * S.q[node];
+		S.r[node] = rv;
+		S.x0[node] = S.w0 * rv / S.dn[node];
+		c = (double) rv * (double) rv;
+	}
+	block_sum(c, S.part);
+}
+
+// res = r - D (x0 - G x0)
+__global__ __launch_bounds__(256) void k_amg_fine_resid(AmgSolve S)
+{
+	if (amg_done(S.st)) return;
+	float g;
+	const int node = fine_row(S.x, S.x0, &g);
+	if (node >= 0) S.res[node] = S.r[node] - S.dn[node] * (S.x0[node] - g);
+}
+
+// x0 += e[agg0] (the coarse correction; rows outside every aggregate keep x0)
+__global__ __launch_bounds__(256) void k_amg_fine_prolong(AmgSolve S, const float* __restrict__ e)
+{
+	if (amg_done(S.st)) return;
+	const int node = slot_node_of(S.x);
+	if (node < 0) return;
+	const int a = S.agg0[node];
+	if (a >= 0) S.x0[node] += e[a];
+}
+
+// z = x0 + w0 (r / D - x0 + G x0); partials of r.z
+__global__ __launch_bounds__(256) void k_amg_fine_post(AmgSolve S)
+{
+	if (amg_done(S.st)) return;
+	float g;
+	const int node = fine_row(S.x, S.x0, &g);
+	double c = 0.0;
+	if (node >= 0) {
+		const float bi = S.r[node];
+		const float xi = S.x0[node];
+		const float v = xi + S.w0 * (bi / S.dn[node] - xi + g);
+		S.z[node] = v;
 		c = (double) bi * (double) v;
 	}
-	block_sum(c, part);
+	block_sum(c, S.part);
 }
 
 // p = z + beta p (beta = 0 on the first iteration)
-__global__ __launch_bounds__(256) void k_amg_fine_dir(AccelArgs x, const double* __restrict__ st,
-                                                      const float* __restrict__ z, float* __restrict__ p, int first)
+__global__ __launch_bounds__(256) void k_amg_fine_dir(AmgSolve S, int first)
 {
-	int slot;
-	const int node = slot_node_of(x, &slot);
+	if (amg_done(S.st)) return;
+	const int node = slot_node_of(S.x);
 	if (node < 0) return;
-	p[node] = first ? z[node] : z[node] + (float) st[kAmgBeta] * p[node];
+	S.p[node] = first ? S.z[node] : S.z[node] + (float) S.st[kAmgBeta] * S.p[node];
 }
 
 // ---- coarse levels (CSR)
 
-__global__ __launch_bounds__(256) void k_amg_pre(int n, const float* __restrict__ dinv, const float* __restrict__ b,
-                                                 float* __restrict__ x0, float w)
+// Coarse rows hold ~20-40 entries: eight lanes share a row (entries k, k+8,
+// ... per lane, then a fixed butterfly over the eight), so the row's column
+// and value loads are contiguous across the lanes.
+constexpr int kCsrLanes = 8;
+
+__device__ __forceinline__ float row_group_sum(float v)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) x0[i] = w * dinv[i] * b[i];
+	v += __shfl_xor(v, 4, 64);
+	v += __shfl_xor(v, 2, 64);
+	v += __shfl_xor(v, 1, 64);
+	return v;
 }
 
-__global__ __launch_bounds__(256) void k_amg_resid(AmgLevel L, const float* __restrict__ b, const float* __restrict__ x0,
-                                                   float* __restrict__ res)
+// pre-smoothing from zero and its residual in one pass: x0 = w Dinv b,
+// res = b - A x0 (x0 at the neighbours recomputed from b)
+__global__ __launch_bounds__(256) void k_amg_pre_resid(const double* __restrict__ st, AmgLevel L, float w,
+                                                       const float* __restrict__ b, float* __restrict__ x0,
+                                                       float* __restrict__ res)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= L.n) return;
+	if (amg_done(st)) return;
+	const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kCsrLanes;
+	const int sub = threadIdx.x % kCsrLanes;
 	float s = 0.0f;
-	for (int k = L.ptr[i]; k < L.ptr[i + 1]; k++) s += L.val[k] * x0[L.col[k]];
-	res[i] = b[i] - s;
+	if (i < L.n)
+		for (int k = L.ptr[i] + sub; k < L.ptr[i + 1]; k += kCsrLanes) {
+			const int j = L.col[k];
+			s += L.val[k] * (w * L.dinv[j] * b[j]);
+		}
+	s = row_group_sum(s);
+	if (i >= L.n || sub != 0) return;
+	const float bi = b[i];
+	x0[i] = w * L.dinv[i] * bi;
+	res[i] = bi - s;
 }
 
-// bc[I] = sum of res over the members of aggregate I (fixed member order)
-__global__ __launch_bounds__(256) void k_amg_restrict(int nc, const int* __restrict__ rptr, const int* __restrict__ ridx,
-                                                      const float* __restrict__ res, float* __restrict__ bc)
+// bc[I] = sum of res over the members of aggregate I (fixed member order per
+// lane, then the butterfly)
+__global__ __launch_bounds__(256) void k_amg_restrict(const double* __restrict__ st, int nc, const int* __restrict__ rptr,
+                                                      const int* __restrict__ ridx, const float* __restrict__ res,
+                                                      float* __restrict__ bc)
 {
-	const int I = blockIdx.x * blockDim.x + threadIdx.x;
-	if (I >= nc) return;
+	if (amg_done(st)) return;
+	const int I = (blockIdx.x * blockDim.x + threadIdx.x) / kCsrLanes;
+	const int sub = threadIdx.x % kCsrLanes;
 	float s = 0.0f;
-	for (int k = rptr[I]; k < rptr[I + 1]; k++) s += res[ridx[k]];
-	bc[I] = s;
+	if (I < nc)
+		for (int k = rptr[I] + sub; k < rptr[I + 1]; k += kCsrLanes) s += res[ridx[k]];
+	s = row_group_sum(s);
+	if (I < nc && sub == 0) bc[I] = s;
 }
 
-__global__ __launch_bounds__(256) void k_amg_prolong(int n, const int* __restrict__ agg, const float* __restrict__ e,
-                                                     float* __restrict__ xv)
+// the coarse correction and the post-smoothing in one pass:
+// xin = x0 + e[agg], xout = xin + w Dinv (b - A xin) (xin at the neighbours
+// recomputed from x0 and e)
+__global__ __launch_bounds__(256) void k_amg_prolong_post(const double* __restrict__ st, AmgLevel L, float w,
+                                                          const int* __restrict__ agg, const float* __restrict__ e,
+                                                          const float* __restrict__ b, const float* __restrict__ x0,
+                                                          float* __restrict__ xout)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
+	if (amg_done(st)) return;
+	const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kCsrLanes;
+	const int sub = threadIdx.x % kCsrLanes;
+	float s = 0.0f;
+	if (i < L.n)
+		for (int k = L.ptr[i] + sub; k < L.ptr[i + 1]; k += kCsrLanes) {
+			const int j = L.col[k];
+			const int a = agg[j];
+			s += L.val[k] * (a >= 0 ? x0[j] + e[a] : x0[j]);
+		}
+	s = row_group_sum(s);
+	if (i >= L.n || sub != 0) return;
 	const int a = agg[i];
-	if (a >= 0) xv[i] += e[a];
-}
-
-// xout = xin + w dinv (b - A xin)
-__global__ __launch_bounds__(256) void k_amg_post(AmgLevel L, const float* __restrict__ b, const float* __restrict__ xin,
-                                                  float* __restrict__ xout, float w)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= L.n) return;
-	float s = 0.0f;
-	for (int k = L.ptr[i]; k < L.ptr[i + 1]; k++) s += L.val[k] * xin[L.col[k]];
-	xout[i] = xin[i] + w * L.dinv[i] * (b[i] - s);
+	const float xi = a >= 0 ? x0[i] + e[a] : x0[i];
+	xout[i] = xi + w * L.dinv[i] * (b[i] - s);
 }
 
 // x = Minv b on the coarsest level (one wavefront per row, fixed-order sum)
-__global__ __launch_bounds__(256) void k_amg_dense(int n, const float* __restrict__ minv, const float* __restrict__ b,
-                                                   float* __restrict__ xv)
+__global__ __launch_bounds__(256) void k_amg_dense(const double* __restrict__ st, int n, const float* __restrict__ minv,
+                                                   const float* __restrict__ b, float* __restrict__ xv)
 {
+	if (amg_done(st)) return;
 	const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
 	const int lane = threadIdx.x & 63;
 	if (row >= n) return;
@@ -227,107 +281,76 @@ __global__ __launch_bounds__(256) void k_amg_dense(int n, const float* __restric
 	if (lane == 0) xv[row] = s;
 }
 
-// the PCG scalars from block partials (one block): mode 0 p.q -> alpha;
-// 1 r.r -> rr (2: also rr0); 3 r.z -> beta and rz (4: first rz)
-__global__ __launch_bounds__(256) void k_amg_scalars(int nblocks, const double* __restrict__ part, double* __restrict__ st,
-                                                     int mode)
-{
-	double v = 0.0;
-	for (int b = threadIdx.x; b < nblocks; b += 256) v += part[b];
-	__shared__ double s[4];
-	v = wave_sumd(v);
-	if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
-	__syncthreads();
-	if (threadIdx.x != 0) return;
-	const double t = (s[0] + s[1]) + (s[2] + s[3]);
-	if (mode == 0) {
-		st[kAmgAlpha] = t != 0.0 ? st[kAmgRz] / t : 0.0;
-	} else if (mode == 1 || mode == 2) {
-		st[kAmgRr] = t;
-		if (mode == 2) st[kAmgRr0] = t;
-	} else {
-		if (mode == 3) st[kAmgBeta] = st[kAmgRz] != 0.0 ? t / st[kAmgRz] : 0.0;
-		st[kAmgRz] = t;
-	}
-}
-
 }  // namespace dev
 
 static inline int blocks_of(long long n) { return (int) ((n + 255) / 256); }
+static inline int csr_blocks(long long rows) { return blocks_of(rows * dev::kCsrLanes); }
 
 int amg_fine_blocks(const AccelArgs& x) { return row_launch_blocks(x.nslots, x.nlong); }
 int amg_slot_blocks(const AccelArgs& x) { return blocks_of(x.nslots); }
 
-int launch_amg_start(hipStream_t s, const AccelArgs& x, const float* dn, const float* r32, float* r, float* delta,
-                     double* part, double* st)
+int launch_amg_start(hipStream_t s, const AmgSolve& S, const float* r32, double tol2)
 {
-	hipLaunchKernelGGL(dev::k_amg_fine_start, dim3(amg_slot_blocks(x)), dim3(256), 0, s, x, dn, r32, r, delta, part);
-	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(256), 0, s, amg_slot_blocks(x), (const double*) part, st, 2);
+	hipLaunchKernelGGL(dev::k_amg_fine_start, dim3(amg_slot_blocks(S.x)), dim3(256), 0, s, S, r32);
+	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(1024), 0, s, amg_slot_blocks(S.x), (const double*) S.part, S.st, 2,
+	                   tol2);
 	return (int) hipGetLastError();
 }
 
-// z = V(b): one V-cycle over the levels; lv[0] is the first coarse level,
-// lv[nlev-1] the coarsest (solved with minv); bl, xl, rl are per-level
-// right-hand side, iterate and scratch vectors
-int launch_amg_vcycle(hipStream_t s, const AccelArgs& x, const float* dn, const int* agg0, float w0, const AmgLevel* lv,
-                      const float* w, int nlev, const float* b, float* x0, float* res, float* z, float* const* bl,
-                      float* const* xl, float* const* rl, const float* minv, double* part)
+// z = V(r): S.x0 holds the level-0 pre-smoothing (formed with r); lv[0] is
+// the first coarse level, lv[nlev-1] the coarsest (solved with minv)
+int launch_amg_vcycle(hipStream_t s, const AmgSolve& S, int mode)
 {
-	const int fb = amg_fine_blocks(x), sb = amg_slot_blocks(x);
-	hipLaunchKernelGGL(dev::k_amg_fine_pre, dim3(sb), dim3(256), 0, s, x, dn, b, x0, w0);
-	hipLaunchKernelGGL(dev::k_amg_fine_resid, dim3(fb), dim3(256), 0, s, x, dn, b, (const float*) x0, res);
-	const float* rfine = res;
-	for (int l = 0; l < nlev; l++) {
-		const AmgLevel& L = lv[l];
-		hipLaunchKernelGGL(dev::k_amg_restrict, dim3(blocks_of(L.n)), dim3(256), 0, s, L.n, L.rptr, L.ridx, rfine, bl[l]);
-		if (l == nlev - 1) {
-			hipLaunchKernelGGL(dev::k_amg_dense, dim3((L.n + 3) / 4), dim3(256), 0, s, L.n, minv, (const float*) bl[l],
-			                   xl[l]);
+	const int fb = amg_fine_blocks(S.x), sb = amg_slot_blocks(S.x);
+	hipLaunchKernelGGL(dev::k_amg_fine_resid, dim3(fb), dim3(256), 0, s, S);
+	const float* rfine = S.res;
+	for (int l = 0; l < S.nlev; l++) {
+		const AmgLevel& L = S.lv[l];
+		hipLaunchKernelGGL(dev::k_amg_restrict, dim3(csr_blocks(L.n)), dim3(256), 0, s, (const double*) S.st, L.n, L.rptr, L.ridx, rfine,
+		                   S.bl[l]);
+		if (l == S.nlev - 1) {
+			hipLaunchKernelGGL(dev::k_amg_dense, dim3((L.n + 3) / 4), dim3(256), 0, s, (const double*) S.st, L.n, S.minv,
+			                   (const float*) S.bl[l], S.xl[l]);
 			break;
 		}
-		hipLaunchKernelGGL(dev::k_amg_pre, dim3(blocks_of(L.n)), dim3(256), 0, s, L.n, L.dinv, (const float*) bl[l], xl[l],
-		                   w[l]);
-		hipLaunchKernelGGL(dev::k_amg_resid, dim3(blocks_of(L.n)), dim3(256), 0, s, L, (const float*) bl[l],
-		                   (const float*) xl[l], rl[l]);
-		rfine = rl[l];
+		hipLaunchKernelGGL(dev::k_amg_pre_resid, dim3(csr_blocks(L.n)), dim3(256), 0, s, (const double*) S.st, L, S.w[l],
+		                   (const float*) S.bl[l], S.xl[l], S.rl[l]);
+		rfine = S.rl[l];
 	}
-	// up: the final iterate of level l+1 corrects level l, then one smoothing
-	// step writes level l's final iterate into its scratch vector
-	const float* fin = xl[nlev - 1];
-	for (int l = nlev - 2; l >= 0; l--) {
-		const AmgLevel& L = lv[l];
-		const AmgLevel& C = lv[l + 1];
-		hipLaunchKernelGGL(dev::k_amg_prolong, dim3(blocks_of(L.n)), dim3(256), 0, s, L.n, C.agg, fin, xl[l]);
-		hipLaunchKernelGGL(dev::k_amg_post, dim3(blocks_of(L.n)), dim3(256), 0, s, L, (const float*) bl[l],
-		                   (const float*) xl[l], rl[l], w[l]);
-		fin = rl[l];
+	// up: level l+1's final iterate corrects level l inside its
+	// post-smoothing, whose result goes to level l's scratch vector
+	const float* fin = S.xl[S.nlev - 1];
+	for (int l = S.nlev - 2; l >= 0; l--) {
+		const AmgLevel& L = S.lv[l];
+		hipLaunchKernelGGL(dev::k_amg_prolong_post, dim3(csr_blocks(L.n)), dim3(256), 0, s, (const double*) S.st, L, S.w[l], S.lv[l + 1].agg,
+		                   fin, (const float*) S.bl[l], (const float*) S.xl[l], S.rl[l]);
+		fin = S.rl[l];
 	}
-	hipLaunchKernelGGL(dev::k_amg_fine_prolong, dim3(sb), dim3(256), 0, s, x, agg0, fin, x0);
-	hipLaunchKernelGGL(dev::k_amg_fine_post, dim3(fb), dim3(256), 0, s, x, dn, b, (const float*) x0, z, w0, part);
+	hipLaunchKernelGGL(dev::k_amg_fine_prolong, dim3(sb), dim3(256), 0, s, S, fin);
+	hipLaunchKernelGGL(dev::k_amg_fine_post, dim3(fb), dim3(256), 0, s, S);
+	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(1024), 0, s, fb, (const double*) S.part, S.st, mode, 0.0);
 	return (int) hipGetLastError();
 }
 
-int launch_amg_scalars(hipStream_t s, int nblocks, const double* part, double* st, int mode)
+int launch_amg_dir(hipStream_t s, const AmgSolve& S, int first)
 {
-	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(256), 0, s, nblocks, part, st, mode);
+	hipLaunchKernelGGL(dev::k_amg_fine_dir, dim3(amg_slot_blocks(S.x)), dim3(256), 0, s, S, first);
 	return (int) hipGetLastError();
 }
 
-int launch_amg_dir(hipStream_t s, const AccelArgs& x, const double* st, const float* z, float* p, int first)
+int launch_amg_fine_matvec(hipStream_t s, const AmgSolve& S)
 {
-	hipLaunchKernelGGL(dev::k_amg_fine_dir, dim3(amg_slot_blocks(x)), dim3(256), 0, s, x, st, z, p, first);
+	hipLaunchKernelGGL(dev::k_amg_fine_matvec, dim3(amg_fine_blocks(S.x)), dim3(256), 0, s, S);
 	return (int) hipGetLastError();
 }
 
-int launch_amg_matvec_update(hipStream_t s, const AccelArgs& x, const float* dn, const float* p, float* q, float* delta,
-                             float* r, double* part, double* st)
+int launch_amg_matvec_update(hipStream_t s, const AmgSolve& S)
 {
-	const int fb = amg_fine_blocks(x), sb = amg_slot_blocks(x);
-	hipLaunchKernelGGL(dev::k_amg_fine_matvec, dim3(fb), dim3(256), 0, s, x, dn, p, q, part);
-	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(256), 0, s, fb, (const double*) part, st, 0);
-	hipLaunchKernelGGL(dev::k_amg_fine_update, dim3(sb), dim3(256), 0, s, x, (const double*) st, p, (const float*) q, delta,
-	                   r, part);
-	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(256), 0, s, sb, (const double*) part, st, 1);
+	const int fb = amg_fine_blocks(S.x), sb = amg_slot_blocks(S.x);
+	hipLaunchKernelGGL(dev::k_amg_fine_matvec, dim3(fb), dim3(256), 0, s, S);
+	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(1024), 0, s, fb, (const double*) S.part, S.st, 0, 0.0);
+	hipLaunchKernelGGL(dev::k_amg_fine_update, dim3(sb), dim3(256), 0, s, S);
+	hipLaunchKernelGGL(dev::k_amg_scalars, dim3(1), dim3(1024), 0, s, sb, (const double*) S.part, S.st, 1, 0.0);
 	return (int) hipGetLastError();
 }
 

@@ -135,23 +135,43 @@ struct AmgLevel {
 	const int* agg;
 };
 // slots of the PCG scalars in device memory
-enum { kAmgAlpha = 0, kAmgBeta = 1, kAmgRz = 2, kAmgRr = 3, kAmgRr0 = 4, kAmgScalars = 8 };
-int amg_fine_blocks(const AccelArgs&);
-int amg_slot_blocks(const AccelArgs&);
-// r = D r32, delta = 0, rr0 = rr = r.r
-int launch_amg_start(hipStream_t, const AccelArgs&, const float* dn, const float* r32, float* r, float* delta,
-                     double* part, double* st);
-// z = V(b) (part receives the block partials of b.z)
-int launch_amg_vcycle(hipStream_t, const AccelArgs&, const float* dn, const int* agg0, float w0, const AmgLevel* lv,
-                      const float* w, int nlev, const float* b, float* x0, float* res, float* z, float* const* bl,
-                      float* const* xl, float* const* rl, const float* minv, double* part);
-// mode 3: beta = b.z / rz, rz = b.z; mode 4: rz = b.z
-int launch_amg_scalars(hipStream_t, int nblocks, const double* part, double* st, int mode);
+// (kAmgDone: set once r.r <= kAmgTol2 * rr0; every later kernel of the
+// solve returns at once; kAmgIters counts the completed iterations)
+enum { kAmgAlpha = 0, kAmgBeta = 1, kAmgRz = 2, kAmgRr = 3, kAmgRr0 = 4, kAmgDone = 5, kAmgTol2 = 6, kAmgIters = 7,
+       kAmgScalars = 8 };
+// one PCG solve's device state: level 0 (the grid's FP32 correction rows,
+// node-indexed vectors), the coarse levels lv[0..nlev) with their Jacobi
+// weights w and per-level vectors (right-hand side, iterate, scratch), the
+// coarsest level's dense inverse, the block partials of the reductions, the
+// scalars
+struct AmgSolve {
+	AccelArgs x;
+	const float* dn;   // row coupling sums D
+	const int* agg0;   // level-0 node -> aggregate of lv[0]
+	float w0;
+	const AmgLevel* lv;
+	const float* w;
+	int nlev;
+	float *r, *z, *p, *q, *x0, *res, *delta;
+	float* const* bl;
+	float* const* xl;
+	float* const* rl;
+	const float* minv;
+	double* part;
+	double* st;
+};
+// r = D r32, delta = 0, x0 = w0 r / D (the V-cycle's pre-smoothing), rr0 = rr = r.r,
+// the stopping target tol2 (on r.r / rr0), done = 0, iterations = 0
+int launch_amg_start(hipStream_t, const AmgSolve&, const float* r32, double tol2);
+// z = V(r); mode 3: beta = r.z / rz, rz = r.z; mode 4: rz = r.z
+int launch_amg_vcycle(hipStream_t, const AmgSolve&, int mode);
 // p = z + beta p (first: p = z)
-int launch_amg_dir(hipStream_t, const AccelArgs&, const double* st, const float* z, float* p, int first);
-// q = A p, alpha = rz / p.q, delta += alpha p, r -= alpha q, rr = r.r
-int launch_amg_matvec_update(hipStream_t, const AccelArgs&, const float* dn, const float* p, float* q, float* delta,
-                             float* r, double* part, double* st);
+int launch_amg_dir(hipStream_t, const AmgSolve&, int first);
+// q = A p, alpha = rz / p.q, delta += alpha p, r -= alpha q, x0 = w0 r / D, rr = r.r,
+// done once rr <= tol2 rr0 (every launch here is a no-op once done is set)
+int launch_amg_matvec_update(hipStream_t, const AmgSolve&);
+// q = A p and the block partials of p.q alone (the kernel's timing)
+int launch_amg_fine_matvec(hipStream_t, const AmgSolve&);
 
 // ---------------------------------------------------------------- warm start
 // the shell of an evaporated atom as the network's graph (rows of the

@@ -19,37 +19,36 @@ __device__ __forceinline__ bf16_t to_bf16(float f)
 	return (bf16_t) (u >> 16);
 }
 
-// scaled FP64 residual J(phi)_i - phi_i of a row (entry order, one thread)
-__device__ double row_residual(const SweepArgs& a, int slot, int node, const double* phi)
+// the scaled FP64 residual J(phi)_i - phi_i of every shell row, one
+// wavefront per row (lane l takes entries l, l+64, ...; a fixed butterfly
+// sums the lanes), and phi_pre; Dirichlet shell nodes get 0
+__global__ __launch_bounds__(256) void k_warm_resid(WarmArgs w, SweepArgs a)
 {
-	const int deg = a.slot_deg[slot];
-	const long long base = a.slice_off[slot >> 6] + (slot & 63);
-	const double* coup = (const double*) a.coup;
-	double lv = 0.0;
-	for (int k = 0; k < deg; k++) {
-		double adj = phi[a.col[base + 64ll * k]];
-		adj *= coup[base + 64ll * k];
-		lv += adj;
+	const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (i >= w.n) return;
+	const int node = w.nodes[i];
+	const int slot = w.slots[i];
+	double acc = 0.0;
+	if (slot >= 0) {
+		const int deg = a.slot_deg[slot];
+		const long long base = a.slice_off[slot >> 6] + (slot & 63);
+		const double* coup = (const double*) a.coup;
+		for (int k = lane; k < deg; k += 64) acc += w.phi[a.col[base + 64ll * k]] * coup[base + 64ll * k];
 	}
-	lv -= a.q[slot];
-	lv /= a.ws[slot];
-	return lv - phi[node];
+	for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+	if (lane != 0) return;
+	const double p = w.phi[node];
+	w.phi_pre[i] = p;
+	w.resid[i] = slot >= 0 ? (acc - a.q[slot]) / a.ws[slot] - p : 0.0;
 }
 
-// one workgroup: residuals over the shell, their maximum s, the features
-__global__ __launch_bounds__(1024) void k_warm_features(WarmArgs w, SweepArgs a)
+// one workgroup: the residuals' maximum s over the shell, the features
+__global__ __launch_bounds__(1024) void k_warm_features(WarmArgs w)
 {
 	__shared__ double sred[16];
 	double rmax = 0.0;
-	for (int i = threadIdx.x; i < w.n; i += blockDim.x) {
-		const int node = w.nodes[i];
-		const double p = w.phi[node];
-		w.phi_pre[i] = p;
-		double r = 0.0;
-		if (w.slots[i] >= 0) r = row_residual(a, w.slots[i], node, w.phi);
-		w.resid[i] = r;
-		rmax = fmax(rmax, fabs(r));
-	}
+	for (int i = threadIdx.x; i < w.n; i += blockDim.x) rmax = fmax(rmax, fabs(w.resid[i]));
 	for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
 	if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = rmax;
 	__syncthreads();
@@ -108,7 +107,8 @@ __global__ void k_warm_targets(WarmArgs w)
 int launch_warm_features(hipStream_t s, const WarmArgs& w, const SweepArgs& a)
 {
 	if (w.n <= 0) return 0;
-	hipLaunchKernelGGL(dev::k_warm_features, dim3(1), dim3(1024), 0, s, w, a);
+	hipLaunchKernelGGL(dev::k_warm_resid, dim3((w.n + 3) / 4), dim3(256), 0, s, w, a);
+	hipLaunchKernelGGL(dev::k_warm_features, dim3(1), dim3(1024), 0, s, w);
 	return (int) hipGetLastError();
 }
 

@@ -68,24 +68,27 @@ def test_multigrid_matches_chebyshev_and_is_deterministic(small_tip):
     """On a 24 k-node tip (several coarse levels): both correction solves reach
     1e-8 V and agree within 2e-8 V / (1 - rho) everywhere (two solutions whose
     Jacobi residuals are below 1e-8 V, rho the spectral radius of the Jacobi
-    operator); two multigrid runs are bit-identical (fixed-order reductions)."""
+    operator).  Two multigrid runs replaying the captured iteration graph and
+    one launching every kernel eagerly are bit-identical (fixed-order
+    reductions; every replay of every refinement enters the result)."""
     mesh, ids = small_tip
     species = tsg.species_table(bench.SPECIES)
-    runs = []
-    for solver in (0, 1, 1):
-        log("small tip relaxation, solver %d" % solver)
+    runs = {}
+    for name in ("chebyshev", "graph", "graph2", "eager"):
+        log("small tip relaxation: %s" % name)
         g = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
         rho = g.accel_setup(0.9)
-        g.accel_multigrid(solver == 1)
+        g.accel_multigrid(name != "chebyshev", eager=name == "eager")
         outer, inner, res = g.accel_relax(1e-8)
-        log("solver %d: levels %d outer %d inner %d residual %.2e"
-            % (solver, g.accel_multigrid(solver == 1), outer, inner, res))
+        log("%s: levels %d outer %d inner %d residual %.2e"
+            % (name, g.accel_multigrid(name != "chebyshev", eager=name == "eager"), outer, inner, res))
         assert res <= 1e-8
-        runs.append(g.phi(0))
-    diff = np.max(np.abs(runs[0] - runs[1]))
+        runs[name] = g.phi(0)
+    diff = np.max(np.abs(runs["chebyshev"] - runs["graph"]))
     log("chebyshev vs multigrid: max difference %.2e V (bound %.2e)" % (diff, 2e-8 / (1.0 - rho)))
     assert diff < 2e-8 / (1.0 - rho)
-    assert np.array_equal(runs[1].view(np.uint64), runs[2].view(np.uint64))
+    assert np.array_equal(runs["graph"].view(np.uint64), runs["graph2"].view(np.uint64))
+    assert np.array_equal(runs["graph"].view(np.uint64), runs["eager"].view(np.uint64))
 
 
 def test_multigrid_events_follow_plain_sequence(small_tip):
