@@ -165,6 +165,10 @@ struct Mesh3 {
 	const int* tv;
 	const int* tn;
 	int m;
+	// one wavefront runs the (uniform) calculation together: the four face
+	// tests of a walk step and the five determinants of the weights go to
+	// lanes 0-4; every lane holds the same results
+	bool wave = false;
 	__device__ const float* P(int i) const { return xyz + 3 * (size_t) i; }
 };
 
@@ -199,8 +203,34 @@ __device__ void rand_unstage(const DevRand& d, int* dst)
 // findTetrahedron walk from t (geometry_3d.cpp:988-1044); -1 outside the
 // mesh, -2 when the walk does not end.  Each step starts at face
 // rand() % 4 when a stream is given, else at face 0.
+// the walk with the four face tests of a step on lanes 0-3 (face lo + lane):
+// the first negative face in the order lo, lo+1, ... is the lowest set lane
+// of the ballot, so the path and the draws are those of the one-lane walk
+__device__ int find_tet_wave(const Mesh3& g, const float* pos, int t, DevRand* rng)
+{
+	const int lane = threadIdx.x & 63;
+	const int limit = min(g.m, 1 << 20);
+	for (int loop = 0; loop < limit; loop++) {
+		if (t < 0) return -1;
+		const int* v = g.tv + 4 * (size_t) t;
+		const int lo = rng ? rng->next() % 4 : 0;
+		bool neg = false;
+		if (lane < 4) {
+			const int index = lo + lane;
+			const int n1 = v[index % 4], n2 = v[3 - (2 + index) % 4], n3 = v[(2 + index) % 4];
+			neg = orient_sign_f(g.P(n2), g.P(n1), g.P(n3), pos) < 0;
+		}
+		const unsigned long long mask = __ballot(neg);
+		if (mask == 0) return t;
+		const int k = __ffsll((unsigned long long) mask) - 1;
+		t = g.tn[4 * (size_t) t + 3 - (lo + k) % 4];
+	}
+	return -2;
+}
+
 __device__ int find_tet(const Mesh3& g, const float* pos, int t, DevRand* rng = nullptr)
 {
+	if (g.wave) return find_tet_wave(g, pos, t, rng);
 	const int limit = min(g.m, 1 << 20);
 	for (int loop = 0; loop < limit; loop++) {
 		if (t < 0) return -1;
@@ -248,8 +278,50 @@ __device__ float tet_volume(const Mesh3& g, int t)
 	return fabsf(value);
 }
 
+// the weights with the determinant of vertex i's opposite sub-tet on lane i
+// (0-3) and the volume on lane 4, gathered to every lane
+__device__ void tet_weights_wave(const Mesh3& g, int t, const float* p, float* w)
+{
+	const int lane = threadIdx.x & 63;
+	const int* v = g.tv + 4 * (size_t) t;
+	float mine = 0.0f;
+	bool ok = true;
+	if (lane < 4) {
+		const int i = lane;
+		float lumat[9];
+		for (int j = 0; j < 3; j++)
+			for (int k = 0; k < 3; k++) lumat[j * 3 + k] = g.P(v[(i + j) % 4])[k] - p[k];
+		float d;
+		int idx[3];
+		if (!lu_decmp<float, 3>(lumat, idx, &d)) {
+			ok = false;
+		} else {
+			float x = lu_det_f(lumat, d);
+			x /= 6.0f;
+			mine = fabsf(x);
+		}
+	} else if (lane == 4) {
+		mine = tet_volume(g, t);
+	}
+	const float volume = __shfl(mine, 4, 64);
+	if (volume == 0.0f) {
+		for (int i = 0; i < 4; i++) w[i] = 0.25f;
+		return;
+	}
+	float wi = 0.0f;
+	if (ok) {
+		wi = mine;
+		wi /= volume;
+	}
+	for (int i = 0; i < 4; i++) w[(i + 3) % 4] = __shfl(wi, i, 64);
+}
+
 __device__ void tet_weights(const Mesh3& g, int t, const float* p, float* w)
 {
+	if (g.wave) {
+		tet_weights_wave(g, t, p, w);
+		return;
+	}
 	const float volume = tet_volume(g, t);
 	if (volume == 0.0f) {
 		for (int i = 0; i < 4; i++) w[i] = 0.25f;
@@ -502,9 +574,10 @@ __device__ void put(const TrajArgs& a, int ion, int k, const Phase& q)
 
 // Trajectory_3d::init / integrate(delta, errorLimit) / extrapolate(Z, zmax)
 // (trajectory_3d.cpp:99-126, 130-205, 303-332) for ion `ion` of the batch
-__device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng)
+__device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = false)
 {
 	Tr tr;
+	tr.g.wave = wave;
 	tr.g.xyz = a.xyz;
 	tr.g.tv = a.tv;
 	tr.g.tn = a.tn;
@@ -786,9 +859,12 @@ __device__ float exit_potential(const Mesh3& g, const float* p, int guess, int w
 	return value;
 }
 
+// one wavefront: every lane runs the flight's uniform calculation (the same
+// values, the stream's table in LDS updated with the same words), lanes 0-4
+// share the face tests and the weights' determinants
 __global__ __launch_bounds__(64) void k_flight(FlightArgs f)
 {
-	if (blockIdx.x != 0 || threadIdx.x != 0) return;
+	if (blockIdx.x != 0) return;
 	__shared__ int table[31];
 	DevRand rng;
 	rand_stage(&rng, table, f.t.rng);
@@ -799,13 +875,14 @@ __global__ __launch_bounds__(64) void k_flight(FlightArgs f)
 		for (int c = 0; c < 3; c++) f.v0[c] = v[c];
 		a.v0 = f.v0;
 	}
-	run_ion(a, 0, &rng);
+	run_ion(a, 0, &rng, true);
 	f.exit[0] = 0.0f;
 	f.exit[1] = 0.0f;
 	if (f.exit_walks > 0 && a.summary && a.summary[11] != 0.0f) {
 		const float p[3] = {a.summary[7], a.summary[8], a.summary[9]};
 		const int guess = (int) a.summary[10];
 		Mesh3 g;
+		g.wave = true;
 		g.xyz = a.xyz;
 		g.tv = a.tv;
 		g.tn = a.tn;
