@@ -114,18 +114,15 @@ def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, sweeps_per_event, budget_s=20
 
 
 def run_events(grid, opts, steps, tag, quiet=False):
-    """steps events in four calls (the flights of consecutive events overlap
-    within one call)"""
+    """steps events in one call: the flight of each event overlaps the next
+    event, and the call returns once the last flight is collected"""
     s0 = grid.sweep_stats()
     p0, e0 = grid.phase_times()
     t0 = time.perf_counter()
-    events = []
-    chunk = max(1, (steps + 3) // 4)
-    while len(events) < steps:
-        events += grid.evaporate(opts, min(chunk, steps - len(events)))
-        if not quiet:
-            log("%s event %d/%d" % (tag, len(events), steps))
+    events = list(grid.evaporate(opts, steps))
     dt = time.perf_counter() - t0
+    if not quiet:
+        log("%s: %d events in %.2fs" % (tag, len(events), dt))
     s1 = grid.sweep_stats()
     p1, e1 = grid.phase_times()
     s1["phases_ms_per_event"] = {k: (p1[k] - p0[k]) / max(1, e1 - e0) for k in p1}
@@ -182,7 +179,7 @@ def main():
     ap.add_argument("--train-events", type=int, default=200, help="events recorded for training the network")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 run (default: steps)")
-    ap.add_argument("--solver", default="chebyshev", choices=("chebyshev", "multigrid", "multigrid-eager"),
+    ap.add_argument("--solver", default="multigrid", choices=("chebyshev", "multigrid", "multigrid-eager"),
                     help="FP32 correction solve of the accelerated paths (multigrid-eager: every kernel "
                          "launched on its own instead of replaying the captured iteration graph)")
     ap.add_argument("--no-cpu", action="store_true")

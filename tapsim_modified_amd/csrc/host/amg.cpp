@@ -387,7 +387,9 @@ int Grid::amg_solve(const AccelArgs& x, float f, int* iters, float** delta)
 	}
 	// batches run ahead of the check: batch k's scalars are read back while
 	// batch k+1 runs (once the target is met every launch returns at once)
-	const int max_batches = 400 / kAmgBatch;
+	// (a solve that stalls in FP32 stops here; the FP64 refinement then
+	// restarts from the true residual)
+	const int max_batches = kAmgMaxIters / kAmgBatch;
 	double* last = nullptr;
 	for (int k = 0; k < max_batches; k++) {
 		if (amg_eager_) {
@@ -404,9 +406,11 @@ int Grid::amg_solve(const AccelArgs& x, float f, int* iters, float** delta)
 		last = amg_host_ + ((k - 1) & 1) * kAmgScalars;
 		if (last[kAmgDone] != 0.0) break;
 	}
+	// the final scalars through the pinned slot, on the grid's stream (a
+	// synchronous copy would go through the null stream)
+	double* fin = amg_host_;
+	HIPCHK(hipMemcpyAsync(fin, S.st, sizeof(double) * kAmgScalars, hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
-	double fin[kAmgScalars];
-	HIPCHK(hipMemcpy(fin, S.st, sizeof(fin), hipMemcpyDeviceToHost));
 	if (!(fin[kAmgRr] == fin[kAmgRr])) return fail(-3, "multigrid: NaN in the correction solve");
 	*iters = (int) fin[kAmgIters];
 	*delta = S.delta;

@@ -218,6 +218,12 @@ Grid::~Grid()
 	if (ev_start_) (void) hipEventDestroy(ev_start_);
 	if (ev_stop_) (void) hipEventDestroy(ev_stop_);
 	if (h_pinned_) (void) hipHostFree(h_pinned_);
+	if (warm_host_) (void) hipHostFree(warm_host_);
+	if (sstream_) (void) hipStreamSynchronize(sstream_);
+	if (sum_src_) (void) hipEventDestroy(sum_src_);
+	if (sum_done_) (void) hipEventDestroy(sum_done_);
+	if (sum_host_) (void) hipHostFree(sum_host_);
+	if (sstream_) (void) hipStreamDestroy(sstream_);
 	amg_graph_drop();
 	for (hipEvent_t& e : amg_ev_)
 		if (e) (void) hipEventDestroy(e);
@@ -984,6 +990,11 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 {
 	if (mode < 0 || mode > 3) return fail(-1, "unknown probability mode");
 	if (mode == 1 && !(temperature_ > 0.0f)) return fail(-1, "Surface_3d::Table::probBoltzmann(): temperature must be positive");
+	// the previous evaluation's normaliser still reads the surface values
+	if (sum_pending_) {
+		HIPCHK(hipEventSynchronize(sum_done_));
+		sum_pending_ = false;
+	}
 	if (int rc = upload_surface()) return rc;
 	const int cnt = (int) surface_vec_.size();
 	if (coord_done_)  // ids may have left the species table (update_coordination)
@@ -1024,8 +1035,24 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 	}
 	if (launch_surface_argmax(stream_, cnt, d_surf_val_.ptr(), f64_, d_argmax_.as<long long>(), d_maxval_.as<double>()))
 		return fail(-2, "argmax launch failed");
-	if (exact_sum && launch_surface_sum(stream_, cnt, d_surf_val_.ptr(), f64_, d_sum_.as<double>()))
-		return fail(-2, "sum launch failed");
+	if (exact_sum) {
+		// the sequential normaliser (one lane, ~1 ms on a large surface) runs on
+		// a side stream; it is read with the event's record (sum_read) and
+		// waited for before the surface values change again
+		if (!sstream_) {
+			HIPCHK(hipStreamCreateWithFlags(&sstream_, hipStreamNonBlocking));
+			HIPCHK(hipEventCreateWithFlags(&sum_src_, hipEventDisableTiming));
+			HIPCHK(hipEventCreateWithFlags(&sum_done_, hipEventDisableTiming));
+			HIPCHK(hipHostMalloc((void**) &sum_host_, sizeof(double), hipHostMallocDefault));
+		}
+		HIPCHK(hipEventRecord(sum_src_, stream_));
+		HIPCHK(hipStreamWaitEvent(sstream_, sum_src_, 0));
+		if (launch_surface_sum(sstream_, cnt, d_surf_val_.ptr(), f64_, d_sum_.as<double>()))
+			return fail(-2, "sum launch failed");
+		HIPCHK(hipMemcpyAsync(sum_host_, d_sum_.ptr(), sizeof(double), hipMemcpyDeviceToHost, sstream_));
+		HIPCHK(hipEventRecord(sum_done_, sstream_));
+		sum_pending_ = true;
+	}
 	int st = 0;
 	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
 	if (mode == 0) HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -1476,22 +1503,18 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 		r.id = ids_[node];
 		r.near_tie = nt;
 		if (int rc = surface_normal(node, r.normal)) return rc;
+		// the chosen site's value now; its probability (value / normaliser)
+		// once the normaliser is in, before the surface is evaluated again
+		double vw = 0.0;
+		float v32 = 0.0f;
 		if (o.exact_probability) {
-			double sum;
-			HIPCHK(hipMemcpyAsync(&sum, d_sum_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
-			double vw;
 			const int slot = (int) (std::lower_bound(surface_vec_.begin(), surface_vec_.end(), node) - surface_vec_.begin());
 			const size_t rs = f64_ ? sizeof(double) : sizeof(float);
-			if (f64_) {
+			if (f64_)
 				HIPCHK(hipMemcpyAsync(&vw, d_surf_val_.as<char>() + rs * slot, sizeof(double), hipMemcpyDeviceToHost, stream_));
-				HIPCHK(hipStreamSynchronize(stream_));
-				r.probability = (float) (vw / sum);
-			} else {
-				float v32;
+			else
 				HIPCHK(hipMemcpyAsync(&v32, d_surf_val_.as<char>() + rs * slot, sizeof(float), hipMemcpyDeviceToHost, stream_));
-				HIPCHK(hipStreamSynchronize(stream_));
-				r.probability = v32 / (float) sum;
-			}
+			HIPCHK(hipStreamSynchronize(stream_));
 		}
 		if (int rc = potential(node, &r.pot_before)) return rc;
 		if (int rc = field_o1(1, &node, r.field_before)) return rc;
@@ -1524,6 +1547,15 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 			}
 		}
 		mark(kPhaseRelax);
+		if (o.exact_probability) {
+			if (!sum_pending_) return fail(-2, "surface normaliser missing");
+			HIPCHK(hipEventSynchronize(sum_done_));
+			const double sum = *sum_host_;
+			if (f64_)
+				r.probability = (float) (vw / sum);
+			else
+				r.probability = v32 / (float) sum;
+		}
 		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode)) return rc;
 		r.surface_size = (int) surface_.size();
 		// V) the ion's flight in the relaxed field (process.cpp:940-975) and
@@ -2028,15 +2060,31 @@ int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 		d_w_scale_.alloc(sizeof(double));
 		if (!d_w_nodes_.ptr() || !d_w_res_.ptr() || !d_w_scale_.ptr()) return fail(-2, "device allocation failed (warm start)");
 	}
-	HIPCHK(hipMemcpyAsync(net.ptr_at(R), lptr.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, stream_));
-	if (ne) {
-		HIPCHK(hipMemcpyAsync(net.col_at(E), lcol.data(), sizeof(int) * ne, hipMemcpyHostToDevice, stream_));
-		HIPCHK(hipMemcpyAsync(net.a_at(E), la.data(), sizeof(float) * ne, hipMemcpyHostToDevice, stream_));
+	// the uploads go through pinned staging (the stream is synchronised
+	// before this function returns, so the staging is free again)
+	const size_t need = 64 + 16 * 7 + sizeof(int) * (ns + 1) + (sizeof(int) + sizeof(float)) * (size_t) ne +
+	                    (sizeof(int) * 2 + 1 + sizeof(float)) * (size_t) ns + 2 * sizeof(double) * 4;
+	if (need > warm_host_bytes_) {
+		if (warm_host_) (void) hipHostFree(warm_host_);
+		warm_host_ = nullptr;
+		warm_host_bytes_ = std::max(need * 2, (size_t) 1 << 20);
+		HIPCHK(hipHostMalloc(&warm_host_, warm_host_bytes_, hipHostMallocDefault));
 	}
-	HIPCHK(hipMemcpyAsync(d_w_nodes_.ptr(), shell.data(), sizeof(int) * ns, hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipMemcpyAsync(d_w_slots_.ptr(), slots.data(), sizeof(int) * ns, hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipMemcpyAsync(d_w_hop_.ptr(), hop.data(), sizeof(signed char) * ns, hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipMemcpyAsync(d_w_wv_.ptr(), wv.data(), sizeof(float) * ns, hipMemcpyHostToDevice, stream_));
+	char* hb = (char*) warm_host_;
+	size_t hoff = 0;
+	auto stage = [&](void* dst, const void* src, size_t bytes) -> int {
+		if (!bytes) return 0;
+		std::memcpy(hb + hoff, src, bytes);
+		const hipError_t e = hipMemcpyAsync(dst, hb + hoff, bytes, hipMemcpyHostToDevice, stream_);
+		hoff += (bytes + 15) & ~(size_t) 15;
+		return e == hipSuccess ? 0 : -1;
+	};
+	if (stage(net.ptr_at(R), lptr.data(), sizeof(int) * (ns + 1)) || stage(net.col_at(E), lcol.data(), sizeof(int) * ne) ||
+	    stage(net.a_at(E), la.data(), sizeof(float) * ne) || stage(d_w_nodes_.ptr(), shell.data(), sizeof(int) * ns) ||
+	    stage(d_w_slots_.ptr(), slots.data(), sizeof(int) * ns) ||
+	    stage(d_w_hop_.ptr(), hop.data(), sizeof(signed char) * ns) || stage(d_w_wv_.ptr(), wv.data(), sizeof(float) * ns))
+		return fail(-2, "warm start: upload failed");
+	double* hscal = (double*) (hb + hoff);  // pinned scalars read back below
 
 	WarmArgs& w = warm_;
 	w.n = ns;
@@ -2071,8 +2119,7 @@ int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 	warm_edges_ = ne;
 	warm_unknown_ = unknown;
 	if (launch_warm_features(stream_, w, sweep_args())) return fail(-2, "warm-start feature launch failed");
-	double s = 0.0;
-	HIPCHK(hipMemcpyAsync(&s, w.scale, sizeof(double), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(hscal, w.scale, sizeof(double), hipMemcpyDeviceToHost, stream_));
 	rec->warm_start = 3;
 	if (predict) {
 		MpnnBatch b;
@@ -2083,10 +2130,11 @@ int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 		if (launch_warm_apply(stream_, w, net.y())) return fail(-2, "warm-start apply launch failed");
 		// the grid's max residual after the prediction
 		SweepArgs a = sweep_args();
-		double red[2];
+		double* red = hscal + 2;
 		if (launch_resid64(stream_, a, w.phi, d_r32_.as<float>(), d_redout_.as<double>())) return fail(-2, "residual launch failed");
-		HIPCHK(hipMemcpyAsync(red, d_redout_.ptr(), sizeof(red), hipMemcpyDeviceToHost, stream_));
+		HIPCHK(hipMemcpyAsync(red, d_redout_.ptr(), 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
 		HIPCHK(hipStreamSynchronize(stream_));
+		const double s = hscal[0];
 		rec->residual_warm = red[0];
 		rec->warm_start = 1;
 		if (!(red[0] <= s)) {  // worse than no prediction (or NaN): start from the removal state
@@ -2098,7 +2146,7 @@ int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 		}
 	} else
 		HIPCHK(hipStreamSynchronize(stream_));
-	rec->residual_cold = s;
+	rec->residual_cold = hscal[0];
 	return 0;
 }
 

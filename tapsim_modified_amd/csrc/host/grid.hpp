@@ -331,6 +331,7 @@ private:
 	bool amg_eager_ = false;
 	// the captured PCG iterations (kAmgBatch per replay) for the current rows
 	static constexpr int kAmgBatch = 2;
+	static constexpr int kAmgMaxIters = 160;
 	hipGraph_t amg_graph_ = nullptr;
 	hipGraphExec_t amg_exec_ = nullptr;
 	AccelArgs amg_graph_args_{};  // the row arguments the graph was captured with
@@ -354,11 +355,18 @@ private:
 	bool net_record_ = false, net_predict_ = false;
 	DevBuf d_w_nodes_, d_w_slots_, d_w_hop_, d_w_wv_, d_w_pre_, d_w_res_, d_w_scale_;
 	int warm_cap_ = 0;
-	std::vector<int> warm_local_;
+	std::vector<int> warm_local_;  // node -> shell index while a shell is built, else -1
+	void* warm_host_ = nullptr;    // pinned staging of the shell's uploads and readbacks
+	size_t warm_host_bytes_ = 0;
 	// the FP64 residual of the current potentials is in d_r32_ (max r32_rmax_):
 	// the warm start's check formed it, the next relaxation starts from it
+	// the surface normaliser on its side stream (surface_evaluate)
+	hipStream_t sstream_ = nullptr;
+	hipEvent_t sum_src_ = nullptr, sum_done_ = nullptr;
+	double* sum_host_ = nullptr;
+	bool sum_pending_ = false;
 	bool r32_ready_ = false;
-	double r32_rmax_ = 0.0;  // node -> shell index while a shell is built, else -1
+	double r32_rmax_ = 0.0;
 	WarmArgs warm_{};
 	long long warm_row0_ = 0, warm_edges_ = 0;
 	int warm_unknown_ = 0;

@@ -23,19 +23,38 @@ __device__ bool field_o1(int node, const float* __restrict__ xyz, const int* __r
 	R b[3] = {0, 0, 0};
 	// the reference forms each of the nine sums in its own loop over the same
 	// ascending neighbour list; the sums are independent, so one pass with
-	// nine accumulators performs exactly the same roundings
-	for (int k = ptr[node]; k < ptr[node + 1]; k++) {
-		if (coup[k] == (R) 0) continue;
-		const int j = idx[k];
-		const V3<R> c = vsub(load_coord<R>(xyz, j), pi);
-		a00 += c.x[0] * c.x[0];
-		a01 += c.x[0] * c.x[1];
-		a02 += c.x[0] * c.x[2];
-		a11 += c.x[1] * c.x[1];
-		a12 += c.x[1] * c.x[2];
-		a22 += c.x[2] * c.x[2];
-		const R dp = phi[j] - p0;
-		for (int i = 0; i < 3; i++) b[i] += dp * c.x[i];
+	// nine accumulators performs exactly the same roundings.  The loads of
+	// eight neighbours are issued before their (ordered) additions.
+	constexpr int U = 8;
+	const int k0 = ptr[node], k1 = ptr[node + 1];
+	for (int kb = k0; kb < k1; kb += U) {
+		int j[U];
+		R cp[U], pj[U];
+		V3<R> xj[U];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const int k = min(kb + u, k1 - 1);
+			j[u] = idx[k];
+			cp[u] = coup[k];
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			xj[u] = load_coord<R>(xyz, j[u]);
+			pj[u] = phi[j[u]];
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (kb + u >= k1 || cp[u] == (R) 0) continue;
+			const V3<R> c = vsub(xj[u], pi);
+			a00 += c.x[0] * c.x[0];
+			a01 += c.x[0] * c.x[1];
+			a02 += c.x[0] * c.x[2];
+			a11 += c.x[1] * c.x[1];
+			a12 += c.x[1] * c.x[2];
+			a22 += c.x[2] * c.x[2];
+			const R dp = pj[u] - p0;
+			for (int i = 0; i < 3; i++) b[i] += dp * c.x[i];
+		}
 	}
 	R a[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
 	R d;
