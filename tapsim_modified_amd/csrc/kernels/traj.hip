@@ -205,25 +205,38 @@ __device__ void rand_unstage(const DevRand& d, int* dst)
 // rand() % 4 when a stream is given, else at face 0.
 // the walk with the four face tests of a step on lanes 0-3 (face lo + lane):
 // the first negative face in the order lo, lo+1, ... is the lowest set lane
-// of the ballot, so the path and the draws are those of the one-lane walk
+// of the ballot, so the path and the draws are those of the one-lane walk.
+// Lane q (0-3) loads vertex q of the tet and the neighbour across face q
+// before the step's draw; each test gathers its three vertices by shuffles.
 __device__ int find_tet_wave(const Mesh3& g, const float* pos, int t, DevRand* rng)
 {
 	const int lane = threadIdx.x & 63;
 	const int limit = min(g.m, 1 << 20);
 	for (int loop = 0; loop < limit; loop++) {
 		if (t < 0) return -1;
-		const int* v = g.tv + 4 * (size_t) t;
-		const int lo = rng ? rng->next() % 4 : 0;
-		bool neg = false;
+		float px = 0.0f, py = 0.0f, pz = 0.0f;
+		int across = -1;
 		if (lane < 4) {
-			const int index = lo + lane;
-			const int n1 = v[index % 4], n2 = v[3 - (2 + index) % 4], n3 = v[(2 + index) % 4];
-			neg = orient_sign_f(g.P(n2), g.P(n1), g.P(n3), pos) < 0;
+			const float* P = g.P(g.tv[4 * (size_t) t + lane]);
+			px = P[0];
+			py = P[1];
+			pz = P[2];
+			across = g.tn[4 * (size_t) t + lane];
 		}
+		const int lo = rng ? rng->next() % 4 : 0;
+		// tetOrient(n1, n2, n3, pos) = orient3d(p2, p1, p3, pos) with
+		// n1 = v[index % 4], n2 = v[3 - (2 + index) % 4], n3 = v[(2 + index) % 4]
+		const int index = lo + (lane & 3);
+		const int i1 = index % 4, i2 = 3 - (2 + index) % 4, i3 = (2 + index) % 4;
+		const float p2[3] = {__shfl(px, i2, 64), __shfl(py, i2, 64), __shfl(pz, i2, 64)};
+		const float p1[3] = {__shfl(px, i1, 64), __shfl(py, i1, 64), __shfl(pz, i1, 64)};
+		const float p3[3] = {__shfl(px, i3, 64), __shfl(py, i3, 64), __shfl(pz, i3, 64)};
+		bool neg = false;
+		if (lane < 4) neg = orient_sign_f(p2, p1, p3, pos) < 0;
 		const unsigned long long mask = __ballot(neg);
 		if (mask == 0) return t;
 		const int k = __ffsll((unsigned long long) mask) - 1;
-		t = g.tn[4 * (size_t) t + 3 - (lo + k) % 4];
+		t = __shfl(across, 3 - (lo + k) % 4, 64);
 	}
 	return -2;
 }
