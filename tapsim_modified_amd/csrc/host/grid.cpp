@@ -1592,10 +1592,11 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 
 namespace tsg {
 
-// algorithmic bytes of one FP64 (or float) Jacobi sweep over the active rows:
-// per entry the column index and the coupling, per row its node index,
-// degree, coupling sum, charge and the written potential, and every
-// referenced potential read once
+// algorithmic bytes of one FP64 (or float) Jacobi sweep over the active rows,
+// B = (4 + R) nnz + (4 + 2R) rows + R referenced (R = 8 for FP64: 12 nnz +
+// 20 rows + 8 referenced): per entry the column index and the coupling, per
+// row its offset, the coupling sum (or charge term) and the written
+// potential, every referenced potential read once
 int Grid::sweep_stats(double* bytes, long long* rows, long long* entries, long long* referenced, double* ms,
                       long long* sweeps, long long* launched) const
 {
@@ -1612,7 +1613,7 @@ int Grid::sweep_stats(double* bytes, long long* rows, long long* entries, long l
 	}
 	long long nref = 0;
 	for (uint8_t v : ref) nref += v;
-	if (bytes) *bytes = (double) ne * (4 + rsz) + (double) nr * (4 + 4 + 8 + 8 + rsz) + (double) nref * rsz;
+	if (bytes) *bytes = (double) ne * (4 + rsz) + (double) nr * (4 + 2 * rsz) + (double) nref * rsz;
 	if (rows) *rows = nr;
 	if (entries) *entries = ne;
 	if (referenced) *referenced = nref;
@@ -1883,8 +1884,9 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 		HIPCHK(hipMemsetAsync(d_d0_.ptr(), 0, sizeof(float) * n, stream_));
 		HIPCHK(hipMemsetAsync(d_d1_.ptr(), 0, sizeof(float) * n, stream_));
 		if (bytes) {
-			// per entry column + weight, per row node, degree, r, delta read and written,
-			// every referenced delta once
+			// the FP32 sweep's 8 nnz + 12 rows + 4 referenced: per entry column +
+			// weight, per row its offset, r and the written iterate, every
+			// referenced iterate once
 			long long nr = 0, ne = 0;
 			std::vector<uint8_t> ref(n, 0);
 			for (int sl = 0; sl < next_spare_; sl++) {
@@ -1898,7 +1900,7 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 			}
 			long long nref = 0;
 			for (uint8_t v : ref) nref += v;
-			*bytes = (double) ne * 8 + (double) nr * (4 + 4 + 4 + 4 + 4) + (double) nref * 4;
+			*bytes = (double) ne * 8 + (double) nr * 12 + (double) nref * 4;
 		}
 	} else if (kind == 2) {
 		if (!accel_)
@@ -1924,8 +1926,9 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 		HIPCHK(hipEventSynchronize(ev_stop_));
 		HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
 		if (bytes) {
-			// per entry column + weight; per row node, degree, inner degree,
-			// row sum, q written; every referenced p once
+			// the FP32 sweep's 8 nnz + 12 rows + 4 referenced: per entry column +
+			// weight, per row its offset, the row sum and q written, every
+			// referenced p once
 			long long nr = 0, ne = 0;
 			std::vector<uint8_t> ref(n, 0);
 			for (int sl = 0; sl < next_spare_; sl++) {
@@ -1939,7 +1942,7 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 			}
 			long long nref = 0;
 			for (uint8_t v : ref) nref += v;
-			*bytes = (double) ne * 8 + (double) nr * (4 + 4 + 4 + 4 + 4) + (double) nref * 4;
+			*bytes = (double) ne * 8 + (double) nr * 12 + (double) nref * 4;
 		}
 	} else
 		return fail(-1, "time_kernel: kind");
