@@ -494,6 +494,51 @@ int main(int argc, char** argv)
 				writeArray("traj_" + tok.at(1) + ".f32", data);
 				writeArray("traj_status_" + tok.at(1) + ".i32", st);
 			}
+			else if (cmd == "writegeom")
+			{
+				// writegeom:NAME:MODE:NUMBERS:POTENTIALS -- File_Io::writeGeomGrid
+				// of the current grid into NAME in the work directory (MODE 0
+				// ASCII, 1 BINARY)
+				File_Io::writeGeomGrid((g_out + "/" + tok.at(1)).c_str(), system.geomTable, system.gridTable,
+				                       std::atoi(tok.at(2).c_str()), std::atoi(tok.at(3).c_str()),
+				                       std::atoi(tok.at(4).c_str()));
+			}
+			else if (cmd == "readgeom")
+			{
+				// readgeom:NAME:TAG -- File_Io::readGeomGrid of NAME in the work
+				// directory into fresh tables; TAG_xyz.f32, TAG_ids.i16,
+				// TAG_numbers.u32, TAG_phi.f32 and TAG_flags.i32 (the reported
+				// withNumbers, withPotentials) hold what it read
+				Geometry_3d::Table geom;
+				Grid_3d::Table grid;
+				int wn = -1, wp = -1;
+				File_Io::readGeomGrid((g_out + "/" + tok.at(1)).c_str(), &geom, &grid, &wn, &wp);
+				const std::string tag = tok.at(2);
+				std::vector<float> xyz, phi;
+				std::vector<short> ids;
+				std::vector<unsigned int> numbers;
+				for (int i = 0; i < grid.numNodes(); i++)
+				{
+					for (int c = 0; c < 3; c++) xyz.push_back(geom.node(i).coords[c]);
+					ids.push_back(grid.node(i).id().toValue());
+					numbers.push_back(grid.node(i).number().toValue());
+					phi.push_back((float) grid.node(i).phi(0));
+				}
+				std::vector<int> flags;
+				flags.push_back(wn);
+				flags.push_back(wp);
+				writeArray(tag + "_xyz.f32", xyz);
+				writeArray(tag + "_ids.i16", ids);
+				writeArray(tag + "_numbers.u32", numbers);
+				writeArray(tag + "_phi.f32", phi);
+				writeArray(tag + "_flags.i32", flags);
+			}
+			else if (cmd == "writeconfig")
+			{
+				// writeconfig:NAME -- File_Io::writeConfig (ASCII) of the
+				// configuration table the driver read
+				File_Io::writeConfig((g_out + "/" + tok.at(1)).c_str(), system.configTable, File_Io::ASCII);
+			}
 			else if (cmd == "coord")
 			{
 				// the fork's update_coordination (surface_3d.cpp:568-593); it

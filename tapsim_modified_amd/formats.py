@@ -1,0 +1,296 @@
+"""TAPSim's on-disk inputs for the hot path: the species configuration
+(`.aly`, ASCII) and the node file (coordinates, ids, optional unique numbers
+and potentials; ASCII or BINARY), read and written as the reference's
+File_Io does (tapsim/file_io.cpp), so that unmodified TAPSim inputs drive
+this library and its outputs can be diffed with the reference's.
+
+    temperature, species = read_config("example.aly")     # {id: dict(...)}
+    table = tapsim_modified_amd.species_table(species)      # -> Grid3D(...)
+    nodes = read_node_file("tip.nodes")                     # xyz, ids, numbers, phi
+"""
+import struct
+
+import numpy as np
+
+# File_Io's configuration keys and units (file_io.cpp:38-53)
+CONFIG_KEYS = (
+    ("TEMPERATURE", "[K]"),
+    ("ID", "[1]"),
+    ("NAME", "[1]"),
+    ("CHARGE_DENSITY", "[C/m³]"),
+    ("DIELECTRICITY", "[1]"),
+    ("REMOVABLE", "[1]"),
+    ("NEUMANN_BOUNDARY", "[1]"),
+    ("DIRICHLET_BOUNDARY", "[1]"),
+    ("POTENTIAL", "[V]"),
+    ("MASS", "[u]"),
+    ("EVAPORATION_CHARGE_STATE", "[1]"),
+    ("EVAPORATION_FIELD_STRENGTH", "[V/m]"),
+    ("EVAPORATION_ACTIVATION_ENERGY", "[eV]"),
+)
+
+# key -> (field of species_table(), parser); name is kept under "name"
+_FIELDS = {
+    "ID": ("id", int),
+    "NAME": ("name", str),
+    "CHARGE_DENSITY": ("charge_density", float),
+    "DIELECTRICITY": ("epsilon", float),
+    "REMOVABLE": ("removable", int),
+    "NEUMANN_BOUNDARY": ("neumann", int),
+    "DIRICHLET_BOUNDARY": ("dirichlet", int),
+    "POTENTIAL": ("phi", float),
+    "MASS": ("mass", float),
+    "EVAPORATION_CHARGE_STATE": ("evap_charge", int),
+    "EVAPORATION_FIELD_STRENGTH": ("evap_field", float),
+    "EVAPORATION_ACTIVATION_ENERGY": ("evap_energy", float),
+}
+
+# Configuration::NodeNumber's invalid value (configuration.h:82-87: numbers
+# start at 1, 0 is invalid)
+INVALID_NUMBER = 0
+
+
+class FormatError(RuntimeError):
+    """The input does not follow the format (the reference throws
+    std::runtime_error("File_Io::...") at the same places)."""
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _parse(kind, text, where):
+    try:
+        if kind is int:
+            return int(text.split()[0])
+        if kind is float:
+            return _f32(float(text.split()[0]))
+        return text
+    except (ValueError, IndexError):
+        raise FormatError("%s: cannot read %r" % (where, text))
+
+
+def read_config(path):
+    """File_Io::readConfig (file_io.cpp:212-463), ASCII files: returns
+    (temperature, {id: fields}) with the fields species_table() takes plus
+    "name".  Blocks are separated by empty lines, lines starting with '#'
+    are skipped, keys and values split at " = "; a block of one key is the
+    temperature, a block of twelve a species."""
+    with open(path, "r", encoding="utf-8", errors="surrogateescape") as f:
+        lines = f.read().split("\n")
+    pos = 0
+    while pos < len(lines) and (not lines[pos] or lines[pos][0] == "#"):
+        pos += 1
+    if pos >= len(lines):
+        raise FormatError("File_Io::readConfig(): no header")
+    if lines[pos] == "BINARY":
+        raise FormatError("File_Io::readConfig(): BINARY configuration files are not read here; write them as ASCII")
+    if lines[pos] != "ASCII":
+        raise FormatError("File_Io::readConfig(): header must be ASCII")
+    pos += 1
+    temperature = 0.0
+    species = {}
+    n = len(lines)
+    while pos < n:
+        while pos < n and (not lines[pos] or lines[pos][0] == "#"):
+            pos += 1
+        block = {}
+        while pos < n and lines[pos]:
+            line = lines[pos]
+            if line[0] != "#":
+                cut = line.find(" = ")
+                if cut < 0:
+                    raise FormatError("File_Io::readConfig(): no ' = ' in %r" % line)
+                key, value = line[:cut].strip(), line[cut + 3:].strip()
+                if key in block:
+                    raise FormatError("File_Io::readConfig(): duplicate key %s" % key)
+                block[key] = value
+            pos += 1
+        if not block:
+            continue
+        if len(block) == 1:
+            key, value = next(iter(block.items()))
+            if key != CONFIG_KEYS[0][0]:
+                raise FormatError("File_Io::readConfig(): a one-line block must be the temperature")
+            temperature = _parse(float, value, key)
+        elif len(block) == 12:
+            entry = {}
+            for key, value in block.items():
+                if key not in _FIELDS:
+                    raise FormatError("File_Io::readConfig(): unknown key %s" % key)
+                field, kind = _FIELDS[key]
+                entry[field] = _parse(kind, value, key)
+            sid = entry.pop("id")
+            if not 0 <= sid <= 127:
+                raise FormatError("File_Io::readConfig(): id %d outside 0..127" % sid)
+            species[sid] = entry
+        else:
+            raise FormatError("File_Io::readConfig(): a block must hold 1 or 12 keys")
+    return temperature, species
+
+
+def _g8(x):
+    """a float as std::ostream with precision(8) writes it (general format)"""
+    return "%.8g" % _f32(x)
+
+
+def write_config(path, temperature, species):
+    """File_Io::writeConfig, ASCII mode (file_io.cpp:116-174): the header
+    comments, then the temperature and every species in ascending id."""
+    out = ["# global parameters", "#", "# %s %s" % CONFIG_KEYS[0], "#", "# species dependant parameters", "#"]
+    out += ["# %s %s" % CONFIG_KEYS[i] for i in range(1, 13)]
+    out += ["ASCII", "", "%s = %s" % (CONFIG_KEYS[0][0], _g8(temperature)), ""]
+    for sid in sorted(species):
+        s = species[sid]
+        out += [
+            "ID = %d" % sid,
+            "NAME = %s" % s.get("name", ""),
+            "CHARGE_DENSITY = %s" % _g8(s.get("charge_density", 0.0)),
+            "DIELECTRICITY = %s" % _g8(s.get("epsilon", 1.0)),
+            "REMOVABLE = %d" % int(bool(s.get("removable", 0))),
+            "NEUMANN_BOUNDARY = %d" % int(bool(s.get("neumann", 0))),
+            "DIRICHLET_BOUNDARY = %d" % int(bool(s.get("dirichlet", 0))),
+            "POTENTIAL = %s" % _g8(s.get("phi", 0.0)),
+            "MASS = %s" % _g8(s.get("mass", 0.0)),
+            "EVAPORATION_CHARGE_STATE = %d" % int(s.get("evap_charge", 0)),
+            "EVAPORATION_FIELD_STRENGTH = %s" % _g8(s.get("evap_field", 0.0)),
+            "EVAPORATION_ACTIVATION_ENERGY = %s" % _g8(s.get("evap_energy", 0.0)),
+            "",
+        ]
+    with open(path, "w", encoding="utf-8", newline="\n") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def _renumber(numbers):
+    """readGeomGrid's renumbering (file_io.cpp:975-1004): valid numbers must
+    be unique and stay; invalid ones get the next numbers after the largest
+    valid one (from 1), in node order."""
+    valid = numbers != INVALID_NUMBER
+    if len(np.unique(numbers[valid])) != int(valid.sum()):
+        raise FormatError("File_Io::readGeomGrid(): non unique numbering detected")
+    nxt = int(numbers[valid].max()) + 1 if valid.any() else 1
+    out = numbers.copy()
+    idx = np.nonzero(~valid)[0]
+    out[idx] = (np.arange(nxt, nxt + len(idx), dtype=np.uint64) & 0xFFFFFFFF).astype(np.uint32)
+    return out
+
+
+def read_node_file(path):
+    """File_Io::readGeomGrid (file_io.cpp:822-1011): the header "ASCII n
+    numbers potentials" or "BINARY ..." (lines starting with '#' before it
+    are skipped), then per node x y z id [number] [phi] (tab separated in
+    ASCII; float32 x3, int16, [uint32], [float32] packed in BINARY).
+    Returns dict(xyz float32 (n,3), ids int16, numbers uint32 (renumbered
+    as the reference does), phi float32 (0 when absent), with_numbers,
+    with_potentials); with_potentials repeats with_numbers, as the
+    reference reports it (file_io.cpp:1008)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    pos = 0
+    while True:
+        end = raw.find(b"\n", pos)
+        if end < 0:
+            raise FormatError("File_Io::readGeomGrid(): no header")
+        line = raw[pos:end].decode("latin-1")
+        pos = end + 1
+        if line and line[0] != "#":
+            break
+    if line.startswith("ASCII"):
+        ascii_mode, rest = True, line[5:]
+    elif line.startswith("BINARY"):
+        ascii_mode, rest = False, line[6:]
+    else:
+        raise FormatError("File_Io::readGeomGrid(): header must start with ASCII or BINARY")
+    try:
+        head = rest.split()
+        n, with_numbers, with_potentials = int(head[0]), int(head[1]) != 0, int(head[2]) != 0
+    except (ValueError, IndexError):
+        raise FormatError("File_Io::readGeomGrid(): bad header %r" % line)
+    xyz = np.zeros((n, 3), np.float32)
+    ids = np.full(n, -1, np.int16)
+    numbers = np.full(n, INVALID_NUMBER, np.uint32)
+    phi = np.zeros(n, np.float32)
+    if ascii_mode:
+        lines = raw[pos:].decode("latin-1").split("\n")
+        li = 0
+        for i in range(n):
+            while li < len(lines) and lines[li][:1] == "#":
+                li += 1
+            if li >= len(lines):
+                raise FormatError("File_Io::readGeomGrid(): file ends at node %d" % i)
+            vals = lines[li].split("\t")
+            li += 1
+            need = 4 + with_numbers + with_potentials
+            if len(vals) < need:
+                raise FormatError("File_Io::readGeomGrid(): node %d has %d values" % (i, len(vals)))
+            xyz[i] = [_parse(float, v, "coords") for v in vals[:3]]
+            ids[i] = _parse(int, vals[3], "id")
+            k = 4
+            if with_numbers:
+                numbers[i] = _parse(int, vals[k], "number")
+                k += 1
+            if with_potentials:
+                phi[i] = _parse(float, vals[k], "potential")
+    else:
+        fmt = "<3fh" + ("I" if with_numbers else "") + ("f" if with_potentials else "")
+        rec = struct.calcsize(fmt)
+        if len(raw) - pos < rec * n:
+            raise FormatError("File_Io::readGeomGrid(): binary body too short")
+        dt = [("xyz", "<f4", 3), ("id", "<i2")]
+        if with_numbers:
+            dt.append(("number", "<u4"))
+        if with_potentials:
+            dt.append(("phi", "<f4"))
+        body = np.frombuffer(raw, dtype=np.dtype(dt), count=n, offset=pos)
+        xyz[:] = body["xyz"]
+        ids[:] = body["id"]
+        if with_numbers:
+            numbers[:] = body["number"]
+        if with_potentials:
+            phi[:] = body["phi"]
+    return dict(xyz=xyz, ids=ids, numbers=_renumber(numbers), phi=phi, with_numbers=with_numbers,
+                with_potentials=with_numbers)
+
+
+def _e8(x):
+    """a float as std::ostream with scientific, showpoint and precision(8)"""
+    return "%.8e" % _f32(x)
+
+
+def write_node_file(path, xyz, ids, numbers=None, phi=None, mode="ASCII"):
+    """File_Io::writeGeomGrid (file_io.cpp:765-820): ASCII (x y z id [number]
+    [phi], tab separated, floats in scientific notation with 8 digits) or
+    BINARY (the same fields packed)."""
+    xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+    ids = np.asarray(ids, np.int16)
+    n = len(xyz)
+    wn, wp = int(numbers is not None), int(phi is not None)
+    head = ("%s %d %d %d\n" % (mode, n, wn, wp)).encode("latin-1")
+    if mode == "ASCII":
+        rows = []
+        for i in range(n):
+            parts = [_e8(xyz[i, 0]), _e8(xyz[i, 1]), _e8(xyz[i, 2]), "%d" % ids[i]]
+            if wn:
+                parts.append("%d" % int(numbers[i]))
+            if wp:
+                parts.append(_e8(phi[i]))
+            rows.append("\t".join(parts) + "\n")
+        body = "".join(rows).encode("latin-1")
+    elif mode == "BINARY":
+        dt = [("xyz", "<f4", 3), ("id", "<i2")]
+        if wn:
+            dt.append(("number", "<u4"))
+        if wp:
+            dt.append(("phi", "<f4"))
+        rec = np.zeros(n, np.dtype(dt))
+        rec["xyz"] = xyz
+        rec["id"] = ids
+        if wn:
+            rec["number"] = np.asarray(numbers, np.uint32)
+        if wp:
+            rec["phi"] = np.asarray(phi, np.float32)
+        body = rec.tobytes()
+    else:
+        raise FormatError("File_Io::writeGeomGrid(): mode must be ASCII or BINARY")
+    with open(path, "wb") as f:
+        f.write(head + body)

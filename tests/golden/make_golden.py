@@ -266,8 +266,61 @@ def make_tetgen_mesh():
     print("tetgen_random: %d tets" % len(tets))
 
 
+def make_formats():
+    """File formats: the reference's own File_Io writes a node file (ASCII
+    with numbers and potentials, BINARY with numbers) and rewrites two
+    configurations (its example.aly and the fixture's); it reads node files
+    written here (ASCII with comment lines and unnumbered nodes, BINARY with
+    potentials) and reports what it read."""
+    from tapsim_modified_amd import formats as F
+    rng = np.random.default_rng(11)
+    n = 300
+    xyz = ((rng.random((n, 3)) - 0.5) * 8e-9).astype(np.float32)
+    ids = rng.choice(np.array([0, 1, 3, 10, 11], np.int16), n)
+    out = {}
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids)
+        # the reference writes
+        run(wd, ["writegeom:w_ascii.txt:0:1:1", "writegeom:w_bin.bin:1:1:0", "writeconfig:w_cfg.aly"])
+        for k in ("w_ascii.txt", "w_bin.bin", "w_cfg.aly"):
+            out[k.replace(".", "_")] = np.frombuffer(open(os.path.join(wd, k), "rb").read(), np.uint8)
+        # the reference reads files written here
+        numbers = (np.arange(n, dtype=np.uint32) * 3 + 7)
+        numbers[rng.choice(n, 40, replace=False)] = F.INVALID_NUMBER
+        phi = (rng.random(n) * 5000.0).astype(np.float32)
+        F.write_node_file(os.path.join(wd, "r_ascii.txt"), xyz, ids, numbers, phi, "ASCII")
+        body = open(os.path.join(wd, "r_ascii.txt"), "rb").read()
+        body = b"# node file written by the fixture generator\n#\n" + body
+        open(os.path.join(wd, "r_ascii.txt"), "wb").write(body)
+        F.write_node_file(os.path.join(wd, "r_bin.bin"), xyz, ids, None, phi, "BINARY")
+        run(wd, ["readgeom:r_ascii.txt:ra", "readgeom:r_bin.bin:rb"])
+        for k in ("r_ascii.txt", "r_bin.bin"):
+            out[k.replace(".", "_")] = np.frombuffer(open(os.path.join(wd, k), "rb").read(), np.uint8)
+        for tag in ("ra", "rb"):
+            out[tag + "_xyz"] = rd(wd, tag + "_xyz.f32", np.float32).reshape(-1, 3)
+            out[tag + "_ids"] = rd(wd, tag + "_ids.i16", np.int16)
+            out[tag + "_numbers"] = rd(wd, tag + "_numbers.u32", np.uint32)
+            out[tag + "_phi"] = rd(wd, tag + "_phi.f32", np.float32)
+            out[tag + "_flags"] = rd(wd, tag + "_flags.i32", np.int32)
+        # the reference's example configuration, as read and rewritten by it
+        example = open("/root/reference/examples/example.aly", "rb").read()
+        out["example_aly"] = np.frombuffer(example, np.uint8)
+        open(os.path.join(wd, "cfg.aly"), "wb").write(example)
+        write_input(os.path.join(wd, "in.bin"), xyz, np.zeros(n, np.int16))  # ids the example defines
+        run(wd, ["writeconfig:x_cfg.aly"])
+        out["x_cfg_aly"] = np.frombuffer(open(os.path.join(wd, "x_cfg.aly"), "rb").read(), np.uint8)
+    out["xyz"], out["ids"] = xyz, ids
+    np.savez_compressed(os.path.join(OUT, "formats.npz"), **out)
+    print("formats: %d nodes, %d bytes of reference output" % (n, sum(v.nbytes for k, v in out.items() if k.startswith(("w_", "x_")))))
+
+
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref"], cwd=ROOT, check=True)
+    if len(sys.argv) > 1 and sys.argv[1] == "formats":
+        make_formats()
+        sys.exit(0)
     make_tetgen_mesh()
     make_small()
+    make_formats()

@@ -1,0 +1,95 @@
+"""TAPSim's file formats against the reference's own File_Io (the fixture
+tests/golden/formats.npz holds files the reference wrote and what it read
+from files written here; tests/golden/make_golden.py makes it)."""
+import os
+
+import numpy as np
+import pytest
+
+from tapsim_modified_amd import formats as F
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def fx():
+    return dict(np.load(os.path.join(HERE, "golden", "formats.npz")))
+
+
+def _file(tmp_path, name, data):
+    p = tmp_path / name
+    p.write_bytes(data.tobytes())
+    return str(p)
+
+
+@pytest.mark.parametrize("name,mode,numbers,potentials", [("w_ascii_txt", "ASCII", True, True),
+                                                          ("w_bin_bin", "BINARY", True, False)])
+def test_node_file_written_as_the_reference_writes_it(fx, tmp_path, name, mode, numbers, potentials):
+    """writeGeomGrid of the fixture's nodes (numbers 1..n, potentials from
+    the configuration): byte for byte, ASCII and BINARY."""
+    n = len(fx["ids"])
+    phi_of = {0: 0.0, 1: 0.0, 3: 0.0, 10: 5000.0, 11: 5000.0}
+    phi = np.array([phi_of[int(i)] for i in fx["ids"]], np.float32)
+    out = str(tmp_path / "mine")
+    F.write_node_file(out, fx["xyz"], fx["ids"], np.arange(1, n + 1, dtype=np.uint32) if numbers else None,
+                      phi if potentials else None, mode)
+    assert open(out, "rb").read() == fx[name].tobytes()
+
+
+@pytest.mark.parametrize("src,tag", [("r_ascii_txt", "ra"), ("r_bin_bin", "rb")])
+def test_node_file_read_as_the_reference_reads_it(fx, tmp_path, src, tag):
+    """readGeomGrid of files written here (ASCII behind comment lines, 40
+    nodes unnumbered; BINARY without numbers): coordinates, ids, numbers
+    after the reference's renumbering, potentials, and the reported flags
+    (withPotentials repeats withNumbers in the reference)."""
+    got = F.read_node_file(_file(tmp_path, "in", fx[src]))
+    assert np.array_equal(got["xyz"].view(np.uint32), fx[tag + "_xyz"].view(np.uint32))
+    assert np.array_equal(got["ids"], fx[tag + "_ids"])
+    assert np.array_equal(got["numbers"], fx[tag + "_numbers"])
+    assert np.array_equal(got["phi"].view(np.uint32), fx[tag + "_phi"].view(np.uint32))
+    assert [int(got["with_numbers"]), int(got["with_potentials"])] == list(fx[tag + "_flags"])
+
+
+def test_node_file_round_trip_of_reference_output(fx, tmp_path):
+    """the reference's own ASCII and BINARY node files read back to the same
+    nodes"""
+    a = F.read_node_file(_file(tmp_path, "a", fx["w_ascii_txt"]))
+    b = F.read_node_file(_file(tmp_path, "b", fx["w_bin_bin"]))
+    assert np.array_equal(a["xyz"], fx["xyz"]) and np.array_equal(b["xyz"], fx["xyz"])
+    assert np.array_equal(a["ids"], fx["ids"]) and np.array_equal(b["ids"], fx["ids"])
+    assert np.array_equal(a["numbers"], np.arange(1, len(fx["ids"]) + 1))
+
+
+@pytest.mark.parametrize("src,ref", [("example_aly", "x_cfg_aly"), ("w_cfg_aly", "w_cfg_aly")])
+def test_config_read_and_rewritten_as_the_reference_does(fx, tmp_path, src, ref):
+    """readConfig of the reference's examples/example.aly (and of the
+    fixture's configuration), then writeConfig: byte for byte the file the
+    reference writes after reading the same input"""
+    t, species = F.read_config(_file(tmp_path, "in.aly", fx[src]))
+    out = str(tmp_path / "out.aly")
+    F.write_config(out, t, species)
+    assert open(out, "rb").read() == fx[ref].tobytes()
+
+
+def test_config_fields_of_the_example(fx, tmp_path):
+    """the example's tungsten entry, field by field"""
+    t, species = F.read_config(_file(tmp_path, "ex.aly", fx["example_aly"]))
+    assert t == 0.0 and sorted(species)[:4] == [0, 1, 2, 3]
+    w = species[10]
+    assert w["name"] == "Tungsten" and w["removable"] == 1 and w["dirichlet"] == 1
+    assert w["epsilon"] == float(np.float32(1.0e6))
+    assert species[2]["neumann"] == 1 and species[2]["phi"] == 1000.0
+
+
+def test_format_errors(tmp_path):
+    bad = tmp_path / "bad"
+    bad.write_text("TEXT 3 0 0\n")
+    with pytest.raises(F.FormatError):
+        F.read_node_file(str(bad))
+    bad.write_text("ASCII\n\nID = 1\n")
+    with pytest.raises(F.FormatError):
+        F.read_config(str(bad))
+    dup = tmp_path / "dup"
+    F.write_node_file(str(dup), np.zeros((2, 3)), np.zeros(2, np.int16), np.array([5, 5], np.uint32), None)
+    with pytest.raises(F.FormatError):
+        F.read_node_file(str(dup))
