@@ -46,7 +46,9 @@
 #include <cmath>
 
 #include "system_3d.h"
+#include <unistd.h>
 #include "file_io.h"
+#include "logResults.h"
 #include "process.h"
 #include "surface_3d.h"
 #include "trajectory_3d.h"
@@ -532,6 +534,46 @@ int main(int argc, char** argv)
 				writeArray(tag + "_numbers.u32", numbers);
 				writeArray(tag + "_phi.f32", phi);
 				writeArray(tag + "_flags.i32", flags);
+			}
+			else if (cmd == "writeresults")
+			{
+				// writeresults:IN:OUT:MODE -- File_Io::LogResults (MODE 0 ASCII,
+				// 1 BINARY) writes the events of IN (27 floats each: index, id,
+				// number, voltage, start xyz, stop xyz, stop time, time scale,
+				// probability, potential and field before, potential and field
+				// after, normal, apex) into OUT of the work directory
+				const std::vector<float> rec = readFloats(tok.at(1));
+				const int K = (int) (rec.size() / 27);
+				{
+					File_Io::LogResults log((g_out + "/" + tok.at(2)).c_str(), std::atoi(tok.at(3).c_str()));
+					for (int e = 0; e < K; e++)
+					{
+						const float* r = &rec[27 * e];
+						File_Io::LogResults::Dataset d;
+						d.simTime = 0.0f;
+						d.eventIndex = (unsigned int) r[0];
+						d.nodeIndex = 0;
+						d.nodeId = Configuration::NodeId::fromValue((short) r[1]);
+						d.nodeNumber = Configuration::NodeNumber::fromValue((unsigned int) r[2]);
+						d.voltage = r[3];
+						Trajectory_3d::phaseVector a(0.0f, MathVector3d<float>(r[4], r[5], r[6]), MathVector3d<float>(0.0f, 0.0f, 0.0f));
+						Trajectory_3d::phaseVector b(r[10], MathVector3d<float>(r[7], r[8], r[9]), MathVector3d<float>(0.0f, 0.0f, 0.0f));
+						const_cast<std::vector<Trajectory_3d::phaseVector>&>(d.trajectory.data()).push_back(a);
+						const_cast<std::vector<Trajectory_3d::phaseVector>&>(d.trajectory.data()).push_back(b);
+						d.timeScale = r[11];
+						d.probability = r[12];
+						d.potentialBefore = r[13];
+						d.fieldBefore = MathVector3d<float>(r[14], r[15], r[16]);
+						d.potentialAfter = r[17];
+						d.fieldAfter = MathVector3d<float>(r[18], r[19], r[20]);
+						d.normal = Geometry_3d::Point(r[21], r[22], r[23]);
+						d.apex = Geometry_3d::Point(r[24], r[25], r[26]);
+						log.push(d);
+					}
+					// the writer thread empties its queue only while it runs; give
+					// it time before the destructor stops it
+					usleep(300000);
+				}
 			}
 			else if (cmd == "writeconfig")
 			{

@@ -294,3 +294,87 @@ def write_node_file(path, xyz, ids, numbers=None, phi=None, mode="ASCII"):
         raise FormatError("File_Io::writeGeomGrid(): mode must be ASCII or BINARY")
     with open(path, "wb") as f:
         f.write(head + body)
+
+
+# ---------------------------------------------------------------- results stream
+
+RESULTS_VERSION = "1.0"  # File_Io::LogResults::_fileVersion (logResults.cpp:34)
+
+# one record per event, in the column order of the results file
+RESULT_FIELDS = [
+    ("index", "<u4"), ("id", "<i2"), ("number", "<u4"), ("voltage", "<f4"), ("start", "<f4", 3), ("stop", "<f4", 3),
+    ("tof", "<f4"), ("probability", "<f4"), ("pot_before", "<f4"), ("field_before", "<f4", 3), ("pot_after", "<f4"),
+    ("field_after", "<f4", 3), ("normal", "<f4", 3), ("apex", "<f4", 3),
+]
+RESULT_DTYPE = np.dtype(RESULT_FIELDS)
+
+_RESULT_COLUMNS = (
+    ("index", "[1]", "int"), ("id", "[1]", "int"), ("number", "[1]", "unsigned int"), ("voltage", "[V]", "float"),
+    ("startX", "[m]", "float"), ("startY", "[m]", "float"), ("startZ", "[m]", "float"),
+    ("stopX", "[m]", "float"), ("stopY", "[m]", "float"), ("stopZ", "[m]", "float"), ("tof", "[s]", "float"),
+    ("probability", "[1]", "float"), ("potentialBefore", "[V]", "float"), ("fieldBeforeX", "[V/m]", "float"),
+    ("fieldBeforeY", "[V/m]", "float"), ("fieldBeforeZ", "[V/m]", "float"), ("potentialAfter", "[V]", "float"),
+    ("fieldAfterX", "[V/m]", "float"), ("fieldAfterY", "[V/m]", "float"), ("fieldAfterZ", "[V/m]", "float"),
+    ("normalX", "[m]", "float"), ("normalY", "[m]", "float"), ("normalZ", "[m]", "float"),
+    ("apexX", "[m]", "float"), ("apexY", "[m]", "float"), ("apexZ", "[m]", "float"),
+)
+
+
+def result_records(events, numbers=None, first_index=1):
+    """Records of the results file from tsg_evaporate's events: the event
+    index counts from first_index (Process::evaporation's ++eventCnt), the
+    unique number is numbers[node] (a node file's numbers) or node + 1."""
+    rec = np.zeros(len(events), RESULT_DTYPE)
+    for k, e in enumerate(events):
+        r = rec[k]
+        r["index"] = first_index + k
+        r["id"] = e.id
+        r["number"] = int(numbers[e.node]) if numbers is not None else e.node + 1
+        r["voltage"] = e.voltage
+        r["start"] = list(e.start)
+        r["stop"] = list(e.stop)
+        r["tof"] = e.tof
+        r["probability"] = e.probability
+        r["pot_before"] = e.pot_before
+        r["field_before"] = list(e.field_before)
+        r["pot_after"] = e.pot_after
+        r["field_after"] = list(e.field_after)
+        r["normal"] = list(e.normal)
+        r["apex"] = list(e.apex)
+    return rec
+
+
+def write_results(path, records, mode="ASCII", header=""):
+    """File_Io::LogResults (logResults.cpp:135-366), one file: "TAPSIM
+    RESULTS DATA", the version, an optional header, the column comments and
+    the mode line, then one line per event (tab separated, floats in
+    scientific notation with six digits) or one packed record per event."""
+    records = np.asarray(records, RESULT_DTYPE)
+    head = ["TAPSIM RESULTS DATA", "VERSION " + RESULTS_VERSION]
+    text = "\n".join(head) + "\n" + header
+    if mode == "BINARY":
+        text += "".join("# %s %s (%s)\n" % c for c in _RESULT_COLUMNS[:12])
+        # the reference labels both potentials "[1]" in binary files
+        text += "# potentialBefore [1] (float)\n"
+        text += "".join("# %s %s (%s)\n" % c for c in _RESULT_COLUMNS[13:16])
+        text += "# potentialAfter [1] (float)\n"
+        text += "".join("# %s %s (%s)\n" % c for c in _RESULT_COLUMNS[17:])
+        text += "BINARY\n"
+        body = records.tobytes()
+    elif mode == "ASCII":
+        text += "".join("# %s %s\n" % c[:2] for c in _RESULT_COLUMNS)
+        text += "ASCII\n"
+        rows = []
+        for r in records:
+            vals = ["%d" % r["index"], "%d" % r["id"], "%d" % r["number"], "%e" % float(r["voltage"])]
+            vals += ["%e" % float(v) for v in r["start"]] + ["%e" % float(v) for v in r["stop"]]
+            vals += ["%e" % float(r[k]) for k in ("tof", "probability", "pot_before")]
+            vals += ["%e" % float(v) for v in r["field_before"]] + ["%e" % float(r["pot_after"])]
+            vals += ["%e" % float(v) for v in r["field_after"]] + ["%e" % float(v) for v in r["normal"]]
+            vals += ["%e" % float(v) for v in r["apex"]]
+            rows.append("\t".join(vals) + "\n")
+        body = "".join(rows).encode("latin-1")
+    else:
+        raise FormatError("File_Io::LogResults::header(): Invalid output mode!")
+    with open(path, "wb") as f:
+        f.write(text.encode("latin-1") + body)

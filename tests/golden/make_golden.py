@@ -310,6 +310,19 @@ def make_formats():
         write_input(os.path.join(wd, "in.bin"), xyz, np.zeros(n, np.int16))  # ids the example defines
         run(wd, ["writeconfig:x_cfg.aly"])
         out["x_cfg_aly"] = np.frombuffer(open(os.path.join(wd, "x_cfg.aly"), "rb").read(), np.uint8)
+        # the reference's results stream of 12 synthetic events (27 floats each)
+        K = 12
+        rec = np.zeros((K, 27), np.float32)
+        rec[:, 0] = np.arange(1, K + 1)
+        rec[:, 1] = rng.choice([10, 11], K)
+        rec[:, 2] = rng.integers(1, 100000, K)
+        rec[:, 3:] = (rng.standard_normal((K, 24)) * 10.0 ** rng.integers(-12, 11, (K, 24))).astype(np.float32)
+        rec[:, 10] = np.abs(rec[:, 10])
+        rec.tofile(os.path.join(wd, "res_in.f32"))
+        run(wd, ["writeresults:res_in.f32:res_ascii.txt:0", "writeresults:res_in.f32:res_bin.bin:1"])
+        out["res_in"] = rec
+        for k in ("res_ascii.txt", "res_bin.bin"):
+            out[k.replace(".", "_")] = np.frombuffer(open(os.path.join(wd, k), "rb").read(), np.uint8)
     out["xyz"], out["ids"] = xyz, ids
     np.savez_compressed(os.path.join(OUT, "formats.npz"), **out)
     print("formats: %d nodes, %d bytes of reference output" % (n, sum(v.nbytes for k, v in out.items() if k.startswith(("w_", "x_")))))

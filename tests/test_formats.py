@@ -93,3 +93,33 @@ def test_format_errors(tmp_path):
     F.write_node_file(str(dup), np.zeros((2, 3)), np.zeros(2, np.int16), np.array([5, 5], np.uint32), None)
     with pytest.raises(F.FormatError):
         F.read_node_file(str(dup))
+
+
+def _records_from_driver_input(rec):
+    """the fixture's 27-float events as result records (tof = stop time x
+    time scale in float, as the reference forms it when writing)"""
+    out = np.zeros(len(rec), F.RESULT_DTYPE)
+    out["index"] = rec[:, 0].astype(np.uint32)
+    out["id"] = rec[:, 1].astype(np.int16)
+    out["number"] = rec[:, 2].astype(np.uint32)
+    out["voltage"] = rec[:, 3]
+    out["start"] = rec[:, 4:7]
+    out["stop"] = rec[:, 7:10]
+    out["tof"] = rec[:, 10] * rec[:, 11]
+    out["probability"] = rec[:, 12]
+    out["pot_before"] = rec[:, 13]
+    out["field_before"] = rec[:, 14:17]
+    out["pot_after"] = rec[:, 17]
+    out["field_after"] = rec[:, 18:21]
+    out["normal"] = rec[:, 21:24]
+    out["apex"] = rec[:, 24:27]
+    return out
+
+
+@pytest.mark.parametrize("mode,name", [("ASCII", "res_ascii_txt"), ("BINARY", "res_bin_bin")])
+def test_results_stream_written_as_the_reference_writes_it(fx, tmp_path, mode, name):
+    """File_Io::LogResults of 12 events (values over 24 decades, both
+    signs): header, column comments and records, byte for byte"""
+    out = str(tmp_path / "res")
+    F.write_results(out, _records_from_driver_input(fx["res_in"]), mode)
+    assert open(out, "rb").read() == fx[name].tobytes()
