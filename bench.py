@@ -188,7 +188,7 @@ def main():
                          "plus a device copy of known size that calibrates the counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)", file=sys.stderr)
+        print("bench.py runs on one GPU: the library has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import tapsim_modified_amd as tsg
