@@ -378,3 +378,73 @@ def write_results(path, records, mode="ASCII", header=""):
         raise FormatError("File_Io::LogResults::header(): Invalid output mode!")
     with open(path, "wb") as f:
         f.write(text.encode("latin-1") + body)
+
+
+# ---------------------------------------------------------------- tapsim.ini
+
+def read_ini(path, delimiter=" = "):
+    """File_Io::readInitialization (file_io.cpp:58-94): "KEY = value" lines,
+    lines starting with '#' and empty lines skipped, keys and values
+    trimmed; a line without the delimiter or a repeated key is an error.
+    As in the reference, a last line without its newline is not read."""
+    with open(path, "r", encoding="utf-8", errors="surrogateescape") as f:
+        text = f.read()
+    values = {}
+    for line in text.split("\n")[:-1]:
+        if not line or line[0] == "#":
+            continue
+        cut = line.find(delimiter)
+        if cut < 0:
+            raise FormatError("File_Io::readInitialization(): no %r in %r" % (delimiter, line))
+        key, value = line[:cut].strip(), line[cut + len(delimiter):].strip()
+        if key in values:
+            raise FormatError("File_Io::readInitialization(): duplicate key %s" % key)
+        values[key] = value
+    return values
+
+
+# EvaporationOptions::setDefaults (process.cpp:340-440): ini key -> (field of
+# tsg_evap_options, parser)
+_EVAP_KEYS = (
+    ("METHOD_EVAPORATION_PROBABILITY", "prob_mode", int),
+    ("METHOD_EVAPORATION_CHOICE", "evap_mode", int),
+    ("TRAJECTORY_INTEGRATOR_TYPE", "integrator", int),
+    ("TRAJECTORY_STEPPER_TYPE", "stepper", int),
+    ("TRAJECTORY_INITIAL_TIME_STEP", "dt0", float),
+    ("TRAJECTORY_MINIMUM_TIME_STEP", "dt_min", float),
+    ("TRAJECTORY_ERROR_THRESHOLD", "err_limit", float),
+    ("VOLTAGE_QUEUE_SIZE", "voltage_queue", int),
+    ("LOCAL_RELAXATION_ERROR_THRESHOLD", "relax_threshold", float),
+    ("LOCAL_RELAXATION_SHELL_SIZE", "relax_shell", int),
+    ("LOCAL_RELAXATION_CYCLE_SIZE", "relax_cycle", int),
+    ("LOCAL_RELAXATION_QUEUE_SIZE", "relax_queue", int),
+    ("GLOBAL_RELAXATION_STEPS", "relax_global_cycles", int),
+    ("REFRESH_RELAXATION_INTERVAL", "refresh_interval", int),
+    ("REFRESH_RELAXATION_ERROR_THRESHOLD", "refresh_threshold", float),
+    ("REFRESH_RELAXATION_CYCLE_SIZE", "refresh_cycle", int),
+    ("REFRESH_RELAXATION_QUEUE_SIZE", "refresh_queue", int),
+)
+
+
+def evap_options_from_ini(values, species):
+    """The hot path's evaporation options from tapsim.ini values (read_ini),
+    as EvaporationOptions::setDefaults reads them (process.cpp:340-440;
+    floats through float, a missing or unreadable key is an error): a dict
+    of tsg_evap_options fields, with the vacuum id looked up by
+    VACUUM_CELL_IDENTIFIER among the configuration's names, the trajectory
+    on, and the thermal start velocity unless TRAJECTORY_NO_INITIAL_VELOCITY."""
+    out = {}
+    for key, field, kind in _EVAP_KEYS:
+        if key not in values:
+            raise FormatError("EvaporationOptions::setDefaults(): '%s' missing" % key)
+        out[field] = _parse(kind, values[key], key)
+    name = values.get("VACUUM_CELL_IDENTIFIER")
+    ids = [sid for sid, s in species.items() if s.get("name") == name]
+    if name is None or len(ids) != 1:
+        raise FormatError("EvaporationOptions::setDefaults(): 'VACUUM_CELL_IDENTIFIER' error!")
+    out["vacuum_id"] = ids[0]
+    if "TRAJECTORY_NO_INITIAL_VELOCITY" not in values:
+        raise FormatError("EvaporationOptions::setDefaults(): 'TRAJECTORY_NO_INITIAL_VELOCITY' missing")
+    out["thermal_velocity"] = 0 if _parse(int, values["TRAJECTORY_NO_INITIAL_VELOCITY"], "velocity") else 1
+    out["trajectory"] = 1
+    return out

@@ -303,6 +303,8 @@ def make_formats():
             out[tag + "_numbers"] = rd(wd, tag + "_numbers.u32", np.uint32)
             out[tag + "_phi"] = rd(wd, tag + "_phi.f32", np.float32)
             out[tag + "_flags"] = rd(wd, tag + "_flags.i32", np.int32)
+        # the reference's own tapsim.ini (data)
+        out["tapsim_ini"] = np.frombuffer(open("/root/reference/tapsim.ini", "rb").read(), np.uint8)
         # the reference's example configuration, as read and rewritten by it
         example = open("/root/reference/examples/example.aly", "rb").read()
         out["example_aly"] = np.frombuffer(example, np.uint8)

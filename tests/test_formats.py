@@ -123,3 +123,34 @@ def test_results_stream_written_as_the_reference_writes_it(fx, tmp_path, mode, n
     out = str(tmp_path / "res")
     F.write_results(out, _records_from_driver_input(fx["res_in"]), mode)
     assert open(out, "rb").read() == fx[name].tobytes()
+
+
+def test_ini_and_evaporation_options_of_the_reference_file(fx, tmp_path):
+    """the reference's own tapsim.ini: every key read, and the hot path's
+    options as EvaporationOptions::setDefaults forms them (floats through
+    float; the vacuum id from the configuration's names)"""
+    vals = F.read_ini(_file(tmp_path, "tapsim.ini", fx["tapsim_ini"]))
+    text = fx["tapsim_ini"].tobytes().decode()
+    keys = [ln.split(" = ")[0].strip() for ln in text.split("\n")[:-1] if ln and ln[0] != "#"]
+    assert sorted(vals) == sorted(keys)
+    _, species = F.read_config(_file(tmp_path, "ex.aly", fx["example_aly"]))
+    o = F.evap_options_from_ini(vals, species)
+    assert o["vacuum_id"] == 0 and o["prob_mode"] == int(vals["METHOD_EVAPORATION_PROBABILITY"])
+    assert o["relax_threshold"] == float(np.float32(float(vals["LOCAL_RELAXATION_ERROR_THRESHOLD"])))
+    assert o["dt0"] == float(np.float32(1e-14)) and o["dt_min"] == float(np.float32(1e-16))
+    assert o["relax_shell"] == 3 and o["voltage_queue"] == 250 and o["thermal_velocity"] == 0
+    import tapsim_modified_amd as tsg
+    opts = tsg.EvapOptions()
+    for k, v in o.items():
+        setattr(opts, k, v)
+    assert opts.relax_cycle == int(vals["LOCAL_RELAXATION_CYCLE_SIZE"])
+
+
+def test_ini_quirks(tmp_path):
+    """a last line without its newline is not read; a repeated key is an error"""
+    p = tmp_path / "a.ini"
+    p.write_text("A = 1\n# c\n\nB = 2")
+    assert F.read_ini(str(p)) == {"A": "1"}
+    p.write_text("A = 1\nA = 2\n")
+    with pytest.raises(F.FormatError):
+        F.read_ini(str(p))
