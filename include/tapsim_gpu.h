@@ -132,6 +132,10 @@ int tsg_reset_potentials(tsg_grid*, int kind, const float* center, float pmin, f
 int tsg_potential(tsg_grid*, int node, double* out);
 /* Grid_3d::Table::field_o1(i) (grid_3d.cpp:909-991), 3 values per node */
 int tsg_field_o1(tsg_grid*, int count, const int* nodes, double* out);
+/* Grid_3d::Table::field_o2(i, geometry, &charge) (grid_3d.cpp:993-1199): the
+   quadratic least-squares field (3 values per node into out) and the charge
+   -eps0 (phi_xx + phi_yy + phi_zz) of the fit (one per node into charge) */
+int tsg_field_o2(tsg_grid*, int count, const int* nodes, double* out, double* charge);
 /* the host mirror of Node::coupling for every directed entry of the CSR */
 int tsg_get_couplings(const tsg_grid*, double* out);
 /* Node::charge of every node (computeCharge, grid_3d.cpp:1373-1390: Voronoi
@@ -213,6 +217,10 @@ typedef struct tsg_evap_options
 	                             0 Chebyshev-Jacobi, 1 conjugate gradients preconditioned by one V-cycle of
 	                             aggregation multigrid (iterations replayed from a captured HIP graph), 2 the
 	                             same launched kernel by kernel; the FP64 refinement and its target are the same */
+	int init_event_cnt;       /* EvaporationOptions::initEventCnt (process.h:98): the event index the run
+	                             starts after; the grid keeps counting across tsg_evaporate calls, so events
+	                             evaporated in chunks get the indices (and refresh relaxations,
+	                             process.cpp:876) of one uninterrupted run */
 } tsg_evap_options;
 
 typedef struct tsg_event
@@ -236,6 +244,7 @@ typedef struct tsg_event
 	float time_scale;         /* sqrt((phi_after - phi(exit)) / voltage) */
 	float tof;                /* time of the last trajectory point times time_scale */
 	int traj_len, traj_status;
+	long long event_index;    /* LogResults::Dataset::eventIndex (process.cpp:696): initEventCnt + events so far */
 } tsg_event;
 
 /* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k

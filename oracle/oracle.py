@@ -34,7 +34,7 @@ class OraEvapOpts(ctypes.Structure):
         ("refresh_queue", ctypes.c_int), ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int),
         ("stepper", ctypes.c_int), ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_uint), ("shrink_limit", ctypes.c_float), ("inplace_order", ctypes.c_int),
-        ("thermal_velocity", ctypes.c_int), ("update_coordination", ctypes.c_int),
+        ("thermal_velocity", ctypes.c_int), ("update_coordination", ctypes.c_int), ("init_event_cnt", ctypes.c_int),
     ]
 
 
@@ -48,7 +48,7 @@ class OraEvent(ctypes.Structure):
         ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int),
         ("local_iter", ctypes.c_longlong), ("refresh_iter", ctypes.c_longlong),
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_float), ("apex", ctypes.c_float * 3),
-        ("normal", ctypes.c_float * 3), ("traj_root_diff", ctypes.c_int),
+        ("normal", ctypes.c_float * 3), ("traj_root_diff", ctypes.c_int), ("event_index", ctypes.c_longlong),
     ]
 
 

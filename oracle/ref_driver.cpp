@@ -31,6 +31,7 @@
  *   vorarea:TAG          voronoiArea(i,j,1.0f) both directions for every edge
  *   evap:...             the steps I-VI of Process::evaporation (process.cpp:680-1037)
  *   traj:TAG:NODE:INT:STEP  one Trajectory_3d from a node (trajectory_3d.cpp:99)
+ *   randreset:SEED       std::srand(SEED), then Grid_3d::Table::randomReset (grid_3d.cpp:811)
  */
 
 #include <cstdio>
@@ -287,6 +288,11 @@ int main(int argc, char** argv)
 				catch (Grid_3d::Table::RelaxationFault& e) { it = -1 - e.iteration(); }
 				relaxIterations.push_back(it);
 				writeArray("local_iter.i64", relaxIterations);
+			}
+			else if (cmd == "randreset")
+			{
+				std::srand((unsigned int) std::atoi(tok.at(1).c_str()));
+				system.gridTable.randomReset(system.configTable);
 			}
 			else if (cmd == "evapnode")
 			{

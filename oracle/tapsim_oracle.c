@@ -67,6 +67,15 @@ struct ora_sys
 	int coord_done;        /* the run-start update_coordination has been applied */
 	int vacuum_id;
 	double scaling_ref;
+	/* Process::evaporation state kept across ora_evaporate calls, so that a
+	   run evaporated in chunks equals one uninterrupted run: the event counter
+	   (process.cpp:676-696), the voltage queue (:983-1010), the initial apex */
+	long long event_cnt;   /* -1 until the first call sets it from initEventCnt */
+	float vq[4096];
+	int vq_head, vq_size;
+	float vq_sum;
+	float apex0;
+	int apex0_set;
 };
 
 /* Configuration::Table::operator[] (configuration.cpp:213-231) throws for an
@@ -221,6 +230,8 @@ ora_sys* ora_create(int mode, int n, const float* xyz, const short* ids,
 {
 	if (n <= 0 || m <= 0 || (mode != ORA_F32 && mode != ORA_F64)) { fail(ORA_E_ARG, "bad arguments"); return 0; }
 	ora_sys* s = (ora_sys*) calloc(1, sizeof(ora_sys));
+	if (!s) return NULL;
+	s->event_cnt = -1;
 	s->mode = mode;
 	s->n = n;
 	s->m = m;
@@ -1154,7 +1165,9 @@ static int stepper_step(traj* tr, float* delta, float err_limit, phase* error, p
 		na = adjacent_tets(tr->s, tr->data[tr->len - 1].tet, allowed, 2048);
 	for (;;)
 	{
-		if (*delta < tr->dt_min) *delta = tr->dt_min;
+		/* NONE integrates with the step as given (trajectory_3d.cpp:155-158);
+		   the other steppers raise it to the step limit first (:362,432,458) */
+		if (tr->stepper != 0 && *delta < tr->dt_min) *delta = tr->dt_min;
 		phase value = do_integration(tr, *delta, error);
 		if (tr->stepper == 0) { *out = value; return 0; }
 		if (tr->stepper == 2)
@@ -1618,14 +1631,20 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 	ora_specific_fields(s);
 	e = ora_compute_probabilities(s, o->prob_mode);
 	if (e) return e;
-	float vq[4096];
-	int vq_head = 0, vq_size = 0;
-	float vq_sum = 0.0f;
+	float* vq = s->vq;
 	const float zmax = zmax_of(s);
-	float apex0 = -FLT_MAX;
-	for (int k = 0; k < s->nsurf; k++)
-		if (s->xyz[3 * (size_t) s->surf[k] + 2] > apex0) apex0 = s->xyz[3 * (size_t) s->surf[k] + 2];
+	if (!s->apex0_set)
+	{
+		float apex0 = -FLT_MAX;
+		for (int k = 0; k < s->nsurf; k++)
+			if (s->xyz[3 * (size_t) s->surf[k] + 2] > apex0) apex0 = s->xyz[3 * (size_t) s->surf[k] + 2];
+		s->apex0 = apex0;
+		s->apex0_set = 1;
+	}
+	const float apex0 = s->apex0;
 	if (o->voltage_queue > 4096) return fail(ORA_E_ARG, "voltage queue too long");
+	if (o->init_event_cnt < 0) return fail(ORA_E_ARG, "negative initial event count");
+	if (s->event_cnt < 0) s->event_cnt = o->init_event_cnt;
 	/* the fork applies update_coordination once after the initial surface
 	   evaluation (process.cpp:645-647) and after every take-away (:770) */
 	if (o->update_coordination && !s->coord_done) { ora_update_coordination(s); s->coord_done = 1; }
@@ -1633,6 +1652,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 	{
 		ora_event* r = &out[ev];
 		memset(r, 0, sizeof(*r));
+		r->event_index = ++s->event_cnt;
 		const float scalingReference = (float) s->scaling_ref;
 		/* I) candidate */
 		const int node = ora_find_candidate(s, o->evap_mode);
@@ -1662,7 +1682,10 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 		                                          o->relax_cycle, o->relax_queue, &e);
 		if (e) return e;
 		if (o->relax_global_cycles > 0) { ora_relax(s, o->relax_global_cycles, &e); if (e) return e; }
-		if (o->refresh_interval > 0 && (ev + 1) % o->refresh_interval == 0)
+		/* process.cpp:876 adds initEventCnt to the already offset eventIndex,
+		   in unsigned int */
+		if (o->refresh_interval > 0 &&
+		    ((unsigned int) o->init_event_cnt + (unsigned int) r->event_index) % (unsigned int) o->refresh_interval == 0)
 		{
 			double fd, fs;
 			r->refresh_iter = ora_relax_threshold(s, (double) o->refresh_threshold, o->refresh_cycle,
@@ -1714,12 +1737,12 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 			v *= s->species[nodeId].evap_field;
 			v /= flen;
 			v /= scalingReference;
-			if (vq_size < (int) o->voltage_queue || o->voltage_queue == 0) { vq[(vq_head + vq_size) % 4096] = v; vq_size++; }
-			else { vq[(vq_head + vq_size) % 4096] = v; vq_size++; }
-			vq_sum += v;
-			while (vq_size > (int) o->voltage_queue) { vq_sum -= vq[vq_head]; vq_head = (vq_head + 1) % 4096; vq_size--; }
-			r->voltage = vq_sum;
-			r->voltage /= (float) vq_size;
+			vq[(s->vq_head + s->vq_size) % 4096] = v;
+			s->vq_size++;
+			s->vq_sum += v;
+			while (s->vq_size > (int) o->voltage_queue) { s->vq_sum -= vq[s->vq_head]; s->vq_head = (s->vq_head + 1) % 4096; s->vq_size--; }
+			r->voltage = s->vq_sum;
+			r->voltage /= (float) s->vq_size;
 			float ts = r->pot_after;
 			ts -= ora_point_potential(s, backp, backt);
 			ts /= r->voltage;

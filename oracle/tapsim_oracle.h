@@ -58,6 +58,7 @@ typedef struct
 	int inplace_order;     /* 0 ascending sequential (one thread), 1 colour classes */
 	int thermal_velocity;  /* 1: evap_initialVelocity from rand(), 0: at rest */
 	int update_coordination; /* 1: the fork's update_coordination at the run's start and after each take-away */
+	int init_event_cnt;    /* EvaporationOptions::initEventCnt (process.h:98) */
 } ora_evap_opts;
 
 typedef struct
@@ -76,6 +77,7 @@ typedef struct
 	float apex[3];
 	float normal[3];
 	int traj_root_diff;    /* flight steps where powf(x, 0.25f) is not the correctly rounded root */
+	long long event_index; /* LogResults::Dataset::eventIndex (process.cpp:696) */
 } ora_event;
 
 typedef struct ora_sys ora_sys;

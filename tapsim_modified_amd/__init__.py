@@ -76,7 +76,7 @@ class EvapOptions(ctypes.Structure):
         ("trajectory", ctypes.c_int), ("integrator", ctypes.c_int), ("stepper", ctypes.c_int),
         ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float), ("update_coordination", ctypes.c_int),
-        ("thermal_velocity", ctypes.c_int), ("accel_solver", ctypes.c_int),
+        ("thermal_velocity", ctypes.c_int), ("accel_solver", ctypes.c_int), ("init_event_cnt", ctypes.c_int),
     ]
 
     def __init__(self, **kw):
@@ -102,7 +102,7 @@ class Event(ctypes.Structure):
         ("start", ctypes.c_float * 3), ("stop", ctypes.c_float * 3), ("apex", ctypes.c_float * 3),
         ("normal", ctypes.c_float * 3),
         ("voltage", ctypes.c_float), ("time_scale", ctypes.c_float), ("tof", ctypes.c_float),
-        ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int),
+        ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int), ("event_index", ctypes.c_longlong),
     ]
 
 
@@ -131,6 +131,7 @@ _SIGS = {
     "tsg_phi_switches": ([_P, _P, _P], ctypes.c_int),
     "tsg_potential": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
+    "tsg_field_o2": ([_P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "tsg_get_couplings": ([_P, _P], ctypes.c_int),
     "tsg_get_charges": ([_P, _P], ctypes.c_int),
     "tsg_get_flags": ([_P, _P], ctypes.c_int),
@@ -261,6 +262,7 @@ class Mesh:
 
     def __init__(self, xyz, tets=None, neighbours=None):
         lib = load()
+        self._lib = lib
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
         h = ctypes.c_void_p()
         if tets is None:
@@ -293,8 +295,9 @@ class Mesh:
         return assoc, bnd
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.tsg_mesh_destroy(self._h)
+        lib = getattr(self, "_lib", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.tsg_mesh_destroy(self._h)
             self._h = None
 
 
@@ -317,6 +320,7 @@ class Grid3D:
 
     def __init__(self, mesh, ids, species, temperature=0.0, mode=PLAIN_F64):
         lib = load()
+        self._lib = lib
         self.mesh = mesh
         self.ids0 = np.ascontiguousarray(ids, dtype=np.int16)
         self.species = species
@@ -329,9 +333,13 @@ class Grid3D:
         self.n, self.nnz = n.value, nnz.value
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.tsg_grid_destroy(self._h)
+        # the grid goes before the network it may hold (a raw pointer on the
+        # device side); the library handle is the one loaded at construction
+        lib = getattr(self, "_lib", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.tsg_grid_destroy(self._h)
             self._h = None
+        self._net = None
 
     # -- Grid_3d::Table::relax / localRelax
     def relax(self, cycles=1):
@@ -378,6 +386,14 @@ class Grid3D:
         out = np.zeros((len(nodes), 3), np.float64)
         _chk(load().tsg_field_o1(self._h, len(nodes), _ptr(nodes), _ptr(out)))
         return out
+
+    def field_o2(self, nodes):
+        """Grid_3d::Table::field_o2: (fields (k, 3), charges (k,)) of the quadratic fit"""
+        nodes = np.ascontiguousarray(np.atleast_1d(nodes), dtype=np.int32)
+        out = np.zeros((len(nodes), 3), np.float64)
+        q = np.zeros(len(nodes), np.float64)
+        _chk(load().tsg_field_o2(self._h, len(nodes), _ptr(nodes), _ptr(out), _ptr(q)))
+        return out, q
 
     def charges(self):
         """Node::charge of every node (computeCharge)"""
@@ -558,12 +574,16 @@ class WarmStartNet:
 
     def __init__(self, max_batch_rows=65536, data_rows=1 << 22, data_edges=1 << 26, seed=1):
         h = ctypes.c_void_p()
-        _chk(load().tsg_mpnn_create(max_batch_rows, data_rows, data_edges, seed, ctypes.byref(h)))
+        self._lib = load()
+        _chk(self._lib.tsg_mpnn_create(max_batch_rows, data_rows, data_edges, seed, ctypes.byref(h)))
         self._h = h
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            load().tsg_mpnn_destroy(self._h)
+        # at interpreter teardown the module's globals may already be gone:
+        # only the library handle loaded at construction is used here
+        lib = getattr(self, "_lib", None)
+        if getattr(self, "_h", None) and lib is not None:
+            lib.tsg_mpnn_destroy(self._h)
             self._h = None
 
     @property

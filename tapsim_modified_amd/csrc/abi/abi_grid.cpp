@@ -115,6 +115,12 @@ int tsg_field_o1(tsg_grid* h, int count, const int* nodes, double* out)
 	return check(h, h->g->field_o1(count, nodes, out));
 }
 
+int tsg_field_o2(tsg_grid* h, int count, const int* nodes, double* out, double* charge)
+{
+	if (!h || (count > 0 && (!nodes || !out || !charge))) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->field_o2(count, nodes, out, charge));
+}
+
 int tsg_get_couplings(const tsg_grid* h, double* out)
 {
 	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
@@ -258,6 +264,7 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.thermal_velocity = o->thermal_velocity;
 	e.update_coordination = o->update_coordination;
 	e.accel_solver = o->accel_solver;
+	e.init_event_cnt = o->init_event_cnt;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
 }
 

@@ -207,28 +207,26 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 
 Grid::~Grid()
 {
-	if (fstream_) (void) hipStreamSynchronize(fstream_);
+	// every stream drains before any event, graph, pinned buffer or stream
+	// goes away: work queued on one stream may wait on an event of another
+	// (the flight waits for the main stream's snapshot, the normaliser for
+	// the surface values) or copy into pinned memory freed below
+	for (hipStream_t s : {stream_, fstream_, sstream_})
+		if (s) (void) hipStreamSynchronize(s);
+	amg_graph_drop();
 	for (FlightSlot& f : fslot_) {
 		if (f.ready) (void) hipEventDestroy(f.ready);
 		if (f.done) (void) hipEventDestroy(f.done);
 		if (f.host) (void) hipHostFree(f.host);
+		f.ready = f.done = nullptr;
+		f.host = nullptr;
 	}
-	if (fstream_) (void) hipStreamDestroy(fstream_);
-	if (stream_) (void) hipStreamSynchronize(stream_);
-	if (ev_start_) (void) hipEventDestroy(ev_start_);
-	if (ev_stop_) (void) hipEventDestroy(ev_stop_);
-	if (h_pinned_) (void) hipHostFree(h_pinned_);
-	if (warm_host_) (void) hipHostFree(warm_host_);
-	if (sstream_) (void) hipStreamSynchronize(sstream_);
-	if (sum_src_) (void) hipEventDestroy(sum_src_);
-	if (sum_done_) (void) hipEventDestroy(sum_done_);
-	if (sum_host_) (void) hipHostFree(sum_host_);
-	if (sstream_) (void) hipStreamDestroy(sstream_);
-	amg_graph_drop();
-	for (hipEvent_t& e : amg_ev_)
+	for (hipEvent_t e : {ev_start_, ev_stop_, sum_src_, sum_done_, amg_ev_[0], amg_ev_[1]})
 		if (e) (void) hipEventDestroy(e);
-	if (amg_host_) (void) hipHostFree(amg_host_);
-	if (stream_) (void) hipStreamDestroy(stream_);
+	for (void* p : {h_pinned_, warm_host_, (void*) sum_host_, (void*) amg_host_})
+		if (p) (void) hipHostFree(p);
+	for (hipStream_t s : {fstream_, sstream_, stream_})
+		if (s) (void) hipStreamDestroy(s);
 }
 
 // Grid_3d::Table::computeCoupling (grid_3d.cpp:1392-1409) from the
@@ -334,6 +332,7 @@ void Grid::row_entries(int node, std::vector<int>* cols, std::vector<double>* co
 int Grid::build_rows()
 {
 	const int n = mesh_.n;
+	ord_dirty_ = true;
 	slot_of_.assign(n, -1);
 	std::vector<int> rows;
 	for (int i = 0; i < n; i++)
@@ -489,17 +488,63 @@ int Grid::arm_ctl(double threshold, int cycles, int queue)
 	return 0;
 }
 
+// the unknown rows' slots in ascending node order: the order of the
+// reference's one-thread in-place sweep (uploaded when the rows changed)
+int Grid::ensure_inplace_order()
+{
+	if (!ord_dirty_ && d_ord_slot_.ptr()) return 0;
+	std::vector<int> ord;
+	ord.reserve(nrows_);
+	for (int i = 0; i < mesh_.n; i++)
+		if (slot_of_[i] >= 0) {
+			ord.push_back(slot_of_[i]);
+			if (slot_deg_[slot_of_[i]] > kInplaceCap)
+				return fail(-7, "relax(cycleSize > 1): a row longer than the in-place staging area");
+		}
+	ord_count_ = (int) ord.size();
+	if (d_ord_slot_.bytes() < sizeof(int) * ord.size()) d_ord_slot_.alloc(sizeof(int) * (ord.size() + 4096));
+	if (!d_ord_slot_.ptr()) return fail(-2, "device allocation failed (in-place order)");
+	HIPCHK(hipMemcpy(d_ord_slot_.ptr(), ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+	ord_dirty_ = false;
+	return 0;
+}
+
+// the cycles of one relax(cycleSize) call starting from _phiSwitch = s
+// (doRelax, grid_3d.cpp:1238-1258): the table's switch stays s for the whole
+// call, so odd cycles are Jacobi (read s, write 1-s) and even cycles update
+// buffer s in place; the decision kernel follows the last cycle
+int Grid::launch_relax_call(const SweepArgs& a, int s, int cycles)
+{
+	for (int c = 1; c <= cycles; c++) {
+		const bool last = c == cycles;
+		if (c & 1) {
+			if (launch_jacobi_cycle(stream_, a, s, f64_, last)) return fail(-2, "sweep launch failed");
+		} else {
+			InplaceArgs P;
+			P.count = ord_count_;
+			P.ord_slot = d_ord_slot_.as<int>();
+			P.buf = s;
+			P.last = last ? 1 : 0;
+			if (launch_inplace_cycle(stream_, a, P, f64_)) return fail(-2, "in-place sweep launch failed");
+		}
+	}
+	return 0;
+}
+
 int Grid::relax_cycles(int cycles, double* deviation)
 {
-	if (cycles != 1) return fail(-1, "relax(cycleSize) on the device supports cycleSize 1 (Jacobi); in-place cycles run in localRelax");
-	if (int rc = arm_ctl(-1.0, 1, 0)) return rc;
+	if (cycles < 1) return fail(-1, "relax(cycleSize): cycleSize must be positive");
+	if (cycles > 1)
+		if (int rc = ensure_inplace_order()) return rc;
+	if (int rc = arm_ctl(-1.0, cycles, 0)) return rc;
 	SweepArgs a = sweep_args();
-	if (launch_jacobi_sweep(stream_, a, sw_, f64_)) return fail(-2, "sweep launch failed");
+	if (int rc = launch_relax_call(a, sw_, cycles)) return rc;
 	RelaxCtl c;
 	HIPCHK(hipMemcpyAsync(&c, d_ctl_.ptr(), offsetof(RelaxCtl, queue), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
+	if (c.done == 4) return fail(-7, "relax(cycleSize > 1): a row longer than the in-place staging area");
 	if (c.done == 3) return fail(-3, "Grid_3d::Table::computePotential(): NaN");
-	sw_ ^= 1;
+	sw_ = (sw_ + cycles) & 1;
 	if (deviation) *deviation = c.last_dev;
 	return 0;
 }
@@ -507,15 +552,20 @@ int Grid::relax_cycles(int cycles, double* deviation)
 int Grid::relax_threshold(double threshold, int cycles, int queue, long long* iterations, double* fault_dev,
                           double* fault_slope, double* last_dev)
 {
-	if (cycles != 1) return fail(-1, "relax(threshold, cycleSize, queue) on the device supports cycleSize 1");
-	if (int rc = arm_ctl(threshold, 1, queue)) return rc;
+	if (cycles < 1) return fail(-1, "relax(threshold, cycleSize, queue): cycleSize must be positive");
+	if (cycles > 1)
+		if (int rc = ensure_inplace_order()) return rc;
+	if (int rc = arm_ctl(threshold, cycles, queue)) return rc;
 	SweepArgs a = sweep_args();
 	const int sw0 = sw_;
 	long long launched = 0;
 	// the first batch is sized from the previous call's sweep count, later
 	// batches are short so that few sweeps are launched after convergence
 	// (those return at once on the device's done flag)
-	int batch = (int) std::min<long long>(4096, std::max<long long>(8, last_sweeps_ * 7 / 8));
+	// (with cycleSize > 1 a "sweep" is one relax(cycleSize) call of that many
+	// cycles; its in-place cycles are long, so batches stay short)
+	const long long cap = cycles > 1 ? 4 : 4096;
+	int batch = (int) std::min<long long>(cap, std::max<long long>(cycles > 1 ? 1 : 8, last_sweeps_ * 7 / 8));
 	RelaxCtl c;
 	if (!ev_start_) {
 		HIPCHK(hipEventCreate(&ev_start_));
@@ -524,13 +574,16 @@ int Grid::relax_threshold(double threshold, int cycles, int queue, long long* it
 	HIPCHK(hipEventRecord(ev_start_, stream_));
 	for (;;) {
 		for (int b = 0; b < batch; b++) {
-			if (launch_jacobi_sweep(stream_, a, (int) ((sw0 + launched) & 1), f64_)) return fail(-2, "sweep launch failed");
+			if (cycles == 1) {
+				if (launch_jacobi_sweep(stream_, a, (int) ((sw0 + launched) & 1), f64_)) return fail(-2, "sweep launch failed");
+			} else if (int rc = launch_relax_call(a, (int) ((sw0 + launched * cycles) & 1), cycles))
+				return rc;
 			launched++;
 		}
 		HIPCHK(hipMemcpyAsync(&c, d_ctl_.ptr(), offsetof(RelaxCtl, queue), hipMemcpyDeviceToHost, stream_));
 		HIPCHK(hipStreamSynchronize(stream_));
 		if (c.done) break;
-		batch = (int) std::min<long long>(64, std::max<long long>(4, (last_sweeps_ - c.sweeps) / 2));
+		batch = (int) std::min<long long>(cycles > 1 ? 4 : 64, std::max<long long>(cycles > 1 ? 1 : 4, (last_sweeps_ - c.sweeps) / 2));
 	}
 	HIPCHK(hipEventRecord(ev_stop_, stream_));
 	HIPCHK(hipEventSynchronize(ev_stop_));
@@ -540,8 +593,9 @@ int Grid::relax_threshold(double threshold, int cycles, int queue, long long* it
 	stat_sweeps_ += c.sweeps;
 	stat_launched_ += launched;
 	last_sweeps_ = c.sweeps;
-	sw_ = (int) ((sw0 + c.sweeps) & 1);
+	sw_ = (int) ((sw0 + c.sweeps * cycles) & 1);
 	if (last_dev) *last_dev = c.last_dev;
+	if (c.done == 4) return fail(-7, "relax(cycleSize > 1): a row longer than the in-place staging area");
 	if (c.done == 3) return fail(-3, "Grid_3d::Table::computePotential(): NaN");
 	if (c.done == 2) {
 		if (iterations) *iterations = -1 - c.iteration;
@@ -587,7 +641,11 @@ int Grid::local_relax(int node, int order, double threshold, int cycles, int que
 		if (iterations) *iterations = cycles;
 		return 0;
 	}
-	// level schedule of the ascending in-place sweep
+	// level schedule of the ascending in-place sweep: a node comes after
+	// every lower-index shell neighbour, whichever of the two reads the
+	// other -- v must see a lower neighbour's new value if it reads it, and a
+	// lower Neumann neighbour that reads v (while v does not read it) must see
+	// v's old value; both need the neighbour's level below v's
 	std::vector<int> level(count, 0), pos_of;
 	int nlev = 1;
 	{
@@ -598,7 +656,6 @@ int Grid::local_relax(int node, int order, double threshold, int cycles, int que
 			for (int k = mesh_.ptr[v]; k < mesh_.ptr[v + 1]; k++) {
 				const int j = mesh_.idx[k];
 				if (j >= v) break;  // ascending: only lower-index neighbours precede
-				if (!participates(v, j)) continue;
 				const auto it = std::lower_bound(nodes.begin(), nodes.end(), j);
 				if (it != nodes.end() && *it == j) lv = std::max(lv, level[it - nodes.begin()] + 1);
 			}
@@ -706,9 +763,10 @@ int Grid::field_o1(int count, const int* nodes, double* out)
 	if (count <= 0) return 0;
 	for (int i = 0; i < count; i++)
 		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "field_o1: bad node");
-	if (d_nodes_.bytes() < sizeof(int) * (size_t) count) {
+	if (d_nodes_.bytes() < sizeof(int) * (size_t) count || d_fields_.bytes() < sizeof(double) * 3 * (size_t) count) {
 		d_nodes_.alloc(sizeof(int) * (size_t) count);
-		d_fields_.alloc(sizeof(double) * 3 * (size_t) count);
+		d_fields_.alloc(sizeof(double) * 4 * (size_t) count);
+		if (!d_nodes_.ptr() || !d_fields_.ptr()) return fail(-2, "device allocation failed (fields)");
 	}
 	HIPCHK(hipMemcpyAsync(d_nodes_.ptr(), nodes, sizeof(int) * count, hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
@@ -722,6 +780,35 @@ int Grid::field_o1(int count, const int* nodes, double* out)
 	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	return 0;
+}
+
+// Grid_3d::Table::field_o2 (grid_3d.cpp:993-1199) of a node list in the
+// potentials potential() reads: fields and the charge of the quadratic fit
+int Grid::field_o2(int count, const int* nodes, double* out, double* charge)
+{
+	if (count <= 0) return 0;
+	for (int i = 0; i < count; i++)
+		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "field_o2: bad node");
+	if (d_nodes_.bytes() < sizeof(int) * (size_t) count || d_fields_.bytes() < sizeof(double) * 4 * (size_t) count) {
+		d_nodes_.alloc(sizeof(int) * (size_t) count);
+		d_fields_.alloc(sizeof(double) * 4 * (size_t) count);
+		if (!d_nodes_.ptr() || !d_fields_.ptr()) return fail(-2, "device allocation failed (fields)");
+	}
+	HIPCHK(hipMemcpyAsync(d_nodes_.ptr(), nodes, sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	double* fo = d_fields_.as<double>();
+	if (launch_node_fields_o2(stream_, count, d_nodes_.as<int>(), d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(),
+	                          d_flags_.as<unsigned char>(), phi, f64_, fo, fo + 3 * (size_t) count, d_status_.as<int>()))
+		return fail(-2, "field launch failed");
+	int st = 0;
+	HIPCHK(hipMemcpyAsync(out, fo, sizeof(double) * 3 * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(charge, fo + 3 * (size_t) count, sizeof(double) * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (st & 4) return fail(-4, "Grid_3d::Table::field_o2(): singular least-squares system");
+	if (st & 2) return fail(-3, "Grid_3d::Table::field_o2(): NaN");
 	return 0;
 }
 
@@ -793,6 +880,7 @@ int Grid::set_id_and_resync(int node, int id)
 	ids_[node] = (int16_t) id;
 	const Species& sp = species_[id];
 	flags_[node] = (mesh_.boundary[node] ? 1 : 0) | (sp.dirichlet ? 2 : 0) | (sp.neumann ? 4 : 0);
+	ord_dirty_ = true;  // rows may appear, move or vanish
 
 	// couplings of the node's faces as computeCoupling(node, neighbour)
 	std::vector<long long> full_addr;
@@ -1066,6 +1154,9 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 
 int Grid::surface_values(double* out)
 {
+	// the values belong to the evaluated site list; after a take-away the
+	// caller's site count (surface_size) no longer describes them
+	if (surface_dirty_) return fail(-6, "surface changed since its last evaluation");
 	const int cnt = (int) surface_vec_.size();
 	if (cnt == 0) return 0;
 	HIPCHK(hipStreamSynchronize(stream_));
@@ -1197,7 +1288,6 @@ int Grid::update_coordination()
 	HIPCHK(hipMemcpy(d_evapF_.ptr(), spec.data(), sizeof(float) * n, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_evapE_.ptr(), spec.data() + n, sizeof(float) * n, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_eps_.ptr(), spec.data() + 2 * (size_t) n, sizeof(float) * n, hipMemcpyHostToDevice));
-	surface_dirty_ = true;
 	return 0;
 }
 
@@ -1484,10 +1574,13 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 		stat_phase_ms_[p] += std::chrono::duration<double, std::milli>(t - t_last).count();
 		t_last = t;
 	};
+	if (o.init_event_cnt < 0) return fail(-1, "negative initial event count");
+	if (event_cnt_ < 0) event_cnt_ = o.init_event_cnt;
 	for (int ev = 0; ev < k; ev++) {
 		if (surface_.empty()) break;
 		EventRecord& r = out[ev];
 		std::memset(&r, 0, sizeof(r));
+		r.event_index = ++event_cnt_;
 		stat_phase_events_++;
 		const float scaling_reference = (float) scaling_ref_;
 		apex_of_surface(r.apex);
@@ -1541,7 +1634,11 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 			if (int rc = local_relax(node, o.relax_shell, o.relax_threshold, o.relax_cycle, o.relax_queue, &r.local_iter)) return rc;
 			for (int c = 0; c < o.relax_global_cycles; c++)
 				if (int rc = relax_cycles(1, nullptr)) return rc;
-			if (o.refresh_interval > 0 && (ev + 1) % o.refresh_interval == 0) {
+			// the reference adds initEventCnt to the already offset eventIndex
+			// (process.cpp:876); the sum is an unsigned int there
+			if (o.refresh_interval > 0 &&
+			    (unsigned int) ((unsigned int) o.init_event_cnt + (unsigned int) r.event_index) %
+			            (unsigned int) o.refresh_interval == 0) {
 				double fd, fs, ld;
 				if (int rc = relax_threshold(o.refresh_threshold, o.refresh_cycle, o.refresh_queue, &r.global_iter, &fd, &fs, &ld)) return rc;
 			}
@@ -2068,6 +2165,7 @@ int Grid::warm_begin(int node, double phi_ref, bool predict, EventRecord* rec)
 	const size_t need = 64 + 16 * 7 + sizeof(int) * (ns + 1) + (sizeof(int) + sizeof(float)) * (size_t) ne +
 	                    (sizeof(int) * 2 + 1 + sizeof(float)) * (size_t) ns + 2 * sizeof(double) * 4;
 	if (need > warm_host_bytes_) {
+		HIPCHK(hipStreamSynchronize(stream_));
 		if (warm_host_) (void) hipHostFree(warm_host_);
 		warm_host_ = nullptr;
 		warm_host_bytes_ = std::max(need * 2, (size_t) 1 << 20);

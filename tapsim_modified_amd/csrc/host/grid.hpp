@@ -52,6 +52,7 @@ struct EventRecord {
 	float start[3], stop[3], apex[3], normal[3];
 	float voltage, time_scale, tof;
 	int traj_len, traj_status;
+	long long event_index;  // LogResults::Dataset::eventIndex (process.cpp:696)
 };
 
 struct EvapOptions {
@@ -74,6 +75,7 @@ struct EvapOptions {
 	float shrink_limit = 1.0f;  // stop once the apex has receded this far (m)
 	int thermal_velocity = 0;   // 1: evap_initialVelocity from the process's rand()
 	int update_coordination = 0;  // 1: the fork's update_coordination (surface_3d.cpp:568-593)
+	int init_event_cnt = 0;       // EvaporationOptions::initEventCnt (process.h:98)
 };
 
 class DevBuf {
@@ -131,6 +133,7 @@ public:
 	int potential(int node, double* out);
 
 	int field_o1(int count, const int* nodes, double* out);
+	int field_o2(int count, const int* nodes, double* out, double* charge);
 	int get_couplings(double* out) const;
 	int get_charges(double* out) const;
 	// the fork's update_coordination (surface_3d.cpp:568-593) on the ids
@@ -234,6 +237,12 @@ private:
 	               std::vector<double>* vr, std::vector<long long>* aw, std::vector<double>* vw);
 	long long entry_index(int row, int col) const;
 	int arm_ctl(double threshold, int cycles, int queue);
+	// relax(cycleSize > 1): the in-place cycles' row order and one call's launches
+	DevBuf d_ord_slot_;
+	int ord_count_ = 0;
+	bool ord_dirty_ = true;
+	int ensure_inplace_order();
+	int launch_relax_call(const SweepArgs& a, int s, int cycles);
 	int resolve_candidate(int* node, int* near_tie);
 
 	int mode_;
@@ -350,6 +359,9 @@ private:
 	float vq_sum_ = 0.0f;
 	float apex0_ = 0.0f;
 	bool apex0_set_ = false;
+	// Process::evaporation's eventCnt (process.cpp:676-696), kept across
+	// evaporate calls; -1 until the first call sets it from initEventCnt
+	long long event_cnt_ = -1;
 	// warm start state of the current event
 	MpnnNet* net_ = nullptr;
 	bool net_record_ = false, net_predict_ = false;

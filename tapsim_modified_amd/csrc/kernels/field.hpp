@@ -78,7 +78,7 @@ __device__ int field_o2(int node, const float* __restrict__ xyz, const int* __re
 	if (flags[node] & 0x01) return 0;
 	const V3<R> pi = load_coord<R>(xyz, node);
 	const R pidx = phi[node];
-	const int k0 = ptr[node], k1 = min(ptr[node + 1], ptr[node] + 256);
+	const int k0 = ptr[node], k1 = ptr[node + 1];
 	double a[81];
 	for (int i = 0; i < 81; i++) a[i] = 0.0;
 	for (int k = k0; k < k1; k++) {

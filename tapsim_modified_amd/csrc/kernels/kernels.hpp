@@ -44,7 +44,7 @@ struct RelaxCtl {
 	double fault_dev, fault_slope;
 	int cycle;                // cycle size (1: every sweep is one relax(1) call)
 	int qhead, qsize;
-	int pad;
+	int nan;                  // sticky: a cycle of the current relax(cycleSize) call produced a NaN
 	double queue[1024];
 };
 
@@ -72,6 +72,24 @@ int row_launch_blocks(int nslots, int nlong);
 // One Jacobi cycle (read buffer `read`, write the other) followed by the
 // one-block decision kernel that reduces max|phi_new - phi_old| into ctl.
 int launch_jacobi_sweep(hipStream_t, const SweepArgs&, int read, bool f64);
+// A Jacobi cycle inside a relax(cycleSize) call: with `decide` the decision
+// kernel follows (the call's last cycle), otherwise only the cycle runs.
+int launch_jacobi_cycle(hipStream_t, const SweepArgs&, int read, bool f64, bool decide);
+
+// The in-place cycles of relax(cycleSize > 1) (doRelax with the table's
+// _phiSwitch held for the whole call, grid_3d.cpp:1238-1258,1458-1514):
+// every even cycle updates buffer `buf` in place in ascending node order, as
+// the reference's one-thread sweep.  One workgroup: the rows go in chunks
+// whose known terms are staged in parallel, then the chain runs in order.
+struct InplaceArgs {
+	int count;                // unknown rows
+	const int* ord_slot;      // their slots in ascending node order
+	int buf;                  // the buffer updated in place (_phiSwitch of the call)
+	int last;                 // 1: the call's last cycle: max |phi0 - phi1| into block_max[0], then the decision
+};
+// entries of one chunk staged in LDS; a row longer than this cannot be swept
+constexpr int kInplaceCap = 4096;
+int launch_inplace_cycle(hipStream_t, const SweepArgs&, const InplaceArgs&, bool f64);
 
 // Local relaxation (Grid_3d::Table::localRelax(index, order, threshold,
 // cycleSize, queueSize), grid_3d.cpp:707-795) of a node shell in one
@@ -318,5 +336,9 @@ int launch_surface_sum(hipStream_t, int count, const void* value, bool f64, doub
 // field_o1 of arbitrary nodes (one thread each)
 int launch_node_fields(hipStream_t, int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
                        const void* coup, const unsigned char* flags, const void* phi, bool f64, double* out, int* status);
+// field_o2 of a node list: fields (3 per node) and charges
+int launch_node_fields_o2(hipStream_t, int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
+                          const unsigned char* flags, const void* phi, bool f64, double* out, double* charge,
+                          int* status);
 
 }  // namespace tsg

@@ -176,7 +176,163 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		a.block_max[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
-		a.block_flag[blockIdx.x] = snan[0] | snan[1] | snan[2] | snan[3];
+		const int f = snan[0] | snan[1] | snan[2] | snan[3];
+		a.block_flag[blockIdx.x] = f;
+		// a NaN in a cycle before the last of a relax(cycleSize) call would be
+		// overwritten by the next cycle's flags: it also sticks in the control
+		// block (rare path, so the atomic costs nothing in a normal sweep)
+		if (f) atomicOr(&a.ctl->nan, 1);
+	}
+}
+
+// One in-place cycle of relax(cycleSize) in one workgroup of 256 threads.
+// Rows go in ascending node order in chunks of at most 256 rows and
+// kInplaceCap staged entries.  For a row of node i in a chunk starting at
+// node lo, the reference's ascending sum (grid_3d.cpp:1324-1366) reads
+//   j < lo          : values already written in this cycle (earlier chunks),
+//   lo <= j < i     : values of this chunk written before i (or Dirichlet),
+//   j > i           : values not yet written in this cycle.
+// Phase A (one thread per row) sums the j < lo part in order and stages the
+// remaining entries: the product for a known value, the coupling and the
+// chunk position for a value written earlier in the chunk.  Phase B (one
+// thread) finishes the rows in order, adding the staged terms in entry
+// order, so every sum is the reference's one-thread sum bit for bit.
+// Phase C writes the chunk back.  Loads and stores of the swept buffer are
+// agent-scope atomics so that later chunks see the earlier chunks' values.
+template <typename R>
+__global__ __launch_bounds__(256) void k_inplace(SweepArgs a, InplaceArgs P)
+{
+	constexpr int CH = 256;
+	R* pw = (R*) (P.buf ? a.phi1 : a.phi0);
+	const R* po = (const R*) (P.buf ? a.phi0 : a.phi1);
+	const R* __restrict__ coup = (const R*) a.coup;
+	__shared__ int s_node[CH];
+	__shared__ int s_e0[CH + 1];
+	__shared__ double s_prefix[CH], s_q[CH], s_ws[CH];
+	__shared__ R s_val[CH];
+	__shared__ double s_term[kInplaceCap];
+	__shared__ short s_pos[kInplaceCap];
+	__shared__ int s_cnt[CH];
+	__shared__ int s_rows, s_nan, s_bad;
+	__shared__ double s_max[4];
+	const int t = threadIdx.x;
+	if (a.ctl->done) return;  // uniform: converged earlier in this batch
+	if (t == 0) { s_nan = 0; s_bad = 0; }
+	double dmax = 0.0;
+	for (int c0 = 0; c0 < P.count;) {
+		const int p = c0 + t;
+		int slot = -1, node = 0x7fffffff, deg = 0;
+		long long base = 0;
+		if (p < P.count) {
+			slot = P.ord_slot[p];
+			node = a.slot_node[slot];
+			deg = a.slot_deg[slot];
+			base = a.slice_off[slot >> 6] + (slot & 63);
+		}
+		s_node[t] = node;
+		__syncthreads();
+		const int lo = s_node[0];
+		// the part before the chunk, in entry order
+		double prefix = 0.0;
+		int k = 0;
+		for (; k < deg; k++) {
+			const int j = a.col[base + 64ll * k];
+			if (j >= lo) break;
+			double adj = (double) __hip_atomic_load(pw + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			adj *= (double) coup[base + 64ll * k];
+			prefix += adj;
+		}
+		const int cnt = deg - k;
+		s_cnt[t] = cnt;
+		__syncthreads();
+		if (t == 0) {
+			// leading rows whose staged entries fit
+			int off = 0, r = 0;
+			for (; r < CH && c0 + r < P.count; r++) {
+				if (off + s_cnt[r] > kInplaceCap) break;
+				s_e0[r] = off;
+				off += s_cnt[r];
+			}
+			s_e0[r] = off;
+			s_rows = r;
+			if (r == 0) s_bad = 1;
+		}
+		__syncthreads();
+		const int rows = s_rows;
+		if (rows == 0) break;  // one row longer than the staging area (reported below)
+		if (t < rows) {
+			s_prefix[t] = prefix;
+			s_q[t] = a.q[slot];
+			s_ws[t] = a.ws[slot];
+			int e = s_e0[t];
+			for (int kk = k; kk < deg; kk++, e++) {
+				const int j = a.col[base + 64ll * kk];
+				const R c = coup[base + 64ll * kk];
+				int pos = -1;
+				if (j < node) {  // an earlier row of this chunk, or a Dirichlet node
+					int l = 0, h = t;
+					while (l < h) {
+						const int m = (l + h) >> 1;
+						if (s_node[m] < j) l = m + 1;
+						else h = m;
+					}
+					if (l < t && s_node[l] == j) pos = l;
+				}
+				if (pos >= 0) {
+					s_term[e] = (double) c;
+				} else {
+					double adj = (double) __hip_atomic_load(pw + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+					adj *= (double) c;
+					s_term[e] = adj;
+				}
+				s_pos[e] = (short) pos;
+			}
+		}
+		__syncthreads();
+		if (t == 0) {
+			for (int r = 0; r < rows; r++) {
+				double lv = s_prefix[r];
+				const int e1 = s_e0[r + 1];
+				for (int e = s_e0[r]; e < e1; e++) {
+					const int pos = s_pos[e];
+					double term = s_term[e];
+					if (pos >= 0) {
+						double adj = (double) s_val[pos];
+						adj *= term;
+						term = adj;
+					}
+					lv += term;
+				}
+				lv -= s_q[r];
+				lv /= s_ws[r];
+				if (lv != lv) s_nan = 1;
+				s_val[r] = (R) lv;
+			}
+		}
+		__syncthreads();
+		if (t < rows) {
+			const R v = s_val[t];
+			__hip_atomic_store(pw + node, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (P.last) {
+				double ld = (double) v;
+				ld -= (double) po[node];
+				dmax = fmax(dmax, fabs(ld));
+			}
+		}
+		__threadfence_block();
+		__syncthreads();
+		c0 += rows;
+	}
+	dmax = wave_max(dmax);
+	if ((t & 63) == 0) s_max[t >> 6] = dmax;
+	__syncthreads();
+	if (t == 0) {
+		if (s_bad) atomicOr(&a.ctl->nan, 2);
+		if (s_nan) atomicOr(&a.ctl->nan, 1);
+		if (P.last) {
+			a.block_max[0] = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+			a.block_flag[0] = s_nan;
+		}
 	}
 }
 
@@ -214,7 +370,8 @@ __global__ __launch_bounds__(256) void k_finalize(int nblocks, RelaxCtl* ctl, co
 	const double d = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
 	ctl->sweeps += 1;
 	ctl->last_dev = d;
-	if (snan[0] | snan[1] | snan[2] | snan[3]) { ctl->done = 3; return; }
+	if (ctl->nan & 2) { ctl->done = 4; return; }  // a row too long for the in-place staging
+	if ((snan[0] | snan[1] | snan[2] | snan[3]) || ctl->nan) { ctl->done = 3; return; }
 	double slope = 0.0;
 	if (push_and_test(ctl->queue, &ctl->qhead, &ctl->qsize, 1024, ctl->queue_size, ctl->cycle, d, &slope)) {
 		ctl->fault_dev = d;
@@ -348,6 +505,26 @@ int launch_jacobi_sweep(hipStream_t s, const SweepArgs& a, int read, bool f64)
 	else hipLaunchKernelGGL(dev::k_jacobi<float>, dim3(blocks), dim3(256), 0, s, a, read);
 	hipLaunchKernelGGL(dev::k_finalize, dim3(1), dim3(256), 0, s, blocks, a.ctl, (const double*) a.block_max,
 	                   (const int*) a.block_flag);
+	return (int) hipGetLastError();
+}
+
+int launch_jacobi_cycle(hipStream_t s, const SweepArgs& a, int read, bool f64, bool decide)
+{
+	if (decide) return launch_jacobi_sweep(s, a, read, f64);
+	const int blocks = row_launch_blocks(a.nslots, a.nlong);
+	if (blocks == 0) return 0;
+	if (f64) hipLaunchKernelGGL(dev::k_jacobi<double>, dim3(blocks), dim3(256), 0, s, a, read);
+	else hipLaunchKernelGGL(dev::k_jacobi<float>, dim3(blocks), dim3(256), 0, s, a, read);
+	return (int) hipGetLastError();
+}
+
+int launch_inplace_cycle(hipStream_t s, const SweepArgs& a, const InplaceArgs& P, bool f64)
+{
+	if (f64) hipLaunchKernelGGL(dev::k_inplace<double>, dim3(1), dim3(256), 0, s, a, P);
+	else hipLaunchKernelGGL(dev::k_inplace<float>, dim3(1), dim3(256), 0, s, a, P);
+	if (P.last)
+		hipLaunchKernelGGL(dev::k_finalize, dim3(1), dim3(256), 0, s, 1, a.ctl, (const double*) a.block_max,
+		                   (const int*) a.block_flag);
 	return (int) hipGetLastError();
 }
 

@@ -110,6 +110,22 @@ __global__ void k_node_fields(int count, const int* nodes, const float* xyz, con
 	for (int i = 0; i < 3; i++) out[3 * (size_t) s + i] = (double) E[i];
 }
 
+// field_o2 (quadratic least squares) of a list of nodes: -grad phi and the
+// charge -eps0 laplacian; status bit 2 for a singular system, bit 1 for NaN
+template <typename R>
+__global__ void k_node_fields_o2(int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
+                                 const unsigned char* flags, const R* phi, double* out, double* charge, int* status)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= count) return;
+	R E[3], q;
+	const int rc = field_o2<R>(nodes[s], xyz, ptr, idx, flags, phi, E, &q);
+	if (rc == 1) atomicOr(status, 4);
+	if (rc == 2) atomicOr(status, 2);
+	for (int i = 0; i < 3; i++) out[3 * (size_t) s + i] = (double) E[i];
+	charge[s] = (double) q;
+}
+
 // first maximum in slot order (slots are in ascending node order); a slot
 // below the winner whose value lies within a few ulps of the maximum could
 // tie with it after the reference's float normalisation (v / sum), which
@@ -197,6 +213,19 @@ int launch_surface_sum(hipStream_t st, int count, const void* value, bool f64, d
 {
 	if (f64) hipLaunchKernelGGL(dev::k_seq_sum<double>, dim3(1), dim3(64), 0, st, count, (const double*) value, out);
 	else hipLaunchKernelGGL(dev::k_seq_sum<float>, dim3(1), dim3(64), 0, st, count, (const float*) value, out);
+	return (int) hipGetLastError();
+}
+
+int launch_node_fields_o2(hipStream_t st, int count, const int* nodes, const float* xyz, const int* ptr, const int* idx,
+                          const unsigned char* flags, const void* phi, bool f64, double* out, double* charge, int* status)
+{
+	if (count <= 0) return 0;
+	if (f64)
+		hipLaunchKernelGGL(dev::k_node_fields_o2<double>, dim3((count + 63) / 64), dim3(64), 0, st, count, nodes, xyz,
+		                   ptr, idx, flags, (const double*) phi, out, charge, status);
+	else
+		hipLaunchKernelGGL(dev::k_node_fields_o2<float>, dim3((count + 63) / 64), dim3(64), 0, st, count, nodes, xyz,
+		                   ptr, idx, flags, (const float*) phi, out, charge, status);
 	return (int) hipGetLastError();
 }
 

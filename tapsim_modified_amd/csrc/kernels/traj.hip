@@ -522,7 +522,9 @@ __device__ Phase stepper_step(Tr& tr, const unsigned char* gbnd, const Phase& ba
 			tr.geom_error = true;
 			return back;
 		}
-		if (*delta < tr.dt_min) *delta = tr.dt_min;
+		// the NONE stepper integrates with the step as given (trajectory_3d.cpp:155-158);
+		// the other three raise it to the step limit first (:362,432,458)
+		if (tr.stepper != 0 && *delta < tr.dt_min) *delta = tr.dt_min;
 		const Phase value = tr.integrator == 0 ? o1_step(tr, back, *delta, error) : o5_step(tr, back, *delta, error);
 		if (tr.stepper == 0) return value;
 		if (tr.stepper == 2) {

@@ -1,3 +1,4 @@
+import faulthandler
 import os
 import sys
 
@@ -7,6 +8,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
+
+
+# a crash in native code prints the Python stack of every thread, naming the test
+faulthandler.enable(all_threads=True)
 
 
 def pytest_configure(config):

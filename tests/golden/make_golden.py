@@ -330,12 +330,84 @@ def make_formats():
     print("formats: %d nodes, %d bytes of reference output" % (n, sum(v.nbytes for k, v in out.items() if k.startswith(("w_", "x_")))))
 
 
+# temperature of the Boltzmann-model fixture, K
+BOLTZMANN_T = 20000.0
+
+
+def make_pinning():
+    """Fixtures for the parts of the path that the first fixture set leaves
+    open, on the same small specimen (inputs from tip_small.npz): relax with
+    cycleSize > 1 inside a threshold relaxation and relax(100), the
+    RelaxationFault of a threshold relaxation that stagnates in float, the
+    randomReset preset, the probability models 1-3 in the event loop, and
+    trajectories with the Euler integrator and the none / geometry-restricted
+    / adaptive steppers."""
+    base = dict(np.load(os.path.join(OUT, "tip_small.npz")))
+    xyz, ids, tv = base["xyz"], base["ids"], base["tets"]
+    node = int(base["local_node"])
+    fx = {}
+
+    def fresh(wd, species=SPECIES, temperature=0.0):
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(species, temperature=temperature))
+        write_input(os.path.join(wd, "in.bin"), xyz, ids, tv)
+
+    with tempfile.TemporaryDirectory() as wd:
+        fresh(wd)
+        # relax(threshold, 4, 5) (odd cycles Jacobi, even cycles in place),
+        # then relax(100) from there
+        run(wd, ["mesh", "sync", "relaxthr:0.5:4:5", "dumpphi:t4", "relax:1:100", "dumpphi:c100"])
+        fx["ref_thr4_iter"] = rd(wd, "relaxthr_iter.i64", np.int64)
+        fx["ref_phi0_t4"] = rd(wd, "phi0_t4.f32", np.float32)
+        fx["ref_phi1_t4"] = rd(wd, "phi1_t4.f32", np.float32)
+        fx["ref_pot_t4"] = rd(wd, "potential_t4.f32", np.float32)
+        fx["ref_dev_c100"] = rd(wd, "relax_dev.f64", np.float64)
+        fx["ref_phi0_c100"] = rd(wd, "phi0_c100.f32", np.float32)
+        fx["ref_phi1_c100"] = rd(wd, "phi1_c100.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        fresh(wd)
+        # from a float fixed point (deviation exactly 0) a negative threshold
+        # never stops the loop: a queue of 25 equal deviations has a zero
+        # slope, which throws RelaxationFault; then the same with cycleSize 4
+        run(wd, ["mesh", "sync", "relax:2000:1", "relaxthr:-1:1:25", "relaxthr:-2:4:5", "dumpphi:fault"])
+        fx["ref_fault_iter"] = rd(wd, "relaxthr_iter.i64", np.int64)
+        fx["ref_fault"] = rd(wd, "relaxthr_fault_-1.f64", np.float64)
+        fx["ref_fault4"] = rd(wd, "relaxthr_fault_-2.f64", np.float64)
+        fx["ref_phi0_fault"] = rd(wd, "phi0_fault.f32", np.float32)
+        fx["ref_phi1_fault"] = rd(wd, "phi1_fault.f32", np.float32)
+    with tempfile.TemporaryDirectory() as wd:
+        fresh(wd)
+        run(wd, ["mesh", "sync", "randreset:7", "dumpphi:rr"])
+        fx["ref_phi0_rr"] = rd(wd, "phi0_rr.f32", np.float32)
+        fx["ref_phi1_rr"] = rd(wd, "phi1_rr.f32", np.float32)
+    for prob in (1, 2, 3):
+        with tempfile.TemporaryDirectory() as wd:
+            fresh(wd, temperature=BOLTZMANN_T if prob == 1 else 0.0)
+            run(wd, ["mesh", "sync", "relax:300:1", "evap:10:%d:0:1e-2:3:5:5:1:5:1e-3:1:0:1:1" % prob])
+            ints = rd(wd, "evap_int.i32", np.int32).reshape(-1, 4)
+            fx["ref_p%d_node" % prob] = ints[:, 0]
+            fx["ref_p%d_f" % prob] = rd(wd, "evap_f.f32", np.float32).reshape(-1, 22)
+            fx["ref_p%d_iter" % prob] = rd(wd, "evap_iter.i64", np.int64).reshape(-1, 2)
+    with tempfile.TemporaryDirectory() as wd:
+        fresh(wd)
+        run(wd, ["mesh", "sync", "relax:300:1", "evapnode:%d" % node, "relax:1:1", "traj:o1:%d:0:1" % node,
+                 "traj:s0:%d:1:0" % node, "traj:s2:%d:1:2" % node, "traj:s3:%d:1:3" % node])
+        for tag in ("o1", "s0", "s2", "s3"):
+            fx["ref_traj_" + tag] = rd(wd, "traj_%s.f32" % tag, np.float32).reshape(-1, 8)
+            fx["ref_traj_status_" + tag] = rd(wd, "traj_status_%s.i32" % tag, np.int32)
+    np.savez_compressed(os.path.join(OUT, "pinning.npz"), **fx)
+    print("pinning: %s" % ", ".join("%s%s" % (k, list(v.shape)) for k, v in sorted(fx.items())))
+
+
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref"], cwd=ROOT, check=True)
     if len(sys.argv) > 1 and sys.argv[1] == "formats":
         make_formats()
         sys.exit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "pinning":
+        make_pinning()
+        sys.exit(0)
     make_tetgen_mesh()
     make_small()
     make_formats()
+    make_pinning()
