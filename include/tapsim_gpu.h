@@ -132,6 +132,16 @@ int tsg_reset_potentials(tsg_grid*, int kind, const float* center, float pmin, f
 int tsg_potential(tsg_grid*, int node, double* out);
 /* Grid_3d::Table::field_o1(i) (grid_3d.cpp:909-991), 3 values per node */
 int tsg_field_o1(tsg_grid*, int count, const int* nodes, double* out);
+/* field_o1 of a node list, as tsg_field_o1, allocating no device memory.
+   No counterpart in the reference: for callers that cannot allocate (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The node list, fields and status word are staged in a
+   file-scope device array of the library (1.8 MB per GPU, held until the
+   process ends); count above TSG_NOALLOC_MAX_NODES fails with TSG_E_ARG.
+   Every call on one GPU shares that array: calls that overlap on one GPU
+   race and must be serialised by the caller. */
+#define TSG_NOALLOC_MAX_NODES 65536
+int tsg_field_o1_noalloc(tsg_grid*, int count, const int* nodes, double* out);
 /* Grid_3d::Table::field_o2(i, geometry, &charge) (grid_3d.cpp:993-1199): the
    quadratic least-squares field (3 values per node into out) and the charge
    -eps0 (phi_xx + phi_yy + phi_zz) of the fit (one per node into charge) */

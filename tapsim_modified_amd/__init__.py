@@ -132,6 +132,7 @@ _SIGS = {
     "tsg_potential": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_field_o2": ([_P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "tsg_field_o1_noalloc": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_get_couplings": ([_P, _P], ctypes.c_int),
     "tsg_get_charges": ([_P, _P], ctypes.c_int),
     "tsg_get_flags": ([_P, _P], ctypes.c_int),
@@ -385,6 +386,14 @@ class Grid3D:
         nodes = np.ascontiguousarray(np.atleast_1d(nodes), dtype=np.int32)
         out = np.zeros((len(nodes), 3), np.float64)
         _chk(load().tsg_field_o1(self._h, len(nodes), _ptr(nodes), _ptr(out)))
+        return out
+
+    def field_o1_noalloc(self, nodes):
+        """field_o1 through tsg_field_o1_noalloc (staging in the library's
+        static device array, no device allocation)"""
+        nodes = np.ascontiguousarray(np.atleast_1d(nodes), dtype=np.int32)
+        out = np.zeros((len(nodes), 3), np.float64)
+        _chk(load().tsg_field_o1_noalloc(self._h, len(nodes), _ptr(nodes), _ptr(out)))
         return out
 
     def field_o2(self, nodes):

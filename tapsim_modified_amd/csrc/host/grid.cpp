@@ -82,6 +82,7 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 		if (f == e || *f != i) { fail(-4, "asymmetric node adjacency"); return; }
 		mirror_[k] = f - mesh_.idx.data();
 	}
+	if (hip_check(hipGetDevice(&device_), "hipGetDevice")) return;
 	if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
 
 	// ---- geometry precompute on the device
@@ -755,6 +756,26 @@ int Grid::potential(int node, double* out)
 		HIPCHK(hipStreamSynchronize(stream_));
 		*out = v;
 	}
+	return 0;
+}
+
+int Grid::field_o1_staged(int count, const int* nodes, double* out, int* d_nodes, double* d_fields, int* d_status)
+{
+	if (count <= 0) return 0;
+	for (int i = 0; i < count; i++)
+		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "field_o1: bad node");
+	HIPCHK(hipMemcpyAsync(d_nodes, nodes, sizeof(int) * count, hipMemcpyHostToDevice, stream_));
+	// the status word is cleared by a kernel (no memset node if the stream is captured)
+	if (launch_fill_i32(stream_, 1, d_status, 0)) return fail(-2, "fill launch failed");
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	if (launch_node_fields(stream_, count, d_nodes, d_xyz_.as<float>(), d_ptr_.as<int>(), d_idx_.as<int>(),
+	                       d_coupfull_.ptr(), d_flags_.as<unsigned char>(), phi, f64_, d_fields, d_status))
+		return fail(-2, "field launch failed");
+	int st = 0;
+	HIPCHK(hipMemcpyAsync(out, d_fields, sizeof(double) * 3 * count, hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(&st, d_status, sizeof(int), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	if (st) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
 	return 0;
 }
 

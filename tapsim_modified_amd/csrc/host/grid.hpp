@@ -134,6 +134,11 @@ public:
 
 	int field_o1(int count, const int* nodes, double* out);
 	int field_o2(int count, const int* nodes, double* out, double* charge);
+	// field_o1 through caller-provided device staging (count node indices,
+	// 3 count doubles, one status int) instead of the grid's own buffers
+	int field_o1_staged(int count, const int* nodes, double* out, int* d_nodes, double* d_fields, int* d_status);
+	// the device the grid was created on (its streams and buffers live there)
+	int device() const { return device_; }
 	int get_couplings(double* out) const;
 	int get_charges(double* out) const;
 	// the fork's update_coordination (surface_3d.cpp:568-593) on the ids
@@ -246,6 +251,7 @@ private:
 	int resolve_candidate(int* node, int* near_tie);
 
 	int mode_;
+	int device_ = 0;
 	bool f64_;
 	int status_ = 0;
 	std::string err_;

@@ -111,6 +111,9 @@ struct LocalArgs {
 };
 int launch_local_relax(hipStream_t, const SweepArgs&, const LocalArgs&, bool f64);
 
+// dst[0..n) = value
+int launch_fill_i32(hipStream_t, int n, int* dst, int value);
+
 // generic scatter of patches: dst[addr[k]] = val[k]
 int launch_scatter_f64(hipStream_t, int n, const long long* addr, const double* val, double* dst);
 int launch_scatter_f32(hipStream_t, int n, const long long* addr, const float* val, float* dst);

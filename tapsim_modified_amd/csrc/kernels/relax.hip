@@ -480,6 +480,11 @@ __global__ void k_scatter_f32(int n, const long long* addr, const float* val, fl
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) dst[addr[i]] = val[i];
 }
+__global__ void k_fill_i32(int n, int* dst, int value)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dst[i] = value;
+}
 __global__ void k_scatter_i32(int n, const long long* addr, const int* val, int* dst)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,6 +550,12 @@ int launch_scatter_f32(hipStream_t s, int n, const long long* addr, const float*
 {
 	if (n <= 0) return 0;
 	hipLaunchKernelGGL(dev::k_scatter_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
+	return (int) hipGetLastError();
+}
+int launch_fill_i32(hipStream_t s, int n, int* dst, int value)
+{
+	if (n <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, n, dst, value);
 	return (int) hipGetLastError();
 }
 int launch_scatter_i32(hipStream_t s, int n, const long long* addr, const int* val, int* dst)
