@@ -111,6 +111,7 @@ _SIGS = {
     "ora_random_reset": ([_P], None),
     "ora_spherical_preset": ([_P, _P, ctypes.c_float, ctypes.c_float], None),
     "ora_relax_threaded": ([_P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_double),
+    "ora_set_threads": ([_P, ctypes.c_int], None),
     "ora_mpnn": ([_P, _P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "ora_bf16": ([ctypes.c_float], ctypes.c_float),
 }
@@ -261,6 +262,11 @@ class Oracle:
     def spherical_preset(self, center, pmin, pmax):
         c = np.ascontiguousarray(center, np.float32)
         load().ora_spherical_preset(self._h, _ptr(c), pmin, pmax)
+
+    def set_threads(self, threads):
+        """worker threads of the cycleSize-1 threshold relaxations (the event
+        loop's refresh relaxation): the reference's chunked threading"""
+        load().ora_set_threads(self._h, threads)
 
     def relax_threaded(self, sweeps, threads):
         """`sweeps` relax(1) calls on `threads` workers (FP64): (deviation, seconds)."""

@@ -32,6 +32,9 @@
  *   evap:...             the steps I-VI of Process::evaporation (process.cpp:680-1037)
  *   traj:TAG:NODE:INT:STEP  one Trajectory_3d from a node (trajectory_3d.cpp:99)
  *   randreset:SEED       std::srand(SEED), then Grid_3d::Table::randomReset (grid_3d.cpp:811)
+ *   threads:T            Grid_3d::Table::setThreadNum(T)  (grid_3d.cpp:194; the driver starts with 1)
+ *
+ * The wall time of every step (steady clock, seconds) goes to step_seconds.f64.
  */
 
 #include <cstdio>
@@ -45,6 +48,7 @@
 #include <queue>
 #include <limits>
 #include <cmath>
+#include <ctime>
 
 #include "system_3d.h"
 #include <unistd.h>
@@ -177,12 +181,15 @@ int main(int argc, char** argv)
 
 		std::vector<double> relaxDeviations;
 		std::vector<long long> relaxIterations;
+		std::vector<double> stepSeconds;
 
 		for (int a = 4; a < argc; a++)
 		{
 			const std::vector<std::string> tok = split(argv[a], ':');
 			const std::string& cmd = tok[0];
 			std::fprintf(stderr, "[ref_driver] step %s\n", argv[a]);
+			struct timespec stepStart;
+			clock_gettime(CLOCK_MONOTONIC, &stepStart);
 
 			if (cmd == "mesh")
 			{
@@ -288,6 +295,10 @@ int main(int argc, char** argv)
 				catch (Grid_3d::Table::RelaxationFault& e) { it = -1 - e.iteration(); }
 				relaxIterations.push_back(it);
 				writeArray("local_iter.i64", relaxIterations);
+			}
+			else if (cmd == "threads")
+			{
+				system.gridTable.setThreadNum((unsigned int) std::atoi(tok.at(1).c_str()));
 			}
 			else if (cmd == "randreset")
 			{
@@ -629,6 +640,11 @@ int main(int argc, char** argv)
 			}
 			else
 				throw std::runtime_error("unknown step " + cmd);
+			struct timespec stepEnd;
+			clock_gettime(CLOCK_MONOTONIC, &stepEnd);
+			stepSeconds.push_back((double) (stepEnd.tv_sec - stepStart.tv_sec) +
+			                      1e-9 * (double) (stepEnd.tv_nsec - stepStart.tv_nsec));
+			writeArray("step_seconds.f64", stepSeconds);
 		}
 	}
 	catch (std::exception& e)

@@ -71,6 +71,7 @@ struct ora_sys
 	   run evaporated in chunks equals one uninterrupted run: the event counter
 	   (process.cpp:676-696), the voltage queue (:983-1010), the initial apex */
 	long long event_cnt;   /* -1 until the first call sets it from initEventCnt */
+	int threads;           /* workers of the cycleSize-1 threshold relaxations (1: calling thread) */
 	float vq[4096];
 	int vq_head, vq_size;
 	float vq_sum;
@@ -232,6 +233,7 @@ ora_sys* ora_create(int mode, int n, const float* xyz, const short* ids,
 	ora_sys* s = (ora_sys*) calloc(1, sizeof(ora_sys));
 	if (!s) return NULL;
 	s->event_cnt = -1;
+	s->threads = 1;
 	s->mode = mode;
 	s->n = n;
 	s->m = m;
@@ -409,18 +411,36 @@ double ora_relax(ora_sys* s, int cycles, int* err)
 	return d;
 }
 
-/* relax(1) calls timed for the CPU baseline, FP64 mode only, with the
-   reference's threading of a cycle (cwrapper_threadedRelax,
-   grid_3d.cpp:1458-1514): T workers each take a contiguous chunk of about
-   N/T nodes, a barrier ends the cycle.  Every call is one Jacobi cycle, so
-   the potentials equal those of ora_relax(s, 1) repeated. */
+/* relax(1) calls with the reference's threading of a cycle
+   (cwrapper_threadedRelax, grid_3d.cpp:1458-1514): T workers each take a
+   contiguous chunk of about N/T nodes, a barrier ends the cycle, the first
+   worker forms the deviation and flips the switch.  A Jacobi cycle's values
+   do not depend on who computes them, so the potentials and deviations equal
+   ora_relax(s, 1) repeated.  With `ctl` the first worker also applies the
+   threshold relaxation's queue test and threshold after every cycle. */
+typedef struct
+{
+	double threshold;
+	unsigned int queue;
+	long long iteration;
+	int fault;
+	double fault_dev, fault_slope;
+} ora_thr_ctl;
+
+typedef struct ora_queue_s ora_queue;
+static int queue_slope_zero(ora_queue* q, double dev, unsigned int qsize, unsigned int cycles, double* slope_out);
+
 typedef struct
 {
 	ora_sys* s;
-	int sweeps, threads, id, err;
+	long long sweeps;
+	int threads, id, err;
 	double dev;
 	pthread_barrier_t* bar;
 	double* devs;
+	int* stop;
+	ora_thr_ctl* ctl;
+	ora_queue* q;
 } ora_relax_job;
 
 static void* relax_worker(void* arg)
@@ -429,49 +449,65 @@ static void* relax_worker(void* arg)
 	ora_sys* s = j->s;
 	const int n = s->n;
 	const int lo = (int) ((long long) n * j->id / j->threads), hi = (int) ((long long) n * (j->id + 1) / j->threads);
-	for (int it = 0; it < j->sweeps; it++)
+	for (long long it = 0; j->ctl || it < j->sweeps; it++)
 	{
-		const double* read = (const double*) (s->sw ? s->phi1 : s->phi0);
-		double* write = (double*) (s->sw ? s->phi0 : s->phi1);
-		int err = 0;
-		for (int i = lo; i < hi; i++) write[i] = compute_potential_f64(s, i, read, &err);
-		if (err) j->err = err;
 		double dev = 0.0;
-		for (int i = lo; i < hi; i++)
-		{
-			double ld = write[i];
-			ld -= read[i];
-			ld = fabs(ld);
-			if (ld > dev) dev = ld;
-		}
-		j->devs[j->id] = dev;
+		const int e = s->mode == ORA_F32 ? jacobi_range_f32(s, lo, hi, &dev) : jacobi_range_f64(s, lo, hi, &dev);
+		if (e) j->err = e;
+		j->devs[j->id] = e ? -1.0 : dev;
 		pthread_barrier_wait(j->bar);
 		if (j->id == 0)
 		{
 			double d = 0.0;
-			for (int t = 0; t < j->threads; t++) if (j->devs[t] > d) d = j->devs[t];
+			int bad = 0;
+			for (int t = 0; t < j->threads; t++)
+			{
+				if (j->devs[t] < 0.0) bad = 1;
+				if (j->devs[t] > d) d = j->devs[t];
+			}
 			j->dev = d;
 			s->sw = (s->sw + 1) % 2;
+			if (bad) *j->stop = 1;
+			else if (j->ctl)
+			{
+				double slope = 0.0;
+				if (queue_slope_zero(j->q, d, j->ctl->queue, 1u, &slope))
+				{
+					j->ctl->fault = 1;
+					j->ctl->fault_dev = d;
+					j->ctl->fault_slope = slope;
+					*j->stop = 1;
+				}
+				else
+				{
+					j->ctl->iteration += 1;
+					if (!(d > j->ctl->threshold)) *j->stop = 1;
+				}
+			}
 		}
 		pthread_barrier_wait(j->bar);
+		if (*j->stop) break;
 	}
 	return NULL;
 }
 
-double ora_relax_threaded(ora_sys* s, int sweeps, int threads, double* seconds, int* err)
+/* runs the workers; returns the last deviation */
+static double run_relax_workers(ora_sys* s, long long sweeps, int threads, ora_thr_ctl* ctl, ora_queue* q,
+                                double* seconds, int* err)
 {
-	if (s->mode != ORA_F64 || threads < 1 || threads > 256 || sweeps < 0) { if (err) *err = ORA_E_ARG; return 0.0; }
 	pthread_barrier_t bar;
 	pthread_barrier_init(&bar, NULL, (unsigned) threads);
 	ora_relax_job jobs[256];
 	double devs[256];
 	pthread_t tid[256];
+	int stop = 0;
 	struct timespec t0, t1;
 	clock_gettime(CLOCK_MONOTONIC, &t0);
 	for (int t = 0; t < threads; t++)
 	{
 		jobs[t].s = s; jobs[t].sweeps = sweeps; jobs[t].threads = threads; jobs[t].id = t; jobs[t].err = 0;
-		jobs[t].dev = 0.0; jobs[t].bar = &bar; jobs[t].devs = devs;
+		jobs[t].dev = 0.0; jobs[t].bar = &bar; jobs[t].devs = devs; jobs[t].stop = &stop; jobs[t].ctl = ctl;
+		jobs[t].q = q;
 		if (t > 0) pthread_create(&tid[t], NULL, relax_worker, &jobs[t]);
 	}
 	relax_worker(&jobs[0]);
@@ -485,9 +521,20 @@ double ora_relax_threaded(ora_sys* s, int sweeps, int threads, double* seconds, 
 	return jobs[0].dev;
 }
 
+double ora_relax_threaded(ora_sys* s, int sweeps, int threads, double* seconds, int* err)
+{
+	if (threads < 1 || threads > 256 || sweeps < 0) { if (err) *err = ORA_E_ARG; return 0.0; }
+	if (sweeps == 0) { if (seconds) *seconds = 0.0; if (err) *err = 0; return 0.0; }
+	return run_relax_workers(s, sweeps, threads, NULL, NULL, seconds, err);
+}
+
+/* worker threads of the threshold relaxations with cycleSize 1 (the event
+   loop's refresh relaxation); 1 runs them on the calling thread */
+void ora_set_threads(ora_sys* s, int threads) { s->threads = threads < 1 ? 1 : (threads > 256 ? 256 : threads); }
+
 /* the deviation queue and least-squares slope test of
    Grid_3d::Table::relax(threshold,cycleSize,queueSize), grid_3d.cpp:611-671 */
-typedef struct { double v[4096]; int head, size; } ora_queue;
+struct ora_queue_s { double v[4096]; int head, size; };
 
 static int queue_slope_zero(ora_queue* q, double dev, unsigned int qsize, unsigned int cycles, double* slope_out)
 {
@@ -522,6 +569,23 @@ long long ora_relax_threshold(ora_sys* s, double threshold, int cycles, int queu
 	long long iteration = 0;
 	double dev;
 	if (queue > 4096) { if (err) *err = ORA_E_ARG; return 0; }
+	if (cycles == 1 && s->threads > 1)
+	{
+		ora_thr_ctl c;
+		c.threshold = threshold; c.queue = (unsigned) queue; c.iteration = 0; c.fault = 0;
+		c.fault_dev = 0.0; c.fault_slope = 0.0;
+		int e = 0;
+		run_relax_workers(s, 0, s->threads, &c, &q, NULL, &e);
+		if (err) *err = e;
+		if (e) return c.iteration;
+		if (c.fault)
+		{
+			if (fault_dev) *fault_dev = c.fault_dev;
+			if (fault_slope) *fault_slope = c.fault_slope;
+			return -1 - c.iteration;
+		}
+		return c.iteration;
+	}
 	do
 	{
 		int e = 0;

@@ -117,6 +117,8 @@ double ora_relax(ora_sys*, int cycles, int* err);
 /* `sweeps` relax(1) calls with `threads` workers (FP64 mode; timing of the
    CPU baseline); wall time in *seconds, the last deviation returned */
 double ora_relax_threaded(ora_sys* s, int sweeps, int threads, double* seconds, int* err);
+/* worker threads of the threshold relaxations with cycleSize 1 (default 1) */
+void ora_set_threads(ora_sys* s, int threads);
 long long ora_relax_threshold(ora_sys*, double threshold, int cycles, int queue,
                               double* fault_dev, double* fault_slope, int* err);
 int ora_find_local_nodes(const ora_sys*, int node, int order, int* out, int cap);

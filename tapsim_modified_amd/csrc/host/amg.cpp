@@ -190,26 +190,29 @@ int Grid::amg_setup()
 	if (!accel_)
 		if (int rc = accel_setup(0.9)) return rc;
 	const int n = mesh_.n;
-	// level 0 in a compact index u over the row nodes, kept node-indexed on the device
-	std::vector<int> unode;
+	// level 0 in a compact index u over the row slots in slot order; on the
+	// device the level-0 vectors are slot-indexed
+	std::vector<int> unode, uslot;
 	std::vector<int> upos(n, -1);
 	for (int slot = 0; slot < next_spare_; slot++) {
 		const int node = slot_node_[slot];
 		if (node < 0) continue;
 		upos[node] = (int) unode.size();
 		unode.push_back(node);
+		uslot.push_back(slot);
 	}
 	Csr A;
 	A.n = (int) unode.size();
 	A.ptr.assign(A.n + 1, 0);
-	std::vector<float> dn(n, 1.0f);
+	const int nv = cap_slots_;
+	std::vector<float> dn(nv, 1.0f);
 	std::vector<int> cols;
 	std::vector<double> coups;
 	for (int u = 0; u < A.n; u++) {
 		const int i = unode[u];
 		double ws;
 		row_entries(i, &cols, &coups, &ws);
-		dn[i] = (float) ws;
+		dn[uslot[u]] = (float) ws;
 		std::vector<std::pair<int, double>> row;
 		row.push_back({u, ws});
 		for (size_t k = 0; k < cols.size(); k++)
@@ -225,7 +228,7 @@ int Grid::amg_setup()
 	// coarse levels (a captured iteration graph holds the old ones' pointers)
 	amg_graph_drop();
 	amg_levels_.clear();
-	std::vector<int> agg0_node(n, -1);
+	std::vector<int> agg0_slot(nv, -1);
 	Csr fine = A;
 	for (int l = 0; l < 12 && (l == 0 || fine.n > kAmgCoarsest); l++) {
 		std::vector<int> agg;
@@ -235,12 +238,12 @@ int Grid::amg_setup()
 		amg_levels_.push_back(std::unique_ptr<AmgLevelHost>(new AmgLevelHost));
 		AmgLevelHost& L = *amg_levels_.back();
 		L.n = nc;
-		// restriction members (finer-level indices; level 0 by node)
+		// restriction members (finer-level indices; level 0 by slot)
 		std::vector<int> rptr(nc + 1, 0), ridx(fine.n);
 		for (int i = 0; i < fine.n; i++) rptr[agg[i] + 1]++;
 		for (int I = 0; I < nc; I++) rptr[I + 1] += rptr[I];
 		std::vector<int> fill(rptr.begin(), rptr.end() - 1);
-		for (int i = 0; i < fine.n; i++) ridx[fill[agg[i]]++] = l == 0 ? unode[i] : i;
+		for (int i = 0; i < fine.n; i++) ridx[fill[agg[i]]++] = l == 0 ? uslot[i] : i;
 		std::vector<float> val(C.val.begin(), C.val.end()), dinv(nc);
 		for (int I = 0; I < nc; I++) {
 			double d = 1.0;
@@ -250,7 +253,7 @@ int Grid::amg_setup()
 		}
 		L.w = (float) (4.0 / (3.0 * jacobi_lmax(C)));
 		if (l == 0)
-			for (int u = 0; u < fine.n; u++) agg0_node[unode[u]] = agg[u];
+			for (int u = 0; u < fine.n; u++) agg0_slot[uslot[u]] = agg[u];
 		if (upload(&L.ptr, C.ptr) || upload(&L.col, C.col) || upload(&L.val, val) || upload(&L.dinv, dinv) ||
 		    upload(&L.rptr, rptr) || upload(&L.ridx, ridx) || (l > 0 && upload(&L.agg, agg)))
 			return fail(-2, "device allocation failed (multigrid levels)");
@@ -269,11 +272,11 @@ int Grid::amg_setup()
 		amg_levels_.clear();
 		return fail(-7, "multigrid: the rows do not coarsen");
 	}
-	if (upload(&amg_dn_, dn) || upload(&amg_agg0_, agg0_node)) return fail(-2, "device allocation failed (multigrid)");
+	if (upload(&amg_dn_, dn) || upload(&amg_agg0_, agg0_slot)) return fail(-2, "device allocation failed (multigrid)");
 	for (DevBuf* b : {&amg_r_, &amg_z_, &amg_p_, &amg_q_, &amg_x0_, &amg_res_, &amg_delta_}) {
-		b->alloc(sizeof(float) * n);
+		b->alloc(sizeof(float) * nv);
 		if (!b->ptr()) return fail(-2, "device allocation failed (multigrid vectors)");
-		HIPCHK(hipMemset(b->ptr(), 0, sizeof(float) * n));
+		HIPCHK(hipMemset(b->ptr(), 0, sizeof(float) * nv));
 	}
 	const int nb = std::max(row_launch_blocks(cap_slots_, long_cap_), (cap_slots_ + 255) / 256) + 8;
 	amg_part_.alloc(sizeof(double) * nb);
@@ -284,10 +287,10 @@ int Grid::amg_setup()
 	return 0;
 }
 
-// rows rewritten by a resync: the new row sums of rows that remain (or
-// became) unknown; a row that was switched off gets a zero residual entry,
+// rows rewritten by a resync: the new row sums (by slot) of rows that remain
+// (or became) unknown; a slot that lost its row gets a zero residual entry,
 // which its aggregate's restriction keeps reading
-int Grid::amg_touch(const std::vector<int>& nodes)
+int Grid::amg_touch(const std::vector<int>& nodes, const std::vector<int>& off_slots)
 {
 	if (!amg_on_) return 0;
 	std::vector<int> cols;
@@ -302,12 +305,12 @@ int Grid::amg_touch(const std::vector<int>& nodes)
 	}
 	const float zero = 0.0f;
 	for (size_t t = 0; t < nodes.size(); t++) {
-		const int node = nodes[t];
-		if (slot_of_[node] >= 0)
-			HIPCHK(hipMemcpyAsync(amg_dn_.as<float>() + node, &val[t], sizeof(float), hipMemcpyHostToDevice, stream_));
-		else
-			HIPCHK(hipMemcpyAsync(amg_res_.as<float>() + node, &zero, sizeof(float), hipMemcpyHostToDevice, stream_));
+		const int slot = slot_of_[nodes[t]];
+		if (slot >= 0)
+			HIPCHK(hipMemcpyAsync(amg_dn_.as<float>() + slot, &val[t], sizeof(float), hipMemcpyHostToDevice, stream_));
 	}
+	for (int slot : off_slots)
+		HIPCHK(hipMemcpyAsync(amg_res_.as<float>() + slot, &zero, sizeof(float), hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
 	return 0;
 }

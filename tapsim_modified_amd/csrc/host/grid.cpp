@@ -927,6 +927,10 @@ int Grid::set_id_and_resync(int node, int id)
 	std::vector<int> touched;
 	touched.push_back(node);
 	for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++) touched.push_back(mesh_.idx[k]);
+	// slots that lose their row (switched off, or moved to a spare slot) and
+	// the rows that moved: the correction rows of a moved row's neighbours
+	// name it by its slot, so they are rewritten too
+	std::vector<int> off_slots, moved;
 	for (int r : touched) {
 		const bool unknown = !(flags_[r] & 2);
 		int slot = slot_of_[r];
@@ -936,6 +940,7 @@ int Grid::set_id_and_resync(int node, int id)
 				set_long(slot, false);
 				ai.push_back((0ll << 60) | slot); vi.push_back(-1);
 				slot_of_[r] = -1;
+				off_slots.push_back(slot);
 			}
 			continue;
 		}
@@ -944,6 +949,8 @@ int Grid::set_id_and_resync(int node, int id)
 			slot_node_[slot] = -1;
 			set_long(slot, false);
 			ai.push_back((0ll << 60) | slot); vi.push_back(-1);
+			off_slots.push_back(slot);
+			moved.push_back(r);
 		}
 		if (next_spare_ >= cap_slots_) return fail(-7, "row slots exhausted (more than 4096 new unknown nodes)");
 		slot = next_spare_++;
@@ -954,11 +961,19 @@ int Grid::set_id_and_resync(int node, int id)
 	// same slots
 	std::vector<long long> fa;
 	std::vector<float> fv;
-	if (accel_)
-		for (int r : touched)
-			if (slot_of_[r] >= 0 && write_inner(slot_of_[r], r, &ai, &vi, &fa, &fv)) return fail(-7, "inner row too wide");
+	if (accel_) {
+		std::vector<int> inner_rows(touched);
+		for (int r : moved)
+			for (int k = mesh_.ptr[r]; k < mesh_.ptr[r + 1]; k++) inner_rows.push_back(mesh_.idx[k]);
+		std::sort(inner_rows.begin(), inner_rows.end());
+		inner_rows.erase(std::unique(inner_rows.begin(), inner_rows.end()), inner_rows.end());
+		for (int r : inner_rows)
+			if (slot_of_[r] >= 0)
+				if (int rc = write_inner(slot_of_[r], r, &ai, &vi, &fa, &fv))
+					return fail(-7, rc == -2 ? "inner row: an unknown neighbour without a row" : "inner row too wide");
+	}
 	if (amg_on_)
-		if (int rc = amg_touch(touched)) return rc;
+		if (int rc = amg_touch(touched, off_slots)) return rc;
 
 	// ship all patches through the pinned staging area
 	const size_t nr = ar.size(), nw = aw.size(), nf = full_addr.size();
@@ -1760,9 +1775,10 @@ AccelArgs Grid::accel_args() const
 	return x;
 }
 
-// the FP32 correction row of a slot: unknown participating neighbours with
-// weight c_ij / (row coupling sum); int patches target arrays 3 (inner_deg)
-// and 4 (inner columns), float patches the weights
+// the FP32 correction row of a slot: unknown participating neighbours (by
+// their slots: the correction vectors are slot-indexed) with weight
+// c_ij / (row coupling sum); int patches target arrays 3 (inner_deg) and 4
+// (inner columns), float patches the weights
 int Grid::write_inner(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* af,
                       std::vector<float>* vf)
 {
@@ -1774,8 +1790,10 @@ int Grid::write_inner(int slot, int node, std::vector<long long>* ai, std::vecto
 	int deg = 0;
 	for (size_t k = 0; k < cols.size(); k++) {
 		if (flags_[cols[k]] & 2) continue;
+		const int js = slot_of_[cols[k]];
+		if (js < 0) return -2;  // an unknown neighbour without a row
 		const long long e = base + (long long) kSlice * deg;
-		ai->push_back((4ll << 60) | e); vi->push_back(cols[k]);
+		ai->push_back((4ll << 60) | e); vi->push_back(js);
 		af->push_back(e); vf->push_back((float) (coups[k] / ws));
 		deg++;
 	}
@@ -1799,44 +1817,46 @@ int Grid::accel_setup(double safety)
 		std::vector<long long> ai, af;
 		std::vector<int> vi;
 		std::vector<float> vf;
-		if (write_inner(slot, node, &ai, &vi, &af, &vf)) return fail(-7, "inner row too wide");
+		if (int rc = write_inner(slot, node, &ai, &vi, &af, &vf))
+			return fail(-7, rc == -2 ? "inner row: an unknown neighbour without a row" : "inner row too wide");
 		for (size_t k = 0; k < af.size(); k++) iw[af[k]] = vf[k];
 		for (size_t k = 0; k < ai.size(); k++)
 			if ((ai[k] >> 60) == 4) icol[ai[k] & ((1ll << 60) - 1)] = vi[k];
 	}
+	const int nv = cap_slots_;  // correction vectors: one entry per row slot
 	d_inner_deg_.alloc(sizeof(int) * cap_slots_);
 	d_icol_.alloc(sizeof(int) * nent);
 	d_iw_.alloc(sizeof(float) * nent);
-	d_d0_.alloc(sizeof(float) * n);
-	d_d1_.alloc(sizeof(float) * n);
-	d_r32_.alloc(sizeof(float) * n);
+	d_d0_.alloc(sizeof(float) * nv);
+	d_d1_.alloc(sizeof(float) * nv);
+	d_r32_.alloc(sizeof(float) * nv);
 	d_redout_.alloc(sizeof(double) * 2);
 	if (!d_icol_.ptr() || !d_iw_.ptr() || !d_d0_.ptr()) return fail(-2, "device allocation failed (accelerated rows)");
 	HIPCHK(hipMemcpy(d_inner_deg_.ptr(), inner_deg_.data(), sizeof(int) * cap_slots_, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_icol_.ptr(), icol.data(), sizeof(int) * nent, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_iw_.ptr(), iw.data(), sizeof(float) * nent, hipMemcpyHostToDevice));
-	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * n));
-	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * n));
-	HIPCHK(hipMemset(d_r32_.ptr(), 0, sizeof(float) * n));
+	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * nv));
+	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * nv));
+	HIPCHK(hipMemset(d_r32_.ptr(), 0, sizeof(float) * nv));
 	accel_ = true;
 
 	// spectral radius of G by power iteration with a D-weighted Rayleigh
 	// quotient (G = D^-1 C is self-adjoint in that inner product)
-	std::vector<float> x(n, 0.0f), y(n, 0.0f);
-	std::vector<double> d(n, 0.0);
+	std::vector<float> x(nv, 0.0f), y(nv, 0.0f);
+	std::vector<double> d(nv, 0.0);
 	for (int slot = 0; slot < next_spare_; slot++) {
 		const int node = slot_node_[slot];
 		if (node < 0) continue;
-		x[node] = 1.0f;
+		x[slot] = 1.0f;
 		double ws = 0.0;
 		for (int k = mesh_.ptr[node]; k < mesh_.ptr[node + 1]; k++)
 			if (participates(node, mesh_.idx[k])) ws += coup_[k];
-		d[node] = ws;
+		d[slot] = ws;
 	}
 	AccelArgs xa = accel_args();
 	double lambda = 0.0;
 	for (int chunk = 0; chunk < 8; chunk++) {
-		HIPCHK(hipMemcpy(d_d0_.ptr(), x.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d_d0_.ptr(), x.data(), sizeof(float) * nv, hipMemcpyHostToDevice));
 		float* in = d_d0_.as<float>();
 		float* out = d_d1_.as<float>();
 		for (int it = 0; it < 150; it++) {
@@ -1845,22 +1865,22 @@ int Grid::accel_setup(double safety)
 		}
 		if (launch_apply_g(stream_, xa, in, out)) return fail(-2, "power iteration launch failed");
 		HIPCHK(hipStreamSynchronize(stream_));
-		HIPCHK(hipMemcpy(x.data(), in, sizeof(float) * n, hipMemcpyDeviceToHost));
-		HIPCHK(hipMemcpy(y.data(), out, sizeof(float) * n, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(x.data(), in, sizeof(float) * nv, hipMemcpyDeviceToHost));
+		HIPCHK(hipMemcpy(y.data(), out, sizeof(float) * nv, hipMemcpyDeviceToHost));
 		double num = 0.0, den = 0.0, mx = 0.0;
-		for (int i = 0; i < n; i++) {
+		for (int i = 0; i < nv; i++) {
 			num += d[i] * (double) x[i] * (double) y[i];
 			den += d[i] * (double) x[i] * (double) x[i];
 			mx = std::max(mx, (double) std::fabs(y[i]));
 		}
 		if (den <= 0.0 || mx <= 0.0) return fail(-4, "power iteration lost its vector");
 		lambda = num / den;
-		for (int i = 0; i < n; i++) x[i] = (float) (y[i] / mx);
+		for (int i = 0; i < nv; i++) x[i] = (float) (y[i] / mx);
 	}
 	rho_ = 1.0 - (1.0 - lambda) * safety;
 	if (!(rho_ > 0.0 && rho_ < 1.0)) return fail(-4, "spectral radius estimate out of (0,1)");
-	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * n));
-	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * n));
+	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * nv));
+	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * nv));
 	return 0;
 }
 
@@ -1999,8 +2019,8 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 		HIPCHK(hipEventRecord(ev_stop_, stream_));
 		HIPCHK(hipEventSynchronize(ev_stop_));
 		HIPCHK(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
-		HIPCHK(hipMemsetAsync(d_d0_.ptr(), 0, sizeof(float) * n, stream_));
-		HIPCHK(hipMemsetAsync(d_d1_.ptr(), 0, sizeof(float) * n, stream_));
+		HIPCHK(hipMemsetAsync(d_d0_.ptr(), 0, sizeof(float) * cap_slots_, stream_));
+		HIPCHK(hipMemsetAsync(d_d1_.ptr(), 0, sizeof(float) * cap_slots_, stream_));
 		if (bytes) {
 			// the FP32 sweep's 8 nnz + 12 rows + 4 referenced: per entry column +
 			// weight, per row its offset, r and the written iterate, every
@@ -2035,7 +2055,7 @@ int Grid::time_kernel(int kind, int reps, double* avg_ms, double* bytes)
 		// a live solve state: done cleared, p = the current FP32 residual
 		const double zero = 0.0;
 		HIPCHK(hipMemcpyAsync(S.st + kAmgDone, &zero, sizeof(double), hipMemcpyHostToDevice, stream_));
-		HIPCHK(hipMemcpyAsync(S.p, d_r32_.ptr(), sizeof(float) * n, hipMemcpyDeviceToDevice, stream_));
+		HIPCHK(hipMemcpyAsync(S.p, d_r32_.ptr(), sizeof(float) * cap_slots_, hipMemcpyDeviceToDevice, stream_));
 		if (launch_amg_fine_matvec(stream_, S)) return fail(-2, "multigrid launch failed");
 		HIPCHK(hipEventRecord(ev_start_, stream_));
 		for (int r = 0; r < reps; r++)

@@ -353,7 +353,7 @@ private:
 	double* amg_host_ = nullptr;  // pinned readback of the scalars, two batches
 	hipEvent_t amg_ev_[2] = {nullptr, nullptr};
 	void amg_graph_drop();
-	int amg_touch(const std::vector<int>& nodes);
+	int amg_touch(const std::vector<int>& nodes, const std::vector<int>& off_slots);
 	int amg_solve(const AccelArgs& x, float f, int* iters, float** delta);
 	// trajectories: mesh arrays for the point location, node-field snapshot
 	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_, d_rng_;

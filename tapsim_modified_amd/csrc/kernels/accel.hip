@@ -23,8 +23,8 @@ __device__ __forceinline__ double wave_maxd(double v)
 	return v;
 }
 
-// FP64 residual of the current potentials; writes r (float) per node and the
-// per-block max |r|; rows without a slot node are skipped
+// FP64 residual of the current potentials; writes r (float) per row slot and
+// the per-block max |r|; slots without a node are skipped
 __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __restrict__ phi, float* __restrict__ r32)
 {
 	const int main_blocks = (a.nslots + 255) >> 8;
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __re
 		lv /= ws;
 		const double r = lv - old;
 		if (r != r) nan = 1;
-		r32[node] = (float) r;
+		r32[slot] = (float) r;
 		rmax = fabs(r);
 	}
 	__shared__ double smax[4];
@@ -112,31 +112,30 @@ __global__ __launch_bounds__(256) void k_reduce_max(int nblocks, const double* b
 __global__ void k_cheb_init(AccelArgs x, const float* __restrict__ r32, float* __restrict__ dprev, float* __restrict__ dcur)
 {
 	const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-	if (slot >= x.nslots) return;
-	const int node = x.slot_node[slot];
-	if (node < 0) return;
-	dprev[node] = 0.0f;
-	dcur[node] = r32[node];
+	if (slot >= x.nslots || x.slot_node[slot] < 0) return;
+	dprev[slot] = 0.0f;
+	dcur[slot] = r32[slot];
 }
 
-// one Chebyshev step in FP32: dnext overwrites dprev node by node
+// one Chebyshev step in FP32: dnext overwrites dprev slot by slot
 __global__ __launch_bounds__(256) void k_cheb32(AccelArgs x, const float* __restrict__ r32, const float* __restrict__ dcur,
                                               float* __restrict__ dprev_next, float omega)
 {
 	float s, r = 0.0f, prev = 0.0f;
-	const int node = accel_row(x, dcur, &s, r32, &r, dprev_next, &prev);
-	if (node < 0) return;
-	dprev_next[node] = fmaf(omega, s + r - prev, prev);
+	const int i = accel_row(x, dcur, &s, r32, &r, dprev_next, &prev);
+	if (i < 0) return;
+	dprev_next[i] = fmaf(omega, s + r - prev, prev);
 }
 
-// phi += delta on the rows, in FP64; both potential buffers take the result
+// phi += delta on the rows, in FP64 (delta by slot, phi by node); both
+// potential buffers take the result
 __global__ void k_apply64(AccelArgs x, const float* __restrict__ delta, double* __restrict__ phi, double* __restrict__ phi_other)
 {
 	const int slot = blockIdx.x * blockDim.x + threadIdx.x;
 	if (slot >= x.nslots) return;
 	const int node = x.slot_node[slot];
 	if (node < 0) return;
-	const double v = phi[node] + (double) delta[node];
+	const double v = phi[node] + (double) delta[slot];
 	phi[node] = v;
 	if (phi_other) phi_other[node] = v;
 }
@@ -145,8 +144,8 @@ __global__ void k_apply64(AccelArgs x, const float* __restrict__ delta, double* 
 __global__ __launch_bounds__(256) void k_apply_g(AccelArgs x, const float* __restrict__ in, float* __restrict__ out)
 {
 	float s;
-	const int node = accel_row(x, in, &s, nullptr, nullptr, nullptr, nullptr);
-	if (node >= 0) out[node] = s;
+	const int i = accel_row(x, in, &s, nullptr, nullptr, nullptr, nullptr);
+	if (i >= 0) out[i] = s;
 }
 
 }  // namespace dev

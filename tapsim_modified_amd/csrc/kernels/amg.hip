@@ -12,8 +12,9 @@
 // Galerkin coarse operators, one damped-Jacobi step before and after each
 // coarse correction, and the coarsest level solved with its dense inverse.
 //
-// Level 0 is the grid's own FP32 correction rows (SELL-64, node-indexed
-// vectors); the coarse levels are CSR.  Every reduction is summed in a fixed
+// Level 0 is the grid's own FP32 correction rows (SELL-64; the vectors are
+// indexed by row slot, dense over the unknown rows, and the rows' columns are
+// slots); the coarse levels are CSR.  Every reduction is summed in a fixed
 // order (block partials, then one block sums them in block order), so the
 // solve is deterministic.  Smoothing steps are fused into the
 // neighbouring kernels: the pre-smoothing from zero (x0 = w b / D) is formed
@@ -81,15 +82,16 @@ __global__ __launch_bounds__(1024) void k_amg_scalars(int nblocks, const double*
 // the solve has met its target: the remaining launches return at once
 __device__ __forceinline__ bool amg_done(const double* st) { return st[kAmgDone] != 0.0; }
 
-// level-0 slot loop: node of this thread's slot (or -1)
-__device__ __forceinline__ int slot_node_of(const AccelArgs& x)
+// level-0 slot loop: this thread's slot when it holds a row (or -1)
+__device__ __forceinline__ int active_slot(const AccelArgs& x)
 {
 	const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-	if (slot >= x.nslots) return -1;
-	return x.slot_node[slot];
+	if (slot >= x.nslots || x.slot_node[slot] < 0) return -1;
+	return slot;
 }
 
-// the row operand of this thread on level 0 (G in over the correction rows)
+// the row operand of this thread on level 0 (G in over the correction rows);
+// returns the slot
 __device__ __forceinline__ int fine_row(const AccelArgs& x, const float* __restrict__ in, float* gin)
 {
 	return accel_row(x, in, gin, nullptr, nullptr, nullptr, nullptr);
@@ -98,14 +100,14 @@ __device__ __forceinline__ int fine_row(const AccelArgs& x, const float* __restr
 // r = D r32, delta = 0, x0 = w0 r / D; partials of r.r
 __global__ __launch_bounds__(256) void k_amg_fine_start(AmgSolve S, const float* __restrict__ r32)
 {
-	const int node = slot_node_of(S.x);
+	const int i = active_slot(S.x);
 	double c = 0.0;
-	if (node >= 0) {
-		const float d = S.dn[node];
-		const float v = d * r32[node];
-		S.r[node] = v;
-		S.delta[node] = 0.0f;
-		S.x0[node] = S.w0 * v / d;
+	if (i >= 0) {
+		const float d = S.dn[i];
+		const float v = d * r32[i];
+		S.r[i] = v;
+		S.delta[i] = 0.0f;
+		S.x0[i] = S.w0 * v / d;
 		c = (double) v * (double) v;
 	}
 	block_sum(c, S.part);
@@ -116,12 +118,12 @@ __global__ __launch_bounds__(256) void k_amg_fine_matvec(AmgSolve S)
 {
 	if (amg_done(S.st)) return;
 	float g;
-	const int node = fine_row(S.x, S.p, &g);
+	const int i = fine_row(S.x, S.p, &g);
 	double c = 0.0;
-	if (node >= 0) {
-		const float pi = S.p[node];
-		const float v = S.dn[node] * (pi - g);
-		S.q[node] = v;
+	if (i >= 0) {
+		const float pi = S.p[i];
+		const float v = S.dn[i] * (pi - g);
+		S.q[i] = v;
 		c = (double) pi * (double) v;
 	}
 	block_sum(c, S.part);
@@ -131,14 +133,14 @@ __global__ __launch_bounds__(256) void k_amg_fine_matvec(AmgSolve S)
 __global__ __launch_bounds__(256) void k_amg_fine_update(AmgSolve S)
 {
 	if (amg_done(S.st)) return;
-	const int node = slot_node_of(S.x);
+	const int i = active_slot(S.x);
 	double c = 0.0;
-	if (node >= 0) {
+	if (i >= 0) {
 		const float alpha = (float) S.st[kAmgAlpha];
-		S.delta[node] += alpha * S.p[node];
-		const float rv = S.r[node] - alpha * S.q[node];
-		S.r[node] = rv;
-		S.x0[node] = S.w0 * rv / S.dn[node];
+		S.delta[i] += alpha * S.p[i];
+		const float rv = S.r[i] - alpha * S.q[i];
+		S.r[i] = rv;
+		S.x0[i] = S.w0 * rv / S.dn[i];
 		c = (double) rv * (double) rv;
 	}
 	block_sum(c, S.part);
@@ -149,18 +151,18 @@ __global__ __launch_bounds__(256) void k_amg_fine_resid(AmgSolve S)
 {
 	if (amg_done(S.st)) return;
 	float g;
-	const int node = fine_row(S.x, S.x0, &g);
-	if (node >= 0) S.res[node] = S.r[node] - S.dn[node] * (S.x0[node] - g);
+	const int i = fine_row(S.x, S.x0, &g);
+	if (i >= 0) S.res[i] = S.r[i] - S.dn[i] * (S.x0[i] - g);
 }
 
 // x0 += e[agg0] (the coarse correction; rows outside every aggregate keep x0)
 __global__ __launch_bounds__(256) void k_amg_fine_prolong(AmgSolve S, const float* __restrict__ e)
 {
 	if (amg_done(S.st)) return;
-	const int node = slot_node_of(S.x);
-	if (node < 0) return;
-	const int a = S.agg0[node];
-	if (a >= 0) S.x0[node] += e[a];
+	const int i = active_slot(S.x);
+	if (i < 0) return;
+	const int a = S.agg0[i];
+	if (a >= 0) S.x0[i] += e[a];
 }
 
 // z = x0 + w0 (r / D - x0 + G x0); partials of r.z
@@ -168,13 +170,13 @@ __global__ __launch_bounds__(256) void k_amg_fine_post(AmgSolve S)
 {
 	if (amg_done(S.st)) return;
 	float g;
-	const int node = fine_row(S.x, S.x0, &g);
+	const int i = fine_row(S.x, S.x0, &g);
 	double c = 0.0;
-	if (node >= 0) {
-		const float bi = S.r[node];
-		const float xi = S.x0[node];
-		const float v = xi + S.w0 * (bi / S.dn[node] - xi + g);
-		S.z[node] = v;
+	if (i >= 0) {
+		const float bi = S.r[i];
+		const float xi = S.x0[i];
+		const float v = xi + S.w0 * (bi / S.dn[i] - xi + g);
+		S.z[i] = v;
 		c = (double) bi * (double) v;
 	}
 	block_sum(c, S.part);
@@ -184,9 +186,9 @@ __global__ __launch_bounds__(256) void k_amg_fine_post(AmgSolve S)
 __global__ __launch_bounds__(256) void k_amg_fine_dir(AmgSolve S, int first)
 {
 	if (amg_done(S.st)) return;
-	const int node = slot_node_of(S.x);
-	if (node < 0) return;
-	S.p[node] = first ? S.z[node] : S.z[node] + (float) S.st[kAmgBeta] * S.p[node];
+	const int i = active_slot(S.x);
+	if (i < 0) return;
+	S.p[i] = first ? S.z[i] : S.z[i] + (float) S.st[kAmgBeta] * S.p[i];
 }
 
 // ---- coarse levels (CSR)

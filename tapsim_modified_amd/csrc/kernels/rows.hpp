@@ -221,9 +221,11 @@ __device__ __forceinline__ float long_row_fma_f32(const int* __restrict__ col, c
 }
 
 // G x over the FP32 correction rows for this thread's slot (or long row):
-// returns the node (-1 when none) and the sum; the per-slot record and the
-// node's extra operands (e1[node], e2[node], either may be null) are loaded
-// in one round before the row
+// returns the slot (-1 when none) and the sum.  The correction vectors are
+// slot-indexed (dense over the unknown rows) and the rows' columns are the
+// neighbours' slots; the per-slot record and the slot's extra operands
+// (e1[slot], e2[slot], either may be null) are loaded in one round before
+// the row
 __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __restrict__ in, float* sum,
                                          const float* __restrict__ e1, float* v1, const float* __restrict__ e2, float* v2)
 {
@@ -255,17 +257,17 @@ __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __rest
 	if (!rb.is_long) {  // block-uniform; the row sum needs the whole wavefront
 		const bool act = node >= 0 && deg <= kLongRow;
 		if (act) {
-			if (e1) *v1 = e1[node];
-			if (e2) *v2 = e2[node];
+			if (e1) *v1 = e1[slot];
+			if (e2) *v2 = e2[slot];
 		}
 		*sum = short_row_fma_f32(x.col, x.w, in, act, ideg, base);
-		return act ? node : -1;
+		return act ? slot : -1;
 	}
 	if (node < 0) return -1;
-	if (e1) *v1 = e1[node];
-	if (e2) *v2 = e2[node];
+	if (e1) *v1 = e1[slot];
+	if (e2) *v2 = e2[slot];
 	*sum = long_row_fma_f32(x.col, x.w, in, ideg, base);
-	return lane_id() == 0 ? node : -1;
+	return lane_id() == 0 ? slot : -1;
 }
 
 }  // namespace dev

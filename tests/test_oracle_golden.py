@@ -183,6 +183,56 @@ def test_threaded_relax_equals_serial(tip_small):
     assert np.array_equal(a.phi(0), b.phi(0)) and np.array_equal(a.phi(1), b.phi(1))
 
 
+@pytest.mark.parametrize("mode", [O.F32, O.F64])
+def test_threaded_threshold_relaxation_equals_serial(tip_small, mode):
+    """Threshold relaxations with cycleSize 1 on worker threads (the event
+    loop's refresh relaxation of the CPU baseline): the serial iterations,
+    potentials and fault, and in float the reference's own"""
+    def make_o(threads):
+        o = O.Oracle(mode, tip_small["xyz"], tip_small["ids"], tip_small["tets"], tip_small["ref_neigh"],
+                     O.species_table(FIXTURE_SPECIES))
+        o.sync()
+        o.set_threads(threads)
+        return o
+    a, b = make_o(1), make_o(5)
+    assert a.relax_threshold(0.5, 1, 4) == b.relax_threshold(0.5, 1, 4)
+    if mode == O.F32:
+        assert b.relax_threshold(0.0, 1, 0)[0] > 0
+        a.relax_threshold(0.0, 1, 0)
+    assert np.array_equal(a.phi(0), b.phi(0)) and np.array_equal(a.phi(1), b.phi(1))
+    if mode == O.F32:
+        assert bits_equal(b.phi(0), a.phi(0))
+    # a stagnating queue: the same RelaxationFault on both
+    assert a.relax_threshold(-1.0, 1, 25) == b.relax_threshold(-1.0, 1, 25)
+    assert a.switches() == b.switches()
+    # the float threaded sweep of the CPU calibration
+    c = make_o(1)
+    d1, _ = c.relax_threaded(9, 3)
+    e = make_o(1)
+    for _ in range(9):
+        d2 = e.relax(1)
+    assert d1 == d2 and np.array_equal(c.phi(0), e.phi(0)) and np.array_equal(c.phi(1), e.phi(1))
+
+
+def test_threaded_event_loop_equals_serial(tip_small):
+    """The event loop with its refresh relaxation on worker threads gives the
+    serial loop's events (the CPU baseline times the threaded one)"""
+    def run(threads):
+        o = make(tip_small)
+        o.sync()
+        for _ in range(300):
+            o.relax(1)
+        o.set_threads(threads)
+        o.srand(99)
+        return o.evaporate(evap_opts(), 8), o.phi(0)
+    e1, p1 = run(1)
+    e4, p4 = run(4)
+    assert [e.node for e in e1] == [e.node for e in e4]
+    assert [e.refresh_iter for e in e1] == [e.refresh_iter for e in e4]
+    assert bits_equal(p1, p4)
+    assert np.array_equal([e.node for e in e1], tip_small["ref_evap_node"][:8])
+
+
 def test_fieldlines(tip_small):
     """Fieldline_3d (init + compute) from six points above the tip, one after
     the other on one rand() stream: to the mesh's end (a), then limited to
