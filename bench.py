@@ -109,7 +109,7 @@ def oracle_options(O, e):
                          inplace_order=0, init_event_cnt=e.init_event_cnt)
 
 
-def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, state0, plain_opts, first_event, phi_check, budget_s=20.0):
+def cpu_baseline(tsg, coup0, xyz, ids, tets, neigh, state0, plain_opts, first_event, check, budget_s=20.0):
     """The FP64 oracle on the host cores: one complete plain event from the
     same relaxed state as the GPU's plain run (choice, take-away and resync,
     local relaxation, relax(1), the refresh relaxation to 1e-8 V on worker
@@ -117,13 +117,15 @@ def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, state0, plain_opts, first_eve
     the ion's flight, voltage), timed whole; its node and sweep count are
     checked against the GPU's first plain event (the two paths are bit-exact).
     Beside it, a bounded sample of threaded relax(1) sweeps, and the oracle's
-    FP64 sweep on `phi_check` (the accelerated path's final potentials): its
-    deviation is the scaled residual max |J(phi) - phi| checked on the CPU."""
+    FP64 sweep on `check` = (potentials, ids, couplings) of the accelerated
+    path's final state: its deviation is the scaled residual max |J(phi) - phi|
+    checked on the CPU."""
     from oracle import oracle as O
     threads = host_cores()
     o = O.Oracle(O.F64, xyz, ids, tets, neigh, O.species_table(SPECIES))
-    # couplings from the device precompute (setup, which the reference's timer also excludes)
-    o.sync_from_couplings(grid.couplings())
+    # couplings from the device precompute of the unevaporated specimen (setup,
+    # which the reference's timer also excludes)
+    o.sync_from_couplings(coup0)
     o.set_phi(*state0)
     o.set_threads(threads)
     o.surface_init(plain_opts.vacuum_id)
@@ -135,6 +137,10 @@ def cpu_baseline(tsg, grid, xyz, ids, tets, neigh, state0, plain_opts, first_eve
     _, t_probe = o.relax_threaded(4, threads)
     k = max(4, int(0.25 * budget_s / max(t_probe / 4, 1e-6)))
     _, t_total = o.relax_threaded(k, threads)
+    phi_check, ids_check, coup_check = check
+    for node in np.nonzero(ids_check != o.ids())[0]:
+        o.set_id(int(node), int(ids_check[node]))
+    o.sync_from_couplings(coup_check)
     o.set_phi(phi_check, phi_check, 0, 0)
     residual, _ = o.relax_threaded(1, threads)
     return residual, dict(value=1.0 / t_event, unit="events/s", cores=threads, kind="port",
@@ -247,6 +253,7 @@ def main():
     rho = plain.accel_setup(0.9)
     outer, inner, res = plain.accel_relax(1e-8)
     state0 = (plain.phi(0), plain.phi(1)) + plain.switches()
+    coup0 = plain.couplings() if not args.no_cpu else None
     log("grid nnz %d; initial relaxation to 1e-8 V: rho %.8f, %d refinements, %d correction steps (%.1fs)"
         % (plain.nnz, rho, outer, inner, time.perf_counter() - t0))
 
@@ -313,6 +320,7 @@ def main():
     # rounding floor is ~1e-11 V at 5 kV)
     sw = learned.switches()[0]
     phi_run = learned.phi(sw)
+    check = (phi_run, learned.ids(), learned.couplings()) if not args.no_cpu else None
     learned.accel_relax(1e-10)
     phi_tight = learned.phi(learned.switches()[0])
     err = float(np.max(np.abs(phi_run - phi_tight)))
@@ -366,8 +374,8 @@ def main():
     }
     if not args.no_cpu:
         tv, tn = mesh.tets()
-        residual, res["cpu_baseline"] = cpu_baseline(tsg, plain, xyz, ids, tv, tn, state0, plain_opts, pev[0],
-                                                     phi_run)
+        residual, res["cpu_baseline"] = cpu_baseline(tsg, coup0, xyz, ids, tv, tn, state0, plain_opts, pev[0],
+                                                     check)
         res["potential_error"]["oracle_fp64_residual_V"] = residual
         res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]
     print(json.dumps(res))

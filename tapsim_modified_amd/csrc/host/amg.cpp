@@ -167,7 +167,7 @@ bool same_rows(const AccelArgs& a, const AccelArgs& b)
 {
 	return a.nslots == b.nslots && a.nlong == b.nlong && a.slot_node == b.slot_node && a.inner_deg == b.inner_deg &&
 	       a.slice_off == b.slice_off && a.col == b.col && a.w == b.w && a.slot_deg == b.slot_deg &&
-	       a.long_slots == b.long_slots;
+	       a.long_slots == b.long_slots && a.long_off == b.long_off && a.licol == b.licol && a.liw == b.liw;
 }
 
 template <typename T>

@@ -413,6 +413,36 @@ int Grid::build_rows()
 		if (slot_deg_[sl] > kLongRow) long_set_.insert(sl);
 	long_cap_ = (int) long_set_.size() + 4096;
 	d_long_.alloc(sizeof(int) * (size_t) long_cap_);
+	d_long_off_.alloc(sizeof(long long) * (size_t) long_cap_);
+	// the long rows' contiguous copies, with room for 4096 new long rows of 64 entries
+	loff_.assign(cap_slots_, -1);
+	lcap_.assign(cap_slots_, 0);
+	lnext_ = 0;
+	for (int sl : long_set_) {
+		loff_[sl] = lnext_;
+		lcap_[sl] = slot_deg_[sl];
+		lnext_ += slot_deg_[sl];
+	}
+	lstore_cap_ = lnext_ + 4096ll * 64;
+	{
+		std::vector<int> lc(lstore_cap_, 0);
+		std::vector<double> lk(lstore_cap_, 0.0);
+		for (int sl : long_set_)
+			for (int k = 0; k < slot_deg_[sl]; k++) {
+				lc[loff_[sl] + k] = ecols[sl][k];
+				lk[loff_[sl] + k] = ecoup[sl][k];
+			}
+		d_lcol_.alloc(sizeof(int) * (size_t) lstore_cap_);
+		d_lcoup_.alloc(rs * (size_t) lstore_cap_);
+		if (!d_lcol_.ptr() || !d_lcoup_.ptr() || !d_long_off_.ptr()) return fail(-2, "device allocation failed (long rows)");
+		HIPCHK(hipMemcpy(d_lcol_.ptr(), lc.data(), sizeof(int) * lc.size(), hipMemcpyHostToDevice));
+		if (f64_) {
+			HIPCHK(hipMemcpy(d_lcoup_.ptr(), lk.data(), sizeof(double) * lk.size(), hipMemcpyHostToDevice));
+		} else {
+			std::vector<float> l32(lk.begin(), lk.end());
+			HIPCHK(hipMemcpy(d_lcoup_.ptr(), l32.data(), sizeof(float) * l32.size(), hipMemcpyHostToDevice));
+		}
+	}
 	// one partial per workgroup of the largest row launch (row_launch_blocks:
 	// slot groups + long-row groups, padded to a multiple of eight)
 	const size_t max_blocks = (size_t) row_launch_blocks(cap_slots_, long_cap_) + 8;
@@ -456,6 +486,9 @@ SweepArgs Grid::sweep_args()
 	a.threshold_mode = 1;
 	a.long_slots = d_long_.as<int>();
 	a.nlong = (int) long_set_.size();
+	a.long_off = d_long_off_.as<long long>();
+	a.lcol = d_lcol_.as<int>();
+	a.lcoup = d_lcoup_.ptr();
 	return a;
 }
 
@@ -471,7 +504,12 @@ int Grid::upload_long()
 	if (!long_dirty_) return 0;
 	if ((int) long_set_.size() > long_cap_) return fail(-7, "long-row list full");
 	std::vector<int> v(long_set_.begin(), long_set_.end());
-	if (!v.empty()) HIPCHK(hipMemcpyAsync(d_long_.ptr(), v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, stream_));
+	std::vector<long long> off(v.size());
+	for (size_t k = 0; k < v.size(); k++) off[k] = loff_[v[k]];
+	if (!v.empty()) {
+		HIPCHK(hipMemcpyAsync(d_long_.ptr(), v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, stream_));
+		HIPCHK(hipMemcpyAsync(d_long_off_.ptr(), off.data(), sizeof(long long) * off.size(), hipMemcpyHostToDevice, stream_));
+	}
 	HIPCHK(hipStreamSynchronize(stream_));
 	long_dirty_ = false;
 	return 0;
@@ -865,6 +903,20 @@ long long Grid::entry_index(int row, int col) const
 	return f - mesh_.idx.data();
 }
 
+// room for a long row of deg entries in the contiguous store: the slot's own
+// if large enough, else fresh room at the end (the offsets list is uploaded
+// again); 0 or -1 when the store is full
+int Grid::long_storage(int slot, int deg)
+{
+	if (loff_[slot] >= 0 && lcap_[slot] >= deg) return 0;
+	if (lnext_ + deg > lstore_cap_) return -1;
+	loff_[slot] = lnext_;
+	lcap_[slot] = deg;
+	lnext_ += deg;
+	long_dirty_ = true;
+	return 0;
+}
+
 // (re)writes a row into a slot: slot arrays (int patches to target 0/1),
 // entries (col int patches target 2, coupling real patches target 0), and
 // ws/q (real patches targets 1/2)
@@ -888,6 +940,13 @@ int Grid::write_slot(int slot, int node, std::vector<long long>* ai, std::vector
 	for (int k = 0; k < deg; k++) {
 		ai->push_back((2ll << 60) | (base + (long long) kSlice * k)); vi->push_back(cols[k]);
 		ar->push_back(base + (long long) kSlice * k); vr->push_back(coups[k]);
+	}
+	if (deg > kLongRow) {  // and its contiguous copy
+		if (long_storage(slot, deg)) return -2;
+		for (int k = 0; k < deg; k++) {
+			lpi_addr_.push_back((0ll << 60) | (loff_[slot] + k)); lpi_val_.push_back(cols[k]);
+			lpr_addr_.push_back(loff_[slot] + k); lpr_val_.push_back(coups[k]);
+		}
 	}
 	aw->push_back((1ll << 60) | slot); vw->push_back(ws);
 	aw->push_back((2ll << 60) | slot); vw->push_back(charge_[node]);
@@ -921,6 +980,7 @@ int Grid::set_id_and_resync(int node, int id)
 	if (charge_[node] != charge_[node]) return fail(-3, "Grid_3d::Table::computeCharge(): NaN");
 
 	// rows touched: the node itself and every neighbour that owns a row
+	lpi_addr_.clear(); lpi_val_.clear(); lpr_addr_.clear(); lpr_val_.clear(); lpf_addr_.clear(); lpf_val_.clear();
 	std::vector<long long> ai, ar, aw;
 	std::vector<int> vi;
 	std::vector<double> vr, vw;
@@ -944,7 +1004,11 @@ int Grid::set_id_and_resync(int node, int id)
 			}
 			continue;
 		}
-		if (slot >= 0 && write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw) == 0) continue;
+		if (slot >= 0) {
+			const int w = write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw);
+			if (w == 0) continue;
+			if (w == -2) return fail(-7, "long-row store full");
+		}
 		if (slot >= 0) {  // does not fit its slice any more
 			slot_node_[slot] = -1;
 			set_long(slot, false);
@@ -954,7 +1018,8 @@ int Grid::set_id_and_resync(int node, int id)
 		}
 		if (next_spare_ >= cap_slots_) return fail(-7, "row slots exhausted (more than 4096 new unknown nodes)");
 		slot = next_spare_++;
-		if (write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw)) return fail(-7, "row wider than the spare slices");
+		if (const int w = write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw))
+			return fail(-7, w == -2 ? "long-row store full" : "row wider than the spare slices");
 		nrows_++;
 	}
 	// the FP32 correction rows of the accelerated relaxation follow the
@@ -978,7 +1043,8 @@ int Grid::set_id_and_resync(int node, int id)
 	// ship all patches through the pinned staging area
 	const size_t nr = ar.size(), nw = aw.size(), nf = full_addr.size();
 	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
-	const size_t need = ai.size() * 12 + nr * (8 + rs) + nw * 16 + nf * (8 + rs) + fa.size() * 12 + 512;
+	const size_t need = ai.size() * 12 + nr * (8 + rs) + nw * 16 + nf * (8 + rs) + fa.size() * 12 + lpi_addr_.size() * 12 +
+	                    lpr_addr_.size() * (8 + rs) + lpf_addr_.size() * 12 + 1024;
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (need > h_pinned_bytes_) {
 		if (h_pinned_) (void) hipHostFree(h_pinned_);
@@ -1020,6 +1086,29 @@ int Grid::set_id_and_resync(int node, int id)
 	}
 	const size_t o_ia3 = put(fa.data(), sizeof(long long) * fa.size());
 	const size_t o_iw3 = put(fv.data(), sizeof(float) * fv.size());
+	// the long rows' contiguous copies: int patches split by target
+	std::vector<long long> la[2];
+	std::vector<int> lv[2];
+	for (size_t t = 0; t < lpi_addr_.size(); t++) {
+		const int arr = (int) (lpi_addr_[t] >> 60);
+		la[arr].push_back(lpi_addr_[t] & ((1ll << 60) - 1));
+		lv[arr].push_back(lpi_val_[t]);
+	}
+	size_t o_la[2], o_lv[2];
+	for (int t = 0; t < 2; t++) {
+		o_la[t] = put(la[t].data(), sizeof(long long) * la[t].size());
+		o_lv[t] = put(lv[t].data(), sizeof(int) * lv[t].size());
+	}
+	const size_t o_lra = put(lpr_addr_.data(), sizeof(long long) * lpr_addr_.size());
+	size_t o_lrv;
+	if (f64_) {
+		o_lrv = put(lpr_val_.data(), sizeof(double) * lpr_val_.size());
+	} else {
+		std::vector<float> l32(lpr_val_.begin(), lpr_val_.end());
+		o_lrv = put(l32.data(), sizeof(float) * l32.size());
+	}
+	const size_t o_lfa = put(lpf_addr_.data(), sizeof(long long) * lpf_addr_.size());
+	const size_t o_lfv = put(lpf_val_.data(), sizeof(float) * lpf_val_.size());
 	const size_t o_ra = put(ar.data(), sizeof(long long) * nr);
 	const size_t o_fa = put(full_addr.data(), sizeof(long long) * nf);
 	size_t o_rv, o_fv;
@@ -1052,6 +1141,20 @@ int Grid::set_id_and_resync(int node, int id)
 	if (rc) return fail(-2, "patch launch failed");
 	if (!fa.empty() &&
 	    launch_scatter_f32(stream_, (int) fa.size(), (const long long*) (d + o_ia3), (const float*) (d + o_iw3), d_iw_.as<float>()))
+		return fail(-2, "patch launch failed");
+	int* larr[2] = {d_lcol_.as<int>(), d_licol_.as<int>()};
+	for (int t = 0; t < 2; t++)
+		if (!la[t].empty() && launch_scatter_i32(stream_, (int) la[t].size(), (const long long*) (d + o_la[t]),
+		                                         (const int*) (d + o_lv[t]), larr[t]))
+			return fail(-2, "patch launch failed");
+	if (!lpr_addr_.empty()) {
+		const int nl = (int) lpr_addr_.size();
+		if (f64_ ? launch_scatter_f64(stream_, nl, (const long long*) (d + o_lra), (const double*) (d + o_lrv), d_lcoup_.as<double>())
+		         : launch_scatter_f32(stream_, nl, (const long long*) (d + o_lra), (const float*) (d + o_lrv), d_lcoup_.as<float>()))
+			return fail(-2, "patch launch failed");
+	}
+	if (!lpf_addr_.empty() &&
+	    launch_scatter_f32(stream_, (int) lpf_addr_.size(), (const long long*) (d + o_lfa), (const float*) (d + o_lfv), d_liw_.as<float>()))
 		return fail(-2, "patch launch failed");
 	HIPCHK(hipMemcpyAsync(d_flags_.as<unsigned char>() + node, &flags_[node], 1, hipMemcpyHostToDevice, stream_));
 	const float spec[3] = {species_[id].evap_field, species_[id].evap_energy, species_[id].epsilon};
@@ -1772,6 +1875,9 @@ AccelArgs Grid::accel_args() const
 	x.slot_deg = d_slot_deg_.as<int>();
 	x.long_slots = d_long_.as<int>();
 	x.nlong = (int) long_set_.size();
+	x.long_off = d_long_off_.as<long long>();
+	x.licol = d_licol_.as<int>();
+	x.liw = d_liw_.as<float>();
 	return x;
 }
 
@@ -1800,6 +1906,16 @@ int Grid::write_inner(int slot, int node, std::vector<long long>* ai, std::vecto
 	if (deg > slice_width_[slot / kSlice]) return -1;
 	inner_deg_[slot] = deg;
 	ai->push_back((3ll << 60) | slot); vi->push_back(deg);
+	if (slot_deg_[slot] > kLongRow) {  // the long row's contiguous copy (room made by write_slot)
+		if (loff_[slot] < 0 || lcap_[slot] < deg) return -1;
+		int k = 0;
+		for (size_t t = 0; t < cols.size(); t++) {
+			if (flags_[cols[t]] & 2) continue;
+			lpi_addr_.push_back((1ll << 60) | (loff_[slot] + k)); lpi_val_.push_back(slot_of_[cols[t]]);
+			lpf_addr_.push_back(loff_[slot] + k); lpf_val_.push_back((float) (coups[t] / ws));
+			k++;
+		}
+	}
 	return 0;
 }
 
@@ -1811,6 +1927,7 @@ int Grid::accel_setup(double safety)
 	inner_deg_.assign(cap_slots_, 0);
 	std::vector<int> icol(nent, 0);
 	std::vector<float> iw(nent, 0.0f);
+	lpi_addr_.clear(); lpi_val_.clear(); lpf_addr_.clear(); lpf_val_.clear();
 	for (int slot = 0; slot < next_spare_; slot++) {
 		const int node = slot_node_[slot];
 		if (node < 0) continue;
@@ -1835,6 +1952,20 @@ int Grid::accel_setup(double safety)
 	HIPCHK(hipMemcpy(d_inner_deg_.ptr(), inner_deg_.data(), sizeof(int) * cap_slots_, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_icol_.ptr(), icol.data(), sizeof(int) * nent, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_iw_.ptr(), iw.data(), sizeof(float) * nent, hipMemcpyHostToDevice));
+	{
+		// the long rows' correction entries, contiguous
+		std::vector<int> lic(lstore_cap_, 0);
+		std::vector<float> liw(lstore_cap_, 0.0f);
+		for (size_t t = 0; t < lpi_addr_.size(); t++)
+			if ((lpi_addr_[t] >> 60) == 1) lic[lpi_addr_[t] & ((1ll << 60) - 1)] = lpi_val_[t];
+		for (size_t t = 0; t < lpf_addr_.size(); t++) liw[lpf_addr_[t]] = lpf_val_[t];
+		d_licol_.alloc(sizeof(int) * (size_t) lstore_cap_);
+		d_liw_.alloc(sizeof(float) * (size_t) lstore_cap_);
+		if (!d_licol_.ptr() || !d_liw_.ptr()) return fail(-2, "device allocation failed (long correction rows)");
+		HIPCHK(hipMemcpy(d_licol_.ptr(), lic.data(), sizeof(int) * lic.size(), hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d_liw_.ptr(), liw.data(), sizeof(float) * liw.size(), hipMemcpyHostToDevice));
+		lpi_addr_.clear(); lpi_val_.clear(); lpf_addr_.clear(); lpf_val_.clear();
+	}
 	HIPCHK(hipMemset(d_d0_.ptr(), 0, sizeof(float) * nv));
 	HIPCHK(hipMemset(d_d1_.ptr(), 0, sizeof(float) * nv));
 	HIPCHK(hipMemset(d_r32_.ptr(), 0, sizeof(float) * nv));

@@ -305,6 +305,21 @@ private:
 	DevBuf d_long_;
 	void set_long(int slot, bool on);
 	int upload_long();
+	// contiguous copies of the long rows (FP64 / faithful rows and FP32
+	// correction rows share the offsets): per slot its offset (-1: none) and
+	// capacity, bump-allocated from a store with spare room for new long rows
+	std::vector<long long> loff_;
+	std::vector<int> lcap_;
+	long long lnext_ = 0, lstore_cap_ = 0;
+	DevBuf d_long_off_, d_lcol_, d_lcoup_, d_licol_, d_liw_;
+	// patches of the contiguous copies collected while rows are rewritten:
+	// int (target 0 columns, 1 correction columns), real (couplings), float
+	// (correction weights)
+	std::vector<long long> lpi_addr_, lpr_addr_, lpf_addr_;
+	std::vector<int> lpi_val_;
+	std::vector<double> lpr_val_;
+	std::vector<float> lpf_val_;
+	int long_storage(int slot, int deg);
 
 	std::set<int> surface_;
 	int vacuum_id_ = -1;

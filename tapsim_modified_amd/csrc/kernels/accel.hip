@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __re
 			slot = a.long_slots[li];
 			node = a.slot_node[slot];
 			deg = a.slot_deg[slot];
-			soff = a.slice_off[slot >> 6];
+			soff = a.long_off[li];  // the contiguous copy
 			q = a.q[slot];
 			ws = a.ws[slot];
 		}
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_resid64(SweepArgs a, const double* __re
 		lv = short_row_dot_exact<double>(a.col, (const double*) a.coup, phi, act, deg, base);
 	} else if (node >= 0) {
 		old = phi[node];
-		lv = long_row_dot_exact<double>(a.col, (const double*) a.coup, phi, deg, base);
+		lv = long_row_dot_exact<double>(a.lcol, (const double*) a.lcoup, phi, deg, soff);
 		if (lane_id() != 0) node = -1;
 	}
 	double rmax = 0.0;

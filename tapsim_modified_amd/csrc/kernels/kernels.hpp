@@ -63,8 +63,13 @@ struct SweepArgs {
 	double* block_max;        // per-block partial max |dphi| of the current cycle
 	int* block_flag;          // per-block NaN flag
 	int threshold_mode;       // 1: use ctl convergence/early exit; 0: plain fixed sweeps
-	const int* long_slots;    // slots whose row has more than 32 entries (one wavefront each)
+	const int* long_slots;    // slots whose row has more than kLongRow entries (one wavefront each), ascending
 	int nlong;
+	// the long rows' entries again, contiguous: long row li (long_slots[li])
+	// has its columns and couplings at long_off[li] + k
+	const long long* long_off;
+	const int* lcol;
+	const void* lcoup;        // R per entry
 };
 // workgroups of a row launch: 256 slots each, then one wavefront per long row
 int row_launch_blocks(int nslots, int nlong);
@@ -132,6 +137,10 @@ struct AccelArgs {
 	const int* slot_deg;      // full row length: the long-row test
 	const int* long_slots;
 	int nlong;
+	// the long rows' correction entries, contiguous at long_off[li] + k
+	const long long* long_off;
+	const int* licol;
+	const float* liw;
 };
 // FP64 scaled residual r = J(phi) - phi into r32, max |r| into out[0], NaN flag into out[1]
 int launch_resid64(hipStream_t, const SweepArgs&, const double* phi, float* r32, double* out);

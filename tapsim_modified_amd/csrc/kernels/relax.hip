@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 			slot = a.long_slots[li];
 			node = a.slot_node[slot];
 			deg = a.slot_deg[slot];
-			soff = a.slice_off[slot >> 6];
+			soff = a.long_off[li];  // the contiguous copy
 			q = a.q[slot];
 			ws = a.ws[slot];
 		}
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_jacobi(SweepArgs a, int read)
 		lv = short_row_dot_exact<R>(a.col, (const R*) a.coup, pin, act, deg, base);
 	} else if (node >= 0) {  // wave-uniform
 		old = pin[node];
-		lv = long_row_dot_exact<R>(a.col, (const R*) a.coup, pin, deg, base);
+		lv = long_row_dot_exact<R>(a.lcol, (const R*) a.lcoup, pin, deg, soff);
 		if (lane_id() != 0) node = -1;
 	}
 	double dev = 0.0;

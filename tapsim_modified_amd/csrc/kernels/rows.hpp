@@ -9,12 +9,15 @@
 // rows are sorted by length within small windows so that a wavefront's
 // rows have similar lengths).  Rows
 // longer than that (the fan of a boundary-plane node under the tip base
-// reaches several hundred entries) are kept on a separate list and each is
+// reaches several thousand entries) are kept on a separate list and each is
 // taken by a whole wavefront in extra workgroups of the same launch: the
 // lanes load 256 entries at a time and every lane adds the products in entry
 // order through v_readlane.  A long row on the per-lane path would keep one
 // wavefront looping for the length of the row after the rest of the launch
-// has finished.
+// has finished.  The bulk sweeps read a long row from a contiguous copy of
+// its entries (in the SELL layout consecutive entries of one row are 64
+// elements apart, so every entry would cost a cache line of its own); the
+// SELL copy stays for the kernels that take rows one by one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -131,11 +134,12 @@ __device__ __forceinline__ double short_row_dot_exact(const int* __restrict__ co
 	return short_row_dot_exact_b<R, kLongRow - 16>(col, coup, p, deg, base, 16, lv);
 }
 
-// the same sum for one long row taken by the whole wavefront (deg and base
-// wave-uniform); every lane returns the sum
+// the same sum for one long row taken by the whole wavefront from its
+// contiguous entries off .. off+deg-1 (deg and off wave-uniform); every lane
+// returns the sum
 template <typename R>
 __device__ __forceinline__ double long_row_dot_exact(const int* __restrict__ col, const R* __restrict__ coup,
-                                                     const R* __restrict__ p, int deg, long long base)
+                                                     const R* __restrict__ p, int deg, long long off)
 {
 	constexpr int W = kLongWindows;
 	double acc = 0.0;
@@ -144,7 +148,7 @@ __device__ __forceinline__ double long_row_dot_exact(const int* __restrict__ col
 		R c[W], v[W];
 #pragma unroll
 		for (int w = 0; w < W; w++) {
-			const long long e = base + 64ll * min(k0 + 64 * w + lane_id(), deg - 1);
+			const long long e = off + min(k0 + 64 * w + lane_id(), deg - 1);
 			j[w] = col[e];
 			c[w] = coup[e];
 		}
@@ -195,8 +199,10 @@ __device__ __forceinline__ float short_row_fma_f32(const int* __restrict__ col, 
 	return short_row_fma_f32_b<kLongRow>(col, w, x, deg, base);
 }
 
+// the long row's contiguous entries off .. off+deg-1: the FP32 sum of a
+// correction row (one wavefront; every lane returns the sum)
 __device__ __forceinline__ float long_row_fma_f32(const int* __restrict__ col, const float* __restrict__ w,
-                                                  const float* __restrict__ x, int deg, long long base)
+                                                  const float* __restrict__ x, int deg, long long off)
 {
 	constexpr int W = kLongWindows;
 	float acc = 0.0f;
@@ -205,7 +211,7 @@ __device__ __forceinline__ float long_row_fma_f32(const int* __restrict__ col, c
 		float c[W], v[W];
 #pragma unroll
 		for (int ww = 0; ww < W; ww++) {
-			const long long e = base + 64ll * min(k0 + 64 * ww + lane_id(), deg - 1);
+			const long long e = off + min(k0 + 64 * ww + lane_id(), deg - 1);
 			j[ww] = col[e];
 			c[ww] = w[e];
 		}
@@ -233,7 +239,7 @@ __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __rest
 	const RowBlock rb = row_block(x.nlong);
 	const int lb = rb.is_long ? -1 : rb.index;
 	int slot = -1, node = -1, deg = 0, ideg = 0;
-	long long soff = 0;
+	long long soff = 0, loff = 0;
 	if (!rb.is_long) {
 		if (lb < main_blocks) slot = lb * 256 + threadIdx.x;
 		if (slot >= 0 && slot < x.nslots) {
@@ -247,9 +253,8 @@ __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __rest
 		if (li < x.nlong) {
 			slot = x.long_slots[li];
 			node = x.slot_node[slot];
-			deg = x.slot_deg[slot];
 			ideg = x.inner_deg[slot];
-			soff = x.slice_off[slot >> 6];
+			loff = x.long_off[li];
 		}
 	}
 	const long long base = soff + (slot & 63);
@@ -266,7 +271,7 @@ __device__ __forceinline__ int accel_row(const AccelArgs& x, const float* __rest
 	if (node < 0) return -1;
 	if (e1) *v1 = e1[slot];
 	if (e2) *v2 = e2[slot];
-	*sum = long_row_fma_f32(x.col, x.w, in, ideg, base);
+	*sum = long_row_fma_f32(x.licol, x.liw, in, ideg, loff);
 	return lane_id() == 0 ? slot : -1;
 }
 
