@@ -155,14 +155,14 @@ __global__ __launch_bounds__(256) void k_amg_fine_resid(AmgSolve S)
 	if (i >= 0) S.res[i] = S.r[i] - S.dn[i] * (S.x0[i] - g);
 }
 
-// x0 += e[agg0] (the coarse correction; rows outside every aggregate keep x0)
+// x0 += wc e[agg0] (the coarse correction; rows outside every aggregate keep x0)
 __global__ __launch_bounds__(256) void k_amg_fine_prolong(AmgSolve S, const float* __restrict__ e)
 {
 	if (amg_done(S.st)) return;
 	const int i = active_slot(S.x);
 	if (i < 0) return;
 	const int a = S.agg0[i];
-	if (a >= 0) S.x0[i] += e[a];
+	if (a >= 0) S.x0[i] += S.wc * e[a];
 }
 
 // z = x0 + w0 (r / D - x0 + G x0); partials of r.z
@@ -245,9 +245,9 @@ __global__ __launch_bounds__(256) void k_amg_restrict(const double* __restrict__
 }
 
 // the coarse correction and the post-smoothing in one pass:
-// xin = x0 + e[agg], xout = xin + w Dinv (b - A xin) (xin at the neighbours
-// recomputed from x0 and e)
-__global__ __launch_bounds__(256) void k_amg_prolong_post(const double* __restrict__ st, AmgLevel L, float w,
+// xin = x0 + wc e[agg], xout = xin + w Dinv (b - A xin) (xin at the
+// neighbours recomputed from x0 and e)
+__global__ __launch_bounds__(256) void k_amg_prolong_post(const double* __restrict__ st, AmgLevel L, float w, float wc,
                                                           const int* __restrict__ agg, const float* __restrict__ e,
                                                           const float* __restrict__ b, const float* __restrict__ x0,
                                                           float* __restrict__ xout)
@@ -260,12 +260,12 @@ __global__ __launch_bounds__(256) void k_amg_prolong_post(const double* __restri
 		for (int k = L.ptr[i] + sub; k < L.ptr[i + 1]; k += kCsrLanes) {
 			const int j = L.col[k];
 			const int a = agg[j];
-			s += L.val[k] * (a >= 0 ? x0[j] + e[a] : x0[j]);
+			s += L.val[k] * (a >= 0 ? x0[j] + wc * e[a] : x0[j]);
 		}
 	s = row_group_sum(s);
 	if (i >= L.n || sub != 0) return;
 	const int a = agg[i];
-	const float xi = a >= 0 ? x0[i] + e[a] : x0[i];
+	const float xi = a >= 0 ? x0[i] + wc * e[a] : x0[i];
 	xout[i] = xi + w * L.dinv[i] * (b[i] - s);
 }
 
@@ -324,7 +324,7 @@ int launch_amg_vcycle(hipStream_t s, const AmgSolve& S, int mode)
 	const float* fin = S.xl[S.nlev - 1];
 	for (int l = S.nlev - 2; l >= 0; l--) {
 		const AmgLevel& L = S.lv[l];
-		hipLaunchKernelGGL(dev::k_amg_prolong_post, dim3(csr_blocks(L.n)), dim3(256), 0, s, (const double*) S.st, L, S.w[l], S.lv[l + 1].agg,
+		hipLaunchKernelGGL(dev::k_amg_prolong_post, dim3(csr_blocks(L.n)), dim3(256), 0, s, (const double*) S.st, L, S.w[l], S.wc, S.lv[l + 1].agg,
 		                   fin, (const float*) S.bl[l], (const float*) S.xl[l], S.rl[l]);
 		fin = S.rl[l];
 	}
