@@ -173,7 +173,9 @@ int tsg_surface_size(const tsg_grid* h, int* count)
 int tsg_surface_nodes(const tsg_grid* h, int* out)
 {
 	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
-	return h->g->surface_nodes(out);
+	const int rc = h->g->surface_nodes(out);
+	if (rc) tsg::set_error(h->g->error());
+	return rc;
 }
 
 int tsg_fieldlines(tsg_grid* h, int count, const float* start, float distance, long long iterations,
@@ -193,7 +195,9 @@ int tsg_update_coordination(tsg_grid* h)
 int tsg_surface_normal(const tsg_grid* h, int node, float out[3])
 {
 	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
-	return h->g->surface_normal(node, out);
+	const int rc = h->g->surface_normal(node, out);
+	if (rc) tsg::set_error(h->g->error());
+	return rc;
 }
 
 int tsg_surface_evaluate_model(tsg_grid* h, int mode, int exact_sum, double* scaling_ref)

@@ -297,29 +297,20 @@ int Grid::amg_setup()
 // rows rewritten by a resync: the new row sums (by slot) of rows that remain
 // (or became) unknown; a slot that lost its row gets a zero residual entry,
 // which its aggregate's restriction keeps reading
-int Grid::amg_touch(const std::vector<int>& nodes, const std::vector<int>& off_slots)
+void Grid::amg_touch(const std::vector<int>& nodes, const std::vector<int>& off_slots, std::vector<long long>* dn_addr,
+                     std::vector<float>* dn_val, std::vector<long long>* res_addr)
 {
-	if (!amg_on_) return 0;
 	std::vector<int> cols;
 	std::vector<double> coups;
-	std::vector<float> val(nodes.size());
-	for (size_t t = 0; t < nodes.size(); t++) {
-		const int node = nodes[t];
-		if (slot_of_[node] < 0) continue;
+	for (const int node : nodes) {
+		const int slot = slot_of_[node];
+		if (slot < 0) continue;
 		double ws;
 		row_entries(node, &cols, &coups, &ws);
-		val[t] = (float) ws;
+		dn_addr->push_back(slot);
+		dn_val->push_back((float) ws);
 	}
-	const float zero = 0.0f;
-	for (size_t t = 0; t < nodes.size(); t++) {
-		const int slot = slot_of_[nodes[t]];
-		if (slot >= 0)
-			HIPCHK(hipMemcpyAsync(amg_dn_.as<float>() + slot, &val[t], sizeof(float), hipMemcpyHostToDevice, stream_));
-	}
-	for (int slot : off_slots)
-		HIPCHK(hipMemcpyAsync(amg_res_.as<float>() + slot, &zero, sizeof(float), hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipStreamSynchronize(stream_));
-	return 0;
+	for (int slot : off_slots) res_addr->push_back(slot);
 }
 
 // PCG with one V-cycle as preconditioner on A delta = D r32, until the

@@ -183,8 +183,6 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 	d_phi0_.alloc(rs * (size_t) n);
 	d_phi1_.alloc(rs * (size_t) n);
 	d_ctl_.alloc(sizeof(RelaxCtl));
-	d_argmax_.alloc(sizeof(long long) * 2);
-	d_maxval_.alloc(sizeof(double));
 	d_sum_.alloc(sizeof(double));
 	d_local_out_.alloc(sizeof(long long) * 4);
 	if (!d_coupfull_.ptr() || !d_phi0_.ptr() || !d_phi1_.ptr()) { fail(-2, "device allocation failed (grid)"); return; }
@@ -222,9 +220,10 @@ Grid::~Grid()
 		f.ready = f.done = nullptr;
 		f.host = nullptr;
 	}
-	for (hipEvent_t e : {ev_start_, ev_stop_, sum_src_, sum_done_, amg_ev_[0], amg_ev_[1]})
+	for (hipEvent_t e : {ev_start_, ev_stop_, sum_src_, sum_done_, amg_ev_[0], amg_ev_[1], sst_ev_, norm_src_, norm_done_,
+	                     patch_ev_})
 		if (e) (void) hipEventDestroy(e);
-	for (void* p : {h_pinned_, warm_host_, (void*) sum_host_, (void*) amg_host_})
+	for (void* p : {h_pinned_, warm_host_, (void*) sum_host_, (void*) amg_host_, (void*) sst_host_, (void*) choice_host_})
 		if (p) (void) hipHostFree(p);
 	for (hipStream_t s : {fstream_, sstream_, stream_})
 		if (s) (void) hipStreamDestroy(s);
@@ -958,6 +957,7 @@ int Grid::set_id_and_resync(int node, int id)
 	if (node < 0 || node >= mesh_.n || id < 0 || id > 127 || !species_[id].defined)
 		return fail(-1, "setId: bad node or species");
 	ids_[node] = (int16_t) id;
+	refresh_vacuum(node);  // the device surface's vacuum bit (the site set itself changes only in take_away)
 	const Species& sp = species_[id];
 	flags_[node] = (mesh_.boundary[node] ? 1 : 0) | (sp.dirichlet ? 2 : 0) | (sp.neumann ? 4 : 0);
 	ord_dirty_ = true;  // rows may appear, move or vanish
@@ -1037,16 +1037,22 @@ int Grid::set_id_and_resync(int node, int id)
 				if (int rc = write_inner(slot_of_[r], r, &ai, &vi, &fa, &fv))
 					return fail(-7, rc == -2 ? "inner row: an unknown neighbour without a row" : "inner row too wide");
 	}
-	if (amg_on_)
-		if (int rc = amg_touch(touched, off_slots)) return rc;
+	std::vector<long long> dn_addr, res_addr;
+	std::vector<float> dn_val;
+	if (amg_on_) amg_touch(touched, off_slots, &dn_addr, &dn_val, &res_addr);
 
 	// ship all patches through the pinned staging area
 	const size_t nr = ar.size(), nw = aw.size(), nf = full_addr.size();
 	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
 	const size_t need = ai.size() * 12 + nr * (8 + rs) + nw * 16 + nf * (8 + rs) + fa.size() * 12 + lpi_addr_.size() * 12 +
-	                    lpr_addr_.size() * (8 + rs) + lpf_addr_.size() * 12 + 1024;
-	HIPCHK(hipStreamSynchronize(stream_));
+	                    lpr_addr_.size() * (8 + rs) + lpf_addr_.size() * 12 + dn_addr.size() * 12 + res_addr.size() * 12 +
+	                    48 * 16 + 1024;
+	// the staging's previous upload is done (nothing else waits for the device)
+	if (patch_ev_) HIPCHK(hipEventSynchronize(patch_ev_));
+	else HIPCHK(hipEventCreateWithFlags(&patch_ev_, hipEventDisableTiming));
 	if (need > h_pinned_bytes_) {
+		// the device copy may still be read by the previous patch kernels
+		HIPCHK(hipStreamSynchronize(stream_));
 		if (h_pinned_) (void) hipHostFree(h_pinned_);
 		h_pinned_bytes_ = std::max(need * 2, (size_t) 1 << 20);
 		HIPCHK(hipHostMalloc(&h_pinned_, h_pinned_bytes_, hipHostMallocDefault));
@@ -1120,8 +1126,22 @@ int Grid::set_id_and_resync(int node, int id)
 		o_rv = put(a32.data(), sizeof(float) * nr);
 		o_fv = put(b32.data(), sizeof(float) * nf);
 	}
+	const size_t o_dna = put(dn_addr.data(), sizeof(long long) * dn_addr.size());
+	const size_t o_dnv = put(dn_val.data(), sizeof(float) * dn_val.size());
+	const size_t o_rsa = put(res_addr.data(), sizeof(long long) * res_addr.size());
+	const std::vector<float> res_zero(res_addr.size(), 0.0f);
+	const size_t o_rsv = put(res_zero.data(), sizeof(float) * res_zero.size());
 	char* d = (char*) d_patch_addr_.ptr();
 	HIPCHK(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipEventRecord(patch_ev_, stream_));
+	// the multigrid's level-0 row sums of the touched rows, and a zero
+	// residual in the slots that lost their row
+	if (!dn_addr.empty() && launch_scatter_f32(stream_, (int) dn_addr.size(), (const long long*) (d + o_dna),
+	                                           (const float*) (d + o_dnv), amg_dn_.as<float>()))
+		return fail(-2, "patch launch failed");
+	if (!res_addr.empty() && launch_scatter_f32(stream_, (int) res_addr.size(), (const long long*) (d + o_rsa),
+	                                            (const float*) (d + o_rsv), amg_res_.as<float>()))
+		return fail(-2, "patch launch failed");
 	int* iarr[5] = {d_slot_node_.as<int>(), d_slot_deg_.as<int>(), d_col_.as<int>(), d_inner_deg_.as<int>(), d_icol_.as<int>()};
 	for (int t = 0; t < 5; t++)
 		if (launch_scatter_i32(stream_, (int) ia[t].size(), (const long long*) (d + o_ia[t]), (const int*) (d + o_iv[t]), iarr[t]))
@@ -1156,85 +1176,259 @@ int Grid::set_id_and_resync(int node, int id)
 	if (!lpf_addr_.empty() &&
 	    launch_scatter_f32(stream_, (int) lpf_addr_.size(), (const long long*) (d + o_lfa), (const float*) (d + o_lfv), d_liw_.as<float>()))
 		return fail(-2, "patch launch failed");
-	HIPCHK(hipMemcpyAsync(d_flags_.as<unsigned char>() + node, &flags_[node], 1, hipMemcpyHostToDevice, stream_));
-	const float spec[3] = {species_[id].evap_field, species_[id].evap_energy, species_[id].epsilon};
-	HIPCHK(hipMemcpyAsync(d_evapF_.as<float>() + node, &spec[0], sizeof(float), hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipMemcpyAsync(d_evapE_.as<float>() + node, &spec[1], sizeof(float), hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipMemcpyAsync(d_eps_.as<float>() + node, &spec[2], sizeof(float), hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipStreamSynchronize(stream_));
+	NodeSpecies ns;
+	ns.node = node;
+	ns.flags = flags_[node];
+	ns.evap_field = species_[id].evap_field;
+	ns.evap_energy = species_[id].evap_energy;
+	ns.epsilon = species_[id].epsilon;
+	ns.d_flags = d_flags_.as<unsigned char>();
+	ns.d_evap_field = d_evapF_.as<float>();
+	ns.d_evap_energy = d_evapE_.as<float>();
+	ns.d_epsilon = d_eps_.as<float>();
+	if (launch_set_node(stream_, ns)) return fail(-2, "node update launch failed");
+	// everything above is queued on the stream: the host goes on
 	return upload_long();
 }
 
 // ---------------------------------------------------------------- surface
 
+// Surface_3d::Table::init (surface_3d.cpp:50-81): a site is a removable
+// non-vacuum node with a vacuum neighbour across a face of positive area
 int Grid::surface_init(int vacuum_id)
 {
-	vacuum_id_ = vacuum_id;
-	surface_.clear();
 	const int n = mesh_.n;
+	if (normal_pending_) {  // it reads the bytes rewritten below
+		HIPCHK(hipEventSynchronize(norm_done_));
+		normal_pending_ = false;
+	}
+	vacuum_id_ = vacuum_id;
+	sstate_.assign(n, 0);
+	surf_count_ = 0;
 	for (int i = 0; i < n; i++) {
-		if (ids_[i] == vacuum_id) continue;
+		if (ids_[i] == vacuum_id) {
+			sstate_[i] = kSurfVacuum;
+			continue;
+		}
 		if (!species_[ids_[i]].removable) continue;
 		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++)
-			if (ids_[mesh_.idx[k]] == vacuum_id && a_one32_[k] > 0.0f) { surface_.insert(i); break; }
+			if (ids_[mesh_.idx[k]] == vacuum_id && a_one32_[k] > 0.0f) {
+				sstate_[i] = kSurfSite;
+				surf_count_++;
+				break;
+			}
 	}
-	surface_dirty_ = true;
+	const size_t nb = (size_t) surface_compact_blocks(n);
+	if (d_sstate_.bytes() < (size_t) n) {
+		d_sstate_.alloc(((size_t) n + 15) / 16 * 16);
+		d_surf_.alloc(sizeof(int) * ((size_t) n + 1));
+		d_surf_blk_.alloc(sizeof(int) * (nb + 1));
+		d_marks_.alloc(sizeof(int) * (size_t) n);
+		d_nshell_.alloc(sizeof(int) * 2 * (size_t) kNormalShellCap);
+		d_choice_.alloc(sizeof(ChoiceRec));
+		if (!d_sstate_.ptr() || !d_surf_.ptr() || !d_surf_blk_.ptr() || !d_marks_.ptr() || !d_nshell_.ptr() ||
+		    !d_choice_.ptr())
+			return fail(-2, "device allocation failed (surface)");
+		// every entry of the list and the shell buffer holds a node index
+		// from the start, the marks no call's stamp
+		if (launch_fill_i32(stream_, n, d_marks_.as<int>(), 0) || launch_fill_i32(stream_, n + 1, d_surf_.as<int>(), 0) ||
+		    launch_fill_i32(stream_, 2 * kNormalShellCap, d_nshell_.as<int>(), 0))
+			return fail(-2, "fill launch failed");
+		normal_stamp_ = 0;
+	}
+	if (!choice_host_) {
+		HIPCHK(hipHostMalloc((void**) &choice_host_, sizeof(ChoiceRec), hipHostMallocDefault));
+		std::memset(choice_host_, 0, sizeof(ChoiceRec));
+	}
+	HIPCHK(hipMemcpyAsync(d_sstate_.ptr(), sstate_.data(), (size_t) n, hipMemcpyHostToDevice, stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	sst_pending_.clear();
+	list_dirty_ = true;
+	choice_valid_ = false;
+	return surface_flush();
+}
+
+void Grid::set_site(int node, bool on)
+{
+	const uint8_t b = (uint8_t) ((sstate_[node] & ~kSurfSite) | (on ? kSurfSite : 0));
+	if (b == sstate_[node]) return;
+	surf_count_ += on ? 1 : -1;
+	sstate_[node] = b;
+	sst_pending_.push_back(node);
+	list_dirty_ = true;
+}
+
+void Grid::refresh_vacuum(int node)
+{
+	if (sstate_.empty()) return;
+	const uint8_t b = (uint8_t) ((sstate_[node] & ~kSurfVacuum) | (ids_[node] == vacuum_id_ ? kSurfVacuum : 0));
+	if (b == sstate_[node]) return;
+	sstate_[node] = b;
+	sst_pending_.push_back(node);
+}
+
+// the changed state bytes to the device (pinned staging, one scatter) and,
+// if a site bit changed, the site list rebuilt there; nothing waits for it
+int Grid::surface_flush()
+{
+	if (!sst_pending_.empty()) {
+		std::sort(sst_pending_.begin(), sst_pending_.end());
+		sst_pending_.erase(std::unique(sst_pending_.begin(), sst_pending_.end()), sst_pending_.end());
+		const size_t cnt = sst_pending_.size();
+		const size_t bytes = cnt * (sizeof(int) + 1);
+		if (!sst_ev_) HIPCHK(hipEventCreateWithFlags(&sst_ev_, hipEventDisableTiming));
+		else HIPCHK(hipEventSynchronize(sst_ev_));  // the staging's previous upload is done
+		if (sst_host_bytes_ < bytes) {
+			if (sst_host_) HIPCHK(hipHostFree(sst_host_));
+			sst_host_ = nullptr;
+			sst_host_bytes_ = std::max(bytes * 2, (size_t) 64 << 10);
+			HIPCHK(hipHostMalloc((void**) &sst_host_, sst_host_bytes_, hipHostMallocDefault));
+		}
+		if (d_sst_upd_.bytes() < bytes) {
+			// the previous scatter reads the old buffer
+			HIPCHK(hipStreamSynchronize(stream_));
+			d_sst_upd_.alloc(sst_host_bytes_);
+			if (!d_sst_upd_.ptr()) return fail(-2, "device allocation failed (surface)");
+		}
+		int* hn = (int*) sst_host_;
+		unsigned char* hv = sst_host_ + cnt * sizeof(int);
+		for (size_t i = 0; i < cnt; i++) {
+			hn[i] = sst_pending_[i];
+			hv[i] = sstate_[sst_pending_[i]];
+		}
+		HIPCHK(hipMemcpyAsync(d_sst_upd_.ptr(), sst_host_, bytes, hipMemcpyHostToDevice, stream_));
+		HIPCHK(hipEventRecord(sst_ev_, stream_));
+		// a normal in flight reads the bytes the scatter changes
+		if (normal_pending_) HIPCHK(hipStreamWaitEvent(stream_, norm_done_, 0));
+		if (launch_sstate_scatter(stream_, (int) cnt, d_sst_upd_.as<int>(), d_sst_upd_.as<unsigned char>() + cnt * sizeof(int),
+		                          d_sstate_.as<unsigned char>()))
+			return fail(-2, "surface scatter launch failed");
+		sst_pending_.clear();
+	}
+	if (list_dirty_) {
+		const int nb = surface_compact_blocks(mesh_.n);
+		if (launch_surface_compact(stream_, mesh_.n, d_sstate_.as<unsigned char>(), d_surf_blk_.as<int>(), d_surf_.as<int>(),
+		                           d_surf_blk_.as<int>() + nb))
+			return fail(-2, "surface compaction launch failed");
+		list_dirty_ = false;
+		host_list_stale_ = true;
+		eval_stale_ = true;
+		choice_valid_ = false;
+	}
 	return 0;
 }
 
-int Grid::surface_nodes(int* out) const
+// the device site list copied to surface_vec_
+int Grid::host_list()
 {
-	int p = 0;
-	for (int v : surface_) out[p++] = v;
+	if (int rc = surface_flush()) return rc;
+	if (!host_list_stale_) return 0;
+	surface_vec_.resize(surf_count_);
+	if (surf_count_ > 0)
+		HIPCHK(hipMemcpyAsync(surface_vec_.data(), d_surf_.ptr(), sizeof(int) * (size_t) surf_count_, hipMemcpyDeviceToHost,
+		                      stream_));
+	HIPCHK(hipStreamSynchronize(stream_));
+	host_list_stale_ = false;
+	return 0;
+}
+
+int Grid::surface_nodes(int* out)
+{
+	if (sstate_.empty()) return 0;
+	if (int rc = host_list()) return rc;
+	std::copy(surface_vec_.begin(), surface_vec_.end(), out);
 	return 0;
 }
 
 int Grid::surface_normal(int node, float out[3])
 {
 	if (node < 0 || node >= mesh_.n) return fail(-1, "Surface_3d::normal: bad node index");
-	if (!tsg::surface_normal(mesh_, surface_, ids_.data(), vacuum_id_, node, out))
-		return fail(-3, "Geometry_3d::find_best_plane(): NaN");
-	return 0;
-}
-
-int Grid::upload_surface()
-{
-	if (!surface_dirty_) return 0;
-	surface_vec_.assign(surface_.begin(), surface_.end());
-	const size_t cnt = surface_vec_.size();
-	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
-	if (d_surf_.bytes() < sizeof(int) * cnt) {
-		d_surf_.alloc(sizeof(int) * (cnt + cnt / 2 + 1024));
-		d_surf_val_.alloc(rs * (cnt + cnt / 2 + 1024));
+	if (sstate_.empty()) {
+		out[0] = out[1] = out[2] = 0.0f;
+		return 0;
 	}
-	HIPCHK(hipMemcpyAsync(d_surf_.ptr(), surface_vec_.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, stream_));
-	HIPCHK(hipStreamSynchronize(stream_));
-	surface_dirty_ = false;
+	if (!tsg::surface_normal(mesh_, sstate_.data(), node, out)) return fail(-3, "Geometry_3d::find_best_plane(): NaN");
 	return 0;
 }
 
-int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
+int Grid::ensure_side_stream()
+{
+	if (sstream_) return 0;
+	HIPCHK(hipStreamCreateWithFlags(&sstream_, hipStreamNonBlocking));
+	HIPCHK(hipEventCreateWithFlags(&sum_src_, hipEventDisableTiming));
+	HIPCHK(hipEventCreateWithFlags(&sum_done_, hipEventDisableTiming));
+	HIPCHK(hipEventCreateWithFlags(&norm_src_, hipEventDisableTiming));
+	HIPCHK(hipEventCreateWithFlags(&norm_done_, hipEventDisableTiming));
+	HIPCHK(hipHostMalloc((void**) &sum_host_, sizeof(double), hipHostMallocDefault));
+	return 0;
+}
+
+// Surface_3d::Table::normal of the event's site on the device, on the side
+// stream (it overlaps the take-away and the relaxation); the state bytes it
+// reads change only after it (the next flush waits for it), and its record
+// is read back with the next surface evaluation
+int Grid::launch_normal(int node)
+{
+	if (int rc = surface_flush()) return rc;
+	if (int rc = ensure_side_stream()) return rc;
+	if (++normal_stamp_ <= 0) {
+		if (launch_fill_i32(stream_, mesh_.n, d_marks_.as<int>(), 0)) return fail(-2, "fill launch failed");
+		normal_stamp_ = 1;
+	}
+	HIPCHK(hipEventRecord(norm_src_, stream_));
+	HIPCHK(hipStreamWaitEvent(sstream_, norm_src_, 0));
+	NormalArgs a;
+	a.node = node;
+	a.ptr = d_ptr_.as<int>();
+	a.idx = d_idx_.as<int>();
+	a.xyz = d_xyz_.as<float>();
+	a.sstate = d_sstate_.as<unsigned char>();
+	a.marks = d_marks_.as<int>();
+	a.stamp = normal_stamp_;
+	a.shell = d_nshell_.as<int>();
+	a.rec = d_choice_.as<ChoiceRec>();
+	if (launch_surface_normal(sstream_, a)) return fail(-2, "normal launch failed");
+	HIPCHK(hipEventRecord(norm_done_, sstream_));
+	normal_pending_ = true;
+	return 0;
+}
+
+int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode, int prev)
 {
 	if (mode < 0 || mode > 3) return fail(-1, "unknown probability mode");
 	if (mode == 1 && !(temperature_ > 0.0f)) return fail(-1, "Surface_3d::Table::probBoltzmann(): temperature must be positive");
+	if (prev >= mesh_.n) return fail(-1, "bad node index");
+	if (sstate_.empty()) return 0;
 	// the previous evaluation's normaliser still reads the surface values
 	if (sum_pending_) {
 		HIPCHK(hipEventSynchronize(sum_done_));
 		sum_pending_ = false;
 	}
-	if (int rc = upload_surface()) return rc;
-	const int cnt = (int) surface_vec_.size();
-	if (coord_done_)  // ids may have left the species table (update_coordination)
+	if (int rc = surface_flush()) return rc;
+	const int cnt = surf_count_;
+	if (coord_done_) {  // ids may have left the species table (update_coordination)
+		if (int rc = host_list()) return rc;
 		for (int v : surface_vec_)
 			if (!species_ok(v)) return fail(-8, "Configuration::Table::operator[]()");
+	}
 	prob_mode_ = mode;
-	if (cnt == 0) return 0;
+	if (cnt == 0 && prev < 0) {
+		choice_valid_ = false;
+		eval_stale_ = false;
+		return 0;
+	}
 	const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+	if (d_surf_val_.bytes() < rs * (size_t) cnt) {
+		// the normaliser (side stream) is done: it was waited for above
+		d_surf_val_.alloc(rs * ((size_t) cnt + cnt / 2 + 1024));
+		if (!d_surf_val_.ptr()) return fail(-2, "device allocation failed (surface)");
+	}
 	if (mode != 0 && d_surf_ef_.bytes() < rs * (size_t) cnt) {
 		d_surf_ef_.alloc(rs * (size_t) std::max(cnt, 1024) * 2);
 		if (!d_surf_ef_.ptr()) return fail(-2, "device allocation failed (surface)");
 	}
-	HIPCHK(hipMemsetAsync(d_status_.ptr(), 0, sizeof(int), stream_));
+	if (launch_fill_i32(stream_, 1, d_status_.as<int>(), 0)) return fail(-2, "fill launch failed");
+	const void* phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
 	FieldArgs f;
 	f.count = cnt;
 	f.nodes = d_surf_.as<int>();
@@ -1243,10 +1437,9 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 	f.idx = d_idx_.as<int>();
 	f.coup = d_coupfull_.ptr();
 	f.flags = d_flags_.as<unsigned char>();
-	f.phi = ((sw_ + 1) % 2) ? d_phi1_.ptr() : d_phi0_.ptr();
+	f.phi = phi;
 	f.evap_field = d_evapF_.as<float>();
 	f.value = d_surf_val_.ptr();
-	f.field = nullptr;
 	f.status = d_status_.as<int>();
 	f.mode = mode;
 	f.temperature = temperature_;
@@ -1254,24 +1447,34 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 	f.epsilon = d_eps_.as<float>();
 	f.value_ef = mode != 0 ? d_surf_ef_.ptr() : nullptr;
 	if (launch_surface_fields(stream_, f, f64_)) return fail(-2, "field launch failed");
-	// evap_compute_specificFields: max |E|/F; the argmax of the model's values
-	if (mode != 0) {
-		if (launch_surface_argmax(stream_, cnt, d_surf_ef_.ptr(), f64_, d_argmax_.as<long long>(), d_maxval_.as<double>()))
-			return fail(-2, "argmax launch failed");
-		HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+	// the first maximum, the apex, max |E|/F and the fields of the chosen
+	// site and of prev in one record
+	ChooseArgs c;
+	c.count = cnt;
+	c.list = d_surf_.as<int>();
+	c.value = d_surf_val_.ptr();
+	c.value_ef = f.value_ef;
+	c.xyz = f.xyz;
+	c.ptr = f.ptr;
+	c.idx = f.idx;
+	c.coup = f.coup;
+	c.flags = f.flags;
+	c.phi = phi;
+	c.status = d_status_.as<int>();
+	c.dev_count = d_surf_blk_.as<int>() + surface_compact_blocks(mesh_.n);
+	c.prev = prev;
+	c.rec = d_choice_.as<ChoiceRec>();
+	// the record shares its memory with the normal of the event
+	if (normal_pending_) {
+		HIPCHK(hipStreamWaitEvent(stream_, norm_done_, 0));
+		normal_pending_ = false;
 	}
-	if (launch_surface_argmax(stream_, cnt, d_surf_val_.ptr(), f64_, d_argmax_.as<long long>(), d_maxval_.as<double>()))
-		return fail(-2, "argmax launch failed");
-	if (exact_sum) {
+	if (launch_surface_choose(stream_, c, f64_)) return fail(-2, "choice launch failed");
+	if (exact_sum && cnt > 0) {
 		// the sequential normaliser (one lane, ~1 ms on a large surface) runs on
-		// a side stream; it is read with the event's record (sum_read) and
-		// waited for before the surface values change again
-		if (!sstream_) {
-			HIPCHK(hipStreamCreateWithFlags(&sstream_, hipStreamNonBlocking));
-			HIPCHK(hipEventCreateWithFlags(&sum_src_, hipEventDisableTiming));
-			HIPCHK(hipEventCreateWithFlags(&sum_done_, hipEventDisableTiming));
-			HIPCHK(hipHostMalloc((void**) &sum_host_, sizeof(double), hipHostMallocDefault));
-		}
+		// the side stream; it is read with the event's record and waited for
+		// before the surface values change again
+		if (int rc = ensure_side_stream()) return rc;
 		HIPCHK(hipEventRecord(sum_src_, stream_));
 		HIPCHK(hipStreamWaitEvent(sstream_, sum_src_, 0));
 		if (launch_surface_sum(sstream_, cnt, d_surf_val_.ptr(), f64_, d_sum_.as<double>()))
@@ -1280,14 +1483,19 @@ int Grid::surface_evaluate(bool exact_sum, double* scaling_ref, int mode)
 		HIPCHK(hipEventRecord(sum_done_, sstream_));
 		sum_pending_ = true;
 	}
-	int st = 0;
-	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
-	if (mode == 0) HIPCHK(hipMemcpyAsync(&scaling_ref_, d_maxval_.ptr(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+	HIPCHK(hipMemcpyAsync(choice_host_, d_choice_.ptr(), sizeof(ChoiceRec), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));
-	if (st & 1) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
-	if (st & 4) return fail(-4, "Grid_3d::Table::field_o2(): singular least-squares system");
-	if (st & 2) return fail(-3, "Grid_3d::Table::field_o1(): NaN");
-	if (scaling_ref) *scaling_ref = scaling_ref_;
+	const ChoiceRec& r = *choice_host_;
+	if (r.count != cnt) return fail(-2, "surface bookkeeping: the device site list differs from the host's count");
+	if (r.status & 17) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
+	if (r.status & 4) return fail(-4, "Grid_3d::Table::field_o2(): singular least-squares system");
+	if (r.status & 2) return fail(-3, "Grid_3d::Table::field_o1(): NaN");
+	eval_stale_ = false;
+	choice_valid_ = cnt > 0;
+	if (cnt > 0) {
+		scaling_ref_ = r.scaling_ref;
+		if (scaling_ref) *scaling_ref = scaling_ref_;
+	}
 	return 0;
 }
 
@@ -1295,8 +1503,8 @@ int Grid::surface_values(double* out)
 {
 	// the values belong to the evaluated site list; after a take-away the
 	// caller's site count (surface_size) no longer describes them
-	if (surface_dirty_) return fail(-6, "surface changed since its last evaluation");
-	const int cnt = (int) surface_vec_.size();
+	if (eval_stale_) return fail(-6, "surface changed since its last evaluation");
+	const int cnt = surf_count_;
 	if (cnt == 0) return 0;
 	HIPCHK(hipStreamSynchronize(stream_));
 	if (f64_) {
@@ -1313,7 +1521,7 @@ int Grid::surface_values(double* out)
 // node order, then value / sum in the element type
 int Grid::surface_probabilities(float* out)
 {
-	const int cnt = (int) surface_vec_.size();
+	const int cnt = surf_count_;
 	std::vector<double> v(cnt);
 	if (int rc = surface_values(v.data())) return rc;
 	if (f64_) {
@@ -1328,16 +1536,15 @@ int Grid::surface_probabilities(float* out)
 	return 0;
 }
 
-int Grid::resolve_candidate(int* node, int* near_tie)
+int Grid::resolve_candidate(int* node, int* near_tie, int* slot)
 {
-	long long am[2];
-	HIPCHK(hipMemcpyAsync(am, d_argmax_.ptr(), sizeof(am), hipMemcpyDeviceToHost, stream_));
-	HIPCHK(hipStreamSynchronize(stream_));
-	int w = (int) am[0];
-	*near_tie = (int) am[1];
-	if (am[1]) {
+	if (!choice_valid_) return fail(-6, "surface changed since its last evaluation");
+	int w = choice_host_->slot;
+	*near_tie = choice_host_->near_tie;
+	*node = choice_host_->node;
+	if (choice_host_->near_tie) {
 		// exact rule: first maximum of the normalised values
-		const int cnt = (int) surface_vec_.size();
+		const int cnt = surf_count_;
 		std::vector<double> v(cnt);
 		if (int rc = surface_values(v.data())) return rc;
 		if (f64_) {
@@ -1355,20 +1562,23 @@ int Grid::resolve_candidate(int* node, int* near_tie)
 				if ((float) v[pick] / sum < (float) v[i] / sum) pick = i;
 			w = pick;
 		}
+		if (int rc = host_list()) return rc;
+		*node = surface_vec_[w];
 	}
-	*node = surface_vec_[w];
+	if (slot) *slot = w;
 	return 0;
 }
 
 // evapMonteCarlo (surface_3d.cpp:433-450): threshold rand()/RAND_MAX on the
 // running float sum of the normalised probabilities in node order; draws
 // from the process's own rand() sequence like the reference
-int Grid::mc_candidate(int* node)
+int Grid::mc_candidate(int* node, int* slot)
 {
-	const int cnt = (int) surface_vec_.size();
+	const int cnt = surf_count_;
 	if (cnt == 0) return fail(-6, "empty surface");
 	std::vector<float> p(cnt);
 	if (int rc = surface_probabilities(p.data())) return rc;
+	if (int rc = host_list()) return rc;
 	double thr = prand();
 	thr /= RAND_MAX;
 	int k = 0;
@@ -1378,14 +1588,15 @@ int Grid::mc_candidate(int* node)
 		acc += p[k];
 	}
 	*node = surface_vec_[k];
+	if (slot) *slot = k;
 	return 0;
 }
 
-int Grid::surface_candidate(int* node, int* near_tie)
+int Grid::surface_candidate(int* node, int* near_tie, int* slot)
 {
-	if (surface_vec_.empty()) return fail(-6, "empty surface");
+	if (surf_count_ == 0) return fail(-6, "empty surface");
 	int nt;
-	const int rc = resolve_candidate(node, &nt);
+	const int rc = resolve_candidate(node, &nt, slot);
 	if (near_tie) *near_tie = nt;
 	return rc;
 }
@@ -1413,7 +1624,9 @@ int Grid::update_coordination()
 		for (int k = mesh_.ptr[i]; k < mesh_.ptr[i + 1]; k++)
 			if (ids_[mesh_.idx[k]] >= 10) count++;
 		if (count == 0) continue;
+		const bool was_vacuum = ids_[i] == vacuum_id_;
 		ids_[i] = (int16_t) (signed char) (ids_[i] + count);
+		if (was_vacuum != (ids_[i] == vacuum_id_)) refresh_vacuum(i);
 		changed = true;
 	}
 	if (!changed) return 0;
@@ -1434,15 +1647,15 @@ int Grid::take_away(int node)
 {
 	if (vacuum_id_ < 0) return fail(-6, "surface not initialised");
 	auto update = [&](int v) -> int {
-		if (surface_.erase(v) != 1) return fail(-6, "Surface_3d::Table::update(): node not on the surface");
+		if (!(sstate_[v] & kSurfSite)) return fail(-6, "Surface_3d::Table::update(): node not on the surface");
+		set_site(v, false);
 		for (int k = mesh_.ptr[v]; k < mesh_.ptr[v + 1]; k++) {
 			const int j = mesh_.idx[k];
 			if (ids_[j] == vacuum_id_) continue;
 			if (!species_ok(j)) return fail(-8, "Configuration::Table::operator[]()");
 			if (!species_[ids_[j]].removable) continue;
-			if (has_vacuum_face(mesh_, ids_, a_one32_, vacuum_id_, j)) surface_.insert(j);
+			if (has_vacuum_face(mesh_, ids_, a_one32_, vacuum_id_, j)) set_site(j, true);
 		}
-		surface_dirty_ = true;
 		return 0;
 	};
 	if (int rc = set_id_and_resync(node, vacuum_id_)) return rc;
@@ -1461,6 +1674,8 @@ int Grid::take_away(int node)
 		if (int rc = set_id_and_resync(j, vacuum_id_)) return rc;
 		if (int rc = update(j)) return rc;
 	}
+	// the state bytes and the site list follow on the device at the next
+	// flush (the relaxation does not read them)
 	return 0;
 }
 
@@ -1683,19 +1898,12 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 	if (o.method >= 1 && !accel_)
 		if (int rc = accel_setup(0.9)) return rc;
 	amg_select(o.accel_solver >= 1, o.accel_solver == 2);
-	if (vacuum_id_ != o.vacuum_id || surface_.empty()) surface_init(o.vacuum_id);
+	if (vacuum_id_ != o.vacuum_id || surf_count_ == 0)
+		if (int rc = surface_init(o.vacuum_id)) return rc;
 	if (int rc = surface_evaluate(o.exact_probability != 0, nullptr, o.prob_mode)) return rc;
 	if (o.trajectory && (o.voltage_queue < 0 || o.voltage_queue > 1 << 20)) return fail(-1, "bad voltage queue length");
-	auto apex_of_surface = [&](float* apex) {
-		int top = *surface_.begin();
-		for (int v : surface_)
-			if (mesh_.xyz[3 * (size_t) v + 2] > mesh_.xyz[3 * (size_t) top + 2]) top = v;
-		for (int c = 0; c < 3; c++) apex[c] = mesh_.xyz[3 * (size_t) top + c];
-	};
-	if (!apex0_set_ && !surface_.empty()) {
-		float a0[3];
-		apex_of_surface(a0);
-		apex0_ = a0[2];
+	if (!apex0_set_ && surf_count_ > 0) {
+		apex0_ = choice_host_->apex[2];
 		apex0_set_ = true;
 	}
 	// the fork applies update_coordination once after the initial surface
@@ -1716,40 +1924,54 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 	if (o.init_event_cnt < 0) return fail(-1, "negative initial event count");
 	if (event_cnt_ < 0) event_cnt_ = o.init_event_cnt;
 	for (int ev = 0; ev < k; ev++) {
-		if (surface_.empty()) break;
+		if (surf_count_ == 0) break;
 		EventRecord& r = out[ev];
 		std::memset(&r, 0, sizeof(r));
 		r.event_index = ++event_cnt_;
 		stat_phase_events_++;
 		const float scaling_reference = (float) scaling_ref_;
-		apex_of_surface(r.apex);
-		int node, nt = 0;
+		// the last evaluation's record: the apex, the first maximum and its
+		// potential and field (read back with that evaluation)
+		const ChoiceRec rec = *choice_host_;
+		for (int c = 0; c < 3; c++) r.apex[c] = rec.apex[c];
+		int node, nt = 0, slot = 0;
 		if (o.evap_mode == 0) {
-			if (int rc = surface_candidate(&node, &nt)) return rc;
+			if (int rc = surface_candidate(&node, &nt, &slot)) return rc;
 		} else {
 			// the choice draws from the rand() stream the flights advance
 			if (int rc = flight_drain(out, o)) return rc;
-			if (int rc = mc_candidate(&node)) return rc;
+			if (int rc = mc_candidate(&node, &slot)) return rc;
 		}
 		r.node = node;
 		r.id = ids_[node];
 		r.near_tie = nt;
-		if (int rc = surface_normal(node, r.normal)) return rc;
+		// the normal on the device, on the surface the site was chosen from
+		if (int rc = launch_normal(node)) return rc;
 		// the chosen site's value now; its probability (value / normaliser)
 		// once the normaliser is in, before the surface is evaluated again
+		const bool in_rec = node == rec.node;
 		double vw = 0.0;
 		float v32 = 0.0f;
 		if (o.exact_probability) {
-			const int slot = (int) (std::lower_bound(surface_vec_.begin(), surface_vec_.end(), node) - surface_vec_.begin());
-			const size_t rs = f64_ ? sizeof(double) : sizeof(float);
-			if (f64_)
-				HIPCHK(hipMemcpyAsync(&vw, d_surf_val_.as<char>() + rs * slot, sizeof(double), hipMemcpyDeviceToHost, stream_));
-			else
-				HIPCHK(hipMemcpyAsync(&v32, d_surf_val_.as<char>() + rs * slot, sizeof(float), hipMemcpyDeviceToHost, stream_));
-			HIPCHK(hipStreamSynchronize(stream_));
+			if (in_rec) {
+				vw = rec.value;
+				v32 = (float) rec.value;
+			} else {
+				const size_t rs = f64_ ? sizeof(double) : sizeof(float);
+				if (f64_)
+					HIPCHK(hipMemcpyAsync(&vw, d_surf_val_.as<char>() + rs * slot, sizeof(double), hipMemcpyDeviceToHost, stream_));
+				else
+					HIPCHK(hipMemcpyAsync(&v32, d_surf_val_.as<char>() + rs * slot, sizeof(float), hipMemcpyDeviceToHost, stream_));
+				HIPCHK(hipStreamSynchronize(stream_));
+			}
 		}
-		if (int rc = potential(node, &r.pot_before)) return rc;
-		if (int rc = field_o1(1, &node, r.field_before)) return rc;
+		if (in_rec) {
+			r.pot_before = rec.pot;
+			for (int c = 0; c < 3; c++) r.field_before[c] = rec.field[c];
+		} else {
+			if (int rc = potential(node, &r.pot_before)) return rc;
+			if (int rc = field_o1(1, &node, r.field_before)) return rc;
+		}
 		mark(kPhaseChoice);
 		if (int rc = take_away(node)) return rc;
 		if (o.update_coordination)
@@ -1792,14 +2014,22 @@ int Grid::evaporate_events(const EvapOptions& o, int k, EventRecord* out, int* d
 			else
 				r.probability = v32 / (float) sum;
 		}
-		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode)) return rc;
-		r.surface_size = (int) surface_.size();
+		// the new surface's evaluation and choice, with the potential and
+		// field of the evaporated node after the event and the event's normal
+		if (int rc = surface_evaluate(o.exact_probability != 0, &r.scaling_ref, o.prob_mode, node)) return rc;
+		r.surface_size = surf_count_;
+		{
+			const ChoiceRec& nr = *choice_host_;
+			if (nr.normal_status == 1) return fail(-3, "Geometry_3d::find_best_plane(): NaN");
+			if (nr.normal_status == 2) return fail(-2, "surface normal: three-hop shell beyond the device buffer");
+			for (int c = 0; c < 3; c++) r.normal[c] = nr.normal[c];
+			r.pot_after = nr.pot_prev;
+			for (int c = 0; c < 3; c++) r.field_after[c] = nr.field_prev[c];
+		}
 		// V) the ion's flight in the relaxed field (process.cpp:940-975) and
 		// VI) voltage and time scale (process.cpp:983-1037)
 		if (o.trajectory && !(r.id >= 0 && r.id < 128 && species_[r.id].defined))
 			return fail(-8, "Configuration::Table::operator[]()");
-		if (int rc = potential(node, &r.pot_after)) return rc;
-		if (int rc = field_o1(1, &node, r.field_after)) return rc;
 		mark(kPhaseSurface);
 		float flen = 0.0f;
 		for (int c = 0; c < 3; c++) flen += (float) r.field_before[c] * (float) r.field_before[c];

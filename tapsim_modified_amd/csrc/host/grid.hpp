@@ -156,16 +156,18 @@ public:
 
 	// surface (Surface_3d::Table, surface_3d.cpp:50-119) and selection
 	int surface_init(int vacuum_id);
-	int surface_size() const { return (int) surface_.size(); }
-	int surface_nodes(int* out) const;
-	int surface_normal(int node, float out[3]);  // Surface_3d::Table::normal
+	int surface_size() const { return surf_count_; }
+	int surface_nodes(int* out);
+	int surface_normal(int node, float out[3]);  // Surface_3d::Table::normal (host)
 	// evap_compute_specificFields + evap_compute_probabilities (mode 0 linear
-	// field, 1 Boltzmann, 2 linear force, 3 Voronoi flux force)
-	int surface_evaluate(bool exact_sum, double* scaling_ref, int mode = 0);
+	// field, 1 Boltzmann, 2 linear force, 3 Voronoi flux force), the choice,
+	// the apex and, for prev >= 0, potential() and field_o1 of node prev
+	int surface_evaluate(bool exact_sum, double* scaling_ref, int mode = 0, int prev = -1);
 	int surface_values(double* out);                  // |E|/F per site, site order
 	int surface_probabilities(float* out);            // normalised as the reference
-	int surface_candidate(int* node, int* near_tie);  // evapMaximum
-	int mc_candidate(int* node);                      // evapMonteCarlo
+	// evapMaximum: node, and its position in the site list
+	int surface_candidate(int* node, int* near_tie, int* slot = nullptr);
+	int mc_candidate(int* node, int* slot = nullptr);  // evapMonteCarlo
 	// evap_takeAway (surface_3d.cpp:507-564)
 	int take_away(int node);
 
@@ -237,7 +239,6 @@ private:
 	bool participates(int row, int col) const;
 	void row_entries(int node, std::vector<int>* cols, std::vector<double>* coups, double* ws) const;
 	int build_rows();
-	int upload_surface();
 	int write_slot(int slot, int node, std::vector<long long>* ai, std::vector<int>* vi, std::vector<long long>* ar,
 	               std::vector<double>* vr, std::vector<long long>* aw, std::vector<double>* vw);
 	long long entry_index(int row, int col) const;
@@ -248,7 +249,7 @@ private:
 	bool ord_dirty_ = true;
 	int ensure_inplace_order();
 	int launch_relax_call(const SweepArgs& a, int s, int cycles);
-	int resolve_candidate(int* node, int* near_tie);
+	int resolve_candidate(int* node, int* near_tie, int* slot);
 
 	int mode_;
 	int device_ = 0;
@@ -321,21 +322,46 @@ private:
 	std::vector<float> lpf_val_;
 	int long_storage(int slot, int deg);
 
-	std::set<int> surface_;
+	// the surface set (Surface_3d::Table's nodes): the host mirror of the
+	// device state bytes (bit 0 site, bit 1 vacuum), the site count, the
+	// nodes whose byte changed since the last upload; the ascending site list
+	// itself is rebuilt on the device and copied to surface_vec_ only where
+	// the host needs it (Monte Carlo choice, exact tie resolution)
 	int vacuum_id_ = -1;
-	bool surface_dirty_ = true;
+	std::vector<uint8_t> sstate_;
+	int surf_count_ = 0;
+	std::vector<int> sst_pending_;
+	bool list_dirty_ = false;        // a site bit changed: the list is rebuilt at the next flush
+	bool host_list_stale_ = true;    // surface_vec_ no longer matches the device list
+	bool eval_stale_ = true;         // the sites changed since the last evaluation
 	std::vector<int> surface_vec_;
 	double scaling_ref_ = 1.0;
+	DevBuf d_sstate_, d_surf_blk_, d_sst_upd_, d_marks_, d_nshell_, d_choice_;
+	unsigned char* sst_host_ = nullptr;  // pinned staging of the state updates
+	size_t sst_host_bytes_ = 0;
+	hipEvent_t sst_ev_ = nullptr;        // the last upload from that staging
+	ChoiceRec* choice_host_ = nullptr;   // pinned readback of the evaluation's choice record
+	bool choice_valid_ = false;          // choice_host_ describes the current sites and potentials
+	int normal_stamp_ = 0;
+	hipEvent_t norm_src_ = nullptr, norm_done_ = nullptr;
+	bool normal_pending_ = false;         // a normal is in flight on the side stream
+	int ensure_side_stream();
+	void set_site(int node, bool on);
+	void refresh_vacuum(int node);
+	int surface_flush();
+	int host_list();
+	int launch_normal(int node);
 
 	hipStream_t stream_ = nullptr;
 	DevBuf d_xyz_, d_ptr_, d_idx_, d_coupfull_, d_flags_, d_evapF_, d_evapE_, d_eps_, d_phi0_, d_phi1_, d_surf_ef_;
 	int prob_mode_ = 0;
 	DevBuf d_slot_node_, d_slot_deg_, d_slice_off_, d_col_, d_coup_, d_ws_, d_q_, d_ctl_, d_block_max_, d_block_flag_;
-	DevBuf d_surf_, d_surf_val_, d_argmax_, d_maxval_, d_sum_, d_status_;
+	DevBuf d_surf_, d_surf_val_, d_sum_, d_status_;
 	DevBuf d_shell_nodes_, d_shell_slots_, d_level_ptr_, d_order_, d_local_out_;
 	DevBuf d_patch_addr_, d_patch_i_, d_patch_r_, d_nodes_, d_fields_;
 	void* h_pinned_ = nullptr;
 	size_t h_pinned_bytes_ = 0;
+	hipEvent_t patch_ev_ = nullptr;       // the last upload from h_pinned_
 	hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
 	// accelerated relaxation state
 	bool accel_ = false;
@@ -368,7 +394,10 @@ private:
 	double* amg_host_ = nullptr;  // pinned readback of the scalars, two batches
 	hipEvent_t amg_ev_[2] = {nullptr, nullptr};
 	void amg_graph_drop();
-	int amg_touch(const std::vector<int>& nodes, const std::vector<int>& off_slots);
+	// the level-0 row sums of the touched rows (slot, value) and the slots
+	// whose residual entry is cleared, shipped with the row patches
+	void amg_touch(const std::vector<int>& nodes, const std::vector<int>& off_slots, std::vector<long long>* dn_addr,
+	               std::vector<float>* dn_val, std::vector<long long>* res_addr);
 	int amg_solve(const AccelArgs& x, float f, int* iters, float** delta);
 	// trajectories: mesh arrays for the point location, node-field snapshot
 	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_, d_rng_;

@@ -71,9 +71,9 @@ void build_topology(Mesh* m, int threads = 0);
 
 // Geometry_3d::find_best_plane over offsets (x,y,z triples) and
 // Surface_3d::Table::normal; both false on a NaN result (normal.cpp).
+// sstate holds per node bit 0 (surface site) and bit 1 (vacuum).
 bool best_plane(const std::vector<float>& off, float out[3]);
-bool surface_normal(const Mesh& mesh, const std::set<int>& surface, const int16_t* ids, int vacuum_id, int node,
-                    float out[3]);
+bool surface_normal(const Mesh& mesh, const uint8_t* sstate, int node, float out[3]);
 
 // Orientation sign of det[b-a, c-a, d-a] with exact fallback.
 int orient_sign(const float* a, const float* b, const float* c, const float* d);

@@ -118,6 +118,15 @@ int launch_local_relax(hipStream_t, const SweepArgs&, const LocalArgs&, bool f64
 
 // dst[0..n) = value
 int launch_fill_i32(hipStream_t, int n, int* dst, int value);
+// one node's flags and species data (values passed with the launch)
+struct NodeSpecies {
+	int node;
+	unsigned char flags;
+	float evap_field, evap_energy, epsilon;
+	unsigned char* d_flags;
+	float *d_evap_field, *d_evap_energy, *d_epsilon;
+};
+int launch_set_node(hipStream_t, const NodeSpecies&);
 
 // generic scatter of patches: dst[addr[k]] = val[k]
 int launch_scatter_f64(hipStream_t, int n, const long long* addr, const double* val, double* dst);
@@ -170,15 +179,16 @@ struct AmgLevel {
 enum { kAmgAlpha = 0, kAmgBeta = 1, kAmgRz = 2, kAmgRr = 3, kAmgRr0 = 4, kAmgDone = 5, kAmgTol2 = 6, kAmgIters = 7,
        kAmgScalars = 8 };
 // one PCG solve's device state: level 0 (the grid's FP32 correction rows,
-// node-indexed vectors), the coarse levels lv[0..nlev) with their Jacobi
+// slot-indexed vectors), the coarse levels lv[0..nlev) with their Jacobi
 // weights w and per-level vectors (right-hand side, iterate, scratch), the
 // coarsest level's dense inverse, the block partials of the reductions, the
 // scalars
 struct AmgSolve {
 	AccelArgs x;
 	const float* dn;   // row coupling sums D
-	const int* agg0;   // level-0 node -> aggregate of lv[0]
+	const int* agg0;   // level-0 slot -> aggregate of lv[0]
 	float w0;
+	float wc;          // scale of every coarse correction (over-correction)
 	const AmgLevel* lv;
 	const float* w;
 	int nlev;
@@ -327,7 +337,6 @@ struct FieldArgs {
 	const void* phi;          // potential() buffer
 	const float* evap_field;  // per node
 	void* value;              // R per surface slot: the probability model's unnormalised value
-	float* field;             // optional 3 floats per slot (E vector) or null
 	int* status;
 	// probability model (Surface_3d::evap_compute_probabilities, surface_3d.cpp:223-417):
 	// 0 linear field |E|/F, 1 Boltzmann, 2 linear force (field_o2), 3 Voronoi flux force
@@ -339,9 +348,72 @@ struct FieldArgs {
 };
 int launch_surface_fields(hipStream_t, const FieldArgs&, bool f64);
 
-// argmax with first-index tie-break plus the near-tie flag
-int launch_surface_argmax(hipStream_t, int count, const void* value, bool f64, long long* out /* [slot, tie_flag] */,
-                          double* maxval);
+// ---------------------------------------------------------------- surface set
+// The surface (Surface_3d::Table's node set) lives on the device as one state
+// byte per node, bit 0 a surface site and bit 1 a vacuum node, and as the
+// ascending list of the sites that a compaction rebuilds from the bytes.
+enum { kSurfSite = 1, kSurfVacuum = 2 };
+// sstate[nodes[i]] = vals[i] (distinct nodes)
+int launch_sstate_scatter(hipStream_t, int count, const int* nodes, const unsigned char* vals, unsigned char* sstate);
+// list = the surface sites in ascending order, *count their number; blk
+// holds one int per 4096 nodes
+int surface_compact_blocks(int n);
+int launch_surface_compact(hipStream_t, int n, const unsigned char* sstate, int* blk, int* list, int* count);
+
+// what the event loop reads of one surface evaluation, in one readback
+struct ChoiceRec {
+	double value;          // the first maximum's unnormalised value
+	double pot;            // potential() of the chosen site
+	double field[3];       // field_o1 of the chosen site
+	double pot_prev;       // potential() and field_o1 of `prev` (the last evaporated atom)
+	double field_prev[3];
+	double scaling_ref;    // max |E|/F (evap_compute_specificFields)
+	float apex[3];         // Surface_3d::Table::apex: the first highest site
+	int node;              // the first maximum's node (-1: empty surface)
+	int slot;              // and its position in the site list
+	int near_tie;          // 1: a value below it may tie after normalisation, or a value is negative / NaN
+	int count;             // sites in the device list
+	int status;            // field status bits (1, 2, 4 as the fields kernel; 16: singular field_o1 of prev)
+	// Surface_3d::Table::normal of the event's site (k_normal, written apart
+	// from the choice): status 0 done, 1 NaN (find_best_plane), 2 shell too large
+	float normal[3];
+	int normal_status;
+};
+struct ChooseArgs {
+	int count;
+	const int* list;
+	const void* value;     // R per slot
+	const void* value_ef;  // R per slot, or null (then value)
+	const float* xyz;
+	const int* ptr;
+	const int* idx;
+	const void* coup;
+	const unsigned char* flags;
+	const void* phi;
+	const int* status;     // the fields kernel's status word
+	const int* dev_count;  // the compaction's count
+	int prev;              // node whose potential and field follow the evaluation, or -1
+	ChoiceRec* rec;
+};
+int launch_surface_choose(hipStream_t, const ChooseArgs&, bool f64);
+
+// Surface_3d::Table::normal of one node on the device: the three-hop shell
+// (marks[] holds the stamp of the call that last reached a node), ascending
+// sort, moments and the shared plane solution (plane.hpp)
+constexpr int kNormalShellCap = 1 << 20;
+struct NormalArgs {
+	int node;
+	const int* ptr;
+	const int* idx;
+	const float* xyz;
+	const unsigned char* sstate;
+	int* marks;
+	int stamp;             // != 0, new per call
+	int* shell;            // 2 * kNormalShellCap ints
+	ChoiceRec* rec;        // normal, normal_status
+};
+int launch_surface_normal(hipStream_t, const NormalArgs&);
+
 // sequential sum in slot order (the reference's float/double normaliser)
 int launch_surface_sum(hipStream_t, int count, const void* value, bool f64, double* out);
 

@@ -485,6 +485,14 @@ __global__ void k_fill_i32(int n, int* dst, int value)
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) dst[i] = value;
 }
+__global__ void k_set_node(NodeSpecies v)
+{
+	if (threadIdx.x != 0) return;
+	v.d_flags[v.node] = v.flags;
+	v.d_evap_field[v.node] = v.evap_field;
+	v.d_evap_energy[v.node] = v.evap_energy;
+	v.d_epsilon[v.node] = v.epsilon;
+}
 __global__ void k_scatter_i32(int n, const long long* addr, const int* val, int* dst)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -556,6 +564,11 @@ int launch_fill_i32(hipStream_t s, int n, int* dst, int value)
 {
 	if (n <= 0) return 0;
 	hipLaunchKernelGGL(dev::k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, n, dst, value);
+	return (int) hipGetLastError();
+}
+int launch_set_node(hipStream_t s, const NodeSpecies& v)
+{
+	hipLaunchKernelGGL(dev::k_set_node, dim3(1), dim3(64), 0, s, v);
 	return (int) hipGetLastError();
 }
 int launch_scatter_i32(hipStream_t s, int n, const long long* addr, const int* val, int* dst)

@@ -2,10 +2,12 @@
 // grid_3d.cpp:909-991), the evaporation score |E| / F_evap of every surface
 // site (Surface_3d::evap_compute_specificFields / probLinear_field,
 // surface_3d.cpp:194-219, 266-297), the first-maximum choice
-// (evapMaximum, surface_3d.cpp:423-431) and the sequential normaliser.
+// (evapMaximum, surface_3d.cpp:423-431) with the apex, the sequential
+// normaliser, the surface normal, and the device copy of the surface set.
 #include "device_math.hpp"
 #include "kernels.hpp"
 #include "field.hpp"
+#include "plane.hpp"
 
 namespace tsg {
 namespace dev {
@@ -95,8 +97,6 @@ __global__ __launch_bounds__(256) void k_surface_fields(FieldArgs f)
 	}
 	((R*) f.value)[s] = value;
 	if (f.value_ef) ((R*) f.value_ef)[s] = ef;
-	if (f.field)
-		for (int i = 0; i < 3; i++) f.field[3 * (size_t) s + i] = (float) E[i];
 }
 
 template <typename R>
@@ -126,24 +126,129 @@ __global__ void k_node_fields_o2(int count, const int* nodes, const float* xyz, 
 	charge[s] = (double) q;
 }
 
-// first maximum in slot order (slots are in ascending node order); a slot
-// below the winner whose value lies within a few ulps of the maximum could
-// tie with it after the reference's float normalisation (v / sum), which
-// sets the near-tie flag for the host to resolve with the exact normaliser
-template <typename R>
-__global__ __launch_bounds__(1024) void k_argmax(int count, const R* __restrict__ v, long long* out, double* maxval)
+// ---- the device surface set
+
+__global__ __launch_bounds__(256) void k_sstate_scatter(int count, const int* __restrict__ nodes,
+                                                        const unsigned char* __restrict__ vals,
+                                                        unsigned char* __restrict__ sstate)
 {
-	__shared__ R sv[1024];
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < count) sstate[nodes[i]] = vals[i];
+}
+
+// the compaction: 256 threads x 16 nodes per block
+constexpr int kCompactPerThread = 16;
+constexpr int kCompactBlock = 256 * kCompactPerThread;
+
+// surface sites among the thread's 16 nodes
+__device__ __forceinline__ int sites_of_thread(int n, const unsigned char* __restrict__ sstate, int first)
+{
+	int c = 0;
+	if (first + kCompactPerThread <= n) {
+		const uint4 w = *(const uint4*) (sstate + first);
+		c = __popc(w.x & 0x01010101u) + __popc(w.y & 0x01010101u) + __popc(w.z & 0x01010101u) +
+		    __popc(w.w & 0x01010101u);
+	} else {
+		for (int v = first; v < n; v++) c += sstate[v] & kSurfSite;
+	}
+	return c;
+}
+
+// exclusive prefix of c over the 256 threads of the block, and the block total
+__device__ __forceinline__ int block_exclusive_scan(int c, int* total)
+{
+	__shared__ int wsum[4];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int inc = c;
+	for (int off = 1; off < 64; off <<= 1) {
+		const int t = __shfl_up(inc, off, 64);
+		if (lane >= off) inc += t;
+	}
+	if (lane == 63) wsum[wave] = inc;
+	__syncthreads();
+	int before = 0;
+	for (int w = 0; w < wave; w++) before += wsum[w];
+	*total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+	__syncthreads();
+	return before + inc - c;
+}
+
+__global__ __launch_bounds__(256) void k_surf_count(int n, const unsigned char* __restrict__ sstate, int* __restrict__ blk)
+{
+	const int first = blockIdx.x * kCompactBlock + threadIdx.x * kCompactPerThread;
+	const int c = first < n ? sites_of_thread(n, sstate, first) : 0;
+	int total;
+	block_exclusive_scan(c, &total);
+	if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_surf_write(int n, const unsigned char* __restrict__ sstate,
+                                                    const int* __restrict__ blk, int* __restrict__ list,
+                                                    int* __restrict__ count)
+{
+	// sites in the blocks before this one
+	int pre = 0;
+	for (int b = threadIdx.x; b < (int) blockIdx.x; b += blockDim.x) pre += blk[b];
+	int ptotal;
+	block_exclusive_scan(pre, &ptotal);
+	const int first = blockIdx.x * kCompactBlock + threadIdx.x * kCompactPerThread;
+	const int c = first < n ? sites_of_thread(n, sstate, first) : 0;
+	int total;
+	int pos = ptotal + block_exclusive_scan(c, &total);
+	if (c)
+		for (int v = first; v < first + kCompactPerThread && v < n; v++)
+			if (sstate[v] & kSurfSite) list[pos++] = v;
+	if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = ptotal + total;
+}
+
+// ---- the choice
+
+// orderable key of a coordinate (-0 counts as +0, as the comparison of the
+// reference's apex scan does) and the slot, larger for a smaller slot
+__device__ __forceinline__ unsigned long long apex_key(float z, int s)
+{
+	const unsigned int u = __float_as_uint(z + 0.0f);
+	const unsigned int k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+	return ((unsigned long long) k << 32) | (unsigned long long) (0xffffffffu - (unsigned int) s);
+}
+
+// evapMaximum (surface_3d.cpp:423-431), the apex (surface_3d.cpp:171-190),
+// the scaling reference (surface_3d.cpp:194-219) and the chosen site's
+// potential and field in one pass over the site list.  The first maximum in
+// slot order (slots ascend with the node index) decides; a slot below the
+// winner whose value lies within a few ulps of the maximum could tie with it
+// after the reference's float normalisation (v / sum), and a negative or NaN
+// value breaks the order of v / sum: both set the near-tie flag for the host
+// to resolve with the exact normaliser.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_choose(ChooseArgs a)
+{
+	__shared__ R sv[1024], se[1024];
 	__shared__ int ss[1024];
-	R best = (R) -1;
-	int bs = 0x7fffffff;
-	for (int s = threadIdx.x; s < count; s += blockDim.x) {
+	__shared__ unsigned long long sa[1024];
+	__shared__ int sneg, snear, sstat;
+	const R* v = (const R*) a.value;
+	const R* ve = a.value_ef ? (const R*) a.value_ef : v;
+	const R ninf = (R) -INFINITY;
+	R best = ninf, emax = ninf;
+	int bs = 0x7fffffff, neg = 0;
+	unsigned long long ak = 0;
+	for (int s = threadIdx.x; s < a.count; s += blockDim.x) {
 		const R x = v[s];
 		if (x > best) { best = x; bs = s; }  // increasing s: ties keep the first
+		if (!(x >= (R) 0)) neg = 1;
+		const R e = ve[s];
+		if (e > emax) emax = e;
+		const unsigned long long k = apex_key(a.xyz[3 * (size_t) a.list[s] + 2], s);
+		if (k > ak) ak = k;
 	}
+	if (threadIdx.x == 0) { sneg = 0; snear = 0; sstat = 0; }
 	sv[threadIdx.x] = best;
 	ss[threadIdx.x] = bs;
+	se[threadIdx.x] = emax;
+	sa[threadIdx.x] = ak;
 	__syncthreads();
+	if (neg) sneg = 1;
 	for (int h = blockDim.x / 2; h > 0; h >>= 1) {
 		if (threadIdx.x < h) {
 			const R o = sv[threadIdx.x + h];
@@ -152,23 +257,204 @@ __global__ __launch_bounds__(1024) void k_argmax(int count, const R* __restrict_
 				sv[threadIdx.x] = o;
 				ss[threadIdx.x] = os;
 			}
+			if (se[threadIdx.x + h] > se[threadIdx.x]) se[threadIdx.x] = se[threadIdx.x + h];
+			if (sa[threadIdx.x + h] > sa[threadIdx.x]) sa[threadIdx.x] = sa[threadIdx.x + h];
 		}
 		__syncthreads();
 	}
 	const R vmax = sv[0];
-	const int w = ss[0];
-	__shared__ int near;
-	if (threadIdx.x == 0) near = 0;
-	__syncthreads();
+	const int w = ss[0] == 0x7fffffff ? 0 : ss[0];
 	const R eps = (sizeof(R) == 4) ? (R) (8.0 / 16777216.0) : (R) (8.0 / 9007199254740992.0);
 	const R lim = vmax - vmax * eps;
 	for (int s = threadIdx.x; s < w; s += blockDim.x)
-		if (v[s] >= lim) near = 1;
+		if (v[s] >= lim) snear = 1;
+	const R* phi = (const R*) a.phi;
+	ChoiceRec* rec = a.rec;
+	if (threadIdx.x == 0 && a.count > 0) {
+		const int node = a.list[w];
+		R E[3];
+		if (!field_o1<R>(node, a.xyz, a.ptr, a.idx, (const R*) a.coup, a.flags, phi, E)) {
+			atomicOr(&sstat, 1);
+			E[0] = E[1] = E[2] = (R) 0;
+		}
+		rec->node = node;
+		rec->slot = w;
+		rec->value = (double) v[w];
+		rec->pot = (double) phi[node];
+		for (int c = 0; c < 3; c++) rec->field[c] = (double) E[c];
+	}
+	if (threadIdx.x == 64 && a.prev >= 0) {
+		R E[3];
+		if (!field_o1<R>(a.prev, a.xyz, a.ptr, a.idx, (const R*) a.coup, a.flags, phi, E)) {
+			atomicOr(&sstat, 16);
+			E[0] = E[1] = E[2] = (R) 0;
+		}
+		rec->pot_prev = (double) phi[a.prev];
+		for (int c = 0; c < 3; c++) rec->field_prev[c] = (double) E[c];
+	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		out[0] = w;
-		out[1] = near;
-		*maxval = (double) vmax;
+		if (a.count == 0) {
+			rec->node = -1;
+			rec->slot = 0;
+		}
+		const int as = (int) (0xffffffffu - (unsigned int) (sa[0] & 0xffffffffu));
+		const int an = a.count > 0 ? a.list[as] : -1;
+		for (int c = 0; c < 3; c++) rec->apex[c] = an >= 0 ? a.xyz[3 * (size_t) an + c] : 0.0f;
+		rec->scaling_ref = (double) se[0];
+		rec->near_tie = (snear || sneg) ? 1 : 0;
+		rec->count = *a.dev_count;
+		rec->status = *a.status | sstat;
+	}
+}
+
+// ---- the surface normal (Surface_3d::Table::normal, surface_3d.cpp:123-169)
+
+constexpr int kNormalSortCap = 8192;  // shells up to this size are sorted in LDS
+constexpr int kNormalChunk = 1024;    // shell entries staged per pass of the sums
+
+__device__ __forceinline__ int load_agent(const int* p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the three-hop shell of a.node (adjacentNodes(node, set, 3),
+// geometry_3d.cpp:1112-1145), in ascending order; the offsets node - site of
+// its surface sites give the scatter moments as float sums in that order,
+// its vacuum nodes the orientation; one workgroup
+__global__ __launch_bounds__(256) void k_normal(NormalArgs a)
+{
+	__shared__ int sorted[kNormalSortCap];
+	__shared__ float cx[kNormalChunk], cy[kNormalChunk], cz[kNormalChunk];
+	__shared__ unsigned char cst[kNormalChunk];
+	__shared__ int s_cnt, s_over, s_lb, s_le;
+	__shared__ float s_acc[9];
+	ChoiceRec* rec = a.rec;
+	const int tid = threadIdx.x;
+	if (!(a.sstate[a.node] & kSurfSite)) {
+		if (tid == 0) {
+			for (int c = 0; c < 3; c++) rec->normal[c] = 0.0f;
+			rec->normal_status = 0;
+		}
+		return;
+	}
+	if (tid == 0) {
+		s_cnt = 0;
+		s_over = 0;
+		atomicExch(&a.marks[a.node], a.stamp);
+	}
+	__syncthreads();
+	// level 1: the node's neighbours; levels 2 and 3: the neighbours of the
+	// previous level's entries
+	for (int lvl = 0; lvl < 3; lvl++) {
+		const int lb = lvl == 0 ? 0 : s_lb, le = lvl == 0 ? 1 : s_le;
+		__syncthreads();
+		if (tid == 0) s_lb = s_cnt;
+		for (int f = lb + tid / 16; f < le; f += blockDim.x / 16) {
+			const int u = lvl == 0 ? a.node : load_agent(a.shell + f);
+			for (int k = a.ptr[u] + (tid & 15); k < a.ptr[u + 1]; k += 16) {
+				const int v = a.idx[k];
+				if (atomicExch(&a.marks[v], a.stamp) != a.stamp) {
+					const int p = atomicAdd(&s_cnt, 1);
+					if (p < kNormalShellCap) a.shell[p] = v;
+					else s_over = 1;
+				}
+			}
+		}
+		// the level's entries visible to every wave before they are read
+		__threadfence();
+		__syncthreads();
+		if (tid == 0) s_le = s_cnt;
+		__syncthreads();
+		if (s_over) break;
+	}
+	const int cnt = s_cnt;
+	if (s_over) {
+		if (tid == 0) rec->normal_status = 2;
+		return;
+	}
+	// ascending order: bitonic sort in LDS, or a rank sort into the second
+	// half of the shell buffer
+	const bool in_lds = cnt <= kNormalSortCap;
+	int* gsorted = a.shell + kNormalShellCap;
+	if (in_lds) {
+		int P = 1;
+		while (P < cnt) P <<= 1;
+		for (int i = tid; i < P; i += blockDim.x) sorted[i] = i < cnt ? load_agent(a.shell + i) : 0x7fffffff;
+		__syncthreads();
+		for (int k = 2; k <= P; k <<= 1)
+			for (int j = k >> 1; j > 0; j >>= 1) {
+				for (int i = tid; i < P; i += blockDim.x) {
+					const int l = i ^ j;
+					if (l > i) {
+						const int x = sorted[i], y = sorted[l];
+						if (((i & k) == 0) == (x > y)) {
+							sorted[i] = y;
+							sorted[l] = x;
+						}
+					}
+				}
+				__syncthreads();
+			}
+	} else {
+		for (int i = tid; i < cnt; i += blockDim.x) {
+			const int x = load_agent(a.shell + i);
+			int r = 0;
+			for (int j = 0; j < cnt; j++) r += load_agent(a.shell + j) < x;
+			gsorted[r] = x;
+		}
+		__threadfence();
+		__syncthreads();
+	}
+	// sums in shell order: threads 0-5 the moments (xx, xy, xz, yy, yz, zz)
+	// of the offsets x0 - x over the surface sites, threads 6-8 the
+	// orientation x - x0 over the vacuum nodes
+	const float x0[3] = {a.xyz[3 * (size_t) a.node], a.xyz[3 * (size_t) a.node + 1], a.xyz[3 * (size_t) a.node + 2]};
+	const int P0 = tid < 3 ? 0 : (tid < 5 ? 1 : 2);
+	const int Q0 = tid == 0 ? 0 : (tid == 1 || tid == 3 ? 1 : 2);
+	float acc = 0.0f;
+	for (int c0 = 0; c0 < cnt; c0 += kNormalChunk) {
+		const int m = min(kNormalChunk, cnt - c0);
+		for (int i = tid; i < m; i += blockDim.x) {
+			const int v = in_lds ? sorted[c0 + i] : load_agent(gsorted + c0 + i);
+			cx[i] = a.xyz[3 * (size_t) v];
+			cy[i] = a.xyz[3 * (size_t) v + 1];
+			cz[i] = a.xyz[3 * (size_t) v + 2];
+			cst[i] = a.sstate[v];
+		}
+		__syncthreads();
+		if (tid < 6) {
+			for (int i = 0; i < m; i++)
+				if (cst[i] & kSurfSite) {
+					const float x[3] = {cx[i], cy[i], cz[i]};
+					const float op = x0[P0] - x[P0], oq = x0[Q0] - x[Q0];
+					acc += op * oq;
+				}
+		} else if (tid < 9) {
+			const int c = tid - 6;
+			for (int i = 0; i < m; i++)
+				if (cst[i] & kSurfVacuum) {
+					const float x[3] = {cx[i], cy[i], cz[i]};
+					acc += x[c] - x0[c];
+				}
+		}
+		__syncthreads();
+	}
+	if (tid < 9) s_acc[tid] = acc;
+	__syncthreads();
+	if (tid == 0) {
+		float mom[6], n[3];
+		for (int t = 0; t < 6; t++) mom[t] = s_acc[t];
+		if (!plane::from_moments(mom, n)) {
+			rec->normal_status = 1;
+			return;
+		}
+		float dot = 0.0f;
+		for (int c = 0; c < 3; c++) dot += n[c] * s_acc[6 + c];
+		if (dot < 0.0f)
+			for (int c = 0; c < 3; c++) n[c] *= -1.0f;
+		for (int c = 0; c < 3; c++) rec->normal[c] = n[c];
+		rec->normal_status = 0;
 	}
 }
 
@@ -202,10 +488,34 @@ int launch_surface_fields(hipStream_t st, const FieldArgs& f, bool f64)
 	return (int) hipGetLastError();
 }
 
-int launch_surface_argmax(hipStream_t st, int count, const void* value, bool f64, long long* out, double* maxval)
+int launch_sstate_scatter(hipStream_t st, int count, const int* nodes, const unsigned char* vals, unsigned char* sstate)
 {
-	if (f64) hipLaunchKernelGGL(dev::k_argmax<double>, dim3(1), dim3(1024), 0, st, count, (const double*) value, out, maxval);
-	else hipLaunchKernelGGL(dev::k_argmax<float>, dim3(1), dim3(1024), 0, st, count, (const float*) value, out, maxval);
+	if (count <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_sstate_scatter, dim3((count + 255) / 256), dim3(256), 0, st, count, nodes, vals, sstate);
+	return (int) hipGetLastError();
+}
+
+int surface_compact_blocks(int n) { return (n + dev::kCompactBlock - 1) / dev::kCompactBlock; }
+
+int launch_surface_compact(hipStream_t st, int n, const unsigned char* sstate, int* blk, int* list, int* count)
+{
+	const int nb = surface_compact_blocks(n);
+	if (nb <= 0) return -1;
+	hipLaunchKernelGGL(dev::k_surf_count, dim3(nb), dim3(256), 0, st, n, sstate, blk);
+	hipLaunchKernelGGL(dev::k_surf_write, dim3(nb), dim3(256), 0, st, n, sstate, (const int*) blk, list, count);
+	return (int) hipGetLastError();
+}
+
+int launch_surface_choose(hipStream_t st, const ChooseArgs& a, bool f64)
+{
+	if (f64) hipLaunchKernelGGL(dev::k_choose<double>, dim3(1), dim3(1024), 0, st, a);
+	else hipLaunchKernelGGL(dev::k_choose<float>, dim3(1), dim3(1024), 0, st, a);
+	return (int) hipGetLastError();
+}
+
+int launch_surface_normal(hipStream_t st, const NormalArgs& a)
+{
+	hipLaunchKernelGGL(dev::k_normal, dim3(1), dim3(256), 0, st, a);
 	return (int) hipGetLastError();
 }
 

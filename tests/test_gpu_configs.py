@@ -99,6 +99,43 @@ def test_c1_plain_reproduces_the_committed_sequence(c1, c1_seq):
     assert np.array_equal([[e.local_iter, e.global_iter] for e in ev], c1_seq["plain_iter"][:n])
 
 
+def test_c1_device_surface_bookkeeping(c1):
+    """the surface set kept on the device over 40 events of the mixed path,
+    one event per call: the site list (rebuilt on the device after every
+    take-away) equals the oracle's Surface_3d set driven with the same atoms;
+    the record's apex is the first highest site of the list before the
+    event, and its normal (the device kernel) is bit-identical to the host
+    restatement's normal of the chosen site on that list (the host
+    restatement is pinned to the reference's normals by the fixture tests)"""
+    g = grid_from(c1)
+    g.accel_setup(0.9)
+    tv, tn = c1["mesh"].tets()
+    o = O.Oracle(O.F64, c1["xyz"], c1["ids"], tv, tn, O.species_table(bench.SPECIES))
+    o.sync()
+    o.surface_init(0)
+    g.surface_init(0)
+    xyz = c1["xyz"]
+    tsg.srand(99)
+    opts = options(1)
+    sizes = []
+    for k in range(40):
+        sites = g.surface_nodes()
+        assert np.array_equal(sites, o.surface_nodes())
+        g.surface_evaluate(True, 0)
+        node, _ = g.candidate()
+        normal = g.surface_normal(node)
+        apex = xyz[sites[np.argmax(xyz[sites, 2])]]
+        (e,) = g.evaporate(opts, 1)
+        assert e.node == node
+        assert np.array_equal(np.asarray(e.normal, np.float32).view(np.uint32), normal.view(np.uint32))
+        assert np.array_equal(np.asarray(e.apex, np.float32).view(np.uint32), apex.view(np.uint32))
+        o.take_away(e.node)
+        assert e.surface_size == len(o.surface_nodes())
+        sizes.append(e.surface_size)
+    assert np.array_equal(g.surface_nodes(), o.surface_nodes())
+    log("c1 surface: %d events, sites %d -> %d" % (len(sizes), sizes[0], sizes[-1]))
+
+
 def test_c1_mixed_and_learned_keep_the_plain_sequence_for_1000_events(c1, c1_seq):
     """the mixed FP32/FP64 path (multigrid correction solve, residual <= 1e-8 V)
     and the learned warm start (trained on its first 200 events) evaporate the
