@@ -1224,7 +1224,7 @@ int Grid::surface_init(int vacuum_id)
 		d_surf_.alloc(sizeof(int) * ((size_t) n + 1));
 		d_surf_blk_.alloc(sizeof(int) * (nb + 1));
 		d_marks_.alloc(sizeof(int) * (size_t) n);
-		d_nshell_.alloc(sizeof(int) * 2 * (size_t) kNormalShellCap);
+		d_nshell_.alloc(surface_normal_workspace(n));
 		d_choice_.alloc(sizeof(ChoiceRec));
 		if (!d_sstate_.ptr() || !d_surf_.ptr() || !d_surf_blk_.ptr() || !d_marks_.ptr() || !d_nshell_.ptr() ||
 		    !d_choice_.ptr())
@@ -1232,7 +1232,7 @@ int Grid::surface_init(int vacuum_id)
 		// every entry of the list and the shell buffer holds a node index
 		// from the start, the marks no call's stamp
 		if (launch_fill_i32(stream_, n, d_marks_.as<int>(), 0) || launch_fill_i32(stream_, n + 1, d_surf_.as<int>(), 0) ||
-		    launch_fill_i32(stream_, 2 * kNormalShellCap, d_nshell_.as<int>(), 0))
+		    launch_fill_i32(stream_, (int) (surface_normal_workspace(n) / sizeof(int)), d_nshell_.as<int>(), 0))
 			return fail(-2, "fill launch failed");
 		normal_stamp_ = 0;
 	}
@@ -1379,13 +1379,18 @@ int Grid::launch_normal(int node)
 	HIPCHK(hipStreamWaitEvent(sstream_, norm_src_, 0));
 	NormalArgs a;
 	a.node = node;
+	a.n = mesh_.n;
 	a.ptr = d_ptr_.as<int>();
 	a.idx = d_idx_.as<int>();
 	a.xyz = d_xyz_.as<float>();
 	a.sstate = d_sstate_.as<unsigned char>();
 	a.marks = d_marks_.as<int>();
 	a.stamp = normal_stamp_;
-	a.shell = d_nshell_.as<int>();
+	char* ws = d_nshell_.as<char>();
+	a.ctl = (NormalCtl*) ws;
+	a.shell = (int*) (ws + ((sizeof(NormalCtl) + 63) / 64) * 64);
+	a.sorted = a.shell + kNormalShellCap;
+	a.blk = a.sorted + kNormalShellCap;
 	a.rec = d_choice_.as<ChoiceRec>();
 	if (launch_surface_normal(sstream_, a)) return fail(-2, "normal launch failed");
 	HIPCHK(hipEventRecord(norm_done_, sstream_));

@@ -398,20 +398,35 @@ struct ChooseArgs {
 int launch_surface_choose(hipStream_t, const ChooseArgs&, bool f64);
 
 // Surface_3d::Table::normal of one node on the device: the three-hop shell
-// (marks[] holds the stamp of the call that last reached a node), ascending
-// sort, moments and the shared plane solution (plane.hpp)
+// gathered level by level (marks[] holds the stamp of the call that last
+// reached a node), put in ascending order by a compaction of the marks over
+// the shell's node range, then the moments and the shared plane solution
+// (plane.hpp) in one workgroup; kernels on one stream, one call after another
 constexpr int kNormalShellCap = 1 << 20;
+struct NormalCtl {
+	int cnt;      // shell entries gathered
+	int over;     // the shell outgrew kNormalShellCap
+	int lo, hi;   // node range of the shell
+	int end[4];   // end of each level's entries (end[0] = 0)
+	int sorted;   // entries of the ascending list
+	int skip;     // the node is not a surface site: zero normal
+};
 struct NormalArgs {
-	int node;
+	int node, n;
 	const int* ptr;
 	const int* idx;
 	const float* xyz;
 	const unsigned char* sstate;
 	int* marks;
 	int stamp;             // != 0, new per call
-	int* shell;            // 2 * kNormalShellCap ints
+	NormalCtl* ctl;
+	int* shell;            // kNormalShellCap ints, gathered order
+	int* sorted;           // kNormalShellCap ints, ascending
+	int* blk;              // surface_compact_blocks(n) ints
 	ChoiceRec* rec;        // normal, normal_status
 };
+// bytes of the device workspace of launch_surface_normal for n nodes
+size_t surface_normal_workspace(int n);
 int launch_surface_normal(hipStream_t, const NormalArgs&);
 
 // sequential sum in slot order (the reference's float/double normaliser)

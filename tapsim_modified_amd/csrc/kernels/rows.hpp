@@ -167,36 +167,42 @@ __device__ __forceinline__ double long_row_dot_exact(const int* __restrict__ col
 
 // sum_k w_k * x[col_k] in FP32 as a chain of fused multiply-adds in entry
 // order; per-lane path, deg <= kLongRow
+// entries k0 .. k0+B-1 continuing the running sum s (loads of entries past
+// the end re-read the last one, so that the batch is issued before the
+// first wait)
 template <int B>
 __device__ __forceinline__ float short_row_fma_f32_b(const int* __restrict__ col, const float* __restrict__ w,
-                                                     const float* __restrict__ x, int deg, long long base)
+                                                     const float* __restrict__ x, int deg, long long base, int k0,
+                                                     float s)
 {
 	int j[B];
 	float c[B], v[B];
 	const int last = max(deg - 1, 0);
 #pragma unroll
-	for (int u = 0; u < B; u++) j[u] = col[base + 64ll * min(u, last)];
+	for (int u = 0; u < B; u++) j[u] = col[base + 64ll * min(k0 + u, last)];
 #pragma unroll
-	for (int u = 0; u < B; u++) c[u] = w[base + 64ll * min(u, last)];
+	for (int u = 0; u < B; u++) c[u] = w[base + 64ll * min(k0 + u, last)];
 #pragma unroll
 	for (int u = 0; u < B; u++) v[u] = x[j[u]];
-	float s = 0.0f;
 #pragma unroll
 	for (int u = 0; u < B; u++) {
 		const float t = fmaf(c[u], v[u], s);
-		s = (u < deg) ? t : s;
+		s = (k0 + u < deg) ? t : s;
 	}
 	return s;
 }
 
+// rows of more than 16 entries take a 16-entry batch and an 8-entry one
+// (one 24-entry batch holds registers that cost a third of the occupancy)
 __device__ __forceinline__ float short_row_fma_f32(const int* __restrict__ col, const float* __restrict__ w,
                                                    const float* __restrict__ x, bool active, int deg, long long base)
 {
 	const int wmax = wave_max_int(active ? deg : 0);
 	if (!active || wmax == 0) return 0.0f;
-	if (wmax <= 8) return short_row_fma_f32_b<8>(col, w, x, deg, base);
-	if (wmax <= 16) return short_row_fma_f32_b<16>(col, w, x, deg, base);
-	return short_row_fma_f32_b<kLongRow>(col, w, x, deg, base);
+	if (wmax <= 8) return short_row_fma_f32_b<8>(col, w, x, deg, base, 0, 0.0f);
+	const float s = short_row_fma_f32_b<16>(col, w, x, deg, base, 0, 0.0f);
+	if (wmax <= 16) return s;
+	return short_row_fma_f32_b<kLongRow - 16>(col, w, x, deg, base, 16, s);
 }
 
 // the long row's contiguous entries off .. off+deg-1: the FP32 sum of a

@@ -310,105 +310,134 @@ __global__ __launch_bounds__(1024) void k_choose(ChooseArgs a)
 
 // ---- the surface normal (Surface_3d::Table::normal, surface_3d.cpp:123-169)
 
-constexpr int kNormalSortCap = 8192;  // shells up to this size are sorted in LDS
-constexpr int kNormalChunk = 1024;    // shell entries staged per pass of the sums
+constexpr int kNormalChunk = 1024;  // shell entries staged per pass of the sums
+constexpr int kNormalExpandBlocks = 256;
 
-__device__ __forceinline__ int load_agent(const int* p)
+__global__ __launch_bounds__(64) void k_nrm_init(NormalArgs a)
 {
-	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	if (threadIdx.x != 0) return;
+	NormalCtl* c = a.ctl;
+	c->cnt = 0;
+	c->over = 0;
+	c->lo = a.n;
+	c->hi = -1;
+	for (int l = 0; l < 4; l++) c->end[l] = 0;
+	c->sorted = 0;
+	c->skip = (a.sstate[a.node] & kSurfSite) ? 0 : 1;
+	a.marks[a.node] = a.stamp;
 }
 
-// the three-hop shell of a.node (adjacentNodes(node, set, 3),
-// geometry_3d.cpp:1112-1145), in ascending order; the offsets node - site of
-// its surface sites give the scatter moments as float sums in that order,
-// its vacuum nodes the orientation; one workgroup
-__global__ __launch_bounds__(256) void k_normal(NormalArgs a)
+// one level of adjacentNodes(node, set, 3) (geometry_3d.cpp:1112-1145): the
+// neighbours of the previous level's entries (level 0: of the node) not
+// reached yet are appended; one wavefront per frontier entry, its lanes over
+// the neighbour list, one append per wavefront
+__global__ __launch_bounds__(256) void k_nrm_expand(NormalArgs a, int level)
 {
-	__shared__ int sorted[kNormalSortCap];
+	const NormalCtl* c = a.ctl;
+	if (c->skip || c->over) return;
+	const int lb = level == 0 ? 0 : c->end[level - 1];
+	const int le = level == 0 ? 1 : c->end[level];
+	const int lane = threadIdx.x & 63;
+	const int waves = gridDim.x * (blockDim.x >> 6);
+	for (int f = lb + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); f < le; f += waves) {
+		const int u = level == 0 ? a.node : a.shell[f];
+		const int k0 = a.ptr[u], k1 = a.ptr[u + 1];
+		for (int kb = k0; kb < k1; kb += 64) {
+			const int k = kb + lane;
+			int v = -1;
+			bool claimed = false;
+			if (k < k1) {
+				v = a.idx[k];
+				claimed = atomicExch(&a.marks[v], a.stamp) != a.stamp;
+			}
+			const unsigned long long m = __ballot(claimed);
+			if (m == 0) continue;
+			int base = 0;
+			if (lane == 0) base = atomicAdd(&a.ctl->cnt, __popcll(m));
+			base = __shfl(base, 0, 64);
+			const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+			int vmin = claimed ? v : 0x7fffffff, vmax = claimed ? v : -1;
+			for (int off = 32; off > 0; off >>= 1) {
+				vmin = min(vmin, __shfl_xor(vmin, off, 64));
+				vmax = max(vmax, __shfl_xor(vmax, off, 64));
+			}
+			if (claimed) {
+				if (pos < kNormalShellCap) a.shell[pos] = v;
+				else a.ctl->over = 1;
+			}
+			if (lane == 0) {
+				atomicMin(&a.ctl->lo, vmin);
+				atomicMax(&a.ctl->hi, vmax);
+			}
+		}
+	}
+}
+
+__global__ __launch_bounds__(64) void k_nrm_close(NormalArgs a, int level)
+{
+	if (threadIdx.x == 0) a.ctl->end[level + 1] = min(a.ctl->cnt, kNormalShellCap);
+}
+
+// the shell's nodes among the thread's 16: marked by this call, not the node
+__device__ __forceinline__ int shell_bits(const NormalArgs& a, int first, int lo, int hi)
+{
+	int bits = 0;
+	for (int t = 0; t < kCompactPerThread; t++) {
+		const int v = first + t;
+		if (v >= lo && v <= hi && a.marks[v] == a.stamp && v != a.node) bits |= 1 << t;
+	}
+	return bits;
+}
+
+__global__ __launch_bounds__(256) void k_nrm_count(NormalArgs a)
+{
+	const NormalCtl* c = a.ctl;
+	const int b0 = blockIdx.x * kCompactBlock;
+	int n = 0;
+	if (!c->skip && !c->over && b0 <= c->hi && b0 + kCompactBlock > c->lo)
+		n = __popc(shell_bits(a, b0 + threadIdx.x * kCompactPerThread, c->lo, c->hi));
+	int total;
+	block_exclusive_scan(n, &total);
+	if (threadIdx.x == 0) a.blk[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_nrm_write(NormalArgs a)
+{
+	const NormalCtl* c = a.ctl;
+	int pre = 0;
+	for (int b = threadIdx.x; b < (int) blockIdx.x; b += blockDim.x) pre += a.blk[b];
+	int ptotal;
+	block_exclusive_scan(pre, &ptotal);
+	const int b0 = blockIdx.x * kCompactBlock;
+	const int first = b0 + threadIdx.x * kCompactPerThread;
+	int bits = 0;
+	if (!c->skip && !c->over && b0 <= c->hi && b0 + kCompactBlock > c->lo) bits = shell_bits(a, first, c->lo, c->hi);
+	int total;
+	int pos = ptotal + block_exclusive_scan(__popc(bits), &total);
+	for (int t = 0; t < kCompactPerThread; t++)
+		if (bits & (1 << t)) a.sorted[pos++] = first + t;
+	if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) a.ctl->sorted = ptotal + total;
+}
+
+// the sums in ascending shell order: threads 0-5 the moments (xx, xy, xz,
+// yy, yz, zz) of the offsets x0 - x over the surface sites, threads 6-8 the
+// orientation x - x0 over the vacuum nodes; then the plane on one thread
+__global__ __launch_bounds__(256) void k_nrm_finish(NormalArgs a)
+{
 	__shared__ float cx[kNormalChunk], cy[kNormalChunk], cz[kNormalChunk];
 	__shared__ unsigned char cst[kNormalChunk];
-	__shared__ int s_cnt, s_over, s_lb, s_le;
 	__shared__ float s_acc[9];
+	const NormalCtl* c = a.ctl;
 	ChoiceRec* rec = a.rec;
 	const int tid = threadIdx.x;
-	if (!(a.sstate[a.node] & kSurfSite)) {
+	if (c->skip || c->over) {
 		if (tid == 0) {
-			for (int c = 0; c < 3; c++) rec->normal[c] = 0.0f;
-			rec->normal_status = 0;
+			for (int k = 0; k < 3; k++) rec->normal[k] = 0.0f;
+			rec->normal_status = c->over ? 2 : 0;
 		}
 		return;
 	}
-	if (tid == 0) {
-		s_cnt = 0;
-		s_over = 0;
-		atomicExch(&a.marks[a.node], a.stamp);
-	}
-	__syncthreads();
-	// level 1: the node's neighbours; levels 2 and 3: the neighbours of the
-	// previous level's entries
-	for (int lvl = 0; lvl < 3; lvl++) {
-		const int lb = lvl == 0 ? 0 : s_lb, le = lvl == 0 ? 1 : s_le;
-		__syncthreads();
-		if (tid == 0) s_lb = s_cnt;
-		for (int f = lb + tid / 16; f < le; f += blockDim.x / 16) {
-			const int u = lvl == 0 ? a.node : load_agent(a.shell + f);
-			for (int k = a.ptr[u] + (tid & 15); k < a.ptr[u + 1]; k += 16) {
-				const int v = a.idx[k];
-				if (atomicExch(&a.marks[v], a.stamp) != a.stamp) {
-					const int p = atomicAdd(&s_cnt, 1);
-					if (p < kNormalShellCap) a.shell[p] = v;
-					else s_over = 1;
-				}
-			}
-		}
-		// the level's entries visible to every wave before they are read
-		__threadfence();
-		__syncthreads();
-		if (tid == 0) s_le = s_cnt;
-		__syncthreads();
-		if (s_over) break;
-	}
-	const int cnt = s_cnt;
-	if (s_over) {
-		if (tid == 0) rec->normal_status = 2;
-		return;
-	}
-	// ascending order: bitonic sort in LDS, or a rank sort into the second
-	// half of the shell buffer
-	const bool in_lds = cnt <= kNormalSortCap;
-	int* gsorted = a.shell + kNormalShellCap;
-	if (in_lds) {
-		int P = 1;
-		while (P < cnt) P <<= 1;
-		for (int i = tid; i < P; i += blockDim.x) sorted[i] = i < cnt ? load_agent(a.shell + i) : 0x7fffffff;
-		__syncthreads();
-		for (int k = 2; k <= P; k <<= 1)
-			for (int j = k >> 1; j > 0; j >>= 1) {
-				for (int i = tid; i < P; i += blockDim.x) {
-					const int l = i ^ j;
-					if (l > i) {
-						const int x = sorted[i], y = sorted[l];
-						if (((i & k) == 0) == (x > y)) {
-							sorted[i] = y;
-							sorted[l] = x;
-						}
-					}
-				}
-				__syncthreads();
-			}
-	} else {
-		for (int i = tid; i < cnt; i += blockDim.x) {
-			const int x = load_agent(a.shell + i);
-			int r = 0;
-			for (int j = 0; j < cnt; j++) r += load_agent(a.shell + j) < x;
-			gsorted[r] = x;
-		}
-		__threadfence();
-		__syncthreads();
-	}
-	// sums in shell order: threads 0-5 the moments (xx, xy, xz, yy, yz, zz)
-	// of the offsets x0 - x over the surface sites, threads 6-8 the
-	// orientation x - x0 over the vacuum nodes
+	const int cnt = c->sorted;
 	const float x0[3] = {a.xyz[3 * (size_t) a.node], a.xyz[3 * (size_t) a.node + 1], a.xyz[3 * (size_t) a.node + 2]};
 	const int P0 = tid < 3 ? 0 : (tid < 5 ? 1 : 2);
 	const int Q0 = tid == 0 ? 0 : (tid == 1 || tid == 3 ? 1 : 2);
@@ -416,7 +445,7 @@ __global__ __launch_bounds__(256) void k_normal(NormalArgs a)
 	for (int c0 = 0; c0 < cnt; c0 += kNormalChunk) {
 		const int m = min(kNormalChunk, cnt - c0);
 		for (int i = tid; i < m; i += blockDim.x) {
-			const int v = in_lds ? sorted[c0 + i] : load_agent(gsorted + c0 + i);
+			const int v = a.sorted[c0 + i];
 			cx[i] = a.xyz[3 * (size_t) v];
 			cy[i] = a.xyz[3 * (size_t) v + 1];
 			cz[i] = a.xyz[3 * (size_t) v + 2];
@@ -431,11 +460,11 @@ __global__ __launch_bounds__(256) void k_normal(NormalArgs a)
 					acc += op * oq;
 				}
 		} else if (tid < 9) {
-			const int c = tid - 6;
+			const int k = tid - 6;
 			for (int i = 0; i < m; i++)
 				if (cst[i] & kSurfVacuum) {
 					const float x[3] = {cx[i], cy[i], cz[i]};
-					acc += x[c] - x0[c];
+					acc += x[k] - x0[k];
 				}
 		}
 		__syncthreads();
@@ -443,17 +472,17 @@ __global__ __launch_bounds__(256) void k_normal(NormalArgs a)
 	if (tid < 9) s_acc[tid] = acc;
 	__syncthreads();
 	if (tid == 0) {
-		float mom[6], n[3];
+		float mom[6], nv[3];
 		for (int t = 0; t < 6; t++) mom[t] = s_acc[t];
-		if (!plane::from_moments(mom, n)) {
+		if (!plane::from_moments(mom, nv)) {
 			rec->normal_status = 1;
 			return;
 		}
 		float dot = 0.0f;
-		for (int c = 0; c < 3; c++) dot += n[c] * s_acc[6 + c];
+		for (int k = 0; k < 3; k++) dot += nv[k] * s_acc[6 + k];
 		if (dot < 0.0f)
-			for (int c = 0; c < 3; c++) n[c] *= -1.0f;
-		for (int c = 0; c < 3; c++) rec->normal[c] = n[c];
+			for (int k = 0; k < 3; k++) nv[k] *= -1.0f;
+		for (int k = 0; k < 3; k++) rec->normal[k] = nv[k];
 		rec->normal_status = 0;
 	}
 }
@@ -513,9 +542,23 @@ int launch_surface_choose(hipStream_t st, const ChooseArgs& a, bool f64)
 	return (int) hipGetLastError();
 }
 
+size_t surface_normal_workspace(int n)
+{
+	return sizeof(NormalCtl) + 64 + 2 * sizeof(int) * (size_t) kNormalShellCap + sizeof(int) * (size_t) surface_compact_blocks(n);
+}
+
 int launch_surface_normal(hipStream_t st, const NormalArgs& a)
 {
-	hipLaunchKernelGGL(dev::k_normal, dim3(1), dim3(256), 0, st, a);
+	const int nb = surface_compact_blocks(a.n);
+	if (nb <= 0) return -1;
+	hipLaunchKernelGGL(dev::k_nrm_init, dim3(1), dim3(64), 0, st, a);
+	for (int l = 0; l < 3; l++) {
+		hipLaunchKernelGGL(dev::k_nrm_expand, dim3(dev::kNormalExpandBlocks), dim3(256), 0, st, a, l);
+		hipLaunchKernelGGL(dev::k_nrm_close, dim3(1), dim3(64), 0, st, a, l);
+	}
+	hipLaunchKernelGGL(dev::k_nrm_count, dim3(nb), dim3(256), 0, st, a);
+	hipLaunchKernelGGL(dev::k_nrm_write, dim3(nb), dim3(256), 0, st, a);
+	hipLaunchKernelGGL(dev::k_nrm_finish, dim3(1), dim3(256), 0, st, a);
 	return (int) hipGetLastError();
 }
 
