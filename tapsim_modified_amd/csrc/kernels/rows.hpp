@@ -135,13 +135,26 @@ __device__ __forceinline__ double short_row_dot_exact(const int* __restrict__ co
 }
 
 // the same sum for one long row taken by the whole wavefront from its
-// contiguous entries off .. off+deg-1 (deg and off wave-uniform); every lane
-// returns the sum
+// contiguous entries off .. off+deg-1 (deg and off wave-uniform; blocks of
+// 256 threads); every lane returns the sum.  The lanes load and multiply
+// 64 W entries at a time and stage the products in LDS; lane 0 then adds
+// them in entry order, 16 per round with the loads of a round issued
+// first.  A round past the row's end adds -0.0, the exact identity of the
+// IEEE addition (s + -0.0 == s for every s, signed zeros included), so the
+// sum is the sequential one bit for bit while the loop count stays a
+// multiple of 16.  (A sum through v_readlane costs about three times as
+// many cycles per entry, and the longest row -- a few thousand entries --
+// bounds the whole sweep.)
 template <typename R>
 __device__ __forceinline__ double long_row_dot_exact(const int* __restrict__ col, const R* __restrict__ coup,
                                                      const R* __restrict__ p, int deg, long long off)
 {
 	constexpr int W = kLongWindows;
+	constexpr int U = 16;
+	static_assert((64 * W) % U == 0, "the staged window holds whole rounds");
+	__shared__ double s_prod[4][64 * W];
+	double* sp = s_prod[(threadIdx.x >> 6) & 3];
+	deg = __builtin_amdgcn_readfirstlane(deg);
 	double acc = 0.0;
 	for (int k0 = 0; k0 < deg; k0 += 64 * W) {
 		int j[W];
@@ -158,11 +171,27 @@ __device__ __forceinline__ double long_row_dot_exact(const int* __restrict__ col
 		for (int w = 0; w < W; w++) {
 			double prod = (double) v[w];
 			prod *= (double) c[w];
-			const int cnt = min(64, deg - (k0 + 64 * w));
-			for (int u = 0; u < cnt; u++) acc += readlane_f64(prod, u);
+			sp[64 * w + lane_id()] = (k0 + 64 * w + lane_id() < deg) ? prod : -0.0;
 		}
+		// the window's products are visible to lane 0 (LDS operations of one
+		// wavefront complete in order; the fence keeps the compiler's order)
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		if (lane_id() == 0) {
+			const int rounds = (min(64 * W, deg - k0) + U - 1) / U;
+			for (int r = 0; r < rounds; r++) {
+				double t[U];
+#pragma unroll
+				for (int u = 0; u < U; u++) t[u] = sp[r * U + u];
+#pragma unroll
+				for (int u = 0; u < U; u++) acc += t[u];
+			}
+		}
+		// the next window overwrites the staging only after lane 0 read it
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		__builtin_amdgcn_wave_barrier();
 	}
-	return acc;
+	return readlane_f64(acc, 0);
 }
 
 // sum_k w_k * x[col_k] in FP32 as a chain of fused multiply-adds in entry
