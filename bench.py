@@ -34,6 +34,8 @@ WORKLOADS = {
     "c2": dict(radius=15e-9, shank=40e-9, box=(150e-9, 150e-9, 300e-9), vacuum=1200000, face=15e-9),
     # c1 geometry, two-phase: solute (25 V/nm) in 2 nm clusters, 1e-3 centres per nm^3 (seeded)
     "c3": dict(radius=5e-9, shank=10e-9, box=(60e-9, 60e-9, 120e-9), vacuum=150000, face=6e-9, clusters=1e24),
+    # full pipeline: r 25 nm, shank 60 nm, ~13 M atoms + ~4 M vacuum, two-phase as c3
+    "c5": dict(radius=25e-9, shank=60e-9, box=(250e-9, 250e-9, 450e-9), vacuum=4000000, face=25e-9, clusters=1e24),
     # small case for quick checks
     "s": dict(radius=2e-9, shank=3e-9, box=(20e-9, 20e-9, 40e-9), vacuum=15000, face=4e-9),
 }
@@ -46,6 +48,19 @@ SPECIES = {
     10: dict(removable=1, dirichlet=1, phi=5000.0, mass=55.845, evap_charge=2, evap_field=3.0e10, evap_energy=1.0),
     11: dict(removable=1, dirichlet=1, phi=5000.0, mass=58.69, evap_charge=2, evap_field=2.5e10, evap_energy=1.0),
 }
+
+
+# the full-pipeline workload (c5): ions drawn per event from these tables --
+# isotopes of the matrix (Fe-like) and solute (Ni-like) species: mass (amu),
+# charge (e), natural abundance, with a share of singly charged matrix ions
+# -- flown by the fourth-order Runge-Kutta integrator and extrapolated to a
+# planar detector 10 cm above the mesh
+ION_TABLES = {
+    10: ([53.9396, 55.9349, 56.9354, 57.9333, 55.9349], [2, 2, 2, 2, 1], [0.05845, 0.85, 0.02119, 0.00282, 0.06754]),
+    11: ([57.9353, 59.9308, 60.9311, 61.9283, 63.9280], [2, 2, 2, 2, 2], [0.6808, 0.2622, 0.0114, 0.0363, 0.0093]),
+}
+ION_SEED = 20261204
+DETECTOR_DISTANCE = 0.1
 
 
 def log(msg):
@@ -63,7 +78,10 @@ def build_specimen(tsg, w, seed=20261203):
     return tsg.generate_specimen(p)
 
 
-def event_options(tsg):
+def event_options(tsg, full_pipeline=False, zmax=0.0):
+    """the events' options (steps I-VI); the full pipeline flies every ion with
+    the fourth-order Runge-Kutta integrator from its species' ion table to a
+    detector plane DETECTOR_DISTANCE above the mesh's top (zmax)"""
     e = tsg.EvapOptions()
     e.prob_mode, e.evap_mode, e.vacuum_id = 0, 0, 0
     e.relax_threshold, e.relax_shell, e.relax_cycle, e.relax_queue = float(np.float32(1e-2)), 3, 5, 5
@@ -71,6 +89,10 @@ def event_options(tsg):
     e.refresh_interval, e.refresh_threshold, e.refresh_cycle, e.refresh_queue = 1, 1e-8, 1, 0
     e.exact_probability = 1
     e.trajectory, e.integrator, e.stepper = 1, 1, 1
+    if full_pipeline:
+        e.integrator = 2
+        e.detector_z = float(np.float32(zmax + DETECTOR_DISTANCE))
+        e.ion_table, e.ion_seed = 1, ION_SEED
     return e
 
 
@@ -242,8 +264,14 @@ def main():
     log("mesh: %d tets (%.1fs)" % (mesh.m, time.perf_counter() - t0))
     species = tsg.species_table(SPECIES)
 
+    full = args.workload == "c5"
+    zmax = float(np.max(xyz[:, 2]))
+
     def new_grid(phi=None):
         g = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
+        if full:
+            for sid, (m, q, wt) in ION_TABLES.items():
+                g.set_ion_table(sid, m, q, wt)
         if phi is not None:
             g.set_phi(*phi)
         return g
@@ -268,13 +296,13 @@ def main():
     # timed events.
     W, K = args.train_events, args.steps
     Kp = args.plain_steps if args.plain_steps is not None else K
-    mixed_opts = event_options(tsg)
+    mixed_opts = event_options(tsg, full, zmax)
     mixed_opts.method, mixed_opts.accel_tol = 1, 1e-8
-    learned_opts = event_options(tsg)
+    learned_opts = event_options(tsg, full, zmax)
     learned_opts.method, learned_opts.accel_tol = 2, 1e-8
     mixed_opts.accel_solver = learned_opts.accel_solver = {"chebyshev": 0, "multigrid": 1, "multigrid-eager": 2}[
         args.solver]
-    plain_opts = event_options(tsg)
+    plain_opts = event_options(tsg, full, zmax)
 
     if args.counters:
         counters_run(tsg, plain)
@@ -372,6 +400,18 @@ def main():
         "roofline": roof_acc,
         "roofline_plain_fp64": roof_plain,
     }
+    if full:
+        ions = {}
+        for e in aev:
+            key = "%d:%.4f/%d+" % (int(ids[e.node]), e.ion_mass, e.ion_charge)
+            ions[key] = ions.get(key, 0) + 1
+        res["full_pipeline"] = {
+            "integrator": "classical RK4 with the embedded third-order estimate, error-restricted steps",
+            "detector_z_m": learned_opts.detector_z, "ion_seed": ION_SEED, "ions_timed_events": ions,
+            "traj_status": sorted(set(int(e.traj_status) for e in aev)),
+            "stop_z_m": [float(e.stop[2]) for e in aev[:3]],
+            "tof_s": {"min": float(min(e.tof for e in aev)), "max": float(max(e.tof for e in aev))},
+            "solute_events": sum(1 for e in aev if int(ids[e.node]) == 11)}
     if not args.no_cpu:
         tv, tn = mesh.tets()
         residual, res["cpu_baseline"] = cpu_baseline(tsg, coup0, xyz, ids, tv, tn, state0, plain_opts, pev[0],

@@ -231,6 +231,13 @@ typedef struct tsg_evap_options
 	                             starts after; the grid keeps counting across tsg_evaporate calls, so events
 	                             evaporated in chunks get the indices (and refresh relaxations,
 	                             process.cpp:876) of one uninterrupted run */
+	/* additions without a reference counterpart */
+	float detector_z;         /* > 0: each flight is extrapolated to the plane z = detector_z (m) instead of the
+	                             mesh's top (process.cpp:960 extrapolates to Geometry_3d::Table::max().z()) */
+	int ion_table;            /* 1: each ion's mass and charge are drawn from its species' ion table
+	                             (tsg_set_ion_table); 0 or an empty table: the species' mass and charge */
+	unsigned int ion_seed;    /* seed of the ion draws: one counter-based draw (splitmix64 of seed and event
+	                             index) per event, independent of the process's rand() stream */
 } tsg_evap_options;
 
 typedef struct tsg_event
@@ -255,6 +262,9 @@ typedef struct tsg_event
 	float tof;                /* time of the last trajectory point times time_scale */
 	int traj_len, traj_status;
 	long long event_index;    /* LogResults::Dataset::eventIndex (process.cpp:696): initEventCnt + events so far */
+	float ion_mass;           /* the ion's mass (amu) and charge (e) in its flight: the species' own, or the
+	                             entry of its ion table drawn for this event */
+	int ion_charge;
 } tsg_event;
 
 /* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k
@@ -312,7 +322,9 @@ int tsg_copy_probe(long long bytes, int reps, double* avg_ms);
  * geometry_3d.cpp:988-1044) for `count` ions, each starting at rest on
  * nodes[i] with charge[i] (C) and mass[i] (kg), in the field of the current
  * potentials.  integrator: 0 first order, 1 Cash-Karp (the reference's
- * default); stepper: 0 none, 1 error-restricted (default), 2
+ * default), 2 the classical fourth-order Runge-Kutta step with the embedded
+ * third-order error estimate (k4 - k5) / 6, k5 the slope at the new point
+ * (an addition, no reference counterpart); stepper: 0 none, 1 error-restricted (default), 2
  * geometry-restricted, 3 adaptive.  out receives, per ion, the first cap
  * phases (t, x, y, z, vx, vy, vz, tet) of its trajectory, the last one the
  * extrapolation to the mesh's largest z; len[i] the number of phases and
@@ -320,6 +332,14 @@ int tsg_copy_probe(long long bytes, int reps, double* avg_ms);
  * mesh, 1 = no field), negative on failure (-4: work limit). */
 int tsg_trajectories(tsg_grid*, int count, const int* nodes, const float* charge, const float* mass, int integrator,
                      int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status);
+
+/* The ion table of a species for the flights of tsg_evaporate (no reference
+ * counterpart; the reference flies every ion of a species with the species'
+ * mass and charge, process.cpp:930-935): up to 8 entries of mass (amu),
+ * charge (e) and weight >= 0; with tsg_evap_options.ion_table = 1 each event
+ * draws one entry with probability weight / sum of weights.  count 0 clears
+ * the table. */
+int tsg_set_ion_table(tsg_grid*, int species, int count, const float* mass_amu, const int* charge, const float* weight);
 
 /* Fieldline_3d::init(start) + compute(distance) (tapsim/fieldline_3d.cpp:55-190)
  * for `count` start points (3 floats each) in the field of the current

@@ -35,6 +35,7 @@ class OraEvapOpts(ctypes.Structure):
         ("stepper", ctypes.c_int), ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_uint), ("shrink_limit", ctypes.c_float), ("inplace_order", ctypes.c_int),
         ("thermal_velocity", ctypes.c_int), ("update_coordination", ctypes.c_int), ("init_event_cnt", ctypes.c_int),
+        ("detector_z", ctypes.c_float), ("ion_table", ctypes.c_int), ("ion_seed", ctypes.c_uint),
     ]
 
 
@@ -49,6 +50,7 @@ class OraEvent(ctypes.Structure):
         ("local_iter", ctypes.c_longlong), ("refresh_iter", ctypes.c_longlong),
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_float), ("apex", ctypes.c_float * 3),
         ("normal", ctypes.c_float * 3), ("traj_root_diff", ctypes.c_int), ("event_index", ctypes.c_longlong),
+        ("ion_mass", ctypes.c_float), ("ion_charge", ctypes.c_int),
     ]
 
 
@@ -114,6 +116,10 @@ _SIGS = {
     "ora_set_threads": ([_P, ctypes.c_int], None),
     "ora_mpnn": ([_P, _P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "ora_bf16": ([ctypes.c_float], ctypes.c_float),
+    "ora_set_ion_table": ([_P, ctypes.c_int, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "ora_ion_pick": ([ctypes.c_uint, ctypes.c_longlong, ctypes.c_int, _P], ctypes.c_int),
+    "ora_rk4_analytic": ([ctypes.c_int, _P, ctypes.c_float, _P, _P, ctypes.c_float, ctypes.c_int, _P, _P],
+                         ctypes.c_int),
 }
 
 _lib = None
@@ -387,6 +393,13 @@ class Oracle:
                                    ctypes.byref(ln))
         return st, out[:min(ln.value, cap)].copy()
 
+    def set_ion_table(self, species, mass, charge, weight):
+        """the species' ion table (mass amu, charge e, weight per entry; at most 8)"""
+        m = np.ascontiguousarray(mass, np.float32)
+        q = np.ascontiguousarray(charge, np.int32)
+        w = np.ascontiguousarray(weight, np.float32)
+        self._chk(load().ora_set_ion_table(self._h, species, len(m), _ptr(m), _ptr(q), _ptr(w)))
+
     def evaporate(self, opts, k):
         out = (OraEvent * max(1, k))()
         done = ctypes.c_int()
@@ -451,6 +464,23 @@ def best_plane(offsets):
     if rc:
         raise RuntimeError(load().ora_last_error().decode())
     return out
+
+
+def ion_pick(seed, event_index, weight):
+    """the index of the ion drawn for an event from a table with these weights"""
+    w = np.ascontiguousarray(weight, np.float32)
+    return load().ora_ion_pick(seed, event_index, len(w), _ptr(w))
+
+
+def rk4_analytic(kind, g, qm, p0, v0, delta, steps):
+    """the fourth-order Runge-Kutta step of the flights in an analytic field
+    (kind 0: E = g; kind 1: E = -(g0 x, g1 y, g2 z)): (t, p, v) after `steps`
+    steps of `delta` and the last step's error estimate (p, v)"""
+    f = lambda v: np.ascontiguousarray(v, np.float32)  # noqa: E731
+    out = np.zeros(7, np.float32)
+    err = np.zeros(6, np.float32)
+    load().ora_rk4_analytic(kind, _ptr(f(g)), qm, _ptr(f(p0)), _ptr(f(v0)), delta, steps, _ptr(out), _ptr(err))
+    return out[0], out[1:4].copy(), out[4:7].copy(), err[:3].copy(), err[3:].copy()
 
 
 def set_root_mode(mode):

@@ -77,6 +77,11 @@ struct ora_sys
 	float vq_sum;
 	float apex0;
 	int apex0_set;
+	/* per species: the ion table of the flights (no reference counterpart) */
+	int ion_n[128];
+	float ion_mass[128][8];
+	int ion_q[128][8];
+	float ion_w[128][8];
 };
 
 /* Configuration::Table::operator[] (configuration.cpp:213-231) throws for an
@@ -1175,8 +1180,121 @@ static phase o1_step(const traj* tr, float delta, phase* error)
 	return next;
 }
 
+/* The classical fourth-order Runge-Kutta step (integrator 2; not in the
+   reference, which offers the Euler and Cash-Karp integrators): slopes at 0,
+   h/2, h/2 and h from the start, from the start plus half of the first and
+   of the second slope, and from the start plus the third; the solution adds
+   (k1 + 2 k2 + 2 k3 + k4) / 6.  The error estimate is the difference to the
+   embedded third-order solution y + (k1 + 2 k2 + 2 k3 + k5) / 6, k5 the
+   slope at the new point: (k4 - k5) / 6.  The field comes from a callback so
+   that the pinning tests can drive the same step with an analytic field. */
+typedef void (*field_fn)(const void* ctx, const float pos[3], int guess, float out[3]);
+
+static phase rk4_step(field_fn field, const void* ctx, const phase* back, float qm, float delta, phase* error)
+{
+	static const float a[4] = { 0.0f, 0.5f, 0.5f, 1.0f };
+	static const float c[4] = { 1.0f / 6.0f, 1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 6.0f };
+	float dv[5][3], dp[5][3];
+	for (int i = 0; i < 4; i++)
+	{
+		float vel[3], pos[3];
+		for (int l = 0; l < 3; l++) { vel[l] = back->v[l]; pos[l] = back->p[l]; }
+		if (i > 0)
+			for (int l = 0; l < 3; l++)
+			{
+				vel[l] += a[i] * dv[i - 1][l];
+				pos[l] += a[i] * dp[i - 1][l];
+			}
+		field(ctx, pos, back->tet, dv[i]);
+		for (int l = 0; l < 3; l++) dv[i][l] *= qm;
+		for (int l = 0; l < 3; l++) dv[i][l] *= delta;
+		for (int l = 0; l < 3; l++) dp[i][l] = vel[l] * delta;
+	}
+	phase next = *back;
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 4; j++)
+		{
+			next.p[i] += c[j] * dp[j][i];
+			next.v[i] += c[j] * dv[j][i];
+		}
+	next.t += delta;
+	field(ctx, next.p, back->tet, dv[4]);
+	for (int l = 0; l < 3; l++) dv[4][l] *= qm;
+	for (int l = 0; l < 3; l++) dv[4][l] *= delta;
+	for (int l = 0; l < 3; l++) dp[4][l] = next.v[l] * delta;
+	if (error)
+	{
+		error->t = 0.0f;
+		for (int i = 0; i < 3; i++)
+		{
+			float e = dp[3][i];
+			e -= dp[4][i];
+			e /= 6.0f;
+			error->p[i] = e;
+			e = dv[3][i];
+			e -= dv[4][i];
+			e /= 6.0f;
+			error->v[i] = e;
+		}
+	}
+	return next;
+}
+
+static void mesh_field(const void* ctx, const float pos[3], int guess, float out[3])
+{
+	fast_field((const ora_sys*) ctx, pos, guess, out);
+}
+
+static phase o4_step(const traj* tr, float delta, phase* error)
+{
+	const phase* back = &tr->data[tr->len - 1];
+	const float qm = tr->charge / tr->mass;
+	phase next = rk4_step(mesh_field, tr->s, back, qm, delta, error);
+	next.tet = ora_find_tetrahedron(tr->s, next.p, back->tet);
+	return next;
+}
+
+/* analytic fields of the integrator's pinning tests: kind 0 uniform,
+   E = (g0, g1, g2); kind 1 linear restoring, E = -(g0 x, g1 y, g2 z) */
+typedef struct { int kind; float g[3]; } analytic_field;
+
+static void analytic_eval(const void* ctx, const float pos[3], int guess, float out[3])
+{
+	(void) guess;
+	const analytic_field* f = (const analytic_field*) ctx;
+	for (int l = 0; l < 3; l++)
+	{
+		if (f->kind == 0) out[l] = f->g[l];
+		else
+		{
+			float v = f->g[l];
+			v *= pos[l];
+			out[l] = -v;
+		}
+	}
+}
+
+int ora_rk4_analytic(int kind, const float g[3], float qm, const float p0[3], const float v0[3], float delta,
+                     int steps, float* out, float* err)
+{
+	analytic_field f;
+	f.kind = kind;
+	for (int l = 0; l < 3; l++) f.g[l] = g[l];
+	phase y;
+	memset(&y, 0, sizeof(y));
+	for (int l = 0; l < 3; l++) { y.p[l] = p0[l]; y.v[l] = v0[l]; }
+	phase e;
+	memset(&e, 0, sizeof(e));
+	for (int s = 0; s < steps; s++) y = rk4_step(analytic_eval, &f, &y, qm, delta, &e);
+	out[0] = y.t;
+	for (int l = 0; l < 3; l++) { out[1 + l] = y.p[l]; out[4 + l] = y.v[l]; }
+	for (int l = 0; l < 3; l++) { err[l] = e.p[l]; err[3 + l] = e.v[l]; }
+	return ORA_OK;
+}
+
 static phase do_integration(const traj* tr, float delta, phase* error)
 {
+	if (tr->integrator == 2) return o4_step(tr, delta, error);
 	return tr->integrator == 0 ? o1_step(tr, delta, error) : o5_step(tr, delta, error);
 }
 
@@ -1394,6 +1512,52 @@ int ora_trajectory(const ora_sys* s, int node, float charge, float mass, int int
 	*len = tr.len;
 	free(tr.data);
 	return status;
+}
+
+/* ------------------------------------------------------------ ion table */
+
+/* splitmix64 (Steele, Lea and Flood 2014): the mixing function of one
+   counter-based draw */
+static unsigned long long splitmix64(unsigned long long x)
+{
+	unsigned long long z = x + 0x9e3779b97f4a7c15ull;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+/* u in [0, 1) from (seed, event index) with 53 random bits; the first entry
+   whose cumulative weight (double, in table order) exceeds u times the
+   total */
+int ora_ion_pick(unsigned int seed, long long event_index, int count, const float* weight)
+{
+	const unsigned long long key = ((unsigned long long) seed << 32) ^ (unsigned long long) event_index;
+	const double u = (double) (splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+	double total = 0.0;
+	for (int k = 0; k < count; k++) total += (double) weight[k];
+	const double target = u * total;
+	double cum = 0.0;
+	for (int k = 0; k < count; k++)
+	{
+		cum += (double) weight[k];
+		if (target < cum) return k;
+	}
+	return count - 1;
+}
+
+int ora_set_ion_table(ora_sys* s, int species, int count, const float* mass, const int* charge, const float* weight)
+{
+	if (species < 0 || species >= 128 || count < 0 || count > 8) return fail(ORA_E_ARG, "ion table: species or count");
+	for (int k = 0; k < count; k++)
+		if (!(weight[k] >= 0.0f) || !(mass[k] > 0.0f)) return fail(ORA_E_ARG, "ion table: mass or weight");
+	s->ion_n[species] = count;
+	for (int k = 0; k < count; k++)
+	{
+		s->ion_mass[species][k] = mass[k];
+		s->ion_q[species][k] = charge[k];
+		s->ion_w[species][k] = weight[k];
+	}
+	return ORA_OK;
 }
 
 /* ------------------------------------------------------------ event loop */
@@ -1696,7 +1860,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 	e = ora_compute_probabilities(s, o->prob_mode);
 	if (e) return e;
 	float* vq = s->vq;
-	const float zmax = zmax_of(s);
+	const float zmax = o->detector_z > 0.0f ? o->detector_z : zmax_of(s);
 	if (!s->apex0_set)
 	{
 		float apex0 = -FLT_MAX;
@@ -1773,9 +1937,19 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 		{
 			if (sp_bad(s, nodeId)) return fail(ORA_E_CONFIG, "Configuration::Table::operator[]()");
 			const ora_species* sp = &s->species[nodeId];
-			float charge = (float) sp->evap_charge;
+			float ion_m = sp->mass;
+			int ion_q = sp->evap_charge;
+			if (o->ion_table && s->ion_n[nodeId] > 0)
+			{
+				const int k = ora_ion_pick(o->ion_seed, r->event_index, s->ion_n[nodeId], s->ion_w[nodeId]);
+				ion_m = s->ion_mass[nodeId][k];
+				ion_q = s->ion_q[nodeId][k];
+			}
+			r->ion_mass = ion_m;
+			r->ion_charge = ion_q;
+			float charge = (float) ion_q;
 			charge *= 1.60217733e-19f;
-			float mass = sp->mass;
+			float mass = ion_m;
 			mass *= 1.660538782e-27f;
 			tr.s = s; tr.charge = charge; tr.mass = mass;
 			tr.integrator = o->integrator; tr.stepper = o->stepper; tr.dt_min = o->dt_min;

@@ -59,6 +59,11 @@ typedef struct
 	int thermal_velocity;  /* 1: evap_initialVelocity from rand(), 0: at rest */
 	int update_coordination; /* 1: the fork's update_coordination at the run's start and after each take-away */
 	int init_event_cnt;    /* EvaporationOptions::initEventCnt (process.h:98) */
+	/* additions without a reference counterpart (integrator 2 is the classical
+	   fourth-order Runge-Kutta step) */
+	float detector_z;      /* > 0: extrapolate the flight to this plane (m) instead of the mesh's top */
+	int ion_table;         /* 1: each ion's mass and charge drawn from its species' ion table */
+	unsigned int ion_seed; /* seed of those draws (one per event, keyed by the event index) */
 } ora_evap_opts;
 
 typedef struct
@@ -78,6 +83,8 @@ typedef struct
 	float normal[3];
 	int traj_root_diff;    /* flight steps where powf(x, 0.25f) is not the correctly rounded root */
 	long long event_index; /* LogResults::Dataset::eventIndex (process.cpp:696) */
+	float ion_mass;        /* the ion's mass (amu) and charge (e) in its flight */
+	int ion_charge;
 } ora_event;
 
 typedef struct ora_sys ora_sys;
@@ -164,6 +171,17 @@ int ora_trajectory(const ora_sys*, int node, float charge, float mass, int integ
                    float* out, int cap, int* len);
 
 int ora_evaporate(ora_sys*, const ora_evap_opts*, int nevents, ora_event* out, int* done);
+
+/* the ion table of a species (no reference counterpart): up to 8 (mass amu,
+   charge e, weight) entries; count 0 clears it */
+int ora_set_ion_table(ora_sys*, int species, int count, const float* mass, const int* charge, const float* weight);
+/* the ion drawn for an event: index into the species' table */
+int ora_ion_pick(unsigned int seed, long long event_index, int count, const float* weight);
+/* the fourth-order Runge-Kutta step in an analytic field (its pinning):
+   kind 0 uniform E = g, kind 1 E = -(g0 x, g1 y, g2 z); steps of delta from
+   (p0, v0); out = t, p[3], v[3]; err = the last step's estimate (p[3], v[3]) */
+int ora_rk4_analytic(int kind, const float g[3], float qm, const float p0[3], const float v0[3], float delta,
+                     int steps, float* out, float* err);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,7 @@ class EvapOptions(ctypes.Structure):
         ("dt0", ctypes.c_float), ("dt_min", ctypes.c_float), ("err_limit", ctypes.c_float),
         ("voltage_queue", ctypes.c_int), ("shrink_limit", ctypes.c_float), ("update_coordination", ctypes.c_int),
         ("thermal_velocity", ctypes.c_int), ("accel_solver", ctypes.c_int), ("init_event_cnt", ctypes.c_int),
+        ("detector_z", ctypes.c_float), ("ion_table", ctypes.c_int), ("ion_seed", ctypes.c_uint),
     ]
 
     def __init__(self, **kw):
@@ -103,6 +104,7 @@ class Event(ctypes.Structure):
         ("normal", ctypes.c_float * 3),
         ("voltage", ctypes.c_float), ("time_scale", ctypes.c_float), ("tof", ctypes.c_float),
         ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int), ("event_index", ctypes.c_longlong),
+        ("ion_mass", ctypes.c_float), ("ion_charge", ctypes.c_int),
     ]
 
 
@@ -132,6 +134,7 @@ _SIGS = {
     "tsg_potential": ([_P, ctypes.c_int, _P], ctypes.c_int),
     "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_field_o2": ([_P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "tsg_set_ion_table": ([_P, ctypes.c_int, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "tsg_field_o1_noalloc": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_get_couplings": ([_P, _P], ctypes.c_int),
     "tsg_get_charges": ([_P, _P], ctypes.c_int),
@@ -562,6 +565,14 @@ class Grid3D:
         record training graphs, method 2 uses the prediction."""
         self._net = net
         _chk(load().tsg_grid_attach_mpnn(self._h, net.handle if net is not None else None, int(record), int(predict)))
+
+    def set_ion_table(self, species, mass, charge, weight):
+        """the species' ion table for the flights (mass amu, charge e, weight
+        per entry; at most 8; empty clears it), used with EvapOptions.ion_table"""
+        m = np.ascontiguousarray(mass, np.float32)
+        q = np.ascontiguousarray(charge, np.int32)
+        w = np.ascontiguousarray(weight, np.float32)
+        _chk(load().tsg_set_ion_table(self._h, species, len(m), _ptr(m), _ptr(q), _ptr(w)))
 
     def evaporate(self, options, k):
         out = (Event * max(1, k))()

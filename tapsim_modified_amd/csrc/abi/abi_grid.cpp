@@ -269,7 +269,17 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.update_coordination = o->update_coordination;
 	e.accel_solver = o->accel_solver;
 	e.init_event_cnt = o->init_event_cnt;
+	e.detector_z = o->detector_z;
+	e.ion_table = o->ion_table;
+	e.ion_seed = o->ion_seed;
 	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
+}
+
+int tsg_set_ion_table(tsg_grid* h, int species, int count, const float* mass_amu, const int* charge,
+                      const float* weight)
+{
+	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
+	return check(h, h->g->set_ion_table(species, count, mass_amu, charge, weight));
 }
 
 int tsg_trajectories(tsg_grid* h, int count, const int* nodes, const float* charge, const float* mass, int integrator,

@@ -1619,6 +1619,47 @@ static bool has_vacuum_face(const Mesh& m, const std::vector<int16_t>& ids, cons
 // for lower indices) is >= 10; the id is a signed char (configuration.h:62)
 // and wraps.  Only the ids change, as in the fork (no flags or couplings are
 // recomputed); the per-node species data of the device follow them.
+// splitmix64 (Steele, Lea and Flood 2014), the mixing function of one
+// counter-based draw
+static unsigned long long splitmix64(unsigned long long x)
+{
+	unsigned long long z = x + 0x9e3779b97f4a7c15ull;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+int ion_pick(unsigned int seed, long long event_index, const IonTable& t)
+{
+	const unsigned long long key = ((unsigned long long) seed << 32) ^ (unsigned long long) event_index;
+	const double u = (double) (splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+	double total = 0.0;
+	for (int k = 0; k < t.count; k++) total += (double) t.weight[k];
+	const double target = u * total;
+	double cum = 0.0;
+	for (int k = 0; k < t.count; k++) {
+		cum += (double) t.weight[k];
+		if (target < cum) return k;
+	}
+	return t.count - 1;
+}
+
+int Grid::set_ion_table(int species, int count, const float* mass, const int* charge, const float* weight)
+{
+	if (species < 0 || species >= 128 || count < 0 || count > kIonTableMax || (count > 0 && (!mass || !charge || !weight)))
+		return fail(-1, "ion table: species, count or arrays");
+	for (int k = 0; k < count; k++)
+		if (!(mass[k] > 0.0f) || !(weight[k] >= 0.0f)) return fail(-1, "ion table: mass or weight");
+	IonTable& t = ions_[species];
+	t.count = count;
+	for (int k = 0; k < count; k++) {
+		t.mass[k] = mass[k];
+		t.charge[k] = charge[k];
+		t.weight[k] = weight[k];
+	}
+	return 0;
+}
+
 int Grid::update_coordination()
 {
 	const int n = mesh_.n;
@@ -1746,9 +1787,22 @@ int Grid::flight_start(const EvapOptions& o, int ev, int node, const Species& sp
 		HIPCHK(hipStreamSynchronize(fstream_));
 		rng_on_device_ = true;
 	}
-	float charge = (float) sp.evap_charge;
+	// the ion: the species' own mass and charge (the reference), or an entry
+	// of its ion table drawn for this event
+	float ion_m = sp.mass;
+	int ion_q = sp.evap_charge;
+	const EventRecord& rec = flight_out_[ev];
+	const IonTable& it = ions_[rec.id];
+	if (o.ion_table && it.count > 0) {
+		const int k = ion_pick(o.ion_seed, rec.event_index, it);
+		ion_m = it.mass[k];
+		ion_q = it.charge[k];
+	}
+	flight_out_[ev].ion_mass = ion_m;
+	flight_out_[ev].ion_charge = ion_q;
+	float charge = (float) ion_q;
 	charge *= 1.60217733e-19f;
-	float mass = sp.mass;
+	float mass = ion_m;
 	mass *= 1.660538782e-27f;
 	int* hi = (int*) f.host;
 	hi[0] = node;
@@ -1770,7 +1824,7 @@ int Grid::flight_start(const EvapOptions& o, int ev, int node, const Species& sp
 	t.dt0 = o.dt0;
 	t.dt_min = o.dt_min;
 	t.err_limit = o.err_limit;
-	t.zmax = zmax_;
+	t.zmax = o.detector_z > 0.0f ? o.detector_z : zmax_;
 	t.max_steps = 1 << 24;
 	t.max_calls = 1ll << 22;
 	t.n = n;
@@ -2685,7 +2739,7 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	if (count <= 0) return 0;
 	if (cap < 0 || !nodes || !charge || !mass || !len || !status || (cap > 0 && !out)) return fail(-1, "trajectory: bad arguments");
 	if (process_rand && count != 1) return fail(-1, "trajectory: the rand() stream is continued for one ion at a time");
-	if (integrator < 0 || integrator > 1 || stepper < 0 || stepper > 3) return fail(-1, "trajectory: unknown integrator or stepper");
+	if (integrator < 0 || integrator > 2 || stepper < 0 || stepper > 3) return fail(-1, "trajectory: unknown integrator or stepper");
 	for (int i = 0; i < count; i++)
 		if (nodes[i] < 0 || nodes[i] >= mesh_.n) return fail(-1, "trajectory: bad node");
 	const int n = mesh_.n, m = mesh_.m();

@@ -53,6 +53,8 @@ struct EventRecord {
 	float voltage, time_scale, tof;
 	int traj_len, traj_status;
 	long long event_index;  // LogResults::Dataset::eventIndex (process.cpp:696)
+	float ion_mass;         // the ion's mass (amu) and charge (e) in its flight
+	int ion_charge;
 };
 
 struct EvapOptions {
@@ -76,7 +78,25 @@ struct EvapOptions {
 	int thermal_velocity = 0;   // 1: evap_initialVelocity from the process's rand()
 	int update_coordination = 0;  // 1: the fork's update_coordination (surface_3d.cpp:568-593)
 	int init_event_cnt = 0;       // EvaporationOptions::initEventCnt (process.h:98)
+	// additions without a reference counterpart (integrator 2 is the classical
+	// fourth-order Runge-Kutta step)
+	float detector_z = 0.0f;      // > 0: extrapolate each flight to this plane (m), else to the mesh's top
+	int ion_table = 0;            // 1: each ion's mass and charge drawn from its species' ion table
+	unsigned int ion_seed = 0;    // seed of those draws (one per event, keyed by the event index)
 };
+
+// a species' ion table: (mass amu, charge e, weight) entries
+constexpr int kIonTableMax = 8;
+struct IonTable {
+	int count = 0;
+	float mass[kIonTableMax];
+	int charge[kIonTableMax];
+	float weight[kIonTableMax];
+};
+// the entry drawn for an event: u in [0, 1) from splitmix64 of (seed, event
+// index), then the first entry whose cumulative weight exceeds u times the
+// total
+int ion_pick(unsigned int seed, long long event_index, const IonTable& t);
 
 class DevBuf {
 public:
@@ -143,6 +163,8 @@ public:
 	int get_charges(double* out) const;
 	// the fork's update_coordination (surface_3d.cpp:568-593) on the ids
 	int update_coordination();
+	// the ion table of a species for the flights (no reference counterpart)
+	int set_ion_table(int species, int count, const float* mass, const int* charge, const float* weight);
 	// the node's id names a defined species (Configuration::Table::operator[]
 	// throws otherwise)
 	bool species_ok(int node) const
@@ -260,6 +282,7 @@ private:
 	std::vector<int16_t> ids_;
 	std::vector<uint8_t> flags_;
 	Species species_[128];
+	IonTable ions_[128];
 	float temperature_;
 	std::vector<long long> row_of_entry_;  // row index of every directed entry (int64 for >2^31)
 	std::vector<long long> mirror_;        // entry of (j,i) for entry (i,j)

@@ -1,0 +1,122 @@
+"""GPU parity for the full-pipeline additions (no reference counterpart; the
+oracle restates them and tests/test_rk4_oracle.py pins it by mathematics):
+the fourth-order Runge-Kutta flight integrator, the extrapolation of each
+flight to a planar detector, and the per-species ion tables whose entry is
+drawn per event from (seed, event index).  Device = oracle bit for bit, the
+oracle's step-control root set to the device's correctly rounded one."""
+import sys
+
+import numpy as np
+import pytest
+
+import tapsim_modified_amd as tsg
+from oracle import oracle as O
+from tests.conftest import FIXTURE_SPECIES
+
+pytestmark = pytest.mark.gpu
+
+Q = 2.0 * np.float32(1.60217733e-19)
+M = np.float32(55.845) * np.float32(1.660538782e-27)
+
+# isotopes of the matrix (Fe-like) and solute (Ni-like) species: mass (amu),
+# charge (e), natural abundance; a share of singly charged ions
+IONS = {
+    10: ([53.9396, 55.9349, 56.9354, 57.9333, 55.9349], [2, 2, 2, 2, 1], [0.05845, 0.85, 0.02119, 0.00282, 0.06754]),
+    11: ([57.9353, 59.9308, 60.9311, 61.9283, 63.9280], [2, 2, 2, 2, 2], [0.6808, 0.2622, 0.0114, 0.0363, 0.0093]),
+}
+
+
+def log(msg):
+    print("[gpu-fullpipe] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def same_bits(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def setup(fx):
+    mesh = tsg.Mesh(fx["xyz"], fx["tets"], fx["ref_neigh"])
+    g = tsg.Grid3D(mesh, fx["ids"], tsg.species_table(FIXTURE_SPECIES), 0.0, tsg.FAITHFUL_F32)
+    o = O.Oracle(O.F32, fx["xyz"], fx["ids"], fx["tets"], fx["ref_neigh"], O.species_table(FIXTURE_SPECIES))
+    o.sync()
+    for _ in range(300):
+        g.relax(1)
+        o.relax(1)
+    return g, o
+
+
+@pytest.mark.parametrize("stepper", [0, 1, 3])
+def test_rk4_flight_matches_oracle(tip_small, stepper):
+    node = int(tip_small["local_node"])
+    log("rk4 flight from node %d, stepper %d" % (node, stepper))
+    g, o = setup(tip_small)
+    g.set_id(node, 0)
+    g.relax(1)
+    o.set_id(node, 0)
+    o.resync(node)
+    o.relax(1)
+    dt0 = 1e-14
+    (st, tr), = g.trajectories([node], Q, M, integrator=2, stepper=stepper, dt0=dt0, cap=20000)
+    try:
+        O.set_root_mode(1)
+        sto, tro = o.trajectory(node, Q, M, integrator=2, stepper=stepper, dt0=dt0, cap=20000)
+    finally:
+        O.set_root_mode(0)
+    log("rk4 stepper %d: status %d / %d, %d / %d phases" % (stepper, st, sto, len(tr), len(tro)))
+    assert st == sto == 5  # left the mesh
+    assert same_bits(tr, tro)
+    # the flight leaves the mesh: the last phase is the extrapolation to its top
+    if len(tr) < 20000:
+        assert tr[-1, 3] == np.float32(tip_small["xyz"][:, 2].max())
+
+
+def test_detector_plane_and_ion_tables_match_oracle(tip_small):
+    """12 events with RK4 flights extrapolated to a detector 10 cm above the
+    mesh and ions drawn from the species' tables: sequence, flights (start,
+    stop on the detector plane), ion mass and charge, voltage, time scale and
+    time of flight equal the oracle's; the draws follow (seed, event index)"""
+    g, o = setup(tip_small)
+    zdet = float(np.float32(tip_small["xyz"][:, 2].max() + 0.1))
+    for sid, (m, q, w) in IONS.items():
+        g.set_ion_table(sid, m, q, w)
+        o.set_ion_table(sid, m, q, w)
+    e = tsg.EvapOptions()
+    e.prob_mode, e.evap_mode, e.vacuum_id = 0, 0, 0
+    e.relax_threshold, e.relax_shell, e.relax_cycle, e.relax_queue = float(np.float32(1e-2)), 3, 5, 5
+    e.relax_global_cycles, e.refresh_interval = 1, 0
+    e.refresh_threshold, e.refresh_cycle, e.refresh_queue, e.exact_probability = 0.0, 1, 0, 1
+    e.trajectory, e.integrator, e.stepper = 1, 2, 1
+    e.detector_z, e.ion_table, e.ion_seed = zdet, 1, 2026
+    oo = O.OraEvapOpts(prob_mode=0, evap_mode=0, vacuum_id=0, relax_threshold=1e-2, relax_shell=3, relax_cycle=5,
+                       relax_queue=5, relax_global_cycles=1, refresh_interval=0, refresh_threshold=0.0,
+                       refresh_cycle=1, refresh_queue=0, trajectory=1, integrator=2, stepper=1, dt0=1e-14,
+                       dt_min=1e-16, err_limit=1e-6, voltage_queue=250, shrink_limit=1.0, inplace_order=0,
+                       detector_z=zdet, ion_table=1, ion_seed=2026)
+    k = 12
+    tsg.srand(99)
+    ev = g.evaporate(e, k)
+    o.srand(99)
+    try:
+        O.set_root_mode(1)
+        eo = o.evaporate(oo, k)
+    finally:
+        O.set_root_mode(0)
+    log("detector: nodes %s; ions %s" % ([x.node for x in ev], [(round(x.ion_mass, 2), x.ion_charge) for x in ev]))
+    assert [x.node for x in ev] == [x.node for x in eo]
+    assert [x.ion_charge for x in ev] == [x.ion_charge for x in eo]
+    assert same_bits([x.ion_mass for x in ev], [x.ion_mass for x in eo])
+    for f in ("start", "stop"):
+        assert same_bits([list(getattr(x, f)) for x in ev], [list(getattr(x, f)) for x in eo]), f
+    for f in ("voltage", "time_scale", "tof"):
+        assert same_bits([getattr(x, f) for x in ev], [getattr(x, f) for x in eo]), f
+    assert [x.traj_len for x in ev] == [x.traj_len for x in eo]
+    # every flight ends on the detector plane, and each ion is its table's draw
+    ids = tip_small["ids"]
+    for x in ev:
+        assert abs(x.stop[2] - zdet) <= 1e-6 * zdet
+        m, q, w = IONS[int(ids[x.node])]
+        j = O.ion_pick(2026, x.event_index, w)
+        assert x.ion_charge == q[j] and x.ion_mass == np.float32(m[j])
