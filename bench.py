@@ -253,6 +253,8 @@ def main():
         print("bench.py runs on one GPU: the library has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c5":
+        os.environ["TSG_VERBOSE"] = "1"  # the mesher reports its progress (minutes at this size)
     import tapsim_modified_amd as tsg
     tsg.load()
     w = WORKLOADS[args.workload]

@@ -265,11 +265,21 @@ typedef struct tsg_event
 	float ion_mass;           /* the ion's mass (amu) and charge (e) in its flight: the species' own, or the
 	                             entry of its ion table drawn for this event */
 	int ion_charge;
+	float err_pos[3];         /* the flight's final error estimate, Trajectory_3d::error_estimate (position,
+	                             velocity; trajectory_3d.cpp:199,327,680-696), which LogTrajectory writes */
+	float err_vel[3];
 } tsg_event;
 
 /* Process::evaporation steps I-IV and VI (process.cpp:680-1037) for up to k
    events; *done is the number recorded in out */
 int tsg_evaporate(tsg_grid*, const tsg_evap_options*, int k, tsg_event* out, int* done);
+/* tsg_evaporate recording each event's trajectory for File_Io::LogTrajectory
+   (process.cpp:976-979 pushes every event's Trajectory_3d): the first cap
+   phases (t, x, y, z, vx, vy, vz, tet; t unscaled, as Trajectory_3d holds
+   it) of event e go to phases + 8 cap e and their full count to
+   phase_len[e]. */
+int tsg_evaporate_traj(tsg_grid*, const tsg_evap_options*, int k, tsg_event* out, int* done, int cap, float* phases,
+                       int* phase_len);
 
 /* Accelerated relaxation (no reference counterpart; replaces step III's
    relaxations when selected): FP32 Chebyshev-accelerated Jacobi correction
@@ -329,9 +339,14 @@ int tsg_copy_probe(long long bytes, int reps, double* avg_ms);
  * phases (t, x, y, z, vx, vy, vz, tet) of its trajectory, the last one the
  * extrapolation to the mesh's largest z; len[i] the number of phases and
  * status[i] the Trajectory_3d status (trajectory_3d.h:89; 5 = left the
- * mesh, 1 = no field), negative on failure (-4: work limit). */
+ * mesh, 1 = no field), negative on failure (-4: work limit);
+ * error_estimate (may be null) 6 floats per ion, Trajectory_3d::error_estimate
+ * (trajectory_3d.cpp:199,327,680-696): the root-sum-square of the accepted
+ * steps' error estimates, position xyz then velocity xyz, the position part
+ * moved along by the extrapolation. */
 int tsg_trajectories(tsg_grid*, int count, const int* nodes, const float* charge, const float* mass, int integrator,
-                     int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status);
+                     int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status,
+                     float* error_estimate);
 
 /* The ion table of a species for the flights of tsg_evaporate (no reference
  * counterpart; the reference flies every ion of a species with the species'

@@ -51,6 +51,7 @@ class OraEvent(ctypes.Structure):
         ("surface_size", ctypes.c_int), ("scaling_ref", ctypes.c_float), ("apex", ctypes.c_float * 3),
         ("normal", ctypes.c_float * 3), ("traj_root_diff", ctypes.c_int), ("event_index", ctypes.c_longlong),
         ("ion_mass", ctypes.c_float), ("ion_charge", ctypes.c_int),
+        ("err_pos", ctypes.c_float * 3), ("err_vel", ctypes.c_float * 3),
     ]
 
 
@@ -108,7 +109,7 @@ _SIGS = {
     "ora_point_potential": ([_P, _P, ctypes.c_int], ctypes.c_float),
     "ora_point_field": ([_P, _P, ctypes.c_int, _P], ctypes.c_int),
     "ora_trajectory": ([_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float,
-                        ctypes.c_float, ctypes.c_float, _P, ctypes.c_int, _P], ctypes.c_int),
+                        ctypes.c_float, ctypes.c_float, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "ora_evaporate": ([_P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "ora_random_reset": ([_P], None),
     "ora_spherical_preset": ([_P, _P, ctypes.c_float, ctypes.c_float], None),
@@ -386,11 +387,17 @@ class Oracle:
             raise OracleError(load().ora_last_error().decode())
         return st, out[:min(ln.value, cap)].copy()
 
-    def trajectory(self, node, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6, cap=200000):
+    def trajectory(self, node, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6, cap=200000,
+                   with_error=False):
+        """(status, phases (k, 8): t, x, y, z, vx, vy, vz, tet) of one flight;
+        with_error adds the final error estimate (position xyz, velocity xyz)"""
         out = np.zeros((cap, 8), np.float32)
+        est = np.zeros(6, np.float32)
         ln = ctypes.c_int()
         st = load().ora_trajectory(self._h, node, charge, mass, integrator, stepper, dt0, dt_min, err, _ptr(out), cap,
-                                   ctypes.byref(ln))
+                                   ctypes.byref(ln), _ptr(est))
+        if with_error:
+            return st, out[:min(ln.value, cap)].copy(), est
         return st, out[:min(ln.value, cap)].copy()
 
     def set_ion_table(self, species, mass, charge, weight):

@@ -30,7 +30,9 @@
  *   field:TAG            field_o1 / field_o2 of every node (grid_3d.cpp:909,993)
  *   vorarea:TAG          voronoiArea(i,j,1.0f) both directions for every edge
  *   evap:...             the steps I-VI of Process::evaporation (process.cpp:680-1037)
- *   traj:TAG:NODE:INT:STEP  one Trajectory_3d from a node (trajectory_3d.cpp:99)
+ *   traj:TAG:NODE:INT:STEP  one Trajectory_3d from a node (trajectory_3d.cpp:99); its phases,
+ *                        status and final error estimate
+ *   writetraj:IN:OUT:MODE:CHUNK  File_Io::LogTrajectory of the datasets in IN
  *   randreset:SEED       std::srand(SEED), then Grid_3d::Table::randomReset (grid_3d.cpp:811)
  *   threads:T            Grid_3d::Table::setThreadNum(T)  (grid_3d.cpp:194; the driver starts with 1)
  *
@@ -49,18 +51,28 @@
 #include <limits>
 #include <cmath>
 #include <ctime>
-
-#include "system_3d.h"
+#include <iostream>
+#include <list>
+#include <map>
+#include <ostream>
+#include <sstream>
+#include <typeinfo>
+#include <pthread.h>
 #include <unistd.h>
+
+// The reference's classes are opened (private -> public; the layout is
+// unchanged) for two things they have no setters for: Fieldline_3d's
+// stepper and iteration limits, which its constructor leaves uninitialised,
+// and the status and error estimate of a Trajectory_3d handed to
+// File_Io::LogTrajectory.  Every standard header they use is included above.
+#define private public
+#include "system_3d.h"
 #include "file_io.h"
 #include "logResults.h"
+#include "logTrajectory.h"
 #include "process.h"
 #include "surface_3d.h"
 #include "trajectory_3d.h"
-// Fieldline_3d has no setters for its stepper and iteration limits and its
-// constructor leaves them uninitialised; the driver opens the class to set
-// them (the layout is unchanged).
-#define private public
 #include "fieldline_3d.h"
 #undef private
 
@@ -512,6 +524,10 @@ int main(int argc, char** argv)
 				std::vector<int> st(1, status);
 				writeArray("traj_" + tok.at(1) + ".f32", data);
 				writeArray("traj_status_" + tok.at(1) + ".i32", st);
+				std::vector<float> err;
+				for (int c = 0; c < 3; c++) err.push_back(t.error_estimate().position(c));
+				for (int c = 0; c < 3; c++) err.push_back(t.error_estimate().velocity(c));
+				writeArray("traj_err_" + tok.at(1) + ".f32", err);
 			}
 			else if (cmd == "writegeom")
 			{
@@ -586,6 +602,46 @@ int main(int argc, char** argv)
 						d.normal = Geometry_3d::Point(r[21], r[22], r[23]);
 						d.apex = Geometry_3d::Point(r[24], r[25], r[26]);
 						log.push(d);
+					}
+					// the writer thread empties its queue only while it runs; give
+					// it time before the destructor stops it
+					usleep(300000);
+				}
+			}
+			else if (cmd == "writetraj")
+			{
+				// writetraj:IN:OUT:MODE:CHUNK -- File_Io::LogTrajectory (MODE 0
+				// ASCII, 1 BINARY; CHUNK events per file, 0 one file) writes the
+				// datasets of IN into OUT of the work directory.  A dataset in IN
+				// (floats): event index, node number, time scale, status, charge
+				// (C), mass (kg), phase count P, error estimate (position xyz,
+				// velocity xyz), then P phases of t, position xyz, velocity xyz,
+				// tet
+				const std::vector<float> rec = readFloats(tok.at(1));
+				{
+					// (name, mode, delay 0, chunk size)
+					File_Io::LogTrajectory log((g_out + "/" + tok.at(2)).c_str(), std::atoi(tok.at(3).c_str()), 0,
+					                           (unsigned int) std::atoi(tok.at(4).c_str()));
+					size_t at = 0;
+					while (at + 13 <= rec.size())
+					{
+						const float* h = &rec[at];
+						Trajectory_3d t;
+						t._charge = h[4];
+						t._mass = h[5];
+						t._status = (int) h[3];
+						const int P = (int) h[6];
+						t._error_estimate = Trajectory_3d::phaseVector(0.0f, MathVector3d<float>(h[7], h[8], h[9]),
+						                                              MathVector3d<float>(h[10], h[11], h[12]));
+						at += 13;
+						t._data.clear();
+						for (int k = 0; k < P; k++, at += 8)
+						{
+							const float* q = &rec[at];
+							t._data.push_back(Trajectory_3d::phaseVector(q[0], MathVector3d<float>(q[1], q[2], q[3]),
+							                                             MathVector3d<float>(q[4], q[5], q[6]), (int) q[7]));
+						}
+						log.push((unsigned int) h[0], (unsigned int) h[1], h[2], t);
 					}
 					// the writer thread empties its queue only while it runs; give
 					// it time before the destructor stops it

@@ -1096,6 +1096,14 @@ static float quarter_pow(traj* tr, float x)
 	return g_root_mode ? c : p;
 }
 
+/* the error estimate's powf(x, 2.0f): glibc's, as the reference calls it
+   (mode 0), or the product x * x that the device forms (mode 1); glibc's
+   powf rounds x^2 correctly except, possibly, at exact ties */
+static float square(float x)
+{
+	return g_root_mode ? x * x : powf(x, 2.0f);
+}
+
 /* Trajectory_3d::o5_integrator, trajectory_3d.cpp:583-676 (Cash-Karp tableau
    in float, Numerical Recipes 16.2) */
 static phase o5_step(const traj* tr, float delta, phase* error)
@@ -1429,13 +1437,14 @@ static int traj_run(traj* tr, const float start[3], const float* v0, int guess, 
 			memset(&error, 0, sizeof(error));
 			stepper_step(tr, &delta, err_limit, &error, &next);
 			traj_push(tr, next);
+			/* Trajectory_3d::cumulative_error, trajectory_3d.cpp:680-696 */
 			for (int i = 0; i < 3; i++)
 			{
-				float a = powf(tr->err_est.p[i], 2.0f);
-				a += powf(error.p[i], 2.0f);
+				float a = square(tr->err_est.p[i]);
+				a += square(error.p[i]);
 				tr->err_est.p[i] = sqrtf(a);
-				float b = powf(tr->err_est.v[i], 2.0f);
-				b += powf(error.v[i], 2.0f);
+				float b = square(tr->err_est.v[i]);
+				b += square(error.v[i]);
 				tr->err_est.v[i] = sqrtf(b);
 			}
 			if (tr->data[tr->len - 1].tet < 0) { status = T_SYSTEM_LIMIT; break; }
@@ -1455,6 +1464,14 @@ static int traj_run(traj* tr, const float start[3], const float* v0, int guess, 
 		for (int k = 0; k < 3; k++) item.v[k] = back->v[k];
 		item.tet = ora_find_tetrahedron(tr->s, item.p, back->tet);
 		traj_push(tr, item);
+		/* the error estimate follows the extrapolation (trajectory_3d.cpp:327) */
+		if (dt > 0.0f)
+			for (int k = 0; k < 3; k++)
+			{
+				float d = dt;
+				d *= tr->err_est.v[k];
+				tr->err_est.p[k] += d;
+			}
 	}
 	return status;
 }
@@ -1495,7 +1512,7 @@ static float zmax_of(const ora_sys* s)
 
 int ora_trajectory(const ora_sys* s, int node, float charge, float mass, int integrator,
                    int stepper, float dt0, float dt_min, float err_limit,
-                   float* out, int cap, int* len)
+                   float* out, int cap, int* len, float* err)
 {
 	traj tr;
 	memset(&tr, 0, sizeof(tr));
@@ -1510,6 +1527,8 @@ int ora_trajectory(const ora_sys* s, int node, float charge, float mass, int int
 		o[7] = (float) tr.data[k].tet;
 	}
 	*len = tr.len;
+	if (err)
+		for (int k = 0; k < 3; k++) { err[k] = tr.err_est.p[k]; err[3 + k] = tr.err_est.v[k]; }
 	free(tr.data);
 	return status;
 }
@@ -1958,6 +1977,7 @@ int ora_evaporate(ora_sys* s, const ora_evap_opts* o, int nevents, ora_event* ou
 			r->traj_status = traj_run(&tr, nc(s, node), v0, s->assoc[node], o->dt0, o->err_limit, zmax);
 			r->traj_len = tr.len;
 			r->traj_root_diff = tr.root_diff;
+			for (int k = 0; k < 3; k++) { r->err_pos[k] = tr.err_est.p[k]; r->err_vel[k] = tr.err_est.v[k]; }
 			for (int k = 0; k < 3; k++) { r->start[k] = tr.data[0].p[k]; r->stop[k] = tr.data[tr.len - 1].p[k]; }
 			for (int k = tr.len - 1; k >= 0; k--)
 				if (tr.data[k].tet >= 0) { have_back = 1; backt = tr.data[k].tet; for (int c = 0; c < 3; c++) backp[c] = tr.data[k].p[c]; break; }

@@ -85,6 +85,8 @@ typedef struct
 	long long event_index; /* LogResults::Dataset::eventIndex (process.cpp:696) */
 	float ion_mass;        /* the ion's mass (amu) and charge (e) in its flight */
 	int ion_charge;
+	float err_pos[3];      /* the flight's final error estimate (Trajectory_3d::error_estimate) */
+	float err_vel[3];
 } ora_event;
 
 typedef struct ora_sys ora_sys;
@@ -168,7 +170,7 @@ float ora_point_potential(const ora_sys*, const float pos[3], int guess);
 int ora_point_field(const ora_sys*, const float pos[3], int guess, float out[3]);
 int ora_trajectory(const ora_sys*, int node, float charge, float mass, int integrator,
                    int stepper, float dt0, float dt_min, float err_limit,
-                   float* out, int cap, int* len);
+                   float* out, int cap, int* len, float* err);
 
 int ora_evaporate(ora_sys*, const ora_evap_opts*, int nevents, ora_event* out, int* done);
 

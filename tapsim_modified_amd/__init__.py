@@ -105,6 +105,7 @@ class Event(ctypes.Structure):
         ("voltage", ctypes.c_float), ("time_scale", ctypes.c_float), ("tof", ctypes.c_float),
         ("traj_len", ctypes.c_int), ("traj_status", ctypes.c_int), ("event_index", ctypes.c_longlong),
         ("ion_mass", ctypes.c_float), ("ion_charge", ctypes.c_int),
+        ("err_pos", ctypes.c_float * 3), ("err_vel", ctypes.c_float * 3),
     ]
 
 
@@ -135,6 +136,7 @@ _SIGS = {
     "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_field_o2": ([_P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "tsg_set_ion_table": ([_P, ctypes.c_int, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "tsg_evaporate_traj": ([_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_field_o1_noalloc": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_get_couplings": ([_P, _P], ctypes.c_int),
     "tsg_get_charges": ([_P, _P], ctypes.c_int),
@@ -180,7 +182,7 @@ _SIGS = {
     "tsg_reset_potentials": ([_P, ctypes.c_int, _P, ctypes.c_float, ctypes.c_float], ctypes.c_int),
     "tsg_copy_probe": ([ctypes.c_longlong, ctypes.c_int, _P], ctypes.c_int),
     "tsg_trajectories": ([_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
-                          ctypes.c_float, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+                          ctypes.c_float, ctypes.c_int, _P, _P, _P, _P], ctypes.c_int),
 }
 
 
@@ -529,9 +531,10 @@ class Grid3D:
         return lv.value
 
     def trajectories(self, nodes, charge, mass, integrator=1, stepper=1, dt0=1e-14, dt_min=1e-16, err=1e-6,
-                     cap=4096):
+                     cap=4096, with_error=False):
         """Ion trajectories from the given nodes (tsg_trajectories): a list of
-        (status, phases[len, 8]) per ion, phases truncated to cap."""
+        (status, phases[len, 8]) per ion, phases truncated to cap; with_error
+        adds each flight's final error estimate (position xyz, velocity xyz)."""
         nodes = np.ascontiguousarray(np.atleast_1d(nodes), np.int32)
         k = len(nodes)
         charge = np.ascontiguousarray(np.broadcast_to(np.float32(charge), (k,)), np.float32)
@@ -539,8 +542,11 @@ class Grid3D:
         out = np.zeros((k, cap, 8), np.float32)
         ln = np.zeros(k, np.int32)
         st = np.zeros(k, np.int32)
+        est = np.zeros((k, 6), np.float32)
         _chk(load().tsg_trajectories(self._h, k, _ptr(nodes), _ptr(charge), _ptr(mass), integrator, stepper, dt0, dt_min,
-                                     err, cap, _ptr(out), _ptr(ln), _ptr(st)))
+                                     err, cap, _ptr(out), _ptr(ln), _ptr(st), _ptr(est)))
+        if with_error:
+            return [(int(st[i]), out[i, :min(int(ln[i]), cap)].copy(), est[i].copy()) for i in range(k)]
         return [(int(st[i]), out[i, :min(int(ln[i]), cap)].copy()) for i in range(k)]
 
     def fieldlines(self, starts, distance=-1.0, iterations=5000, stepper_limit=1000, process_rand=True, cap=4096):
@@ -574,11 +580,23 @@ class Grid3D:
         w = np.ascontiguousarray(weight, np.float32)
         _chk(load().tsg_set_ion_table(self._h, species, len(m), _ptr(m), _ptr(q), _ptr(w)))
 
-    def evaporate(self, options, k):
+    def evaporate(self, options, k, trajectories=0):
+        """Process::evaporation steps I-VI for up to k events (tsg_evaporate);
+        trajectories > 0 also returns each event's flight phases (at most that
+        many per event, t unscaled) for the trajectory stream
+        (tsg_evaporate_traj): (events, [phases (len, 8) per event])"""
         out = (Event * max(1, k))()
         done = ctypes.c_int()
-        _chk(load().tsg_evaporate(self._h, ctypes.byref(options), k, out, ctypes.byref(done)))
-        return [out[i] for i in range(done.value)]
+        if trajectories <= 0:
+            _chk(load().tsg_evaporate(self._h, ctypes.byref(options), k, out, ctypes.byref(done)))
+            return [out[i] for i in range(done.value)]
+        cap = int(trajectories)
+        phases = np.zeros((max(1, k), cap, 8), np.float32)
+        lens = np.zeros(max(1, k), np.int32)
+        _chk(load().tsg_evaporate_traj(self._h, ctypes.byref(options), k, out, ctypes.byref(done), cap, _ptr(phases),
+                                       _ptr(lens)))
+        ev = [out[i] for i in range(done.value)]
+        return ev, [phases[i, :min(int(lens[i]), cap)].copy() for i in range(done.value)]
 
 
 # ---------------------------------------------------------------- learned warm start

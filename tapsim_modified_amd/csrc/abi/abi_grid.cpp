@@ -245,9 +245,10 @@ int tsg_take_away(tsg_grid* h, int node)
 	return check(h, h->g->take_away(node));
 }
 
-int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out, int* done)
+namespace {
+
+tsg::EvapOptions options_of(const tsg_evap_options* o)
 {
-	if (!h || !o || !out || !done) return tsg::fail(TSG_E_ARG, "null argument");
 	tsg::EvapOptions e;
 	e.prob_mode = o->prob_mode; e.evap_mode = o->evap_mode; e.vacuum_id = o->vacuum_id;
 	e.relax_threshold = o->relax_threshold; e.relax_shell = o->relax_shell; e.relax_cycle = o->relax_cycle;
@@ -272,7 +273,24 @@ int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out,
 	e.detector_z = o->detector_z;
 	e.ion_table = o->ion_table;
 	e.ion_seed = o->ion_seed;
-	return check(h, h->g->evaporate(e, k, reinterpret_cast<tsg::EventRecord*>(out), done));
+	return e;
+}
+
+}  // namespace
+
+int tsg_evaporate(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out, int* done)
+{
+	if (!h || !o || !out || !done) return tsg::fail(TSG_E_ARG, "null argument");
+	return check(h, h->g->evaporate(options_of(o), k, reinterpret_cast<tsg::EventRecord*>(out), done));
+}
+
+int tsg_evaporate_traj(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event* out, int* done, int cap, float* phases,
+                       int* phase_len)
+{
+	if (!h || !o || !out || !done || cap < 0 || (cap > 0 && (!phases || !phase_len)))
+		return tsg::fail(TSG_E_ARG, "tsg_evaporate_traj: bad arguments");
+	return check(h, h->g->evaporate(options_of(o), k, reinterpret_cast<tsg::EventRecord*>(out), done, cap, phases,
+	                                phase_len));
 }
 
 int tsg_set_ion_table(tsg_grid* h, int species, int count, const float* mass_amu, const int* charge,
@@ -283,11 +301,12 @@ int tsg_set_ion_table(tsg_grid* h, int species, int count, const float* mass_amu
 }
 
 int tsg_trajectories(tsg_grid* h, int count, const int* nodes, const float* charge, const float* mass, int integrator,
-                     int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status)
+                     int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status,
+                     float* error_estimate)
 {
 	if (!h) return tsg::fail(TSG_E_ARG, "null grid");
 	return check(h, h->g->trajectories(count, nodes, charge, mass, integrator, stepper, dt0, dt_min, err_limit, cap, out,
-	                                   len, status, nullptr));
+	                                   len, status, nullptr, nullptr, false, error_estimate));
 }
 
 int tsg_accel_setup(tsg_grid* h, double safety, double* rho)

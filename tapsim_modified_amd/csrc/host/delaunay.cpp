@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -299,8 +301,16 @@ bool Builder::run(Mesh* out, std::string* err)
 	std::iota(order.begin(), order.end(), 0);
 	std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
 
-	for (size_t i = 0; i < n_; i++)
+	// a build of millions of points takes minutes: with TSG_VERBOSE set,
+	// every 2 M insertions are reported on stderr
+	const bool verbose = std::getenv("TSG_VERBOSE") != nullptr;
+	for (size_t i = 0; i < n_; i++) {
 		if (!insert(order[i], err)) return false;
+		if (verbose && (i + 1) % 2000000 == 0) {
+			std::fprintf(stderr, "[tsg mesh] %zu of %zu points inserted, %zu tets\n", i + 1, n_, tets_.size());
+			std::fflush(stderr);
+		}
+	}
 
 	// drop tets touching the enclosing vertices, compact, re-encode neighbours
 	std::vector<int> remap(tets_.size(), -1);

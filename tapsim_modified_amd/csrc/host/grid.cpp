@@ -217,8 +217,10 @@ Grid::~Grid()
 		if (f.ready) (void) hipEventDestroy(f.ready);
 		if (f.done) (void) hipEventDestroy(f.done);
 		if (f.host) (void) hipHostFree(f.host);
+		if (f.phases_host) (void) hipHostFree(f.phases_host);
 		f.ready = f.done = nullptr;
 		f.host = nullptr;
+		f.phases_host = nullptr;
 	}
 	for (hipEvent_t e : {ev_start_, ev_stop_, sum_src_, sum_done_, amg_ev_[0], amg_ev_[1], sst_ev_, norm_src_, norm_done_,
 	                     patch_ev_})
@@ -1764,12 +1766,12 @@ int Grid::flight_start(const EvapOptions& o, int ev, int node, const Species& sp
 	if (f.active)
 		if (int rc = flight_finish(f, nullptr, o)) return rc;
 	if (!f.host) {
-		HIPCHK(hipHostMalloc((void**) &f.host, 24 * sizeof(float)));
+		HIPCHK(hipHostMalloc((void**) &f.host, 32 * sizeof(float)));
 		HIPCHK(hipEventCreateWithFlags(&f.ready, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
 		f.nfield.alloc(sizeof(float) * 3 * (size_t) n);
 		f.phi.alloc(rs * (size_t) n);
-		f.io.alloc(24 * sizeof(float));
+		f.io.alloc(32 * sizeof(float));
 		if (!f.nfield.ptr() || !f.phi.ptr() || !f.io.ptr()) return fail(-2, "device allocation failed (flight slot)");
 	}
 	int* io = f.io.as<int>();
@@ -1840,6 +1842,21 @@ int Grid::flight_start(const EvapOptions& o, int ev, int node, const Species& sp
 	t.status = io + 5;
 	t.summary = (float*) (io + 6);
 	t.rng = d_rng_.as<int>();
+	t.err_out = (float*) (io + 24);
+	if (rec_cap_ > 0) {
+		// the phases of the flight, for the caller's trajectory record
+		if (f.phases_cap < rec_cap_) {
+			if (f.phases_host) (void) hipHostFree(f.phases_host);
+			f.phases_host = nullptr;
+			f.phases_cap = 0;
+			f.phases.alloc(sizeof(float) * 8 * (size_t) rec_cap_);
+			HIPCHK(hipHostMalloc((void**) &f.phases_host, sizeof(float) * 8 * (size_t) rec_cap_));
+			if (!f.phases.ptr()) return fail(-2, "device allocation failed (flight phases)");
+			f.phases_cap = rec_cap_;
+		}
+		t.out = f.phases.as<float>();
+		t.cap = rec_cap_;
+	}
 	a.thermal = o.thermal_velocity;
 	a.temperature = temperature_;
 	a.mass_amu = sp.mass;
@@ -1851,7 +1868,10 @@ int Grid::flight_start(const EvapOptions& o, int ev, int node, const Species& sp
 	a.f64 = f64_;
 	a.exit = (float*) (io + 18);
 	if (launch_flight(fstream_, a)) return fail(-2, "flight launch failed");
-	HIPCHK(hipMemcpyAsync(f.host + 4, io + 4, 20 * sizeof(float), hipMemcpyDeviceToHost, fstream_));
+	HIPCHK(hipMemcpyAsync(f.host + 4, io + 4, 26 * sizeof(float), hipMemcpyDeviceToHost, fstream_));
+	if (rec_cap_ > 0)
+		HIPCHK(hipMemcpyAsync(f.phases_host, f.phases.ptr(), sizeof(float) * 8 * (size_t) rec_cap_, hipMemcpyDeviceToHost,
+		                      fstream_));
 	HIPCHK(hipEventRecord(f.done, fstream_));
 	f.active = true;
 	f.seq = flight_seq_++;
@@ -1876,6 +1896,15 @@ int Grid::flight_finish(FlightSlot& f, EventRecord* out, const EvapOptions& o)
 	if (hi[23]) return fail(-4, "Grid_3d::Table::field_o1(): singular least-squares system");
 	r.traj_len = hi[4];
 	r.traj_status = hi[5];
+	for (int c = 0; c < 3; c++) {
+		r.err_pos[c] = f.host[24 + c];
+		r.err_vel[c] = f.host[27 + c];
+	}
+	if (rec_phases_ && rec_cap_ > 0) {
+		const int keep = std::min(r.traj_len, rec_cap_);
+		std::memcpy(rec_phases_ + 8 * (size_t) rec_cap_ * (size_t) f.ev, f.phases_host, sizeof(float) * 8 * (size_t) keep);
+		rec_len_[f.ev] = r.traj_len;
+	}
 	const float* sm = f.host + 6;
 	for (int c = 0; c < 3; c++) {
 		r.start[c] = sm[c];
@@ -1935,13 +1964,23 @@ int Grid::flight_drain(EventRecord* out, const EvapOptions& o)
 	return rc;
 }
 
-int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done)
+int Grid::evaporate(const EvapOptions& o, int k, EventRecord* out, int* done, int phase_cap, float* phases,
+                    int* phase_len)
 {
+	if (phase_cap < 0 || (phase_cap > 0 && (!phases || !phase_len))) return fail(-1, "evaporate: trajectory record");
 	flight_out_ = out;
+	rec_cap_ = phase_cap;
+	rec_phases_ = phases;
+	rec_len_ = phase_len;
+	if (phase_len)
+		for (int e = 0; e < k; e++) phase_len[e] = 0;
 	const int rc = evaporate_events(o, k, out, done);
 	// collect the flights still in the air (their records are in out)
 	const int rd = flight_drain(out, o);
 	flight_out_ = nullptr;
+	rec_cap_ = 0;
+	rec_phases_ = nullptr;
+	rec_len_ = nullptr;
 	return rc ? rc : rd;
 }
 
@@ -2734,7 +2773,7 @@ namespace tsg {
 // reads); one device thread per ion
 int Grid::trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
                        float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status, float* summary,
-                       const float* v0, bool process_rand)
+                       const float* v0, bool process_rand, float* error_estimate)
 {
 	if (count <= 0) return 0;
 	if (cap < 0 || !nodes || !charge || !mass || !len || !status || (cap > 0 && !out)) return fail(-1, "trajectory: bad arguments");
@@ -2750,8 +2789,8 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	                          d_flags_.as<unsigned char>(), phi, f64_, d_nfield_.as<float>(), d_status_.as<int>()))
 		return fail(-2, "field snapshot launch failed");
 	// per ion: node, snapshot, charge, mass, start velocity in; len, status,
-	// 12-float summary out
-	const size_t in_bytes = 28 * (size_t) count, res_bytes = 56 * (size_t) count;
+	// 12-float summary and the 6-float error estimate out
+	const size_t in_bytes = 28 * (size_t) count, res_bytes = 80 * (size_t) count;
 	const size_t out_bytes = sizeof(float) * 8 * (size_t) count * (size_t) std::max(cap, 1);
 	if (d_traj_in_.bytes() < in_bytes) d_traj_in_.alloc(in_bytes * 2);
 	if (d_traj_res_.bytes() < res_bytes) d_traj_res_.alloc(res_bytes * 2);
@@ -2800,12 +2839,15 @@ int Grid::trajectories(int count, const int* nodes, const float* charge, const f
 	a.summary = summary ? (float*) (res + 2 * count) : nullptr;
 	a.v0 = v0 ? dv0 : nullptr;
 	a.rng = process_rand ? d_rng_.as<int>() : nullptr;
+	a.err_out = error_estimate ? (float*) (res + 14 * count) : nullptr;
 	if (launch_trajectories(stream_, a)) return fail(-2, "trajectory launch failed");
 	int st = 0;
 	if (cap > 0) HIPCHK(hipMemcpyAsync(out, d_traj_out_.ptr(), out_bytes, hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipMemcpyAsync(len, res, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipMemcpyAsync(status, res + count, sizeof(int) * count, hipMemcpyDeviceToHost, stream_));
 	if (summary) HIPCHK(hipMemcpyAsync(summary, res + 2 * count, sizeof(float) * 12 * count, hipMemcpyDeviceToHost, stream_));
+	if (error_estimate)
+		HIPCHK(hipMemcpyAsync(error_estimate, res + 14 * count, sizeof(float) * 6 * count, hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipMemcpyAsync(&st, d_status_.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream_));
 	if (process_rand) HIPCHK(hipMemcpyAsync(&prs, d_rng_.ptr(), sizeof(RandState), hipMemcpyDeviceToHost, stream_));
 	HIPCHK(hipStreamSynchronize(stream_));

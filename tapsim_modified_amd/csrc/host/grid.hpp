@@ -55,6 +55,7 @@ struct EventRecord {
 	long long event_index;  // LogResults::Dataset::eventIndex (process.cpp:696)
 	float ion_mass;         // the ion's mass (amu) and charge (e) in its flight
 	int ion_charge;
+	float err_pos[3], err_vel[3];  // the flight's final error estimate (Trajectory_3d::error_estimate)
 };
 
 struct EvapOptions {
@@ -219,7 +220,8 @@ public:
 	}
 
 	// Process::evaporation steps I-IV and VI (process.cpp:680-1037) for k events
-	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done);
+	int evaporate(const EvapOptions& o, int k, EventRecord* out, int* done, int phase_cap = 0, float* phases = nullptr,
+	              int* phase_len = nullptr);
 
 	// ion trajectories (Trajectory_3d, trajectory_3d.cpp:99-332) of a batch of
 	// ions from their nodes in the current potentials; out holds cap phases
@@ -228,7 +230,8 @@ public:
 	// continue the process's rand() stream (one ion only)
 	int trajectories(int count, const int* nodes, const float* charge, const float* mass, int integrator, int stepper,
 	                 float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status,
-	                 float* summary = nullptr, const float* v0 = nullptr, bool process_rand = false);
+	                 float* summary = nullptr, const float* v0 = nullptr, bool process_rand = false,
+	                 float* error_estimate = nullptr);
 	// Grid_3d::potential(pos, guess): barycentric potential of a point; with
 	// process_rand the point is located `walks` times drawing from the
 	// process's rand() stream, as that many calls of the reference would
@@ -300,14 +303,23 @@ private:
 		int ev = 0;
 		float flen = 0.0f, pot_before = 0.0f, pot_after = 0.0f, evap_field = 0.0f, scaling = 0.0f;
 		DevBuf nfield, phi, io;
-		float* host = nullptr;  // pinned: inputs [0, 4), outputs [4, 24)
+		float* host = nullptr;  // pinned: inputs [0, 4), outputs [4, 30)
 		hipEvent_t ready = nullptr, done = nullptr;
+		// the flight's phases when the caller records trajectories
+		DevBuf phases;
+		float* phases_host = nullptr;  // pinned, phases_cap phases of 8 floats
+		int phases_cap = 0;
 	};
 	FlightSlot fslot_[2];
 	hipStream_t fstream_ = nullptr;
 	bool rng_on_device_ = false;      // the flight stream holds the process's rand() stream
 	long long flight_seq_ = 0;
 	EventRecord* flight_out_ = nullptr;  // the records of the evaporate call in progress
+	// trajectory recording of the evaporate call in progress (cap 0: off):
+	// event e's phases go to rec_phases_ + 8 cap e, their count to rec_len_[e]
+	float* rec_phases_ = nullptr;
+	int* rec_len_ = nullptr;
+	int rec_cap_ = 0;
 	int flight_start(const EvapOptions& o, int ev, int node, const Species& sp, float flen, float pot_before,
 	                 float pot_after, float scaling);
 	int flight_finish(FlightSlot& f, EventRecord* out, const EvapOptions& o);

@@ -17,8 +17,6 @@
 namespace tsg {
 namespace dev {
 
-constexpr int kMaxRing = 96;
-
 // Geometry_3d::Table::tetCenter, geometry_3d.cpp:830-870: double LU solve
 // relative to vertex 3; the float result is (float)(x3 + rhs)
 __global__ void k_circumcentres(int m, const float* __restrict__ xyz, const int* __restrict__ tv,
@@ -74,15 +72,35 @@ __device__ __forceinline__ bool local_slots(const int* __restrict__ tv, int t, i
 	return true;
 }
 
+// The ring of tets around the edge (n1, n2), walked without storing it:
+// `front` is its first tet, `len` the number of tets in ring order (a
+// closed ring repeats its start at the end, as the reference's list does),
+// and the ring is re-walked from `front` by ring_next.  The order is the
+// reference's: from the start tet forward (the neighbour opposite slot
+// li[2]) until the ring closes or opens; an open ring then extends backwards
+// from the start (opposite li[3]) to its front.  Any number of tets: the
+// fans of the boundary-plane nodes under a wide tip base exceed any fixed
+// buffer.
 struct Ring {
-	int tets[kMaxRing];
+	int front;
 	int len;
 	bool closed;
 	int front_open_vertex;  // vertex defining d1 (open rings)
 	int back_open_vertex;   // vertex defining d2
+	int n1, n2;
 };
 
-// returns false on a walk the ring buffer cannot hold or a broken mesh
+constexpr int kMaxRingWalk = 1 << 20;  // bounds a walk on a broken mesh
+
+// the tet after t in ring order (-1 at an open end, -2 on a broken mesh)
+__device__ __forceinline__ int ring_next(const int* __restrict__ tv, const int* __restrict__ tn, const Ring& r, int t)
+{
+	int li[4];
+	if (!local_slots(tv, t, r.n1, r.n2, li)) return -2;
+	return tn[4 * (size_t) t + li[2]];
+}
+
+// returns false on a broken mesh
 __device__ bool build_ring(const int* __restrict__ tv, const int* __restrict__ tn, const int* __restrict__ assoc,
                            const int* __restrict__ nt_ptr, const int* __restrict__ nt_idx, int n1, int n2, Ring& r)
 {
@@ -94,47 +112,48 @@ __device__ bool build_ring(const int* __restrict__ tv, const int* __restrict__ t
 			if (local_slots(tv, nt_idx[k], n1, n2, li)) { start = nt_idx[k]; break; }
 		if (start < 0) return false;
 	}
-	r.len = 0;
-	r.tets[r.len++] = start;
+	r.n1 = n1;
+	r.n2 = n2;
+	r.front = start;
+	r.len = 1;
 	bool finite = true;
 	{
 		const int nb = tn[4 * (size_t) start + li[2]];
-		if (nb != -1) r.tets[r.len++] = nb;
-		else { r.back_open_vertex = tv[4 * (size_t) start + li[3]]; finite = false; }
-	}
-	if (finite) {
-		while (r.tets[0] != r.tets[r.len - 1]) {
-			const int t = r.tets[r.len - 1];
-			if (!local_slots(tv, t, n1, n2, li)) return false;
-			const int nb = tn[4 * (size_t) t + li[2]];
-			if (nb != -1) {
-				if (r.len >= kMaxRing) return false;
-				r.tets[r.len++] = nb;
-			} else {
-				r.back_open_vertex = tv[4 * (size_t) t + li[3]];
-				finite = false;
-				break;
+		if (nb != -1) {
+			r.len++;
+			int t = nb;
+			while (t != start) {
+				if (!local_slots(tv, t, n1, n2, li) || r.len > kMaxRingWalk) return false;
+				const int nx = tn[4 * (size_t) t + li[2]];
+				if (nx == -1) {
+					r.back_open_vertex = tv[4 * (size_t) t + li[3]];
+					finite = false;
+					break;
+				}
+				r.len++;
+				t = nx;
 			}
+		} else {
+			r.back_open_vertex = tv[4 * (size_t) start + li[3]];
+			finite = false;
 		}
 	}
 	if (!finite) {
-		// walk backwards from the front; prepend by shifting
+		// extend backwards from the start to the front
+		int t = start;
 		for (;;) {
-			const int t = r.tets[0];
-			if (!local_slots(tv, t, n1, n2, li)) return false;
+			if (!local_slots(tv, t, n1, n2, li) || r.len > kMaxRingWalk) return false;
 			const int nb = tn[4 * (size_t) t + li[3]];
-			if (nb != -1) {
-				if (r.len >= kMaxRing) return false;
-				for (int q = r.len; q > 0; q--) r.tets[q] = r.tets[q - 1];
-				r.tets[0] = nb;
-				r.len++;
-			} else {
+			if (nb == -1) {
 				r.front_open_vertex = tv[4 * (size_t) t + li[2]];
+				r.front = t;
 				break;
 			}
+			r.len++;
+			t = nb;
 		}
 	}
-	r.closed = r.len > 1 && r.tets[0] == r.tets[r.len - 1];
+	r.closed = finite && r.len > 1;
 	return true;
 }
 
@@ -147,12 +166,14 @@ __device__ __forceinline__ V3<R> centre_of(const double* cc64, const float* cc32
 		return v3<R>(cc64[3 * (size_t) t], cc64[3 * (size_t) t + 1], cc64[3 * (size_t) t + 2]);
 }
 
-// VoronoiFace::finiteArea(inf) on a ring (inf > 0 or a closed ring)
+// VoronoiFace::finiteArea(inf) on a ring (inf > 0 or a closed ring): the
+// centre of the ring's circumcentres (with the two closing points of an
+// open ring), then the fan of triangles about it, both in ring order
 template <typename R>
-__device__ R ring_area(const Ring& r, const float* __restrict__ xyz, const double* cc64, const float* cc32,
-                       int n1, int n2, R inf)
+__device__ R ring_area(const Ring& r, const float* __restrict__ xyz, const int* __restrict__ tv,
+                       const int* __restrict__ tn, const double* cc64, const float* cc32, R inf)
 {
-	const V3<R> p1 = load_coord<R>(xyz, n1), p2 = load_coord<R>(xyz, n2);
+	const V3<R> p1 = load_coord<R>(xyz, r.n1), p2 = load_coord<R>(xyz, r.n2);
 	V3<R> d1 = v3<R>(0, 0, 0), d2 = v3<R>(0, 0, 0);
 	if (!r.closed) {
 		const V3<R> qb = load_coord<R>(xyz, r.back_open_vertex);
@@ -163,10 +184,15 @@ __device__ R ring_area(const Ring& r, const float* __restrict__ xyz, const doubl
 		if (vlen(d1) > (R) 0) { const R l = vlen(d1); for (int k = 0; k < 3; k++) d1.x[k] /= l; }
 	}
 	V3<R> c = v3<R>(0, 0, 0);
-	for (int k = 0; k < r.len; k++) c = vadd(c, centre_of<R>(cc64, cc32, r.tets[k]));
+	int t = r.front, last = r.front;
+	for (int k = 0; k < r.len; k++) {
+		c = vadd(c, centre_of<R>(cc64, cc32, t));
+		last = t;
+		if (k + 1 < r.len) t = ring_next(tv, tn, r, t);
+	}
 	int cnt = r.len;
-	const V3<R> vf = centre_of<R>(cc64, cc32, r.tets[0]);
-	const V3<R> vb = centre_of<R>(cc64, cc32, r.tets[r.len - 1]);
+	const V3<R> vf = centre_of<R>(cc64, cc32, r.front);
+	const V3<R> vb = centre_of<R>(cc64, cc32, last);
 	if (!r.closed) {
 		c = vadd(c, vf);
 		c = vadd(c, vscale(inf, d1));
@@ -177,8 +203,10 @@ __device__ R ring_area(const Ring& r, const float* __restrict__ xyz, const doubl
 	for (int k = 0; k < 3; k++) c.x[k] /= (R) cnt;
 	R value = (R) 0;
 	V3<R> cur = vf;
+	t = r.front;
 	for (int k = 0; k + 1 < r.len; k++) {
-		const V3<R> nxt = centre_of<R>(cc64, cc32, r.tets[k + 1]);
+		t = ring_next(tv, tn, r, t);
+		const V3<R> nxt = centre_of<R>(cc64, cc32, t);
 		value += vlen(vcross(vsub(cur, c), vsub(nxt, c))) / (R) 2.0;
 		cur = nxt;
 	}
@@ -225,9 +253,9 @@ __global__ void k_edge_areas(int n, long long nnz, const float* __restrict__ xyz
 	// computeCoupling: distance = 0.5 * dist (double product stored to float)
 	const float half32 = (float) (0.5 * (double) d32);
 	const double half64 = 0.5 * d64;
-	a_half32[k] = ring_area<float>(r, xyz, cc64, cc32, i, j, half32);
-	a_half64[k] = ring_area<double>(r, xyz, cc64, cc32, i, j, half64);
-	a_one32[k] = ring_area<float>(r, xyz, cc64, cc32, i, j, 1.0f);
+	a_half32[k] = ring_area<float>(r, xyz, tv, tn, cc64, cc32, half32);
+	a_half64[k] = ring_area<double>(r, xyz, tv, tn, cc64, cc32, half64);
+	a_one32[k] = ring_area<float>(r, xyz, tv, tn, cc64, cc32, 1.0f);
 }
 
 // Geometry_3d::Table::voronoiVolume(i, inf) with computeCharge's closing
@@ -271,11 +299,11 @@ __global__ void k_cell_volumes(int n, const float* __restrict__ xyz, const int* 
 			vol64[i] = 0.0;
 			return;
 		}
-		float t32 = ring_area<float>(r, xyz, cc64, cc32, i, idx[k], m32);
+		float t32 = ring_area<float>(r, xyz, tv, tn, cc64, cc32, m32);
 		t32 *= dist32[k];
 		t32 /= 3.0f * 2.0f;
 		v32 += t32;
-		double t64 = ring_area<double>(r, xyz, cc64, cc32, i, idx[k], m64);
+		double t64 = ring_area<double>(r, xyz, tv, tn, cc64, cc32, m64);
 		t64 *= dist64[k];
 		t64 /= 3.0 * 2.0;
 		v64 += t64;

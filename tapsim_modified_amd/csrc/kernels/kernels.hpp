@@ -266,6 +266,8 @@ struct TrajArgs {
 	const float* v0;          // optional start velocities, 3 per ion (m/s); at rest when null
 	int* rng;                 // optional rand() stream (31-word table + index, see host/prand.hpp):
 	                          // the walks draw their first face from it; count must be 1
+	float* err_out;           // optional, 6 floats per ion: the final error estimate
+	                          // (Trajectory_3d::error_estimate: position xyz, velocity xyz)
 };
 // One event's flight (steps V-VI of Process::evaporation, process.cpp:940-1037)
 // in one thread, on the process's rand() stream: the thermal start velocity

@@ -390,7 +390,24 @@ struct Tr {
 	bool geom_error;       // the work limit ended the integration
 	long long calls, max_calls;
 	DevRand* rng;          // the process's rand() stream, or null
+	float ep[3], ev[3];    // the cumulative error estimate (position, velocity)
 };
+
+// Trajectory_3d::cumulative_error (trajectory_3d.cpp:680-696): per component
+// sqrt(e^2 + step^2), the squares as the reference's powf(x, 2.0f)
+__device__ __forceinline__ void cumulate_error(Tr& tr, const Phase& e)
+{
+	for (int i = 0; i < 3; i++) {
+		float a = tr.ep[i] * tr.ep[i];
+		float b = e.p[i] * e.p[i];
+		a += b;
+		tr.ep[i] = sqrtf(a);
+		a = tr.ev[i] * tr.ev[i];
+		b = e.v[i] * e.v[i];
+		a += b;
+		tr.ev[i] = sqrtf(a);
+	}
+}
 
 // Trajectory_3d::o5_integrator (trajectory_3d.cpp:583-676): Cash-Karp
 // embedded Runge-Kutta 4(5), tableau of Cash & Karp (1990)
@@ -665,6 +682,7 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 	tr.calls = 0;
 	tr.max_calls = a.max_calls;
 	tr.rng = rng;
+	for (int k = 0; k < 3; k++) tr.ep[k] = tr.ev[k] = 0.0f;
 	const int node = a.nodes[ion];
 	Phase back;
 	back.t = 0.0f;
@@ -677,6 +695,8 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 	int len = 0;
 	if (a.summary)
 		for (int c = 0; c < 12; c++) a.summary[12 * (size_t) ion + c] = 0.0f;
+	if (a.err_out)
+		for (int c = 0; c < 6; c++) a.err_out[6 * (size_t) ion + c] = 0.0f;
 	if (back.tet < 0) {
 		a.len[ion] = 0;
 		a.status[ion] = -10;  // initial point outside the mesh
@@ -695,6 +715,7 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 			if (tr.geom_error) break;
 			back = next;
 			put(a, ion, len++, back);
+			cumulate_error(tr, error);
 			if (back.tet < 0) {
 				status = 5;  // SYSTEM_LIMIT: the ion left the mesh
 				break;
@@ -708,6 +729,11 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 	if (tr.geom_error) {
 		a.len[ion] = len;
 		a.status[ion] = -4;  // work limit
+		if (a.err_out)
+			for (int c = 0; c < 3; c++) {
+				a.err_out[6 * (size_t) ion + c] = tr.ep[c];
+				a.err_out[6 * (size_t) ion + 3 + c] = tr.ev[c];
+			}
 		return;
 	}
 	// extrapolate(Z, zmax)
@@ -723,9 +749,21 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 		for (int k = 0; k < 3; k++) item.v[k] = back.v[k];
 		item.tet = find_tet(tr.g, item.p, back.tet, rng);
 		put(a, ion, len++, item);
+		// the error estimate follows the extrapolation (trajectory_3d.cpp:327)
+		if (dt > 0.0f)
+			for (int k = 0; k < 3; k++) {
+				float d = dt;
+				d *= tr.ev[k];
+				tr.ep[k] += d;
+			}
 	}
 	a.len[ion] = len;
 	a.status[ion] = status;
+	if (a.err_out)
+		for (int c = 0; c < 3; c++) {
+			a.err_out[6 * (size_t) ion + c] = tr.ep[c];
+			a.err_out[6 * (size_t) ion + 3 + c] = tr.ev[c];
+		}
 }
 
 __global__ __launch_bounds__(64) void k_trajectory(TrajArgs a)

@@ -380,6 +380,95 @@ def write_results(path, records, mode="ASCII", header=""):
         f.write(text.encode("latin-1") + body)
 
 
+# ---------------------------------------------------------------- trajectory stream
+
+TRAJECTORY_VERSION = "1.0"
+_TRAJ_COLUMNS = (("time", "[s]", "float"), ("positionX", "[m]", "float"), ("positionY", "[m]", "float"),
+                 ("positionZ", "[m]", "float"), ("velocityX", "[m/s]", "float"), ("velocityY", "[m/s]", "float"),
+                 ("velocityZ", "[m/s]", "float"), ("tetrahedron", "[1]", "int"))
+# Trajectory_3d::status_str (trajectory_3d.cpp:64-83)
+TRAJECTORY_STATUS = {0: "NO_INTEGRATION", 1: "NO_FIELD", 2: "STEPPER_LIMIT_PASSED", 3: "ITERATION_LIMIT_PASSED",
+                     4: "GEOMETRY_LIMIT_PASSED", 5: "SYSTEM_LIMIT_PASSED"}
+_E_CHARGE = np.float32(1.60217733e-19)
+_AMU = np.float32(1.660538782e-27)
+
+
+def trajectory_datasets(events, phases, numbers=None):
+    """Datasets of the trajectory stream from tsg_evaporate_traj's events and
+    phases: (event index, unique number, time scale, status, charge C, mass
+    kg, phases (k, 8), error estimate (6)) per event, the charge and mass in
+    float as the event's flight had them (process.cpp:930-935)."""
+    out = []
+    for e, ph in zip(events, phases):
+        charge = np.float32(np.float32(e.ion_charge) * _E_CHARGE)
+        mass = np.float32(np.float32(e.ion_mass) * _AMU)
+        number = int(numbers[e.node]) if numbers is not None else e.node + 1
+        err = np.array(list(e.err_pos) + list(e.err_vel), np.float32)
+        out.append((int(e.event_index), number, np.float32(e.time_scale), int(e.traj_status), charge, mass,
+                    np.asarray(ph, np.float32), err))
+    return out
+
+
+def _trajectory_header(mode):
+    text = "TAPSIM TRAJECTORY DATA\nVERSION " + TRAJECTORY_VERSION + "\n"
+    if mode == "BINARY":
+        text += "".join("# %s %s (%s)\n" % col for col in _TRAJ_COLUMNS)
+    elif mode == "ASCII":
+        text += "".join("# %s %s\n" % col[:2] for col in _TRAJ_COLUMNS)
+    else:
+        raise FormatError("File_Io::LogTrajectory::header(): Invalid output mode!")
+    return (text + "\n").encode("latin-1")
+
+
+def _trajectory_block(d, mode):
+    index, number, scale, status, charge, mass, ph, err = d
+    scale = np.float32(scale)
+    text = "# index = %d\n# unique number = %d\n# time scale = %e\n" % (index, number, float(scale))
+    text += "# integrator status = %d => %s\n" % (status, TRAJECTORY_STATUS.get(status, "INVALID"))
+    text += "# charge [C] / mass [kg] = %e / %e\n" % (float(charge), float(mass))
+    ph = np.asarray(ph, np.float32).reshape(-1, 8)
+    t = (ph[:, 0] * scale).astype(np.float32)  # time() * timeScale in float
+    if mode == "BINARY":
+        rec = np.zeros(len(ph), np.dtype([("t", "<f4"), ("pv", "<f4", 6), ("tet", "<i4")]))
+        rec["t"] = t
+        rec["pv"] = ph[:, 1:7]
+        rec["tet"] = ph[:, 7].astype(np.int32)
+        body = ("BINARY %d\n" % len(ph)).encode("latin-1") + rec.tobytes() + b"\n"
+    else:
+        rows = ["%e\t%s%d\n" % (float(t[k]), "".join("%e\t" % float(v) for v in ph[k, 1:7]), int(ph[k, 7]))
+                for k in range(len(ph))]
+        body = ("ASCII %d\n" % len(ph) + "".join(rows)).encode("latin-1")
+    err = np.asarray(err, np.float32)
+    tail = "# final error estimate (position) = (%e/%e/%e)\n" % tuple(float(v) for v in err[:3])
+    tail += "# final error estimate (velocity) = (%e/%e/%e)\n\n" % tuple(float(v) for v in err[3:])
+    return text.encode("latin-1") + body + tail.encode("latin-1")
+
+
+def write_trajectories(path, datasets, mode="ASCII", chunk=0):
+    """File_Io::LogTrajectory (logTrajectory.cpp:137-299): the header, then
+    per event its index, unique number, time scale, integrator status,
+    charge / mass, its phases (time scaled by the time scale; tab-separated
+    scientific text or packed 32-byte records) and the final error estimate.
+    chunk > 0: a new file path.%08u (the event index) whenever the index is a
+    multiple of chunk, and at the first event.  Returns the files written."""
+    files, cur, name = [], None, None
+    try:
+        for d in datasets:
+            index = int(d[0])
+            if cur is None or (chunk > 0 and index % chunk == 0):
+                if cur is not None:
+                    cur.close()
+                name = path if chunk == 0 else "%s.%08u" % (path, index)
+                cur = open(name, "wb")
+                files.append(name)
+                cur.write(_trajectory_header(mode))
+            cur.write(_trajectory_block(d, mode))
+    finally:
+        if cur is not None:
+            cur.close()
+    return files
+
+
 # ---------------------------------------------------------------- tapsim.ini
 
 def read_ini(path, delimiter=" = "):

@@ -325,6 +325,27 @@ def make_formats():
         out["res_in"] = rec
         for k in ("res_ascii.txt", "res_bin.bin"):
             out[k.replace(".", "_")] = np.frombuffer(open(os.path.join(wd, k), "rb").read(), np.uint8)
+        # the reference's trajectory stream (File_Io::LogTrajectory) of 10
+        # synthetic flights, one file in each mode and ASCII in chunks of 4
+        trng = np.random.default_rng(23)
+        flat = []
+        for e in range(10):
+            P = int(trng.integers(1, 7))
+            head = [e + 1, int(trng.integers(1, 100000)), float(trng.random() * 1e-3 + 1e-6),
+                    [5, 1, 3, -1, 0, 5, 2, 4, 5, 6][e], 3.2043547e-19, 9.27e-26] + [P]
+            head += list((trng.standard_normal(6) * 10.0 ** trng.integers(-14, -3, 6)))
+            ph = (trng.standard_normal((P, 8)) * 10.0 ** trng.integers(-12, 5, (P, 8))).astype(np.float32)
+            ph[:, 0] = np.abs(ph[:, 0])
+            ph[:, 7] = trng.integers(-1, 2000000, P)
+            flat += head + list(ph.ravel())
+        traj = np.asarray(flat, np.float32)
+        traj.tofile(os.path.join(wd, "traj_in.f32"))
+        run(wd, ["writetraj:traj_in.f32:traj_ascii.txt:0:0", "writetraj:traj_in.f32:traj_bin.bin:1:0",
+                 "writetraj:traj_in.f32:traj_chunk.txt:0:4"])
+        out["traj_in"] = traj
+        for k in ("traj_ascii.txt", "traj_bin.bin", "traj_chunk.txt.00000001", "traj_chunk.txt.00000004",
+                  "traj_chunk.txt.00000008"):
+            out[k.replace(".", "_")] = np.frombuffer(open(os.path.join(wd, k), "rb").read(), np.uint8)
     out["xyz"], out["ids"] = xyz, ids
     np.savez_compressed(os.path.join(OUT, "formats.npz"), **out)
     print("formats: %d nodes, %d bytes of reference output" % (n, sum(v.nbytes for k, v in out.items() if k.startswith(("w_", "x_")))))
@@ -390,10 +411,13 @@ def make_pinning():
     with tempfile.TemporaryDirectory() as wd:
         fresh(wd)
         run(wd, ["mesh", "sync", "relax:300:1", "evapnode:%d" % node, "relax:1:1", "traj:o1:%d:0:1" % node,
-                 "traj:s0:%d:1:0" % node, "traj:s2:%d:1:2" % node, "traj:s3:%d:1:3" % node])
-        for tag in ("o1", "s0", "s2", "s3"):
+                 "traj:s0:%d:1:0" % node, "traj:s2:%d:1:2" % node, "traj:s3:%d:1:3" % node,
+                 "traj:o5:%d:1:1" % node])
+        for tag in ("o1", "s0", "s2", "s3", "o5"):
             fx["ref_traj_" + tag] = rd(wd, "traj_%s.f32" % tag, np.float32).reshape(-1, 8)
             fx["ref_traj_status_" + tag] = rd(wd, "traj_status_%s.i32" % tag, np.int32)
+            # Trajectory_3d::error_estimate after the extrapolation: position xyz, velocity xyz
+            fx["ref_traj_err_" + tag] = rd(wd, "traj_err_%s.f32" % tag, np.float32)
     np.savez_compressed(os.path.join(OUT, "pinning.npz"), **fx)
     print("pinning: %s" % ", ".join("%s%s" % (k, list(v.shape)) for k, v in sorted(fx.items())))
 

@@ -154,3 +154,39 @@ def test_ini_quirks(tmp_path):
     p.write_text("A = 1\nA = 2\n")
     with pytest.raises(F.FormatError):
         F.read_ini(str(p))
+
+
+def _trajectory_datasets(flat):
+    """the datasets of the trajectory-stream fixture's input (the layout the
+    reference driver's writetraj step reads)"""
+    out, at = [], 0
+    while at + 13 <= len(flat):
+        h = flat[at:at + 13]
+        P = int(h[6])
+        ph = flat[at + 13:at + 13 + 8 * P].reshape(P, 8)
+        out.append((int(h[0]), int(h[1]), np.float32(h[2]), int(h[3]), np.float32(h[4]), np.float32(h[5]), ph,
+                    h[7:13]))
+        at += 13 + 8 * P
+    return out
+
+
+@pytest.mark.parametrize("mode,name", [("ASCII", "traj_ascii_txt"), ("BINARY", "traj_bin_bin")])
+def test_trajectory_stream_written_as_the_reference_writes_it(fx, tmp_path, mode, name):
+    """File_Io::LogTrajectory's files of ten flights (statuses of every kind,
+    the time scaled in float), byte for byte"""
+    ds = _trajectory_datasets(fx["traj_in"])
+    assert len(ds) == 10
+    files = F.write_trajectories(str(tmp_path / "t"), ds, mode)
+    assert files == [str(tmp_path / "t")]
+    assert open(files[0], "rb").read() == bytes(fx[name])
+
+
+def test_trajectory_stream_in_chunks(fx, tmp_path):
+    """chunks of 4 events: a new file path.%08u at the first event and at
+    every event index divisible by 4"""
+    ds = _trajectory_datasets(fx["traj_in"])
+    files = F.write_trajectories(str(tmp_path / "c"), ds, "ASCII", chunk=4)
+    assert [os.path.basename(f) for f in files] == ["c.00000001", "c.00000004", "c.00000008"]
+    for f in files:
+        key = "traj_chunk_txt_" + f.rsplit(".", 1)[1]
+        assert open(f, "rb").read() == bytes(fx[key])

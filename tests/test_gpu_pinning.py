@@ -180,12 +180,14 @@ def test_field_o2(tip_small, mode, omode, dt):
         assert same_bits(qg, tip_small["ref_charge2_r6"], np.float32)
 
 
-@pytest.mark.parametrize("tag,integrator,stepper", [("o1", 0, 1), ("s0", 1, 0), ("s2", 1, 2), ("s3", 1, 3)])
+@pytest.mark.parametrize("tag,integrator,stepper", [("o1", 0, 1), ("s0", 1, 0), ("s2", 1, 2), ("s3", 1, 3),
+                                                    ("o5", 1, 1)])
 def test_trajectory_integrators_and_steppers(tip_small, pin, tag, integrator, stepper):
     """one flight per integrator / stepper from the apex atom after its
-    removal: every phase equals the oracle with the device's correctly rounded
-    step-control root, and the reference where the oracle's two root modes
-    agree (the none and geometry-restricted steppers take no root)"""
+    removal: every phase and the final error estimate equal the oracle with
+    the device's correctly rounded step-control root, and the reference where
+    the oracle's two root modes agree (the none and geometry-restricted
+    steppers take no root)"""
     node = int(tip_small["local_node"])
     log("trajectory %s from node %d" % (tag, node))
     g = grid(tip_small, tsg.FAITHFUL_F32)
@@ -198,18 +200,20 @@ def test_trajectory_integrators_and_steppers(tip_small, pin, tag, integrator, st
     o.set_id(node, 0)
     o.resync(node)
     o.relax(1)
-    (st, tr), = g.trajectories([node], Q, M, integrator=integrator, stepper=stepper, cap=8192)
+    (st, tr, est), = g.trajectories([node], Q, M, integrator=integrator, stepper=stepper, cap=8192, with_error=True)
     try:
         O.set_root_mode(1)
-        sto, tro = o.trajectory(node, Q, M, integrator=integrator, stepper=stepper)
+        sto, tro, esto = o.trajectory(node, Q, M, integrator=integrator, stepper=stepper, with_error=True)
     finally:
         O.set_root_mode(0)
-    st0, tr0 = o.trajectory(node, Q, M, integrator=integrator, stepper=stepper)
+    st0, tr0, est0 = o.trajectory(node, Q, M, integrator=integrator, stepper=stepper, with_error=True)
     assert st == sto and same_bits(tr, tro, np.float32)
+    assert same_bits(est, esto, np.float32)
     ref = pin["ref_traj_" + tag]
     assert st == pin["ref_traj_status_" + tag][0]
-    if same_bits(tro, tr0, np.float32):
+    if same_bits(tro, tr0, np.float32) and same_bits(esto, est0, np.float32):
         assert same_bits(tr, ref, np.float32)
+        assert same_bits(est, pin["ref_traj_err_" + tag], np.float32)
     log("%s: status %d, %d phases" % (tag, st, len(tr)))
 
 

@@ -121,8 +121,11 @@ def test_probability_models_in_event_loop(tip_small, pin, prob):
     assert bits_equal([e.scaling_ref for e in ev], F[:, 21])
 
 
-@pytest.mark.parametrize("tag,integrator,stepper", [("o1", 0, 1), ("s0", 1, 0), ("s2", 1, 2), ("s3", 1, 3)])
+@pytest.mark.parametrize("tag,integrator,stepper", [("o1", 0, 1), ("s0", 1, 0), ("s2", 1, 2), ("s3", 1, 3),
+                                                    ("o5", 1, 1)])
 def test_trajectory_integrators_and_steppers(tip_small, pin, tag, integrator, stepper):
+    """every phase, the status and the final error estimate (the root-sum
+    of the steps' estimates, moved along by the extrapolation) of one flight"""
     node = int(tip_small["local_node"])
     o = make(tip_small)
     o.sync()
@@ -134,6 +137,7 @@ def test_trajectory_integrators_and_steppers(tip_small, pin, tag, integrator, st
     o.srand(99)
     q = 2.0 * np.float32(1.60217733e-19)
     m = np.float32(55.845) * np.float32(1.660538782e-27)
-    st, tr = o.trajectory(node, q, m, integrator=integrator, stepper=stepper)
+    st, tr, est = o.trajectory(node, q, m, integrator=integrator, stepper=stepper, with_error=True)
     assert st == pin["ref_traj_status_" + tag][0]
     assert bits_equal(tr, pin["ref_traj_" + tag])
+    assert bits_equal(est, pin["ref_traj_err_" + tag])
