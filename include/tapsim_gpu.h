@@ -348,6 +348,12 @@ int tsg_trajectories(tsg_grid*, int count, const int* nodes, const float* charge
                      int stepper, float dt0, float dt_min, float err_limit, int cap, float* out, int* len, int* status,
                      float* error_estimate);
 
+/* Geometry_3d::Table's extents (geometry_3d.cpp:1555-1610), as the dump of
+   File_Io::writeSystem holds them: out[0..2] min, [3..5] max of the node
+   coordinates, [6..8] and [9..11] the same widened by the first and last
+   vertices of every open Voronoi face (setConstrainedExtents) */
+int tsg_grid_extents(tsg_grid*, float* out);
+
 /* The ion table of a species for the flights of tsg_evaporate (no reference
  * counterpart; the reference flies every ion of a species with the species'
  * mass and charge, process.cpp:930-935): up to 8 entries of mass (amu),

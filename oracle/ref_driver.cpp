@@ -33,6 +33,7 @@
  *   traj:TAG:NODE:INT:STEP  one Trajectory_3d from a node (trajectory_3d.cpp:99); its phases,
  *                        status and final error estimate
  *   writetraj:IN:OUT:MODE:CHUNK  File_Io::LogTrajectory of the datasets in IN
+ *   writedump:NAME:INDEX  File_Io::writeSystem (configuration, geometry, grid)
  *   randreset:SEED       std::srand(SEED), then Grid_3d::Table::randomReset (grid_3d.cpp:811)
  *   threads:T            Grid_3d::Table::setThreadNum(T)  (grid_3d.cpp:194; the driver starts with 1)
  *
@@ -647,6 +648,13 @@ int main(int argc, char** argv)
 					// it time before the destructor stops it
 					usleep(300000);
 				}
+			}
+			else if (cmd == "writedump")
+			{
+				// writedump:NAME:INDEX -- File_Io::writeSystem of the current
+				// system into NAME in the work directory
+				File_Io::writeSystem((g_out + "/" + tok.at(1)).c_str(), system,
+				                     (unsigned int) std::atoi(tok.at(2).c_str()));
 			}
 			else if (cmd == "writeconfig")
 			{

@@ -136,6 +136,7 @@ _SIGS = {
     "tsg_field_o1": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_field_o2": ([_P, ctypes.c_int, _P, _P, _P], ctypes.c_int),
     "tsg_set_ion_table": ([_P, ctypes.c_int, ctypes.c_int, _P, _P, _P], ctypes.c_int),
+    "tsg_grid_extents": ([_P, _P], ctypes.c_int),
     "tsg_evaporate_traj": ([_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_field_o1_noalloc": ([_P, ctypes.c_int, _P, _P], ctypes.c_int),
     "tsg_get_couplings": ([_P, _P], ctypes.c_int),
@@ -571,6 +572,12 @@ class Grid3D:
         record training graphs, method 2 uses the prediction."""
         self._net = net
         _chk(load().tsg_grid_attach_mpnn(self._h, net.handle if net is not None else None, int(record), int(predict)))
+
+    def extents(self):
+        """Geometry_3d::Table min, max, constrained min, constrained max (4 x 3 floats)"""
+        out = np.zeros(12, np.float32)
+        _chk(load().tsg_grid_extents(self._h, _ptr(out)))
+        return out.reshape(4, 3)
 
     def set_ion_table(self, species, mass, charge, weight):
         """the species' ion table for the flights (mass amu, charge e, weight

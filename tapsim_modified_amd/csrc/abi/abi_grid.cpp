@@ -293,6 +293,13 @@ int tsg_evaporate_traj(tsg_grid* h, const tsg_evap_options* o, int k, tsg_event*
 	                                phase_len));
 }
 
+int tsg_grid_extents(tsg_grid* h, float* out)
+{
+	if (!h || !out) return tsg::fail(TSG_E_ARG, "null argument");
+	for (int k = 0; k < 12; k++) out[k] = h->g->extents()[k];
+	return 0;
+}
+
 int tsg_set_ion_table(tsg_grid* h, int species, int count, const float* mass_amu, const int* charge,
                       const float* weight)
 {

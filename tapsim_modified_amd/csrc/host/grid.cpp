@@ -3,6 +3,7 @@
 #include "prand.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -124,8 +125,19 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 	    hip_check(hipMemcpy(d_rowof.ptr(), rowof32.data(), d_rowof.bytes(), hipMemcpyHostToDevice), "upload row_of") ||
 	    hip_check(hipMemsetAsync(d_status_.ptr(), 0, d_status_.bytes(), stream_), "clear status"))
 		return;
+	// the extents of Geometry_3d::Table (setExtents, setConstrainedExtents;
+	// geometry_3d.cpp:1555-1610): the coordinates' bounding box, widened by
+	// the circumcentres that open Voronoi faces begin and end with
+	DevBuf d_ext;
+	d_ext.alloc(sizeof(int) * 6);
+	const int ext0[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+	if (!d_ext.ptr() || hip_check(hipMemcpy(d_ext.ptr(), ext0, sizeof(ext0), hipMemcpyHostToDevice), "extents")) {
+		fail(-2, "device allocation failed (extents)");
+		return;
+	}
 	if (launch_circumcentres(stream_, m, d_xyz_.as<float>(), d_tv.as<int>(), d_cc64.as<double>(), d_cc32.as<float>(),
 	                         d_status_.as<int>()) ||
+	    launch_hull_extents(stream_, m, d_tn.as<int>(), d_cc32.as<float>(), d_ext.as<int>()) ||
 	    launch_edge_areas(stream_, n, nnz, d_xyz_.as<float>(), d_tv.as<int>(), d_tn.as<int>(), d_assoc.as<int>(),
 	                      d_ntp.as<int>(), d_nti.as<int>(), d_ptr_.as<int>(), d_idx_.as<int>(), d_rowof.as<int>(),
 	                      d_cc64.as<double>(), d_cc32.as<float>(), d_ah32.as<float>(), d_ah64.as<double>(),
@@ -169,6 +181,28 @@ Grid::Grid(int mode, const Mesh& mesh, const int16_t* ids, const Species* specie
 		if (hip_check(hipMemcpy(cell_vol32_.data(), d_v32.ptr(), d_v32.bytes(), hipMemcpyDeviceToHost), "volumes") ||
 		    hip_check(hipMemcpy(cell_vol64_.data(), d_v64.ptr(), d_v64.bytes(), hipMemcpyDeviceToHost), "volumes"))
 			return;
+	}
+	{
+		int ext[6];
+		if (hip_check(hipMemcpy(ext, d_ext.ptr(), sizeof(ext), hipMemcpyDeviceToHost), "extents")) return;
+		for (int k = 0; k < 3; k++) {
+			float lo = mesh_.xyz[k], hi = mesh_.xyz[k];
+			for (int i = 1; i < n; i++) {
+				const float v = mesh_.xyz[3 * (size_t) i + k];
+				if (v < lo) lo = v;
+				else if (v > hi) hi = v;
+			}
+			extents_[k] = lo;
+			extents_[3 + k] = hi;
+			float clo = lo, chi = hi;
+			if (ext[k] != INT_MAX) {
+				const float a = float_of_key(ext[k]), b = float_of_key(ext[3 + k]);
+				if (a < clo) clo = a;
+				if (b > chi) chi = b;
+			}
+			extents_[6 + k] = clo;
+			extents_[9 + k] = chi;
+		}
 	}
 	if (gst & 1) { fail(-4, "degenerate tetrahedron (circumcentre)"); return; }
 	if (gst & 2) { fail(-4, "Voronoi face walk failed"); return; }

@@ -164,6 +164,8 @@ public:
 	int get_charges(double* out) const;
 	// the fork's update_coordination (surface_3d.cpp:568-593) on the ids
 	int update_coordination();
+	// Geometry_3d::Table::min() / max() and the constrained extents
+	const float* extents() const { return extents_; }
 	// the ion table of a species for the flights (no reference counterpart)
 	int set_ion_table(int species, int count, const float* mass, const int* charge, const float* weight);
 	// the node's id names a defined species (Configuration::Table::operator[]
@@ -437,6 +439,7 @@ private:
 	// trajectories: mesh arrays for the point location, node-field snapshot
 	DevBuf d_tv_, d_tn_, d_assoc_, d_gbnd_, d_nfield_, d_ppos_, d_traj_in_, d_traj_res_, d_traj_out_, d_rng_;
 	float zmax_ = 0.0f;
+	float extents_[12] = {};  // Geometry_3d::Table min, max, constrained min, constrained max
 	int ensure_point_location();
 	// Process::evaporation's voltage queue (process.cpp:983-1037) and initial apex
 	std::vector<float> vq_;

@@ -11,6 +11,8 @@
 // (geometry_3d.cpp:1211-1382) and VoronoiFace::finiteCenter/finiteArea
 // (:244-360) including the start tet, the walk direction given by the
 // permutation parity, and the float operation order.
+#include <cstring>
+
 #include "device_math.hpp"
 #include "kernels.hpp"
 
@@ -329,6 +331,46 @@ int launch_circumcentres(hipStream_t s, int m, const float* xyz, const int* tv, 
 {
 	if (m <= 0) return 0;
 	hipLaunchKernelGGL(dev::k_circumcentres, dim3((m + 255) / 256), dim3(256), 0, s, m, xyz, tv, cc64, cc32, status);
+	return (int) hipGetLastError();
+}
+
+namespace dev {
+
+// order-preserving integer key of a float (no NaN): negative values have
+// their magnitude bits flipped
+__device__ __forceinline__ int key_of(float f)
+{
+	const int b = __float_as_int(f);
+	return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+
+__global__ void k_hull_extents(int m, const int* __restrict__ tn, const float* __restrict__ cc32, int* __restrict__ ext)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= m) return;
+	const int* nb = tn + 4 * (size_t) t;
+	if (nb[0] >= 0 && nb[1] >= 0 && nb[2] >= 0 && nb[3] >= 0) return;
+	for (int k = 0; k < 3; k++) {
+		const int key = key_of(cc32[3 * (size_t) t + k]);
+		atomicMin(ext + k, key);
+		atomicMax(ext + 3 + k, key);
+	}
+}
+
+}  // namespace dev
+
+float float_of_key(int k)
+{
+	const int b = k >= 0 ? k : (k ^ 0x7fffffff);
+	float f;
+	std::memcpy(&f, &b, sizeof(f));
+	return f;
+}
+
+int launch_hull_extents(hipStream_t s, int m, const int* tn, const float* cc32, int* ext)
+{
+	if (m <= 0) return 0;
+	hipLaunchKernelGGL(dev::k_hull_extents, dim3((m + 255) / 256), dim3(256), 0, s, m, tn, cc32, ext);
 	return (int) hipGetLastError();
 }
 

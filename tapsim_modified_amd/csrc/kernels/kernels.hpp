@@ -11,6 +11,14 @@ namespace tsg {
 
 // ---------------------------------------------------------------- geometry
 int launch_circumcentres(hipStream_t, int m, const float* xyz, const int* tv, double* cc64, float* cc32, int* status);
+// Geometry_3d::Table::setConstrainedExtents (geometry_3d.cpp:1575-1610): the
+// float circumcentres of the tets with a hull face are the first and last
+// vertices of the open Voronoi faces; their per-axis min and max go to
+// ext[0..2] and ext[3..5] as order-preserving integer keys (set ext to
+// INT_MAX x3, INT_MIN x3 before)
+int launch_hull_extents(hipStream_t, int m, const int* tn, const float* cc32, int* ext);
+// the float of an order-preserving key
+float float_of_key(int k);
 // per node: the Voronoi cell volume closed at half the mean neighbour
 // distance (computeCharge), float and double; needs dist32/dist64 of
 // launch_edge_areas

@@ -469,6 +469,184 @@ def write_trajectories(path, datasets, mode="ASCII", chunk=0):
     return files
 
 
+# ---------------------------------------------------------------- dump
+
+DUMP_VERSION = "1.0"  # Configuration, Geometry_3d and Grid_3d tables' binary version
+GRID_INFINITY_FACTOR = np.float32(1e2)  # Grid_3d::Table::_infinityFactor (grid_3d.h:287)
+
+
+def write_dump(path, last_index, temperature, species, xyz, boundary, assoc, tets, tet_neighbours, extents, ids,
+               phi0, phi1, phi_switch, charge, flags, ptr, idx, couplings, numbers=None):
+    """File_Io::writeSystem (file_io.cpp:1013-1025): "TAPSIM DUMP INFORMATION",
+    "LAST_INDEX <index>", then the binary blocks of the configuration
+    (configuration.cpp:162-212), the geometry (geometry_3d.cpp:639-689) and
+    the grid (grid_3d.cpp:333-389).  Two fields the reference writes from
+    memory it does not own are written as zeros: the two bytes after each
+    species' evaporation charge (a short written as a float) and the four
+    bytes of the uninitialised relaxation factor (a float written as a double
+    whose second half is the potential switch).  species: {id: dict(...)} with
+    the fields of read_config; extents: 4 x 3 floats (min, max, constrained
+    min, constrained max); flags: bit 1 boundary, 2 Dirichlet, 4 Neumann;
+    potentials, charges and couplings are stored as float."""
+    out = [b"TAPSIM DUMP INFORMATION\n", ("LAST_INDEX %d\n" % int(last_index)).encode("latin-1")]
+    # configuration
+    out.append(DUMP_VERSION.encode("latin-1") + b"\0")
+    out.append(struct.pack("<fi", float(np.float32(temperature)), len(species)))
+    for sid in sorted(species, key=lambda s: np.int8(s)):
+        s = species[sid]
+        out.append(struct.pack("<b", int(sid)) + str(s.get("name", "")).encode("latin-1") + b"\0")
+        out.append(struct.pack("<ff???ffhxxff", s.get("charge_density", 0.0), s.get("epsilon", 1.0),
+                               bool(s.get("removable", 0)), bool(s.get("neumann", 0)), bool(s.get("dirichlet", 0)),
+                               s.get("phi", 0.0), s.get("mass", 0.0), int(s.get("evap_charge", 0)),
+                               s.get("evap_field", 0.0), s.get("evap_energy", 0.0)))
+    # geometry
+    n = len(xyz)
+    out.append(DUMP_VERSION.encode("latin-1") + b"\0")
+    out.append(np.asarray(extents, np.float32).reshape(12).tobytes())
+    node = np.zeros(n, np.dtype([("p", "<f4", 3), ("b", "u1"), ("t", "<i4")], align=False))
+    node["p"] = np.asarray(xyz, np.float32).reshape(n, 3)
+    node["b"] = np.asarray(boundary, bool)
+    node["t"] = np.asarray(assoc, np.int32)
+    out.append(struct.pack("<I", n) + node.tobytes())
+    tv = np.asarray(tets, np.int32).reshape(-1, 4)
+    tn = np.asarray(tet_neighbours, np.int32).reshape(-1, 4)
+    out.append(struct.pack("<I", len(tv)) + np.concatenate([tv, tn], axis=1).astype("<i4").tobytes())
+    # grid
+    out.append(DUMP_VERSION.encode("latin-1") + b"\0")
+    out.append(struct.pack("<f4xii", float(GRID_INFINITY_FACTOR), int(phi_switch), int(phi_switch)))
+    out.append(struct.pack("<I", n))
+    ptr = np.asarray(ptr, np.int64)
+    idx = np.asarray(idx, np.int32)
+    coup = np.asarray(couplings, np.float32)
+    ids8 = np.asarray(ids).astype(np.int8)
+    numbers = np.arange(1, n + 1, dtype=np.uint32) if numbers is None else np.asarray(numbers, np.uint32)
+    p0 = np.asarray(phi0, np.float32)
+    p1 = np.asarray(phi1, np.float32)
+    q = np.asarray(charge, np.float32)
+    fl = np.asarray(flags, np.uint8)
+    head = np.dtype([("id", "i1"), ("num", "<u4"), ("p0", "<f4"), ("p1", "<f4"), ("q", "<f4"), ("b", "u1"),
+                     ("d", "u1"), ("ne", "u1"), ("k", "<i4")], align=False)
+    pair = np.dtype([("j", "<i4"), ("c", "<f4")], align=False)
+    body = []
+    for i in range(n):
+        h = np.zeros(1, head)
+        h["id"], h["num"], h["p0"], h["p1"], h["q"] = ids8[i], numbers[i], p0[i], p1[i], q[i]
+        h["b"], h["d"], h["ne"] = (fl[i] & 1) != 0, (fl[i] & 2) != 0, (fl[i] & 4) != 0
+        k0, k1 = int(ptr[i]), int(ptr[i + 1])
+        h["k"] = k1 - k0
+        e = np.zeros(k1 - k0, pair)
+        e["j"] = idx[k0:k1]
+        e["c"] = coup[k0:k1]
+        body.append(h.tobytes() + e.tobytes())
+    out.append(b"".join(body))
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+def dump_of_grid(path, grid, mesh, temperature, species, last_index=0, numbers=None):
+    """write_dump of a Grid3D's state (its potentials, couplings and charges
+    stored as float) with its Mesh"""
+    tv, tn = mesh.tets()
+    assoc, bnd = mesh.nodeinfo()
+    ptr, idx = mesh.csr()
+    sw = grid.switches()[0]
+    write_dump(path, last_index, temperature, species, mesh.xyz, bnd, assoc, tv, tn, grid.extents(), grid.ids(),
+               grid.phi(0), grid.phi(1), sw, grid.charges(), grid.flags(), ptr, idx, grid.couplings(), numbers)
+
+
+def read_dump(path):
+    """File_Io::readSystem (file_io.cpp:1026-1062): the last index and the
+    three tables.  As the reference reads them: a species' Dirichlet flag is
+    read but not kept (configuration.cpp:135-137: the caller rereads the
+    configuration file, tapsim.cpp:161-170), the evaporation charge is the
+    short in the first two of its four bytes, the relaxation factor's eight
+    bytes end with the potential switch."""
+    data = open(path, "rb").read()
+    pos = 0
+
+    def line():
+        nonlocal pos
+        end = data.index(b"\n", pos)
+        s = data[pos:end].decode("latin-1")
+        pos = end + 1
+        return s
+
+    def cstr():
+        nonlocal pos
+        end = data.index(b"\0", pos)
+        s = data[pos:end].decode("latin-1")
+        pos = end + 1
+        return s
+
+    def take(fmt):
+        nonlocal pos
+        v = struct.unpack_from(fmt, data, pos)
+        pos += struct.calcsize(fmt)
+        return v
+
+    if not line().startswith("TAPSIM DUMP INFORMATION"):
+        raise FormatError("File_Io::readSystem(): wrong file type")
+    head = line()
+    if not head.startswith("LAST_INDEX"):
+        raise FormatError("File_Io::readSystem()")
+    last_index = int(head[10:].strip())
+    out = {"last_index": last_index}
+    if cstr() != DUMP_VERSION:
+        raise FormatError("Configuration::Table<<(): wrong version!")
+    temperature, count = take("<fi")
+    species = {}
+    for _ in range(count):
+        (sid,) = take("<b")
+        name = cstr()
+        cd, eps, rem, neu, _dir, phi, mass, q, _pad, ef, ee = take("<ff???ffh2sff")
+        species[sid] = dict(name=name, charge_density=cd, epsilon=eps, removable=int(rem), neumann=int(neu),
+                            dirichlet=0, phi=phi, mass=mass, evap_charge=q, evap_field=ef, evap_energy=ee)
+    out["temperature"], out["species"] = temperature, species
+    if cstr() != DUMP_VERSION:
+        raise FormatError("Geometry_3d::Table<<(): wrong version!")
+    out["extents"] = np.frombuffer(data, np.float32, 12, pos).reshape(4, 3).copy()
+    pos += 48
+    (n,) = take("<I")
+    node = np.frombuffer(data, np.dtype([("p", "<f4", 3), ("b", "u1"), ("t", "<i4")]), n, pos)
+    pos += node.nbytes
+    out["xyz"], out["boundary"], out["assoc"] = node["p"].copy(), node["b"].astype(bool), node["t"].copy()
+    (m,) = take("<I")
+    tt = np.frombuffer(data, "<i4", 8 * m, pos).reshape(m, 8)
+    pos += tt.nbytes
+    out["tets"], out["tet_neighbours"] = tt[:, :4].copy(), tt[:, 4:].copy()
+    if cstr() != DUMP_VERSION:
+        raise FormatError("Grid_3d::Table<<(): wrong version!")
+    inf, _rf, _sw8, sw = take("<f4sii")
+    if np.float32(inf) != GRID_INFINITY_FACTOR:
+        raise FormatError("Grid_3d::Table::operator<<()")
+    out["phi_switch"] = sw
+    (nn,) = take("<I")
+    head = np.dtype([("id", "i1"), ("num", "<u4"), ("p0", "<f4"), ("p1", "<f4"), ("q", "<f4"), ("b", "u1"),
+                     ("d", "u1"), ("ne", "u1"), ("k", "<i4")])
+    ids = np.zeros(nn, np.int16)
+    numbers = np.zeros(nn, np.uint32)
+    phi = np.zeros((2, nn), np.float32)
+    q = np.zeros(nn, np.float32)
+    flags = np.zeros(nn, np.uint8)
+    ptr = np.zeros(nn + 1, np.int64)
+    cols, coups = [], []
+    for i in range(nn):
+        h = np.frombuffer(data, head, 1, pos)[0]
+        pos += head.itemsize
+        k = int(h["k"])
+        e = np.frombuffer(data, np.dtype([("j", "<i4"), ("c", "<f4")]), k, pos)
+        pos += e.nbytes
+        ids[i], numbers[i], phi[0, i], phi[1, i], q[i] = h["id"], h["num"], h["p0"], h["p1"], h["q"]
+        flags[i] = (1 if h["b"] else 0) | (2 if h["d"] else 0) | (4 if h["ne"] else 0)
+        ptr[i + 1] = ptr[i] + k
+        cols.append(e["j"].copy())
+        coups.append(e["c"].copy())
+    out.update(ids=ids, numbers=numbers, phi0=phi[0], phi1=phi[1], charge=q, flags=flags, ptr=ptr,
+               idx=np.concatenate(cols) if cols else np.zeros(0, np.int32),
+               couplings=np.concatenate(coups) if coups else np.zeros(0, np.float32))
+    return out
+
+
 # ---------------------------------------------------------------- tapsim.ini
 
 def read_ini(path, delimiter=" = "):

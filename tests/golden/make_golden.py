@@ -422,6 +422,20 @@ def make_pinning():
     print("pinning: %s" % ", ".join("%s%s" % (k, list(v.shape)) for k, v in sorted(fx.items())))
 
 
+def make_dump():
+    """The reference's dump (File_Io::writeSystem) of the small specimen after
+    six Jacobi cycles, with last index 42 (tests/golden/dump.npz)"""
+    base = dict(np.load(os.path.join(OUT, "tip_small.npz")))
+    with tempfile.TemporaryDirectory() as wd:
+        open(os.path.join(wd, "cfg.aly"), "w").write(aly_text(SPECIES))
+        write_input(os.path.join(wd, "in.bin"), base["xyz"], base["ids"], base["tets"])
+        run(wd, ["mesh", "sync", "relax:6:1", "writedump:dump.bin:42"])
+        fx = dict(dump=np.frombuffer(open(os.path.join(wd, "dump.bin"), "rb").read(), np.uint8),
+                  cfg_aly=np.frombuffer(open(os.path.join(wd, "cfg.aly"), "rb").read(), np.uint8))
+    np.savez_compressed(os.path.join(OUT, "dump.npz"), **fx)
+    print("dump: %d bytes" % len(fx["dump"]))
+
+
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref"], cwd=ROOT, check=True)
@@ -430,6 +444,9 @@ if __name__ == "__main__":
         sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "pinning":
         make_pinning()
+        sys.exit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "dump":
+        make_dump()
         sys.exit(0)
     make_tetgen_mesh()
     make_small()

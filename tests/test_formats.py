@@ -190,3 +190,84 @@ def test_trajectory_stream_in_chunks(fx, tmp_path):
     for f in files:
         key = "traj_chunk_txt_" + f.rsplit(".", 1)[1]
         assert open(f, "rb").read() == bytes(fx[key])
+
+
+def undefined_bytes(data):
+    """offsets of the dump bytes the reference writes from memory it does not
+    own: the two bytes after each species' evaporation charge and the four of
+    the uninitialised relaxation factor"""
+    pos = data.index(b"\n", data.index(b"\n") + 1) + 1
+    pos += 4 + 4  # version "1.0\0", temperature
+    count = int(np.frombuffer(data, "<i4", 1, pos)[0])
+    pos += 4
+    out = []
+    for _ in range(count):
+        pos += 1
+        pos = data.index(b"\0", pos) + 1
+        pos += 4 + 4 + 3 + 4 + 4
+        out += [pos + 2, pos + 3]
+        pos += 4 + 4 + 4
+    pos += 4 + 48
+    n = int(np.frombuffer(data, "<u4", 1, pos)[0])
+    pos += 4 + 17 * n
+    m = int(np.frombuffer(data, "<u4", 1, pos)[0])
+    pos += 4 + 32 * m + 4 + 4
+    return out + [pos, pos + 1, pos + 2, pos + 3]
+
+
+def masked(data, offsets):
+    b = bytearray(data)
+    for o in offsets:
+        b[o] = 0
+    return bytes(b)
+
+
+@pytest.fixture(scope="module")
+def dump_fx():
+    return dict(np.load(os.path.join(HERE, "golden", "dump.npz")))
+
+
+def test_dump_read_as_the_reference_wrote_it(dump_fx, tip_small, tmp_path):
+    """File_Io::readSystem of the reference's dump of the small specimen after
+    six Jacobi cycles: geometry, couplings, potentials, flags and ids equal
+    the reference's own dumps of the same state"""
+    p = tmp_path / "dump.bin"
+    p.write_bytes(bytes(dump_fx["dump"]))
+    r = F.read_dump(str(p))
+    assert r["last_index"] == 42 and r["phi_switch"] == 0
+    bits = lambda a: np.asarray(a, np.float32).view(np.uint32)  # noqa: E731
+    assert np.array_equal(bits(r["xyz"]), bits(tip_small["xyz"]))
+    assert np.array_equal(r["tets"], tip_small["tets"]) and np.array_equal(r["tet_neighbours"], tip_small["ref_neigh"])
+    assert np.array_equal(r["assoc"], tip_small["ref_assoc"])
+    assert np.array_equal(r["boundary"].astype(np.uint8), tip_small["ref_boundary"])
+    assert np.array_equal(r["ptr"], tip_small["ref_ptr"]) and np.array_equal(r["idx"], tip_small["ref_idx"])
+    assert np.array_equal(bits(r["couplings"]), bits(tip_small["ref_coupling"]))
+    assert np.array_equal(bits(r["phi0"]), bits(tip_small["ref_phi0_r6"]))
+    assert np.array_equal(bits(r["phi1"]), bits(tip_small["ref_phi1_r6"]))
+    assert np.array_equal(r["flags"], tip_small["ref_flags"]) and np.array_equal(r["ids"], tip_small["ids"])
+    assert np.array_equal(r["numbers"], np.arange(1, len(r["ids"]) + 1))
+    # the species as the reference reads them: the Dirichlet flag dropped
+    assert sorted(r["species"]) == [0, 1, 2, 3, 10, 11]
+    assert all(s["dirichlet"] == 0 for s in r["species"].values())
+    assert r["species"][10]["evap_charge"] == 2 and r["species"][2]["neumann"] == 1
+
+
+def test_dump_written_as_the_reference_writes_it(dump_fx, tmp_path):
+    """the dump rewritten from what was read, with the configuration's
+    Dirichlet flags from the configuration file: byte for byte the
+    reference's, but for the bytes it writes from memory it does not own"""
+    src = tmp_path / "ref.bin"
+    src.write_bytes(bytes(dump_fx["dump"]))
+    r = F.read_dump(str(src))
+    (tmp_path / "cfg.aly").write_bytes(bytes(dump_fx["cfg_aly"]))
+    temperature, species = F.read_config(str(tmp_path / "cfg.aly"))
+    out = tmp_path / "ours.bin"
+    F.write_dump(str(out), r["last_index"], temperature, species, r["xyz"], r["boundary"], r["assoc"], r["tets"],
+                 r["tet_neighbours"], r["extents"], r["ids"], r["phi0"], r["phi1"], r["phi_switch"], r["charge"],
+                 r["flags"], r["ptr"], r["idx"], r["couplings"], r["numbers"])
+    ref = bytes(dump_fx["dump"])
+    ours = out.read_bytes()
+    assert len(ours) == len(ref)
+    und = undefined_bytes(ref)
+    assert len(und) == 2 * len(species) + 4
+    assert masked(ours, und) == masked(ref, und)
