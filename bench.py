@@ -60,6 +60,8 @@ ION_TABLES = {
     11: ([57.9353, 59.9308, 60.9311, 61.9283, 63.9280], [2, 2, 2, 2, 2], [0.6808, 0.2622, 0.0114, 0.0363, 0.0093]),
 }
 ION_SEED = 20261204
+# nodes per recorded shell graph the training store is sized for
+SHELL_NODES = {"c1": 4000, "c3": 4000, "s": 4000, "c2": 1500, "c5": 14000}
 DETECTOR_DISTANCE = 0.1
 
 
@@ -175,13 +177,21 @@ def cpu_baseline(tsg, coup0, xyz, ids, tets, neigh, state0, plain_opts, first_ev
                 sweep_ms=t_total / k * 1e3)
 
 
-def run_events(grid, opts, steps, tag, quiet=False):
-    """steps events in one call: the flight of each event overlaps the next
-    event, and the call returns once the last flight is collected"""
+def run_events(grid, opts, steps, tag, quiet=False, chunk=1000):
+    """steps events, in one call when short (the flight of each event
+    overlaps the next event, and the call returns once the last flight is
+    collected), in calls of `chunk` events with a progress line each when
+    long (the grid counts events across calls, so the events are the same)"""
     s0 = grid.sweep_stats()
     p0, e0 = grid.phase_times()
     t0 = time.perf_counter()
-    events = list(grid.evaporate(opts, steps))
+    if steps <= 2 * chunk:
+        events = list(grid.evaporate(opts, steps))
+    else:
+        events = []
+        while len(events) < steps:
+            events += list(grid.evaporate(opts, min(chunk, steps - len(events))))
+            log("%s: %d of %d events (%.1fs)" % (tag, len(events), steps, time.perf_counter() - t0))
     dt = time.perf_counter() - t0
     if not quiet:
         log("%s: %d events in %.2fs" % (tag, len(events), dt))
@@ -319,8 +329,11 @@ def main():
     mixed.accel_setup(0.9)
     learned = new_grid(state0)
     learned.accel_setup(0.9)
-    net = tsg.WarmStartNet(max_batch_rows=65536, data_rows=max(1 << 20, 4000 * (W + 8)),
-                           data_edges=max(1 << 24, 60000 * (W + 8)), seed=1)
+    # the training store: the recorded shells hold ~200 nodes (c1), ~1000
+    # (c2) and ~12.5 k (c5), ~10 edges a node; edge offsets stay within int32
+    shell = SHELL_NODES.get(args.workload, 4000)
+    net = tsg.WarmStartNet(max_batch_rows=65536, data_rows=max(1 << 20, shell * (W + 8)),
+                           data_edges=min((1 << 31) - 1, max(1 << 24, 12 * shell * (W + 8))), seed=1)
     learned.attach(net, record=True, predict=False)
     t0 = time.perf_counter()
     rec_l, _, _, _ = run_events(learned, mixed_opts, W, "record", quiet=True)

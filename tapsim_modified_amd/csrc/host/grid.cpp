@@ -14,7 +14,10 @@ namespace tsg {
 
 namespace {
 constexpr int kSlice = 64;
-constexpr int kSpareSlices = 64;  // 4096 rows for atoms that turn into vacuum
+// spare row slots for atoms that turn into vacuum: one per removable node, at
+// least 4096 and at most 2^20 (memory: ~20 B per entry of the removable
+// nodes' widest row, ~300 MB at the cap for BCC atoms of 14 neighbours)
+constexpr int kSpareMin = 4096, kSpareMax = 1 << 20;
 constexpr int kSortWindow = 256;  // rows sorted by length within each window of this many
 constexpr double kEpsilon0 = 8.85418781762e-12;  // grid_3d.h:285
 }  // namespace
@@ -375,10 +378,18 @@ int Grid::build_rows()
 		if (!(flags_[i] & 2)) rows.push_back(i);
 	nrows_ = (int) rows.size();
 	const int nslices_main = (nrows_ + kSlice - 1) / kSlice;
-	int maxdeg = 0;
-	for (int i = 0; i < n; i++) maxdeg = std::max(maxdeg, mesh_.ptr[i + 1] - mesh_.ptr[i]);
+	// new unknown rows come from removable atoms taken away (or purged): the
+	// spare slices are as wide as the widest of their rows
+	int maxdeg = 1;
+	long long removable = 0;
+	for (int i = 0; i < n; i++)
+		if (species_[ids_[i]].removable && (flags_[i] & 2)) {
+			maxdeg = std::max(maxdeg, mesh_.ptr[i + 1] - mesh_.ptr[i]);
+			removable++;
+		}
 	spare_width_ = maxdeg;
-	const int nslices = nslices_main + kSpareSlices;
+	const long long spare_rows = std::min<long long>(kSpareMax, std::max<long long>(kSpareMin, removable + kSlice));
+	const int nslices = nslices_main + (int) ((spare_rows + kSlice - 1) / kSlice);
 	cap_slots_ = nslices * kSlice;
 	nslots_ = nslices_main * kSlice;
 	next_spare_ = nslots_;
@@ -1052,7 +1063,7 @@ int Grid::set_id_and_resync(int node, int id)
 			off_slots.push_back(slot);
 			moved.push_back(r);
 		}
-		if (next_spare_ >= cap_slots_) return fail(-7, "row slots exhausted (more than 4096 new unknown nodes)");
+		if (next_spare_ >= cap_slots_) return fail(-7, "row slots exhausted (more new unknown nodes than spare slots)");
 		slot = next_spare_++;
 		if (const int w = write_slot(slot, r, &ai, &vi, &ar, &vr, &aw, &vw))
 			return fail(-7, w == -2 ? "long-row store full" : "row wider than the spare slices");

@@ -475,6 +475,33 @@ DUMP_VERSION = "1.0"  # Configuration, Geometry_3d and Grid_3d tables' binary ve
 GRID_INFINITY_FACTOR = np.float32(1e2)  # Grid_3d::Table::_infinityFactor (grid_3d.h:287)
 
 
+# a node of the grid block: a 24-byte header (id, number, both potentials,
+# charge, three flags, neighbour count) and 8 bytes per neighbour (index,
+# coupling): both whole 8-byte units, so the block is an array of them
+_GRID_HEAD = np.dtype([("id", "i1"), ("num", "<u4"), ("p0", "<f4"), ("p1", "<f4"), ("q", "<f4"), ("b", "u1"),
+                       ("d", "u1"), ("ne", "u1"), ("k", "<i4")], align=False)
+_GRID_PAIR = np.dtype([("j", "<i4"), ("c", "<f4")], align=False)
+
+
+def _grid_records(ids8, numbers, p0, p1, q, fl, ptr, idx, coup):
+    n = len(ids8)
+    head = np.zeros(n, _GRID_HEAD)
+    head["id"], head["num"], head["p0"], head["p1"], head["q"] = ids8, numbers, p0, p1, q
+    head["b"], head["d"], head["ne"] = (fl & 1) != 0, (fl & 2) != 0, (fl & 4) != 0
+    deg = np.diff(ptr)
+    head["k"] = deg
+    pair = np.zeros(len(idx), _GRID_PAIR)
+    pair["j"], pair["c"] = idx, coup
+    units = np.zeros(3 * n + len(idx), "<u8")
+    at = 3 * np.arange(n, dtype=np.int64) + ptr[:-1]
+    hu = head.view("<u8").reshape(n, 3)
+    for w in range(3):
+        units[at + w] = hu[:, w]
+    row = np.repeat(np.arange(n, dtype=np.int64), deg)
+    units[3 * (row + 1) + np.arange(len(idx), dtype=np.int64)] = pair.view("<u8")
+    return units
+
+
 def write_dump(path, last_index, temperature, species, xyz, boundary, assoc, tets, tet_neighbours, extents, ids,
                phi0, phi1, phi_switch, charge, flags, ptr, idx, couplings, numbers=None):
     """File_Io::writeSystem (file_io.cpp:1013-1025): "TAPSIM DUMP INFORMATION",
@@ -524,21 +551,7 @@ def write_dump(path, last_index, temperature, species, xyz, boundary, assoc, tet
     p1 = np.asarray(phi1, np.float32)
     q = np.asarray(charge, np.float32)
     fl = np.asarray(flags, np.uint8)
-    head = np.dtype([("id", "i1"), ("num", "<u4"), ("p0", "<f4"), ("p1", "<f4"), ("q", "<f4"), ("b", "u1"),
-                     ("d", "u1"), ("ne", "u1"), ("k", "<i4")], align=False)
-    pair = np.dtype([("j", "<i4"), ("c", "<f4")], align=False)
-    body = []
-    for i in range(n):
-        h = np.zeros(1, head)
-        h["id"], h["num"], h["p0"], h["p1"], h["q"] = ids8[i], numbers[i], p0[i], p1[i], q[i]
-        h["b"], h["d"], h["ne"] = (fl[i] & 1) != 0, (fl[i] & 2) != 0, (fl[i] & 4) != 0
-        k0, k1 = int(ptr[i]), int(ptr[i + 1])
-        h["k"] = k1 - k0
-        e = np.zeros(k1 - k0, pair)
-        e["j"] = idx[k0:k1]
-        e["c"] = coup[k0:k1]
-        body.append(h.tobytes() + e.tobytes())
-    out.append(b"".join(body))
+    out.append(_grid_records(ids8, numbers, p0, p1, q, fl, ptr, idx, coup).tobytes())
     with open(path, "wb") as f:
         f.write(b"".join(out))
 
@@ -621,29 +634,26 @@ def read_dump(path):
         raise FormatError("Grid_3d::Table::operator<<()")
     out["phi_switch"] = sw
     (nn,) = take("<I")
-    head = np.dtype([("id", "i1"), ("num", "<u4"), ("p0", "<f4"), ("p1", "<f4"), ("q", "<f4"), ("b", "u1"),
-                     ("d", "u1"), ("ne", "u1"), ("k", "<i4")])
-    ids = np.zeros(nn, np.int16)
-    numbers = np.zeros(nn, np.uint32)
-    phi = np.zeros((2, nn), np.float32)
-    q = np.zeros(nn, np.float32)
-    flags = np.zeros(nn, np.uint8)
-    ptr = np.zeros(nn + 1, np.int64)
-    cols, coups = [], []
+    # the neighbour counts chain the records: walk them, then take every
+    # field at once
+    heads = np.zeros(nn, np.int64)
+    at = pos
     for i in range(nn):
-        h = np.frombuffer(data, head, 1, pos)[0]
-        pos += head.itemsize
-        k = int(h["k"])
-        e = np.frombuffer(data, np.dtype([("j", "<i4"), ("c", "<f4")]), k, pos)
-        pos += e.nbytes
-        ids[i], numbers[i], phi[0, i], phi[1, i], q[i] = h["id"], h["num"], h["p0"], h["p1"], h["q"]
-        flags[i] = (1 if h["b"] else 0) | (2 if h["d"] else 0) | (4 if h["ne"] else 0)
-        ptr[i + 1] = ptr[i] + k
-        cols.append(e["j"].copy())
-        coups.append(e["c"].copy())
-    out.update(ids=ids, numbers=numbers, phi0=phi[0], phi1=phi[1], charge=q, flags=flags, ptr=ptr,
-               idx=np.concatenate(cols) if cols else np.zeros(0, np.int32),
-               couplings=np.concatenate(coups) if coups else np.zeros(0, np.float32))
+        heads[i] = at
+        at += 24 + 8 * struct.unpack_from("<i", data, at + 20)[0]
+    units = np.frombuffer(data, "<u8", (at - pos) // 8, pos)
+    u = (heads - pos) // 8
+    head = np.stack([units[u], units[u + 1], units[u + 2]], axis=1).copy().view(_GRID_HEAD).reshape(nn)
+    deg = head["k"].astype(np.int64)
+    ptr = np.zeros(nn + 1, np.int64)
+    ptr[1:] = np.cumsum(deg)
+    row = np.repeat(np.arange(nn, dtype=np.int64), deg)
+    pair = units[3 * (row + 1) + np.arange(ptr[-1], dtype=np.int64)].copy().view(_GRID_PAIR)
+    flags = (head["b"] != 0).astype(np.uint8) | ((head["d"] != 0).astype(np.uint8) << 1) | \
+        ((head["ne"] != 0).astype(np.uint8) << 2)
+    out.update(ids=head["id"].astype(np.int16), numbers=head["num"].copy(), phi0=head["p0"].copy(),
+               phi1=head["p1"].copy(), charge=head["q"].copy(), flags=flags, ptr=ptr, idx=pair["j"].copy(),
+               couplings=pair["c"].copy())
     return out
 
 

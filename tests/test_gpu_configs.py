@@ -233,3 +233,73 @@ def test_c2_events_one_sequence():
     res, _ = o.relax_threaded(1, oracle_threads())
     log("c2: mixed path's final residual on the CPU oracle %.3g V" % res)
     assert res <= 1e-8
+
+
+def test_c5_full_pipeline_events(capsys, monkeypatch):
+    """c5 (r 25 nm, 60 nm shank: 12.75 M atoms, 3 % solute in clusters,
+    4.67 M vacuum nodes, 107.6 M tets): plain FP64, mixed and learned+mixed
+    evaporate the same 6 atoms, every ion drawn from its species' isotope
+    table and flown by the RK4 integrator to the detector plane 10 cm above
+    the mesh; the oracle's FP64 sweep on the CPU confirms the mixed path's
+    residual target.  The steps report their progress as they go (the
+    mesher's every 2 M points, a line per phase): the test takes minutes."""
+    monkeypatch.setenv("TSG_VERBOSE", "1")
+    with capsys.disabled():
+        xyz, ids, counts, mesh, species, state0 = relaxed_state(tsg, "c5")
+        log("c5: %d nodes %s, %d tets, relaxed" % (len(xyz), counts, mesh.m))
+    zmax = float(np.max(xyz[:, 2]))
+
+    def grid(method):
+        g = tsg.Grid3D(mesh, ids, species, 0.0, tsg.PLAIN_F64)
+        g.set_phi(*state0)
+        for sid, (m, q, w) in bench.ION_TABLES.items():
+            g.set_ion_table(sid, m, q, w)
+        if method:
+            g.accel_setup(0.9)
+        return g
+
+    def opts(method):
+        e = bench.event_options(tsg, True, zmax)
+        if method:
+            e.method, e.accel_tol, e.accel_solver = method, 1e-8, 1
+        return e
+
+    n = 6
+    p = grid(0)
+    tsg.srand(99)
+    pe = p.evaporate(opts(0), n)
+    with capsys.disabled():
+        log("c5 plain: %s" % [e.node for e in pe])
+    m = grid(1)
+    tsg.srand(99)
+    me = m.evaporate(opts(1), n)
+    lg = grid(2)
+    net = tsg.WarmStartNet(max_batch_rows=65536, data_rows=14000 * 16, data_edges=12 * 14000 * 16, seed=1)
+    lg.attach(net, record=True, predict=False)
+    tsg.srand(99)
+    le = lg.evaporate(opts(1), n // 2)
+    graphs, _, _ = net.dataset()
+    net.train(0, graphs, epochs=20, batch_graphs=16, lr=2e-3, seed=5)
+    lg.attach(net, record=False, predict=True)
+    le += lg.evaporate(opts(2), n - n // 2)
+    log("c5 sequences: plain %s mixed %s learned %s; ions %s" % (
+        [e.node for e in pe], [e.node for e in me], [e.node for e in le],
+        [(round(e.ion_mass, 3), e.ion_charge) for e in le]))
+    assert [e.node for e in pe] == [e.node for e in me] == [e.node for e in le]
+    zdet = np.float32(zmax + bench.DETECTOR_DISTANCE)
+    for e in pe + me + le:
+        assert e.traj_status == 5 and abs(e.stop[2] - zdet) <= 1e-6 * zdet
+        masses, charges, weights = bench.ION_TABLES[int(ids[e.node])]
+        j = O.ion_pick(bench.ION_SEED, e.event_index, weights)
+        assert e.ion_charge == charges[j] and e.ion_mass == np.float32(masses[j])
+    phi_m = m.phi(m.switches()[0])
+    with capsys.disabled():
+        log("c5: sequences equal, flights on the detector plane; the oracle's check follows")
+    tv, tn = mesh.tets()
+    o = O.Oracle(O.F64, xyz, m.ids(), tv, tn, O.species_table(bench.SPECIES))
+    o.sync_from_couplings(m.couplings())
+    o.set_phi(phi_m, phi_m, 0, 0)
+    res, _ = o.relax_threaded(1, oracle_threads())
+    with capsys.disabled():
+        log("c5: mixed path's final residual on the CPU oracle %.3g V" % res)
+    assert res <= 1e-8

@@ -195,3 +195,41 @@ def test_dump_of_the_device_grid_is_the_references(tip_small, tmp_path):
     for _ in range(4):
         assert g.relax(1) == g2.relax(1)
     assert same_bits(g.phi(0), g2.phi(0)) and same_bits(g.phi(1), g2.phi(1))
+
+
+def test_run_resumed_from_its_dump_continues_the_sequence(tip_small, tmp_path):
+    """a run dumped after 4 events and resumed from the dump (as the
+    reference's --input-dump: mesh, ids and both potential buffers, a fresh
+    synchronisation and surface, the event counter from LAST_INDEX)
+    evaporates the same next 4 atoms as the run that went on, with the
+    event indices counted on and the potentials equal to float rounding"""
+    from tapsim_modified_amd import formats as F
+    mesh = tsg.Mesh(tip_small["xyz"], tip_small["tets"], tip_small["ref_neigh"])
+    g = tsg.Grid3D(mesh, tip_small["ids"], tsg.species_table(FIXTURE_SPECIES), 0.0, tsg.FAITHFUL_F32)
+    for _ in range(300):
+        g.relax(1)
+    e = tsg.EvapOptions()
+    e.prob_mode, e.evap_mode, e.vacuum_id = 0, 0, 0
+    e.relax_threshold, e.relax_shell, e.relax_cycle, e.relax_queue = float(np.float32(1e-2)), 3, 5, 5
+    e.relax_global_cycles, e.refresh_interval = 1, 0
+    e.refresh_threshold, e.refresh_cycle, e.refresh_queue, e.exact_probability = 0.0, 1, 0, 1
+    e.trajectory = 0
+    first = g.evaporate(e, 4)
+    species = {k: dict(v, name="s%d" % k) for k, v in FIXTURE_SPECIES.items()}
+    F.dump_of_grid(str(tmp_path / "d.bin"), g, mesh, 0.0, species, last_index=first[-1].event_index)
+    went_on = g.evaporate(e, 4)
+    r = F.read_dump(str(tmp_path / "d.bin"))
+    m2 = tsg.Mesh(r["xyz"], r["tets"], r["tet_neighbours"])
+    g2 = tsg.Grid3D(m2, r["ids"], tsg.species_table(FIXTURE_SPECIES), 0.0, tsg.FAITHFUL_F32)
+    g2.set_phi(r["phi0"], r["phi1"], r["phi_switch"], 0)
+    e.init_event_cnt = r["last_index"]
+    resumed = g2.evaporate(e, 4)
+    log("resume: went on %s, resumed %s" % ([x.node for x in went_on], [x.node for x in resumed]))
+    assert [x.node for x in resumed] == [x.node for x in went_on]
+    assert [x.event_index for x in resumed] == [x.event_index for x in went_on]
+    # a fresh synchronisation computes the removed atoms' couplings as
+    # computeCoupling(lower, higher), the run that went on as
+    # computeCoupling(atom, neighbour), which round the face area differently
+    # (so does the reference's resumption): the potentials agree to float
+    # rounding, not bit for bit
+    np.testing.assert_allclose(g2.phi(0), g.phi(0), rtol=1e-5, atol=1e-3)
