@@ -373,6 +373,147 @@ __device__ void fast_field(const Mesh3& g, const float* nfield, const float* pos
 		for (int c = 0; c < 3; c++) out[c] += w[k] * nfield[3 * (size_t) v[k] + c];
 }
 
+// ---------------------------------------------------------------- the flight's tet cache
+
+// The tets a one-wavefront flight's walks visit, kept in the wavefront's
+// registers: slot s (0-15) lives on lanes 4s..4s+3, lane 4s+q holding vertex
+// q of the slot's tet (its index, coordinates and field-snapshot value) and
+// the neighbour across face q.  The walks of an integration step all start
+// at the step's tet and mostly end in it or a neighbour, so most walk steps
+// find their tet here instead of behind two dependent loads (tet -> vertices
+// -> coordinates).  The cached values are the loaded ones, so the walks, the
+// draws and the weights are those of the uncached walk.  Slots are refilled
+// round-robin.
+struct TetCache {
+	int tet = -1;
+	int across = -1;
+	float x = 0.0f, y = 0.0f, z = 0.0f;
+	float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+	int next = 0;  // the slot the next miss fills (uniform)
+};
+
+// the first lane of t's slot, filled from memory on a miss (t >= 0)
+__device__ int cache_slot(TetCache& c, const Mesh3& g, const float* nfield, int t)
+{
+	const unsigned long long hit = __ballot(c.tet == t);
+	if (hit != 0) return __ffsll(hit) - 1;
+	const int s = c.next;
+	c.next = (s + 1) & 15;
+	const int lane = threadIdx.x & 63;
+	if ((lane >> 2) == s) {
+		const int q = lane & 3;
+		const int v = g.tv[4 * (size_t) t + q];
+		c.tet = t;
+		c.across = g.tn[4 * (size_t) t + q];
+		const float* P = g.P(v);
+		const float* F = nfield + 3 * (size_t) v;
+		c.x = P[0];
+		c.y = P[1];
+		c.z = P[2];
+		c.f0 = F[0];
+		c.f1 = F[1];
+		c.f2 = F[2];
+	}
+	return 4 * s;
+}
+
+// find_tet_wave with the tets taken from the cache; *slot is the found
+// tet's first lane
+__device__ int find_tet_cached(TetCache& c, const Mesh3& g, const float* nfield, const float* pos, int t, DevRand* rng,
+                               int* slot)
+{
+	const int lane = threadIdx.x & 63;
+	const int limit = min(g.m, 1 << 20);
+	for (int loop = 0; loop < limit; loop++) {
+		if (t < 0) return -1;
+		const int base = cache_slot(c, g, nfield, t);
+		const int lo = rng ? rng->next() % 4 : 0;
+		const int index = lo + (lane & 3);
+		const int i1 = base + index % 4, i2 = base + 3 - (2 + index) % 4, i3 = base + (2 + index) % 4;
+		const float p2[3] = {__shfl(c.x, i2, 64), __shfl(c.y, i2, 64), __shfl(c.z, i2, 64)};
+		const float p1[3] = {__shfl(c.x, i1, 64), __shfl(c.y, i1, 64), __shfl(c.z, i1, 64)};
+		const float p3[3] = {__shfl(c.x, i3, 64), __shfl(c.y, i3, 64), __shfl(c.z, i3, 64)};
+		bool neg = false;
+		if (lane < 4) neg = orient_sign_f(p2, p1, p3, pos) < 0;
+		const unsigned long long mask = __ballot(neg);
+		if (mask == 0) {
+			*slot = base;
+			return t;
+		}
+		const int k = __ffsll(mask) - 1;
+		t = __shfl(c.across, base + 3 - (lo + k) % 4, 64);
+	}
+	return -2;
+}
+
+// tet_weights_wave from the cached slot: lane i (0-3) takes vertices i, i+1,
+// i+2 (mod 4), lane 4 the four vertices of the volume
+__device__ void tet_weights_cached(const TetCache& c, int base, const float* p, float* w)
+{
+	const int lane = threadIdx.x & 63;
+	float X[4][3];
+	for (int j = 0; j < 4; j++) {
+		const int src = base + (lane < 4 ? (lane + j) % 4 : j);
+		X[j][0] = __shfl(c.x, src, 64);
+		X[j][1] = __shfl(c.y, src, 64);
+		X[j][2] = __shfl(c.z, src, 64);
+	}
+	float mine = 0.0f;
+	bool ok = true;
+	if (lane < 4) {
+		float lumat[9];
+		for (int j = 0; j < 3; j++)
+			for (int k = 0; k < 3; k++) lumat[j * 3 + k] = X[j][k] - p[k];
+		float d;
+		int idx[3];
+		if (!lu_decmp<float, 3>(lumat, idx, &d)) {
+			ok = false;
+		} else {
+			float x = lu_det_f(lumat, d);
+			x /= 6.0f;
+			mine = fabsf(x);
+		}
+	} else if (lane == 4) {
+		float lumat[9];
+		for (int i = 0; i < 3; i++)
+			for (int j = 0; j < 3; j++) lumat[i * 3 + j] = X[i][j] - X[3][j];
+		float d;
+		int idx[3];
+		if (lu_decmp<float, 3>(lumat, idx, &d)) {
+			float value = lu_det_f(lumat, d);
+			value /= 6.0f;
+			mine = fabsf(value);
+		}
+	}
+	const float volume = __shfl(mine, 4, 64);
+	if (volume == 0.0f) {
+		for (int i = 0; i < 4; i++) w[i] = 0.25f;
+		return;
+	}
+	float wi = 0.0f;
+	if (ok) {
+		wi = mine;
+		wi /= volume;
+	}
+	for (int i = 0; i < 4; i++) w[(i + 3) % 4] = __shfl(wi, i, 64);
+}
+
+// fast_field with the walk and the weights on the cache
+__device__ void fast_field_cached(TetCache& c, const Mesh3& g, const float* nfield, const float* pos, int guess,
+                                  float* out, DevRand* rng)
+{
+	out[0] = out[1] = out[2] = 0.0f;
+	int base = 0;
+	const int t = find_tet_cached(c, g, nfield, pos, guess, rng, &base);
+	if (t < 0) return;
+	float w[4];
+	tet_weights_cached(c, base, pos, w);
+	for (int k = 0; k < 4; k++) {
+		const float f[3] = {__shfl(c.f0, base + k, 64), __shfl(c.f1, base + k, 64), __shfl(c.f2, base + k, 64)};
+		for (int i = 0; i < 3; i++) out[i] += w[k] * f[i];
+	}
+}
+
 // ---------------------------------------------------------------- integration
 
 struct Phase {
@@ -391,7 +532,22 @@ struct Tr {
 	long long calls, max_calls;
 	DevRand* rng;          // the process's rand() stream, or null
 	float ep[3], ev[3];    // the cumulative error estimate (position, velocity)
+	TetCache cache;        // the walks' tets (one-wavefront flights)
 };
+
+// the field at pos and the tet of pos, walked from guess
+__device__ void tr_field(Tr& tr, const float* pos, int guess, float* out)
+{
+	if (tr.g.wave) fast_field_cached(tr.cache, tr.g, tr.nfield, pos, guess, out, tr.rng);
+	else fast_field(tr.g, tr.nfield, pos, guess, out, tr.rng);
+}
+
+__device__ int tr_locate(Tr& tr, const float* pos, int guess)
+{
+	if (!tr.g.wave) return find_tet(tr.g, pos, guess, tr.rng);
+	int base;
+	return find_tet_cached(tr.cache, tr.g, tr.nfield, pos, guess, tr.rng, &base);
+}
 
 // Trajectory_3d::cumulative_error (trajectory_3d.cpp:680-696): per component
 // sqrt(e^2 + step^2), the squares as the reference's powf(x, 2.0f)
@@ -434,7 +590,7 @@ __device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
 				vel[l] += b[i][j] * dv[j][l];
 				pos[l] += b[i][j] * dp[j][l];
 			}
-		fast_field(tr.g, tr.nfield, pos, back.tet, dv[i], tr.rng);
+		tr_field(tr, pos, back.tet, dv[i]);
 		for (int l = 0; l < 3; l++) dv[i][l] *= qm;
 		for (int l = 0; l < 3; l++) dv[i][l] *= delta;
 		for (int l = 0; l < 3; l++) dp[i][l] = vel[l] * delta;
@@ -455,7 +611,7 @@ __device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
 			error->v[i] += (c1[j] - c2[j]) * dv[j][i];
 		}
 	}
-	next.tet = find_tet(tr.g, next.p, back.tet, tr.rng);
+	next.tet = tr_locate(tr, next.p, back.tet);
 	return next;
 }
 
@@ -463,7 +619,7 @@ __device__ Phase o5_step(Tr& tr, const Phase& back, float delta, Phase* error)
 __device__ Phase o1_step(Tr& tr, const Phase& back, float delta, Phase* error)
 {
 	float f[3];
-	fast_field(tr.g, tr.nfield, back.p, back.tet, f, tr.rng);
+	tr_field(tr, back.p, back.tet, f);
 	for (int i = 0; i < 3; i++) f[i] *= tr.charge;
 	for (int i = 0; i < 3; i++) f[i] /= tr.mass;
 	Phase next = back;
@@ -478,7 +634,7 @@ __device__ Phase o1_step(Tr& tr, const Phase& back, float delta, Phase* error)
 		error->p[i] = next.p[i] * d2;
 		error->v[i] = next.v[i] * d2;
 	}
-	next.tet = find_tet(tr.g, next.p, back.tet, tr.rng);
+	next.tet = tr_locate(tr, next.p, back.tet);
 	return next;
 }
 
@@ -507,7 +663,7 @@ __device__ Phase o4_step(Tr& tr, const Phase& back, float delta, Phase* error)
 				vel[l] += a[i] * dv[i - 1][l];
 				pos[l] += a[i] * dp[i - 1][l];
 			}
-		fast_field(tr.g, tr.nfield, pos, back.tet, dv[i], tr.rng);
+		tr_field(tr, pos, back.tet, dv[i]);
 		for (int l = 0; l < 3; l++) dv[i][l] *= qm;
 		for (int l = 0; l < 3; l++) dv[i][l] *= delta;
 		for (int l = 0; l < 3; l++) dp[i][l] = vel[l] * delta;
@@ -519,7 +675,7 @@ __device__ Phase o4_step(Tr& tr, const Phase& back, float delta, Phase* error)
 			next.v[i] += c[j] * dv[j][i];
 		}
 	next.t += delta;
-	fast_field(tr.g, tr.nfield, next.p, back.tet, dv[4], tr.rng);
+	tr_field(tr, next.p, back.tet, dv[4]);
 	for (int l = 0; l < 3; l++) dv[4][l] *= qm;
 	for (int l = 0; l < 3; l++) dv[4][l] *= delta;
 	for (int l = 0; l < 3; l++) dp[4][l] = next.v[l] * delta;
@@ -534,7 +690,7 @@ __device__ Phase o4_step(Tr& tr, const Phase& back, float delta, Phase* error)
 		e /= 6.0f;
 		error->v[i] = e;
 	}
-	next.tet = find_tet(tr.g, next.p, back.tet, tr.rng);
+	next.tet = tr_locate(tr, next.p, back.tet);
 	return next;
 }
 
@@ -690,7 +846,7 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 		back.p[k] = a.xyz[3 * (size_t) node + k];
 		back.v[k] = a.v0 ? a.v0[3 * (size_t) ion + k] : 0.0f;
 	}
-	back.tet = find_tet(tr.g, back.p, a.assoc[node], rng);
+	back.tet = tr_locate(tr, back.p, a.assoc[node]);
 	int status;
 	int len = 0;
 	if (a.summary)
@@ -704,7 +860,7 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 	}
 	put(a, ion, len++, back);
 	float f[3];
-	fast_field(tr.g, tr.nfield, back.p, back.tet, f, rng);
+	tr_field(tr, back.p, back.tet, f);
 	if (f[0] == 0.0f && f[1] == 0.0f && f[2] == 0.0f)
 		status = 1;  // NO_FIELD
 	else {
@@ -747,7 +903,7 @@ __device__ void run_ion(const TrajArgs& a, int ion, DevRand* rng, bool wave = fa
 		for (int k = 0; k < 3; k++) item.p[k] = back.p[k];
 		for (int k = 0; k < 3; k++) item.p[k] += dt * back.v[k];
 		for (int k = 0; k < 3; k++) item.v[k] = back.v[k];
-		item.tet = find_tet(tr.g, item.p, back.tet, rng);
+		item.tet = tr_locate(tr, item.p, back.tet);
 		put(a, ion, len++, item);
 		// the error estimate follows the extrapolation (trajectory_3d.cpp:327)
 		if (dt > 0.0f)
@@ -972,7 +1128,8 @@ __device__ float exit_potential(const Mesh3& g, const float* p, int guess, int w
 
 // one wavefront: every lane runs the flight's uniform calculation (the same
 // values, the stream's table in LDS updated with the same words), lanes 0-4
-// share the face tests and the weights' determinants
+// share the face tests and the weights' determinants, the walks' tets are
+// cached in the lanes' registers
 __global__ __launch_bounds__(64) void k_flight(FlightArgs f)
 {
 	if (blockIdx.x != 0) return;
