@@ -389,6 +389,7 @@ struct TetCache {
 	int across = -1;
 	float x = 0.0f, y = 0.0f, z = 0.0f;
 	float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+	float vol = -1.0f;  // the tet's volume once computed (-1 before)
 	int next = 0;  // the slot the next miss fills (uniform)
 };
 
@@ -404,6 +405,7 @@ __device__ int cache_slot(TetCache& c, const Mesh3& g, const float* nfield, int 
 		const int q = lane & 3;
 		const int v = g.tv[4 * (size_t) t + q];
 		c.tet = t;
+		c.vol = -1.0f;
 		c.across = g.tn[4 * (size_t) t + q];
 		const float* P = g.P(v);
 		const float* F = nfield + 3 * (size_t) v;
@@ -447,53 +449,59 @@ __device__ int find_tet_cached(TetCache& c, const Mesh3& g, const float* nfield,
 }
 
 // tet_weights_wave from the cached slot: lane i (0-3) takes vertices i, i+1,
-// i+2 (mod 4), lane 4 the four vertices of the volume
-__device__ void tet_weights_cached(const TetCache& c, int base, const float* p, float* w)
+// i+2 (mod 4); the volume is computed once per cached tet (lane 4, from the
+// four vertices) and kept in the slot
+__device__ void tet_weights_cached(TetCache& c, int base, const float* p, float* w)
 {
 	const int lane = threadIdx.x & 63;
-	float X[4][3];
-	for (int j = 0; j < 4; j++) {
-		const int src = base + (lane < 4 ? (lane + j) % 4 : j);
+	float volume = __shfl(c.vol, base, 64);
+	if (volume < 0.0f) {
+		float X[4][3];
+		for (int j = 0; j < 4; j++) {
+			X[j][0] = __shfl(c.x, base + j, 64);
+			X[j][1] = __shfl(c.y, base + j, 64);
+			X[j][2] = __shfl(c.z, base + j, 64);
+		}
+		float v = 0.0f;
+		if (lane == 4) {
+			float lumat[9];
+			for (int i = 0; i < 3; i++)
+				for (int j = 0; j < 3; j++) lumat[i * 3 + j] = X[i][j] - X[3][j];
+			float d;
+			int idx[3];
+			if (lu_decmp<float, 3>(lumat, idx, &d)) {
+				float value = lu_det_f(lumat, d);
+				value /= 6.0f;
+				v = fabsf(value);
+			}
+		}
+		volume = __shfl(v, 4, 64);
+		if ((lane >> 2) == (base >> 2)) c.vol = volume;
+	}
+	if (volume == 0.0f) {
+		for (int i = 0; i < 4; i++) w[i] = 0.25f;
+		return;
+	}
+	float X[3][3];
+	for (int j = 0; j < 3; j++) {
+		const int src = base + ((lane & 3) + j) % 4;
 		X[j][0] = __shfl(c.x, src, 64);
 		X[j][1] = __shfl(c.y, src, 64);
 		X[j][2] = __shfl(c.z, src, 64);
 	}
-	float mine = 0.0f;
-	bool ok = true;
+	float wi = 0.0f;
 	if (lane < 4) {
 		float lumat[9];
 		for (int j = 0; j < 3; j++)
 			for (int k = 0; k < 3; k++) lumat[j * 3 + k] = X[j][k] - p[k];
 		float d;
 		int idx[3];
-		if (!lu_decmp<float, 3>(lumat, idx, &d)) {
-			ok = false;
-		} else {
+		if (lu_decmp<float, 3>(lumat, idx, &d)) {
 			float x = lu_det_f(lumat, d);
 			x /= 6.0f;
-			mine = fabsf(x);
+			wi = fabsf(x);
+			wi /= volume;
 		}
-	} else if (lane == 4) {
-		float lumat[9];
-		for (int i = 0; i < 3; i++)
-			for (int j = 0; j < 3; j++) lumat[i * 3 + j] = X[i][j] - X[3][j];
-		float d;
-		int idx[3];
-		if (lu_decmp<float, 3>(lumat, idx, &d)) {
-			float value = lu_det_f(lumat, d);
-			value /= 6.0f;
-			mine = fabsf(value);
-		}
-	}
-	const float volume = __shfl(mine, 4, 64);
-	if (volume == 0.0f) {
-		for (int i = 0; i < 4; i++) w[i] = 0.25f;
-		return;
-	}
-	float wi = 0.0f;
-	if (ok) {
-		wi = mine;
-		wi /= volume;
 	}
 	for (int i = 0; i < 4; i++) w[(i + 3) % 4] = __shfl(wi, i, 64);
 }
