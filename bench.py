@@ -320,7 +320,8 @@ def main():
         counters_run(tsg, plain)
         return
 
-    pev, pdt, ps0, ps1 = run_events(plain, plain_opts, Kp, "plain")
+    # the plain path runs about one c2 event per second: long runs report every 50 events
+    pev, pdt, ps0, ps1 = run_events(plain, plain_opts, Kp, "plain", chunk=50)
     psweeps = (ps1["sweeps"] - ps0["sweeps"]) / max(1, Kp)
     roof_plain = roofline(plain, 0, "k_jacobi<double>", args.workload)
     log("plain FP64: %.3f events/s, %.0f sweeps/event" % (Kp / pdt, psweeps))

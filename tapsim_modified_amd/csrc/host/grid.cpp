@@ -1181,49 +1181,35 @@ int Grid::set_id_and_resync(int node, int id)
 	char* d = (char*) d_patch_addr_.ptr();
 	HIPCHK(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, stream_));
 	HIPCHK(hipEventRecord(patch_ev_, stream_));
-	// the multigrid's level-0 row sums of the touched rows, and a zero
-	// residual in the slots that lost their row
-	if (!dn_addr.empty() && launch_scatter_f32(stream_, (int) dn_addr.size(), (const long long*) (d + o_dna),
-	                                           (const float*) (d + o_dnv), amg_dn_.as<float>()))
-		return fail(-2, "patch launch failed");
-	if (!res_addr.empty() && launch_scatter_f32(stream_, (int) res_addr.size(), (const long long*) (d + o_rsa),
-	                                            (const float*) (d + o_rsv), amg_res_.as<float>()))
-		return fail(-2, "patch launch failed");
-	int* iarr[5] = {d_slot_node_.as<int>(), d_slot_deg_.as<int>(), d_col_.as<int>(), d_inner_deg_.as<int>(), d_icol_.as<int>()};
-	for (int t = 0; t < 5; t++)
-		if (launch_scatter_i32(stream_, (int) ia[t].size(), (const long long*) (d + o_ia[t]), (const int*) (d + o_iv[t]), iarr[t]))
-			return fail(-2, "patch launch failed");
-	double* warr[3] = {nullptr, d_ws_.as<double>(), d_q_.as<double>()};
-	for (int t = 1; t < 3; t++)
-		if (launch_scatter_f64(stream_, (int) wa[t].size(), (const long long*) (d + o_wa[t]), (const double*) (d + o_wv[t]), warr[t]))
-			return fail(-2, "patch launch failed");
-	int rc;
-	if (f64_) {
-		rc = launch_scatter_f64(stream_, (int) nr, (const long long*) (d + o_ra), (const double*) (d + o_rv), d_coup_.as<double>()) |
-		     launch_scatter_f64(stream_, (int) nf, (const long long*) (d + o_fa), (const double*) (d + o_fv), d_coupfull_.as<double>());
-	} else {
-		rc = launch_scatter_f32(stream_, (int) nr, (const long long*) (d + o_ra), (const float*) (d + o_rv), d_coup_.as<float>()) |
-		     launch_scatter_f32(stream_, (int) nf, (const long long*) (d + o_fa), (const float*) (d + o_fv), d_coupfull_.as<float>());
-	}
-	if (rc) return fail(-2, "patch launch failed");
-	if (!fa.empty() &&
-	    launch_scatter_f32(stream_, (int) fa.size(), (const long long*) (d + o_ia3), (const float*) (d + o_iw3), d_iw_.as<float>()))
-		return fail(-2, "patch launch failed");
-	int* larr[2] = {d_lcol_.as<int>(), d_licol_.as<int>()};
-	for (int t = 0; t < 2; t++)
-		if (!la[t].empty() && launch_scatter_i32(stream_, (int) la[t].size(), (const long long*) (d + o_la[t]),
-		                                         (const int*) (d + o_lv[t]), larr[t]))
-			return fail(-2, "patch launch failed");
-	if (!lpr_addr_.empty()) {
-		const int nl = (int) lpr_addr_.size();
-		if (f64_ ? launch_scatter_f64(stream_, nl, (const long long*) (d + o_lra), (const double*) (d + o_lrv), d_lcoup_.as<double>())
-		         : launch_scatter_f32(stream_, nl, (const long long*) (d + o_lra), (const float*) (d + o_lrv), d_lcoup_.as<float>()))
-			return fail(-2, "patch launch failed");
-	}
-	if (!lpf_addr_.empty() &&
-	    launch_scatter_f32(stream_, (int) lpf_addr_.size(), (const long long*) (d + o_lfa), (const float*) (d + o_lfv), d_liw_.as<float>()))
-		return fail(-2, "patch launch failed");
-	NodeSpecies ns;
+	// every patch below in one launch: the multigrid's level-0 row sums of the
+	// touched rows and a zero residual in the slots that lost their row, the
+	// slot tables, the row sums and charges, the couplings, the inner and long
+	// rows, and the node's species values
+	PatchBatch pb;
+	auto job = [&](size_t n, int width, size_t o_addr, size_t o_val, void* dst) {
+		if (n == 0) return;
+		PatchJob& jb = pb.job[pb.count++];
+		jb.n = (int) n;
+		jb.width = width;
+		jb.addr = (const long long*) (d + o_addr);
+		jb.val = d + o_val;
+		jb.dst = dst;
+	};
+	const int rw = f64_ ? 8 : 4;
+	job(dn_addr.size(), 4, o_dna, o_dnv, amg_dn_.ptr());
+	job(res_addr.size(), 4, o_rsa, o_rsv, amg_res_.ptr());
+	void* iarr[5] = {d_slot_node_.ptr(), d_slot_deg_.ptr(), d_col_.ptr(), d_inner_deg_.ptr(), d_icol_.ptr()};
+	for (int t = 0; t < 5; t++) job(ia[t].size(), 4, o_ia[t], o_iv[t], iarr[t]);
+	void* warr[3] = {nullptr, d_ws_.ptr(), d_q_.ptr()};
+	for (int t = 1; t < 3; t++) job(wa[t].size(), 8, o_wa[t], o_wv[t], warr[t]);
+	job(nr, rw, o_ra, o_rv, d_coup_.ptr());
+	job(nf, rw, o_fa, o_fv, d_coupfull_.ptr());
+	job(fa.size(), 4, o_ia3, o_iw3, d_iw_.ptr());
+	void* larr[2] = {d_lcol_.ptr(), d_licol_.ptr()};
+	for (int t = 0; t < 2; t++) job(la[t].size(), 4, o_la[t], o_lv[t], larr[t]);
+	job(lpr_addr_.size(), rw, o_lra, o_lrv, d_lcoup_.ptr());
+	job(lpf_addr_.size(), 4, o_lfa, o_lfv, d_liw_.ptr());
+	NodeSpecies& ns = pb.node;
 	ns.node = node;
 	ns.flags = flags_[node];
 	ns.evap_field = species_[id].evap_field;
@@ -1233,7 +1219,7 @@ int Grid::set_id_and_resync(int node, int id)
 	ns.d_evap_field = d_evapF_.as<float>();
 	ns.d_evap_energy = d_evapE_.as<float>();
 	ns.d_epsilon = d_eps_.as<float>();
-	if (launch_set_node(stream_, ns)) return fail(-2, "node update launch failed");
+	if (launch_patch_batch(stream_, pb)) return fail(-2, "patch launch failed");
 	// everything above is queued on the stream: the host goes on
 	return upload_long();
 }

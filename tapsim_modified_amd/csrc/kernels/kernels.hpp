@@ -134,12 +134,24 @@ struct NodeSpecies {
 	unsigned char* d_flags;
 	float *d_evap_field, *d_evap_energy, *d_epsilon;
 };
-int launch_set_node(hipStream_t, const NodeSpecies&);
 
-// generic scatter of patches: dst[addr[k]] = val[k]
-int launch_scatter_f64(hipStream_t, int n, const long long* addr, const double* val, double* dst);
-int launch_scatter_f32(hipStream_t, int n, const long long* addr, const float* val, float* dst);
-int launch_scatter_i32(hipStream_t, int n, const long long* addr, const int* val, int* dst);
+// the patches of one take-away in one launch: job j writes dst_j[addr_j[k]]
+// = val_j[k] for k < n_j, elements of `width` bytes (4 or 8), and the node's
+// species values are written beside them; the jobs write distinct arrays,
+// so their order does not matter
+constexpr int kPatchJobs = 16;
+struct PatchJob {
+	int n = 0, width = 4;
+	const long long* addr = nullptr;
+	const void* val = nullptr;
+	void* dst = nullptr;
+};
+struct PatchBatch {
+	int count = 0;
+	PatchJob job[kPatchJobs];
+	NodeSpecies node;
+};
+int launch_patch_batch(hipStream_t, const PatchBatch&);
 
 // ---------------------------------------------------------------- accelerated relaxation
 // FP32 correction rows: the unknown neighbours of every row slot with the

@@ -12,6 +12,8 @@
 //     lv = sum_k (double) phi[j_k] * c_k ;  lv -= charge ;  lv /= sum_k c_k
 // (double accumulation for both float and double storage; the build uses
 // -ffp-contract=off so no product is fused into an add).
+#include <algorithm>
+
 #include "device_math.hpp"
 #include "kernels.hpp"
 #include "rows.hpp"
@@ -470,33 +472,33 @@ __global__ __launch_bounds__(1024) void k_local(SweepArgs a, LocalArgs L)
 	}
 }
 
-__global__ void k_scatter_f64(int n, const long long* addr, const double* val, double* dst)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) dst[addr[i]] = val[i];
-}
-__global__ void k_scatter_f32(int n, const long long* addr, const float* val, float* dst)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) dst[addr[i]] = val[i];
-}
 __global__ void k_fill_i32(int n, int* dst, int value)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) dst[i] = value;
 }
-__global__ void k_set_node(NodeSpecies v)
+
+// row y < count: job y's scatter (the values moved as bits); row count: the
+// node's species values
+__global__ __launch_bounds__(256) void k_patch_batch(PatchBatch b)
 {
-	if (threadIdx.x != 0) return;
-	v.d_flags[v.node] = v.flags;
-	v.d_evap_field[v.node] = v.evap_field;
-	v.d_evap_energy[v.node] = v.evap_energy;
-	v.d_epsilon[v.node] = v.epsilon;
-}
-__global__ void k_scatter_i32(int n, const long long* addr, const int* val, int* dst)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) dst[addr[i]] = val[i];
+	const int j = blockIdx.y;
+	if (j == b.count) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			const NodeSpecies& v = b.node;
+			v.d_flags[v.node] = v.flags;
+			v.d_evap_field[v.node] = v.evap_field;
+			v.d_evap_energy[v.node] = v.evap_energy;
+			v.d_epsilon[v.node] = v.epsilon;
+		}
+		return;
+	}
+	const PatchJob p = b.job[j];
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += gridDim.x * blockDim.x) {
+		const long long a = p.addr[i];
+		if (p.width == 8) ((unsigned long long*) p.dst)[a] = ((const unsigned long long*) p.val)[i];
+		else ((unsigned int*) p.dst)[a] = ((const unsigned int*) p.val)[i];
+	}
 }
 
 }  // namespace dev
@@ -548,33 +550,21 @@ int launch_local_relax(hipStream_t s, const SweepArgs& a, const LocalArgs& L, bo
 	return (int) hipGetLastError();
 }
 
-int launch_scatter_f64(hipStream_t s, int n, const long long* addr, const double* val, double* dst)
-{
-	if (n <= 0) return 0;
-	hipLaunchKernelGGL(dev::k_scatter_f64, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
-	return (int) hipGetLastError();
-}
-int launch_scatter_f32(hipStream_t s, int n, const long long* addr, const float* val, float* dst)
-{
-	if (n <= 0) return 0;
-	hipLaunchKernelGGL(dev::k_scatter_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
-	return (int) hipGetLastError();
-}
 int launch_fill_i32(hipStream_t s, int n, int* dst, int value)
 {
 	if (n <= 0) return 0;
 	hipLaunchKernelGGL(dev::k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, n, dst, value);
 	return (int) hipGetLastError();
 }
-int launch_set_node(hipStream_t s, const NodeSpecies& v)
+int launch_patch_batch(hipStream_t s, const PatchBatch& b)
 {
-	hipLaunchKernelGGL(dev::k_set_node, dim3(1), dim3(64), 0, s, v);
-	return (int) hipGetLastError();
-}
-int launch_scatter_i32(hipStream_t s, int n, const long long* addr, const int* val, int* dst)
-{
-	if (n <= 0) return 0;
-	hipLaunchKernelGGL(dev::k_scatter_i32, dim3((n + 255) / 256), dim3(256), 0, s, n, addr, val, dst);
+	if (b.count < 0 || b.count > kPatchJobs) return (int) hipErrorInvalidValue;
+	int blocks = 1;
+	for (int j = 0; j < b.count; j++) {
+		if (b.job[j].n < 0 || (b.job[j].width != 4 && b.job[j].width != 8)) return (int) hipErrorInvalidValue;
+		blocks = std::max(blocks, std::min(1024, (b.job[j].n + 255) / 256));
+	}
+	hipLaunchKernelGGL(dev::k_patch_batch, dim3(blocks, b.count + 1), dim3(256), 0, s, b);
 	return (int) hipGetLastError();
 }
 
