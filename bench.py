@@ -250,6 +250,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--train-events", type=int, default=200, help="events recorded for training the network")
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3, help="the network's learning rate (Adam)")
     ap.add_argument("--plain-steps", type=int, default=None, help="events of the plain FP64 run (default: steps)")
     ap.add_argument("--solver", default="multigrid", choices=("chebyshev", "multigrid", "multigrid-eager"),
                     help="FP32 correction solve of the accelerated paths (multigrid-eager: every kernel "
@@ -344,7 +345,7 @@ def main():
     t0 = time.perf_counter()
     held = max(1, graphs // 10)
     untrained = net.evaluate(graphs - held, graphs)
-    first, last = net.train(0, graphs - held, epochs=args.epochs, batch_graphs=16, lr=2e-3, seed=5)
+    first, last = net.train(0, graphs - held, epochs=args.epochs, batch_graphs=16, lr=args.lr, seed=5)
     heldout = net.evaluate(graphs - held, graphs)
     t_train = time.perf_counter() - t0
     log("trained on the GPU in %.1fs: loss %.3g -> %.3g, held-out %.3g (untrained %.3g)"

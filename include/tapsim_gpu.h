@@ -422,8 +422,9 @@ int tsg_mpnn_add_graph(tsg_mpnn*, int n, const int* ptr, const int* col, const f
 int tsg_mpnn_dataset(const tsg_mpnn*, int* graphs, long long* rows, long long* edges);
 int tsg_mpnn_clear(tsg_mpnn*);
 /* Adam over graphs [g_begin, g_end), batches of up to batch_graphs
- * consecutive graphs in shuffled order; mean batch loss of the first and
- * last epoch */
+ * consecutive graphs in shuffled order, the learning rate lr decayed over
+ * the epochs on a cosine (epoch e of E uses lr (1 + cos(pi e / E)) / 2);
+ * mean batch loss of the first and last epoch */
 int tsg_mpnn_train(tsg_mpnn*, int g_begin, int g_end, int epochs, int batch_graphs, float lr, unsigned seed,
                    double* first_loss, double* last_loss);
 int tsg_mpnn_evaluate(tsg_mpnn*, int g_begin, int g_end, double* loss);

@@ -676,6 +676,9 @@ class WarmStartNet:
         _chk(load().tsg_mpnn_clear(self._h))
 
     def train(self, g_begin, g_end, epochs, batch_graphs=16, lr=1e-3, seed=1):
+        """Adam over graphs [g_begin, g_end) (tsg_mpnn_train), the rate lr
+        decayed over the epochs on a cosine; the mean batch loss of the
+        first and the last epoch"""
         f, l = ctypes.c_double(), ctypes.c_double()
         _chk(load().tsg_mpnn_train(self._h, g_begin, g_end, epochs, batch_graphs, lr, seed, ctypes.byref(f),
                                    ctypes.byref(l)))

@@ -327,11 +327,14 @@ int MpnnNet::train(int g_begin, int g_end, int epochs, int batch_graphs, float l
 	std::vector<float> lh(nbatch);
 	for (int ep = 0; ep < epochs; ep++) {
 		std::shuffle(batches.begin(), batches.end(), rng);
+		// cosine decay of the learning rate over the epochs (from lr down to
+		// lr / 2 (1 + cos(pi (epochs - 1) / epochs)) in the last)
+		const float lr_ep = (float) (0.5 * lr * (1.0 + std::cos(3.14159265358979323846 * ep / epochs)));
 		for (int k = 0; k < nbatch; k++) {
 			const MpnnBatch b = batch(batches[k].first, batches[k].second);
 			if (int rc = forward(b)) return rc;
 			if (int rc = backward(b, k)) return rc;
-			if (int rc = adam(lr)) return rc;
+			if (int rc = adam(lr_ep)) return rc;
 		}
 		if (ep == 0 || ep == epochs - 1) {
 			MCHK(hipMemcpyAsync(lh.data(), d_losses_.ptr(), sizeof(float) * nbatch, hipMemcpyDeviceToHost, stream_));
