@@ -84,6 +84,37 @@ __global__ __launch_bounds__(256) void k_gemm(int n, int K, int N, const bf16_t*
 }
 
 // one wavefront per row, one lane per channel; edges in CSR order
+// sum over the row's edges e (in CSR order) of a[e] x[col[e] - row0][o]:
+// the wavefront loads 64 edges' columns and weights at once (lane k holds
+// edge c + k) and gathers the neighbours' rows eight at a time, so the
+// gathers do not wait on one column load each; the additions stay in edge
+// order
+__device__ __forceinline__ float edge_sum(int e0, int e1, int row0, const int* __restrict__ col,
+                                          const float* __restrict__ a, const float* x, int stride, int o, float m)
+{
+	for (int c = e0; c < e1; c += 64) {
+		const int cnt = min(64, e1 - c);
+		int cj = 0;
+		float aj = 0.0f;
+		if (o < cnt) {
+			cj = col[c + o] - row0;
+			aj = a[c + o];
+		}
+		for (int k0 = 0; k0 < cnt; k0 += 8) {
+			float v[8];
+#pragma unroll
+			for (int k = 0; k < 8; k++) {
+				const int j = __shfl(cj, k0 + k, 64);
+				v[k] = k0 + k < cnt ? x[(size_t) j * stride + o] : 0.0f;
+			}
+#pragma unroll
+			for (int k = 0; k < 8; k++)
+				if (k0 + k < cnt) m += __shfl(aj, k0 + k, 64) * v[k];
+		}
+	}
+	return m;
+}
+
 __global__ __launch_bounds__(256) void k_aggregate(int n, int row0, const int* __restrict__ ptr, const int* __restrict__ col,
                                                    const float* __restrict__ a, const float* __restrict__ h,
                                                    bf16_t* __restrict__ cat)
@@ -92,8 +123,7 @@ __global__ __launch_bounds__(256) void k_aggregate(int n, int row0, const int* _
 	const int o = threadIdx.x & 63;
 	if (i >= n) return;
 	const int e0 = ptr[row0 + i], e1 = ptr[row0 + i + 1];
-	float m = 0.0f;
-	for (int e = e0; e < e1; e++) m += a[e] * h[(size_t) (col[e] - row0) * kMpnnHid + o];
+	const float m = edge_sum(e0, e1, row0, col, a, h, kMpnnHid, o, 0.0f);
 	cat[(size_t) i * kMpnnCat + kMpnnHid + o] = f2bf(m);
 }
 
@@ -105,9 +135,8 @@ __global__ __launch_bounds__(256) void k_aggregate_bwd(int n, int row0, const in
 	const int o = threadIdx.x & 63;
 	if (i >= n) return;
 	const int e0 = ptr[row0 + i], e1 = ptr[row0 + i + 1];
-	float d = dh[(size_t) i * kMpnnHid + o] + dcat[(size_t) i * kMpnnCat + o];
-	for (int e = e0; e < e1; e++) d += a[e] * dcat[(size_t) (col[e] - row0) * kMpnnCat + kMpnnHid + o];
-	dh_prev[(size_t) i * kMpnnHid + o] = d;
+	const float d0 = dh[(size_t) i * kMpnnHid + o] + dcat[(size_t) i * kMpnnCat + o];
+	dh_prev[(size_t) i * kMpnnHid + o] = edge_sum(e0, e1, row0, col, a, dcat + kMpnnHid, kMpnnCat, o, d0);
 }
 
 __global__ __launch_bounds__(256) void k_out(int n, const float* __restrict__ h3, const bf16_t* __restrict__ wout,
